@@ -1,0 +1,330 @@
+"""ctypes binding of the gfx950 C ABI (include/giaohip.h).
+
+The library ``giaoint_amd/libgiaohip.so`` is the only compute path: when it
+is missing or no GPU is visible, every call raises. There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY = Path(__file__).resolve().parent / "libgiaohip.so"
+
+OK, INVALID_ARGUMENT, LENGTH_ERROR, OUT_OF_RANGE, BOYS_ERROR, DEVICE_ERROR, UNSUPPORTED = range(7)
+
+
+class BoysNumericalError(ArithmeticError):
+    """Raised where the reference raises giao::BoysNumericalError (boys.hpp:31-34)."""
+
+
+class GpuUnavailableError(RuntimeError):
+    """The HIP library is not built or no GPU is visible."""
+
+
+_EXC = {
+    INVALID_ARGUMENT: ValueError,
+    LENGTH_ERROR: ValueError,
+    OUT_OF_RANGE: IndexError,
+    BOYS_ERROR: BoysNumericalError,
+    DEVICE_ERROR: RuntimeError,
+    UNSUPPORTED: RuntimeError,
+}
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("requested_quartets", C.c_uint64),
+        ("computed_quartets", C.c_uint64),
+        ("screened_quartets", C.c_uint64),
+        ("primitive_quartets", C.c_uint64),
+        ("seed_ms", C.c_double),
+        ("transform_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("launches", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+# name: (argtypes, restype)
+_SIGNATURES = {
+    "giaohip_create": ([C.c_int, C.POINTER(_vp)], C.c_int),
+    "giaohip_destroy": ([_vp], C.c_int),
+    "giaohip_last_error": ([_vp, C.c_char_p, C.c_size_t], C.c_size_t),
+    "giaohip_stream": ([_vp], _vp),
+    "giaohip_last_stats": ([_vp, C.POINTER(Stats)], C.c_int),
+    "giaohip_device_alloc": ([_vp, C.c_size_t, C.POINTER(_vp)], C.c_int),
+    "giaohip_device_free": ([_vp, _vp], C.c_int),
+    "giaohip_memcpy_to_host": ([_vp, _vp, _vp, C.c_size_t], C.c_int),
+    "giaohip_synchronize": ([_vp], C.c_int),
+    "giaohip_basis_create": ([_vp, C.c_int, _dp, _ip, _ip, _ip, _dp, _dp, C.POINTER(_vp)], C.c_int),
+    "giaohip_basis_destroy": ([_vp, _vp], C.c_int),
+    "giaohip_basis_ao_count": ([_vp], C.c_size_t),
+    "giaohip_normalize_shells": ([C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, C.c_char_p, C.c_size_t], C.c_int),
+    "giaohip_primitive_normalization": ([C.c_double, C.c_int, C.c_int, C.c_int], C.c_double),
+    "giaohip_boys": ([_vp, _dp, C.c_size_t, C.c_int, C.c_int, _dp, _ip, _dp, _ip], C.c_int),
+    "giaohip_eri_batch": ([_vp, _vp, _u32p, C.c_size_t, _dp, _u64p, _vp, C.c_int], C.c_int),
+    "giaohip_eri_batch_size": ([_vp, _u32p, C.c_size_t, _u64p], C.c_int),
+    "giaohip_eri_tensor": ([_vp, _vp, _dp, _vp, C.c_int], C.c_int),
+    "giaohip_primitive_eri": ([_vp, _dp, _ip, _dp, _dp], C.c_int),
+    "giaohip_cholesky_diagonal": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp], C.c_int),
+    "giaohip_cholesky_columns": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp, C.c_int], C.c_int),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load_library(path: Path | None = None):
+    """Load libgiaohip.so and declare its signatures (no GPU needed)."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path else LIBRARY
+        if not p.exists():
+            raise GpuUnavailableError(
+                f"{p} is not built; run `python -m giaoint_amd.build` (hipcc, gfx950)"
+            )
+        lib = C.CDLL(str(p))
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def normalize_shells(L, nprim, ncontr, exps, coefs) -> np.ndarray:
+    """Contraction-normalize many shells at once (host code in the library)."""
+    lib = load_library()
+    L = np.ascontiguousarray(L, dtype=np.int32)
+    nprim = np.ascontiguousarray(nprim, dtype=np.int32)
+    ncontr = np.ascontiguousarray(ncontr, dtype=np.int32)
+    exps = np.ascontiguousarray(exps, dtype=np.float64)
+    coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    out = np.empty_like(coefs)
+    msg = C.create_string_buffer(256)
+    st = lib.giaohip_normalize_shells(
+        int(L.size), _p(L, C.c_int), _p(nprim, C.c_int), _p(ncontr, C.c_int), _p(exps), _p(coefs), _p(out), msg, 256
+    )
+    if st != OK:
+        raise ValueError(msg.value.decode())
+    return out
+
+
+def primitive_normalization(exponent: float, angular) -> float:
+    if not np.isfinite(exponent) or exponent <= 0.0:
+        raise ValueError("Gaussian exponent must be finite and positive")
+    ax, ay, az = (int(v) for v in angular)
+    return float(load_library().giaohip_primitive_normalization(float(exponent), ax, ay, az))
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def field6(B=(0.0, 0.0, 0.0), origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    f = np.ascontiguousarray([*map(float, B), *map(float, origin)], dtype=np.float64)
+    if f.shape != (6,):
+        raise ValueError("field needs three B components and three origin coordinates")
+    return f
+
+
+class Context:
+    """One HIP stream on one device (giaohip_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        handle = _vp()
+        st = self.lib.giaohip_create(int(device), C.byref(handle))
+        if st != OK:
+            raise GpuUnavailableError(f"giaohip_create(device={device}) failed with status {st}: no usable GPU")
+        self.handle = handle
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.giaohip_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, status: int):
+        if status == OK:
+            return
+        buf = C.create_string_buffer(1024)
+        self.lib.giaohip_last_error(self.handle, buf, 1024)
+        raise _EXC.get(status, RuntimeError)(buf.value.decode())
+
+    def stats(self) -> dict:
+        s = Stats()
+        self.check(self.lib.giaohip_last_stats(self.handle, C.byref(s)))
+        return s.as_dict()
+
+    # --- device memory -------------------------------------------------
+    def alloc(self, nbytes: int) -> int:
+        p = _vp()
+        self.check(self.lib.giaohip_device_alloc(self.handle, C.c_size_t(int(nbytes)), C.byref(p)))
+        return p.value
+
+    def free(self, ptr: int):
+        self.check(self.lib.giaohip_device_free(self.handle, _vp(ptr)))
+
+    def to_host(self, ptr: int, out: np.ndarray):
+        self.check(self.lib.giaohip_memcpy_to_host(self.handle, out.ctypes.data_as(_vp), _vp(ptr), out.nbytes))
+        return out
+
+    def synchronize(self):
+        self.check(self.lib.giaohip_synchronize(self.handle))
+
+    # --- Boys ------------------------------------------------------------
+    def boys(self, z, nmax: int, scaled: bool = False):
+        z = np.ascontiguousarray(np.atleast_1d(np.asarray(z, dtype=np.complex128)))
+        n = z.size
+        out = np.zeros((n, nmax + 1), dtype=np.complex128) if nmax >= 0 else np.zeros((n, 1), np.complex128)
+        region = np.zeros(n, dtype=np.int32)
+        err = np.zeros(n)
+        seg = np.zeros(n, dtype=np.int32)
+        self.check(
+            self.lib.giaohip_boys(
+                self.handle, _p(z.view(np.float64)), C.c_size_t(n), C.c_int(nmax), C.c_int(int(bool(scaled))),
+                _p(out.view(np.float64)), _p(region, C.c_int), _p(err), _p(seg, C.c_int),
+            )
+        )
+        return out, region, err, seg
+
+    def primitive_eri(self, prims, field) -> complex:
+        p = np.zeros((4, 6))
+        a = np.zeros((4, 3), dtype=np.int32)
+        for i, d in enumerate(prims):
+            p[i] = [d["exponent"], *d["center"], d.get("coefficient", 1.0), float(bool(d.get("normalized", True)))]
+            a[i] = d.get("angular", (0, 0, 0))
+        out = np.zeros(2)
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.check(self.lib.giaohip_primitive_eri(self.handle, _p(p), _p(a, C.c_int), _p(f), _p(out)))
+        return complex(out[0], out[1])
+
+
+class DeviceBasis:
+    """A basis uploaded to one context (giaohip_basis)."""
+
+    def __init__(self, ctx: Context, centers, L, nprim, ncontr, exps, coefs):
+        self.ctx = ctx
+        self._centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1)
+        self._L = np.ascontiguousarray(L, dtype=np.int32)
+        self._nprim = np.ascontiguousarray(nprim, dtype=np.int32)
+        self._ncontr = np.ascontiguousarray(ncontr, dtype=np.int32)
+        self._exps = np.ascontiguousarray(exps, dtype=np.float64)
+        self._coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+        self.nshell = int(self._L.size)
+        self.ao_counts = ((self._L + 1) * (self._L + 2) // 2 * self._ncontr).astype(np.int64)
+        self.nao = int(self.ao_counts.sum())
+        handle = _vp()
+        ctx.check(
+            ctx.lib.giaohip_basis_create(
+                ctx.handle, C.c_int(self.nshell), _p(self._centers), _p(self._L, C.c_int), _p(self._nprim, C.c_int),
+                _p(self._ncontr, C.c_int), _p(self._exps), _p(self._coefs), C.byref(handle),
+            )
+        )
+        self.handle = handle
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.giaohip_basis_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def quartet_sizes(self, quartets) -> np.ndarray:
+        q = np.asarray(quartets, dtype=np.int64).reshape(-1, 4)
+        return np.prod(self.ao_counts[q], axis=1) if len(q) else np.zeros(0, np.int64)
+
+    def eri_batch(self, quartets, field, out=None, device_out: int | None = None):
+        """Packed complex128 blocks of the quartets (host array), or into device_out."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.uint32).reshape(-1, 4))
+        sizes = self.quartet_sizes(q)
+        total = int(sizes.sum())
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        ctx = self.ctx
+        if device_out is not None:
+            ctx.check(
+                ctx.lib.giaohip_eri_batch(
+                    ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)), _p(f), None, _vp(device_out), 1
+                )
+            )
+            return None
+        if out is None:
+            out = np.empty(total, dtype=np.complex128)
+        ctx.check(
+            ctx.lib.giaohip_eri_batch(
+                ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)), _p(f), None,
+                out.ctypes.data_as(_vp), 0,
+            )
+        )
+        return out
+
+    def eri_tensor(self, field):
+        n = self.nao
+        out = np.empty((n, n, n, n), dtype=np.complex128)
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.giaohip_eri_tensor(self.ctx.handle, self.handle, _p(f), out.ctypes.data_as(_vp), 0))
+        return out
+
+    def cholesky_diagonal(self, pairs, field):
+        n = self.nao
+        p = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint64).reshape(-1, 2))
+        out = np.empty((n, n))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.ctx.check(
+            self.ctx.lib.giaohip_cholesky_diagonal(
+                self.ctx.handle, self.handle, _p(p, C.c_uint64), C.c_size_t(len(p)), _p(f), _p(out)
+            )
+        )
+        return out
+
+    def cholesky_columns(self, pivots, field, real: bool = False):
+        n = self.nao
+        p = np.ascontiguousarray(np.asarray(pivots, dtype=np.uint64).reshape(-1, 2))
+        out = np.empty((len(p), n * n), dtype=np.float64 if real else np.complex128)
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.ctx.check(
+            self.ctx.lib.giaohip_cholesky_columns(
+                self.ctx.handle, self.handle, _p(p, C.c_uint64), C.c_size_t(len(p)), _p(f),
+                out.ctypes.data_as(_dp), C.c_int(int(real)),
+            )
+        )
+        return out
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    with _lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx

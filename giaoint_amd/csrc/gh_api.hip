@@ -1,0 +1,969 @@
+// C ABI of the gfx950 London-orbital ERI library (declared in include/giaohip.h).
+//
+// Host responsibilities: validation with the reference's error semantics,
+// per-shell tables (normalizations, London wave vectors), planning of device
+// work (seed chunks bounded in memory, work items grouped by angular-momentum
+// class, general contractions split into contraction rows), and copies.
+// Every integral is computed by the kernels in gh_eri.hip.
+#include "giaohip.h"
+#include "gh_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <set>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using gh::DevError;
+using gh::DevShell;
+using gh::SeedQuartet;
+using gh::WorkItem;
+using gh::zc;
+
+namespace {
+
+struct DevBuffer {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct Status {
+    int code = GIAOHIP_OK;
+    std::string msg;
+};
+
+constexpr double kPi = 3.141592653589793;
+constexpr size_t kMaxAuxEntries = 4000000u;  // eri.cpp:24
+
+}  // namespace
+
+struct giaohip_basis {
+    int ns = 0;
+    long long nao = 0;
+    std::vector<DevShell> shells;
+    std::vector<long long> ao_off;
+    std::vector<double> exps, coefs, norms;
+    std::vector<int> owner;
+    DevBuffer d_shells, d_exps, d_coefs, d_norms, d_aooff, d_owner;
+};
+
+struct giaohip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    giaohip_stats stats{};
+    DevBuffer seeds, squartets, items, kappa, errrec, staging, blocks, misc_a, misc_b, misc_c;
+    hipEvent_t ev[4]{};
+    std::set<void*> user_allocs;
+    size_t seed_budget_bytes = size_t(4) << 30;
+    size_t staging_budget_bytes = size_t(1) << 30;
+};
+
+namespace {
+
+int set_error(giaohip_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(giaohip_ctx* ctx, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_error(ctx, GIAOHIP_DEVICE_ERROR, m);
+}
+
+#define HIP_TRY(expr, what)                                  \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
+    } while (0)
+
+// types.cpp:168-175,207-219
+double log_odd_double_factorial(int l) {
+    if (l == 0) return 0.0;
+    const double n = (double)l;
+    return n * std::log(2.0) + std::lgamma(n + 0.5) - 0.5 * std::log(kPi);
+}
+
+double primitive_normalization(double exponent, int ax, int ay, int az) {
+    const double total = (double)(ax + ay + az);
+    const double log_den = log_odd_double_factorial(ax) + log_odd_double_factorial(ay) + log_odd_double_factorial(az);
+    const double log_norm = 0.75 * std::log(2.0 * exponent / kPi) + 0.5 * (total * std::log(4.0 * exponent) - log_den);
+    return std::exp(log_norm);
+}
+
+std::vector<std::array<int, 3>> cartesian_components(int l) {
+    std::vector<std::array<int, 3>> out;
+    for (int x = l; x >= 0; --x)
+        for (int y = l - x; y >= 0; --y) out.push_back({x, y, l - x - y});
+    return out;
+}
+
+// MagneticField::london_wave_vector (types.cpp:326-333)
+std::vector<double> wave_vectors(const giaohip_basis* b, const double f[6]) {
+    std::vector<double> k(3 * (size_t)b->ns);
+    for (int s = 0; s < b->ns; ++s) {
+        const double* c = b->shells[s].center;
+        const double dx = c[0] - f[3], dy = c[1] - f[4], dz = c[2] - f[5];
+        k[3 * s + 0] = 0.5 * (f[1] * dz - f[2] * dy);
+        k[3 * s + 1] = 0.5 * (f[2] * dx - f[0] * dz);
+        k[3 * s + 2] = 0.5 * (f[0] * dy - f[1] * dx);
+    }
+    return k;
+}
+
+int nmax_bucket(int L) {
+    for (int b : {4, 8, 12, 16, 24}) if (L <= b) return b;
+    return 32;
+}
+
+// Maximum over component quartets of (L+1) * prod over the six axes of
+// (power sum + 1), the reference's auxiliary admission size (eri.cpp:85-98).
+size_t max_aux_entries(int la, int lb, int lc, int ld) {
+    auto pair_max = [](int l1, int l2) {
+        size_t best = 0;
+        for (auto& a : cartesian_components(l1))
+            for (auto& b : cartesian_components(l2)) {
+                size_t v = 1;
+                for (int i = 0; i < 3; ++i) v *= (size_t)(a[i] + b[i] + 1);
+                best = std::max(best, v);
+            }
+        return best;
+    };
+    return (size_t)(la + lb + lc + ld + 1) * pair_max(la, lb) * pair_max(lc, ld);
+}
+
+// A request: one shell quartet, optionally restricted to fixed ket
+// contraction rows and a ket component range, written with AO strides.
+struct Request {
+    int sh[4];
+    int xfix[4];  // -1 = every contraction row
+    int cd0, cd1; // ket component range; cd1 < 0 = all
+    long long out_base;
+    long long stride[4];
+};
+
+int check_quartet_supported(giaohip_ctx* ctx, const giaohip_basis* b, const int sh[4]) {
+    int l[4];
+    for (int i = 0; i < 4; ++i) l[i] = b->shells[sh[i]].l;
+    const int L = l[0] + l[1] + l[2] + l[3];
+    if (L > 32) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "electron repulsion supports combined angular order through 32");
+    if (max_aux_entries(l[0], l[1], l[2], l[3]) > kMaxAuxEntries)
+        return set_error(ctx, GIAOHIP_LENGTH_ERROR, "ERI auxiliary exceeds the workspace limit");
+    for (int i = 0; i < 4; ++i)
+        if (l[i] > gh::kMaxShellL)
+            return set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support shell angular momentum through 8");
+    if (L > gh::kMaxTotalL)
+        return set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support combined angular order through 24");
+    return GIAOHIP_OK;
+}
+
+int boys_error(giaohip_ctx* ctx, const DevError& e) {
+    switch (e.code) {
+        case gh::kErrBoysDomain: {
+            std::ostringstream m;
+            m << "complex Boys argument " << std::complex<double>(e.zr, e.zi) << " at order " << e.order
+              << " is outside the verified direct domain |z| <= " << 160.0
+              << " and the conservative large-positive asymptotic sector";
+            return set_error(ctx, GIAOHIP_BOYS_ERROR, m.str());
+        }
+        case gh::kErrBoysSeries: return set_error(ctx, GIAOHIP_BOYS_ERROR, "complex Boys power series did not converge");
+        case gh::kErrBoysQuadrature:
+            return set_error(ctx, GIAOHIP_BOYS_ERROR, "adaptive complex Boys quadrature did not reach tolerance");
+        case gh::kErrBoysScaling:
+            return set_error(ctx, GIAOHIP_BOYS_ERROR, "requested Boys scaling is not representable for this argument");
+        case gh::kErrBoysUnscaled:
+            return set_error(ctx, GIAOHIP_BOYS_ERROR, "unscaled Boys value is not representable for this argument");
+        default: return set_error(ctx, GIAOHIP_BOYS_ERROR, "complex Boys evaluation failed");
+    }
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    return ms;
+}
+
+// Runs a list of requests; outputs go to d_out (device). Accumulates stats.
+int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
+                 zc* d_out) {
+    if (reqs.empty()) return GIAOHIP_OK;
+    hipStream_t s = ctx->stream;
+    const std::vector<double> kap = wave_vectors(b, field);
+    HIP_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
+    HIP_TRY(hipMemcpyAsync(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+    HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
+    HIP_TRY(hipMemsetAsync(ctx->errrec.p, 0, sizeof(DevError), s), "hipMemsetAsync");
+
+    const int maxw = gh::transform_max_w(), maxo = gh::transform_max_o();
+    size_t begin = 0;
+    while (begin < reqs.size()) {
+        // Chunk so that the seed buffer stays within budget.
+        size_t end = begin;
+        size_t seed_zc = 0;
+        long long pq_total = 0;
+        int maxL = 0;
+        std::vector<SeedQuartet> sq;
+        while (end < reqs.size()) {
+            const Request& r = reqs[end];
+            const DevShell &A = b->shells[r.sh[0]], &B = b->shells[r.sh[1]], &C = b->shells[r.sh[2]],
+                           &D = b->shells[r.sh[3]];
+            const int L = A.l + B.l + C.l + D.l;
+            const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
+            const size_t need = (size_t)(kab * kcd) * (size_t)(L + 4);
+            if (end > begin && (seed_zc + need) * sizeof(zc) > ctx->seed_budget_bytes) break;
+            SeedQuartet q{};
+            for (int i = 0; i < 4; ++i) q.sh[i] = r.sh[i];
+            q.L = L;
+            q.kab = (int)kab;
+            q.kcd = (int)kcd;
+            q.pq_begin = pq_total;
+            q.seed_off = (long long)seed_zc;
+            sq.push_back(q);
+            seed_zc += need;
+            pq_total += kab * kcd;
+            maxL = std::max(maxL, L);
+            ++end;
+        }
+        HIP_TRY(ctx->seeds.ensure(seed_zc * sizeof(zc)), "hipMalloc(seeds)");
+        HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
+        HIP_TRY(hipMemcpyAsync(ctx->squartets.p, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+
+        // Work items grouped by (class, tile).
+        std::map<std::tuple<int, int, int, int, int>, std::vector<WorkItem>> groups;
+        for (size_t i = begin; i < end; ++i) {
+            const Request& r = reqs[i];
+            const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
+            const int ncd_all = S[2]->ncart * S[3]->ncart;
+            const int cd0 = r.cd1 < 0 ? 0 : r.cd0, cd1 = r.cd1 < 0 ? ncd_all : r.cd1;
+            const int nab = S[0]->ncart * S[1]->ncart;
+            const int NHab = gh::nherm(S[0]->l + S[1]->l);
+            int tile = std::min({cd1 - cd0, maxw / NHab, maxo / nab});
+            if (tile < 1) return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class too large for the GPU transform kernel");
+            gh::ClassArgs ca{{S[0]->l, S[1]->l, S[2]->l, S[3]->l}, tile};
+            while (tile > 1 && gh::transform_lds_bytes(ca) > 160 * 1024) ca.tile = --tile;
+            if (gh::transform_lds_bytes(ca) > 160 * 1024)
+                return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class exceeds the LDS capacity of the transform kernel");
+            auto& vec = groups[{S[0]->l, S[1]->l, S[2]->l, S[3]->l, tile}];
+            int lo[4], hi[4];
+            for (int k = 0; k < 4; ++k) {
+                lo[k] = r.xfix[k] < 0 ? 0 : r.xfix[k];
+                hi[k] = r.xfix[k] < 0 ? S[k]->ncontr : r.xfix[k] + 1;
+            }
+            for (int xa = lo[0]; xa < hi[0]; ++xa)
+                for (int xb = lo[1]; xb < hi[1]; ++xb)
+                    for (int xc = lo[2]; xc < hi[2]; ++xc)
+                        for (int xd = lo[3]; xd < hi[3]; ++xd) {
+                            WorkItem w{};
+                            for (int k = 0; k < 4; ++k) w.sh[k] = r.sh[k];
+                            w.x[0] = xa;
+                            w.x[1] = xb;
+                            w.x[2] = xc;
+                            w.x[3] = xd;
+                            w.cd0 = cd0;
+                            w.cd1 = cd1;
+                            w.seed_off = sq[i - begin].seed_off;
+                            w.out_base = r.out_base;
+                            for (int k = 0; k < 4; ++k) w.stride[k] = r.stride[k];
+                            vec.push_back(w);
+                        }
+        }
+        std::vector<WorkItem> all;
+        std::vector<std::pair<gh::ClassArgs, std::pair<size_t, size_t>>> launches;
+        for (auto& [key, vec] : groups) {
+            gh::ClassArgs ca{{std::get<0>(key), std::get<1>(key), std::get<2>(key), std::get<3>(key)}, std::get<4>(key)};
+            launches.push_back({ca, {all.size(), vec.size()}});
+            all.insert(all.end(), vec.begin(), vec.end());
+        }
+        HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
+        HIP_TRY(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+
+        HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
+        HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>(), (int)sq.size(), pq_total,
+                                b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
+                                ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), s),
+                "seed kernel launch");
+        HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
+        for (auto& [ca, range] : launches) {
+            HIP_TRY(gh::launch_transform(ca, ctx->items.as<WorkItem>() + range.first, (int)range.second,
+                                         b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                                         b->d_norms.as<double>(), ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_out, s),
+                    "transform kernel launch");
+            ctx->stats.launches += 1;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[2], s), "hipEventRecord");
+        HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
+        ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
+        ctx->stats.transform_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+        ctx->stats.launches += 1;
+        ctx->stats.primitive_quartets += (uint64_t)pq_total;
+        DevError de{};
+        HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (de.code != 0) return boys_error(ctx, de);
+        begin = end;
+    }
+    return GIAOHIP_OK;
+}
+
+void reset_stats(giaohip_ctx* ctx) { ctx->stats = giaohip_stats{}; }
+
+long long ao_count(const giaohip_basis* b, int s) { return b->shells[s].ao_count; }
+
+}  // namespace
+
+extern "C" {
+
+double giaohip_primitive_normalization(double exponent, int ax, int ay, int az) {
+    return primitive_normalization(exponent, ax, ay, az);
+}
+
+// types.cpp:177-188 (same-centre overlap of two normalized primitives)
+static double same_center_overlap(double alpha, double beta, int l) {
+    const double norm = primitive_normalization(alpha, l, 0, 0) * primitive_normalization(beta, l, 0, 0);
+    const double p = alpha + beta;
+    const double lm = std::lgamma((double)l + 0.5) - ((double)l + 0.5) * std::log(p);
+    return norm * std::exp(lm) * kPi / p;
+}
+
+int giaohip_normalize_shells(int n, const int* L, const int* nprim, const int* ncontr, const double* exps,
+                             const double* coefs_in, double* coefs_out, char* message, size_t message_size) {
+    auto fail = [&](const char* m) {
+        if (message && message_size) {
+            std::snprintf(message, message_size, "%s", m);
+        }
+        return GIAOHIP_INVALID_ARGUMENT;
+    };
+    size_t po = 0, co = 0;
+    for (int s = 0; s < n; ++s) {
+        const int np = nprim[s], nc = ncontr[s];
+        if (np < 1) return fail("a shell requires at least one primitive");
+        if (nc < 1) return fail("a shell requires at least one contraction");
+        const double* e = exps + po;
+        for (int i = 0; i < np; ++i)
+            if (!std::isfinite(e[i]) || e[i] <= 0.0) return fail("shell exponents must be finite and positive");
+        for (int i = 0; i < np * nc; ++i)
+            if (!std::isfinite(coefs_in[co + i])) return fail("shell coefficients must be finite");
+        if (coefs_out != coefs_in) std::memcpy(coefs_out + co, coefs_in + co, sizeof(double) * (size_t)(np * nc));
+        for (int x = 0; x < nc; ++x) {
+            double* c = coefs_out + co + (size_t)x * np;
+            bool any = false;
+            for (int i = 0; i < np; ++i) any = any || c[i] != 0.0;
+            if (!any) return fail("a shell contraction cannot have all-zero coefficients");
+            double norm2 = 0.0;
+            for (int i = 0; i < np; ++i)
+                for (int j = 0; j < np; ++j) norm2 += c[i] * c[j] * same_center_overlap(e[i], e[j], L[s]);
+            if (!std::isfinite(norm2) || norm2 <= 0.0) return fail("contraction has a non-positive or non-finite norm");
+            const double scale = 1.0 / std::sqrt(norm2);
+            for (int i = 0; i < np; ++i) c[i] *= scale;
+        }
+        po += (size_t)np;
+        co += (size_t)np * nc;
+    }
+    return GIAOHIP_OK;
+}
+
+int giaohip_create(int device, giaohip_ctx** out) {
+    if (!out) return GIAOHIP_INVALID_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GIAOHIP_DEVICE_ERROR;
+    if (device < 0 || device >= count) return GIAOHIP_INVALID_ARGUMENT;
+    auto* ctx = new giaohip_ctx;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+    if (e == hipSuccess) e = gh::upload_constants();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        delete ctx;
+        return GIAOHIP_DEVICE_ERROR;
+    }
+    *out = ctx;
+    return GIAOHIP_OK;
+}
+
+int giaohip_destroy(giaohip_ctx* ctx) {
+    if (!ctx) return GIAOHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuffer* b : {&ctx->seeds, &ctx->squartets, &ctx->items, &ctx->kappa, &ctx->errrec, &ctx->staging,
+                         &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c})
+        b->release();
+    for (void* p : ctx->user_allocs) (void)hipFree(p);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GIAOHIP_OK;
+}
+
+size_t giaohip_last_error(const giaohip_ctx* ctx, char* buffer, size_t size) {
+    if (!ctx) return 0;
+    if (buffer && size) {
+        const size_t n = std::min(size - 1, ctx->err.size());
+        std::memcpy(buffer, ctx->err.data(), n);
+        buffer[n] = '\0';
+    }
+    return ctx->err.size();
+}
+
+void* giaohip_stream(giaohip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int giaohip_last_stats(const giaohip_ctx* ctx, giaohip_stats* out) {
+    if (!ctx || !out) return GIAOHIP_INVALID_ARGUMENT;
+    *out = ctx->stats;
+    return GIAOHIP_OK;
+}
+
+int giaohip_device_alloc(giaohip_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, std::max<size_t>(bytes, 16)), "hipMalloc");
+    ctx->user_allocs.insert(p);
+    *out = p;
+    return GIAOHIP_OK;
+}
+
+int giaohip_device_free(giaohip_ctx* ctx, void* ptr) {
+    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->user_allocs.erase(ptr)) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "pointer not allocated by this context");
+    HIP_TRY(hipFree(ptr), "hipFree");
+    return GIAOHIP_OK;
+}
+
+int giaohip_memcpy_to_host(giaohip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return GIAOHIP_OK;
+}
+
+int giaohip_synchronize(giaohip_ctx* ctx) {
+    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return GIAOHIP_OK;
+}
+
+int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, const int* L, const int* nprim,
+                         const int* ncontr, const double* exps, const double* coefs, giaohip_basis** out) {
+    if (!ctx || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = nullptr;
+    if (nshell < 0) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell count must be non-negative");
+    auto* b = new giaohip_basis;
+    b->ns = nshell;
+    int po = 0, co = 0, no = 0;
+    long long ao = 0;
+    for (int s = 0; s < nshell; ++s) {
+        if (L[s] < 0 || nprim[s] < 1 || ncontr[s] < 1) {
+            delete b;
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "invalid shell description");
+        }
+        DevShell d{};
+        for (int i = 0; i < 3; ++i) d.center[i] = centers[3 * s + i];
+        d.l = L[s];
+        d.nprim = nprim[s];
+        d.ncontr = ncontr[s];
+        d.ncart = gh::ncart(L[s]);
+        d.prim_off = po;
+        d.coef_off = co;
+        d.norm_off = no;
+        d.ao_count = d.ncart * d.ncontr;
+        for (int p = 0; p < d.nprim; ++p) {
+            const double e = exps[po + p];
+            if (!std::isfinite(e) || e <= 0.0) {
+                delete b;
+                return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell exponents must be finite and positive");
+            }
+            b->exps.push_back(e);
+        }
+        for (int k = 0; k < d.nprim * d.ncontr; ++k) b->coefs.push_back(coefs[co + k]);
+        const auto comps = cartesian_components(d.l);
+        for (const auto& c : comps)
+            for (int p = 0; p < d.nprim; ++p) b->norms.push_back(primitive_normalization(exps[po + p], c[0], c[1], c[2]));
+        b->shells.push_back(d);
+        b->ao_off.push_back(ao);
+        for (int k = 0; k < d.ao_count; ++k) b->owner.push_back(s);
+        ao += d.ao_count;
+        po += d.nprim;
+        co += d.nprim * d.ncontr;
+        no += (int)comps.size() * d.nprim;
+    }
+    b->nao = ao;
+    (void)hipSetDevice(ctx->device);
+    auto up = [&](DevBuffer& buf, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = buf.ensure(bytes);
+        if (e != hipSuccess) return e;
+        if (bytes) e = hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice);
+        return e;
+    };
+    hipError_t e = up(b->d_shells, b->shells.data(), b->shells.size() * sizeof(DevShell));
+    if (e == hipSuccess) e = up(b->d_exps, b->exps.data(), b->exps.size() * sizeof(double));
+    if (e == hipSuccess) e = up(b->d_coefs, b->coefs.data(), b->coefs.size() * sizeof(double));
+    if (e == hipSuccess) e = up(b->d_norms, b->norms.data(), b->norms.size() * sizeof(double));
+    if (e == hipSuccess) e = up(b->d_aooff, b->ao_off.data(), b->ao_off.size() * sizeof(long long));
+    if (e == hipSuccess) e = up(b->d_owner, b->owner.data(), b->owner.size() * sizeof(int));
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(ctx, e, "basis upload");
+    }
+    *out = b;
+    return GIAOHIP_OK;
+}
+
+int giaohip_basis_destroy(giaohip_ctx* ctx, giaohip_basis* b) {
+    if (!b) return GIAOHIP_OK;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner}) d->release();
+    delete b;
+    return GIAOHIP_OK;
+}
+
+size_t giaohip_basis_ao_count(const giaohip_basis* b) { return b ? (size_t)b->nao : 0; }
+
+int giaohip_eri_batch_size(const giaohip_basis* b, const uint32_t* quartets, size_t n, uint64_t* total) {
+    if (!b || !total) return GIAOHIP_INVALID_ARGUMENT;
+    uint64_t t = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t sz = 1;
+        for (int k = 0; k < 4; ++k) {
+            if (quartets[4 * i + k] >= (uint32_t)b->ns) return GIAOHIP_OUT_OF_RANGE;
+            sz *= (uint64_t)b->shells[quartets[4 * i + k]].ao_count;
+        }
+        t += sz;
+    }
+    *total = t;
+    return GIAOHIP_OK;
+}
+
+int giaohip_boys(giaohip_ctx* ctx, const double* z, size_t n, int nmax, int scaled, double* out, int* region,
+                 double* error_estimate, int* segments) {
+    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (nmax < 0) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "Boys output must contain at least F_0");
+    if (nmax > gh::kMaxBoysOrder) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "complex Boys supports orders through 32");
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(z[2 * i]) || !std::isfinite(z[2 * i + 1]))
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "complex Boys argument must be finite");
+    if (n == 0) return GIAOHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t zb = n * sizeof(zc), ob = n * (size_t)(nmax + 1) * sizeof(zc);
+    HIP_TRY(ctx->misc_a.ensure(zb + ob), "hipMalloc");
+    HIP_TRY(ctx->misc_b.ensure(n * (2 * sizeof(int) + sizeof(double))), "hipMalloc");
+    HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
+    char* a = ctx->misc_a.as<char>();
+    char* bb = ctx->misc_b.as<char>();
+    int* d_region = reinterpret_cast<int*>(bb);
+    int* d_seg = d_region + n;
+    double* d_err = reinterpret_cast<double*>(bb + 2 * n * sizeof(int));
+    HIP_TRY(hipMemsetAsync(ctx->errrec.p, 0, sizeof(DevError), s), "hipMemsetAsync");
+    HIP_TRY(hipMemcpyAsync(a, z, zb, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    HIP_TRY(gh::launch_boys_probe(reinterpret_cast<zc*>(a), (long long)n, nmax, scaled, reinterpret_cast<zc*>(a + zb),
+                                  d_region, d_err, d_seg, ctx->errrec.as<DevError>(), s),
+            "Boys kernel launch");
+    HIP_TRY(hipMemcpyAsync(out, a + zb, ob, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    std::vector<int> reg(n), seg(n);
+    std::vector<double> er(n);
+    HIP_TRY(hipMemcpyAsync(reg.data(), d_region, n * sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(seg.data(), d_seg, n * sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(er.data(), d_err, n * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    DevError de{};
+    HIP_TRY(hipMemcpyAsync(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "Boys kernel execution");
+    if (de.code != 0) return boys_error(ctx, de);
+    if (region) std::copy(reg.begin(), reg.end(), region);
+    if (segments) std::copy(seg.begin(), seg.end(), segments);
+    if (error_estimate) std::copy(er.begin(), er.end(), error_estimate);
+    return GIAOHIP_OK;
+}
+
+int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
+                      const double field[6], const uint64_t* offsets, double* out, int out_on_device) {
+    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    ctx->stats.requested_quartets = n;
+    for (size_t i = 0; i < 4 * n; ++i)
+        if (quartets[i] >= (uint32_t)b->ns) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "ERI shell-quartet index is out of range");
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(field[k])) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "field values must be finite");
+    std::vector<uint64_t> offs(n), sizes(n);
+    uint64_t run = 0;
+    for (size_t i = 0; i < n; ++i) {
+        int sh[4];
+        for (int k = 0; k < 4; ++k) sh[k] = (int)quartets[4 * i + k];
+        const int st = check_quartet_supported(ctx, b, sh);
+        if (st) return st;
+        sizes[i] = (uint64_t)ao_count(b, sh[0]) * ao_count(b, sh[1]) * ao_count(b, sh[2]) * ao_count(b, sh[3]);
+        offs[i] = offsets ? offsets[i] : run;
+        run += sizes[i];
+    }
+    ctx->stats.computed_quartets = n;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipEventRecord(ctx->ev[3], s), "hipEventRecord");
+    hipEvent_t t0;
+    HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
+    HIP_TRY(hipEventRecord(t0, s), "hipEventRecord");
+    auto make_req = [&](size_t i, long long base) {
+        Request r{};
+        for (int k = 0; k < 4; ++k) {
+            r.sh[k] = (int)quartets[4 * i + k];
+            r.xfix[k] = -1;
+        }
+        r.cd0 = 0;
+        r.cd1 = -1;
+        r.out_base = base;
+        const long long nb = ao_count(b, r.sh[1]), nc = ao_count(b, r.sh[2]), nd = ao_count(b, r.sh[3]);
+        r.stride[0] = nb * nc * nd;
+        r.stride[1] = nc * nd;
+        r.stride[2] = nd;
+        r.stride[3] = 1;
+        return r;
+    };
+    int st = GIAOHIP_OK;
+    if (out_on_device) {
+        std::vector<Request> reqs;
+        reqs.reserve(n);
+        for (size_t i = 0; i < n; ++i) reqs.push_back(make_req(i, (long long)offs[i]));
+        st = run_requests(ctx, b, field, reqs, reinterpret_cast<zc*>(out));
+    } else {
+        // Stage groups of quartets through device memory, then copy back.
+        const size_t budget = ctx->staging_budget_bytes / sizeof(zc);
+        size_t i = 0;
+        std::vector<zc> host_tmp;
+        while (i < n && st == GIAOHIP_OK) {
+            size_t j = i;
+            uint64_t tot = 0;
+            while (j < n && (j == i || tot + sizes[j] <= budget)) tot += sizes[j++];
+            if (hipError_t e = ctx->staging.ensure(tot * sizeof(zc)); e != hipSuccess) {
+                st = hip_fail(ctx, e, "hipMalloc(staging)");
+                break;
+            }
+            std::vector<Request> reqs;
+            uint64_t local = 0;
+            for (size_t k = i; k < j; ++k) {
+                reqs.push_back(make_req(k, (long long)local));
+                local += sizes[k];
+            }
+            st = run_requests(ctx, b, field, reqs, ctx->staging.as<zc>());
+            if (st) break;
+            bool packed = true;
+            for (size_t k = i + 1; k < j; ++k) packed = packed && offs[k] == offs[k - 1] + sizes[k - 1];
+            if (packed) {
+                if (hipError_t e = hipMemcpy(out + 2 * offs[i], ctx->staging.p, tot * sizeof(zc), hipMemcpyDeviceToHost);
+                    e != hipSuccess) {
+                    st = hip_fail(ctx, e, "hipMemcpy");
+                    break;
+                }
+            } else {
+                host_tmp.resize(tot);
+                if (hipError_t e = hipMemcpy(host_tmp.data(), ctx->staging.p, tot * sizeof(zc), hipMemcpyDeviceToHost);
+                    e != hipSuccess) {
+                    st = hip_fail(ctx, e, "hipMemcpy");
+                    break;
+                }
+                uint64_t lo = 0;
+                for (size_t k = i; k < j; ++k) {
+                    std::memcpy(out + 2 * offs[k], host_tmp.data() + lo, sizes[k] * sizeof(zc));
+                    lo += sizes[k];
+                }
+            }
+            i = j;
+        }
+    }
+    hipEvent_t t1;
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t1, s);
+    (void)hipEventSynchronize(t1);
+    ctx->stats.total_ms = elapsed(t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return st;
+}
+
+int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out, int out_on_device) {
+    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    const int ns = b->ns;
+    const long long nao = b->nao;
+    // canonical quartets (eri.cpp:414-429,825-843)
+    std::vector<int> qs;
+    std::vector<Request> reqs;
+    std::vector<long long> boff;
+    long long total = 0;
+    for (int a = 0; a < ns; ++a)
+        for (int bb = 0; bb < ns; ++bb)
+            for (int c = 0; c < ns; ++c)
+                for (int d = 0; d < ns; ++d) {
+                    const std::array<int, 4> q{a, bb, c, d};
+                    const std::array<std::array<int, 4>, 4> cand{q, std::array<int, 4>{c, d, a, bb},
+                                                                 std::array<int, 4>{bb, a, d, c},
+                                                                 std::array<int, 4>{d, c, bb, a}};
+                    if (*std::min_element(cand.begin(), cand.end()) != q) continue;
+                    const int st = check_quartet_supported(ctx, b, q.data());
+                    if (st) return st;
+                    Request r{};
+                    for (int k = 0; k < 4; ++k) {
+                        r.sh[k] = q[k];
+                        r.xfix[k] = -1;
+                        qs.push_back(q[k]);
+                    }
+                    r.cd0 = 0;
+                    r.cd1 = -1;
+                    const long long nb = ao_count(b, bb), nc = ao_count(b, c), nd = ao_count(b, d);
+                    r.out_base = total;
+                    r.stride[0] = nb * nc * nd;
+                    r.stride[1] = nc * nd;
+                    r.stride[2] = nd;
+                    r.stride[3] = 1;
+                    reqs.push_back(r);
+                    boff.push_back(total);
+                    total += ao_count(b, a) * nb * nc * nd;
+                }
+    ctx->stats.requested_quartets = ctx->stats.computed_quartets = reqs.size();
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t tensor_bytes = (size_t)nao * nao * nao * nao * sizeof(zc);
+    HIP_TRY(ctx->blocks.ensure(std::max<size_t>(total, 1) * sizeof(zc)), "hipMalloc(blocks)");
+    HIP_TRY(ctx->misc_a.ensure(boff.size() * sizeof(long long) + qs.size() * sizeof(int) + 64), "hipMalloc");
+    zc* d_tensor = nullptr;
+    if (out_on_device) d_tensor = reinterpret_cast<zc*>(out);
+    else {
+        HIP_TRY(ctx->staging.ensure(std::max<size_t>(tensor_bytes, 16)), "hipMalloc(tensor)");
+        d_tensor = ctx->staging.as<zc>();
+    }
+    int st = run_requests(ctx, b, field, reqs, ctx->blocks.as<zc>());
+    if (st) return st;
+    long long* d_boff = ctx->misc_a.as<long long>();
+    int* d_q = reinterpret_cast<int*>(d_boff + boff.size());
+    HIP_TRY(hipMemcpyAsync(d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(d_q, qs.data(), qs.size() * sizeof(int), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    HIP_TRY(hipMemsetAsync(d_tensor, 0, tensor_bytes, s), "hipMemsetAsync");
+    HIP_TRY(gh::launch_tensor_scatter(d_boff, d_q, (int)boff.size(), total, b->d_shells.as<DevShell>(),
+                                      b->d_aooff.as<long long>(), nao, ctx->blocks.as<zc>(), d_tensor, s),
+            "scatter launch");
+    if (!out_on_device)
+        HIP_TRY(hipMemcpyAsync(out, d_tensor, tensor_bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "tensor execution");
+    return GIAOHIP_OK;
+}
+
+int giaohip_primitive_eri(giaohip_ctx* ctx, const double prim[4][6], const int ang[4][3], const double field[6],
+                          double out[2]) {
+    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    double centers[12], exps[4], coefs[4] = {1.0, 1.0, 1.0, 1.0};
+    int L[4], np[4] = {1, 1, 1, 1}, nc[4] = {1, 1, 1, 1};
+    for (int i = 0; i < 4; ++i) {
+        if (!std::isfinite(prim[i][0]) || prim[i][0] <= 0.0)
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "Gaussian exponent must be finite and positive");
+        exps[i] = prim[i][0];
+        for (int k = 0; k < 3; ++k) centers[3 * i + k] = prim[i][1 + k];
+        L[i] = ang[i][0] + ang[i][1] + ang[i][2];
+    }
+    giaohip_basis* b = nullptr;
+    int st = giaohip_basis_create(ctx, 4, centers, L, np, nc, exps, coefs, &b);
+    if (st) return st;
+    const uint32_t q[4] = {0, 1, 2, 3};
+    uint64_t total = 0;
+    giaohip_eri_batch_size(b, q, 1, &total);
+    std::vector<double> blk(2 * total);
+    st = giaohip_eri_batch(ctx, b, q, 1, field, nullptr, blk.data(), 0);
+    if (st == GIAOHIP_OK) {
+        int idx[4];
+        for (int i = 0; i < 4; ++i) {
+            const auto comps = cartesian_components(L[i]);
+            idx[i] = 0;
+            for (size_t k = 0; k < comps.size(); ++k)
+                if (comps[k][0] == ang[i][0] && comps[k][1] == ang[i][1] && comps[k][2] == ang[i][2]) idx[i] = (int)k;
+        }
+        const size_t k = (((size_t)idx[0] * gh::ncart(L[1]) + idx[1]) * gh::ncart(L[2]) + idx[2]) * gh::ncart(L[3]) + idx[3];
+        double scale = 1.0;
+        for (int i = 0; i < 4; ++i) {
+            scale *= prim[i][4];
+            if (prim[i][5] == 0.0) scale /= primitive_normalization(prim[i][0], ang[i][0], ang[i][1], ang[i][2]);
+        }
+        out[0] = scale * blk[2 * k];
+        out[1] = scale * blk[2 * k + 1];
+    }
+    giaohip_basis_destroy(ctx, b);
+    return st;
+}
+
+int giaohip_cholesky_diagonal(giaohip_ctx* ctx, const giaohip_basis* b, const uint64_t* pairs, size_t npairs,
+                              const double field[6], double* out) {
+    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    const int ns = b->ns;
+    const long long nao = b->nao;
+    std::vector<char> seen((size_t)ns * ns + 1, 0);
+    std::vector<int> pr;
+    std::vector<Request> reqs;
+    std::vector<long long> boff;
+    long long total = 0;
+    for (size_t i = 0; i < npairs; ++i) {
+        const uint64_t a = pairs[2 * i], bb = pairs[2 * i + 1];
+        if (a >= (uint64_t)ns || bb >= (uint64_t)ns || a > bb)
+            return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky diagonal shell pair must be valid and ordered");
+        if (seen[a * ns + bb]) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "duplicate Cholesky diagonal shell pair");
+        seen[a * ns + bb] = 1;
+    }
+    for (size_t i = 0; i < npairs; ++i) {
+        const int a = (int)pairs[2 * i], bb = (int)pairs[2 * i + 1];
+        const int q[4] = {a, bb, bb, a};
+        const int st = check_quartet_supported(ctx, b, q);
+        if (st) return st;
+        Request r{};
+        for (int k = 0; k < 4; ++k) {
+            r.sh[k] = q[k];
+            r.xfix[k] = -1;
+        }
+        r.cd0 = 0;
+        r.cd1 = -1;
+        const long long na = ao_count(b, a), nb = ao_count(b, bb);
+        r.out_base = total;
+        r.stride[0] = nb * nb * na;
+        r.stride[1] = nb * na;
+        r.stride[2] = na;
+        r.stride[3] = 1;
+        reqs.push_back(r);
+        boff.push_back(total);
+        pr.push_back(a);
+        pr.push_back(bb);
+        total += na * nb * nb * na;
+    }
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t dbytes = (size_t)nao * nao * sizeof(double);
+    HIP_TRY(ctx->blocks.ensure(std::max<long long>(total, 1) * sizeof(zc)), "hipMalloc(blocks)");
+    HIP_TRY(ctx->misc_a.ensure(boff.size() * sizeof(long long) + pr.size() * sizeof(int) + 64), "hipMalloc");
+    HIP_TRY(ctx->staging.ensure(std::max<size_t>(dbytes, 16)), "hipMalloc");
+    int st = run_requests(ctx, b, field, reqs, ctx->blocks.as<zc>());
+    if (st) return st;
+    long long* d_boff = ctx->misc_a.as<long long>();
+    int* d_pr = reinterpret_cast<int*>(d_boff + boff.size());
+    if (!boff.empty()) {
+        HIP_TRY(hipMemcpyAsync(d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        HIP_TRY(hipMemcpyAsync(d_pr, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    }
+    HIP_TRY(hipMemsetAsync(ctx->staging.p, 0, dbytes, s), "hipMemsetAsync");
+    HIP_TRY(gh::launch_diag_extract(d_boff, d_pr, (int)npairs, b->d_shells.as<DevShell>(), b->d_aooff.as<long long>(), nao,
+                                    ctx->blocks.as<zc>(), ctx->staging.as<double>(), s),
+            "diagonal launch");
+    HIP_TRY(hipMemcpyAsync(out, ctx->staging.p, dbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "diagonal execution");
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* b, const uint64_t* pivots, size_t npiv,
+                             const double field[6], double* out, int real_output) {
+    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    const bool zero_field = field[0] == 0.0 && field[1] == 0.0 && field[2] == 0.0;
+    if (real_output && !zero_field)
+        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "real Cholesky columns require zero phase field");
+    if (npiv == 0) return GIAOHIP_OK;
+    const long long n = b->nao;
+    for (size_t i = 0; i < npiv; ++i)
+        if (pivots[2 * i] >= (uint64_t)n || pivots[2 * i + 1] >= (uint64_t)n)
+            return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky pivot AO index is out of range");
+    const int sa = b->owner[pivots[0]], sb = b->owner[pivots[1]];
+    for (size_t i = 0; i < npiv; ++i)
+        if (b->owner[pivots[2 * i]] != sa || b->owner[pivots[2 * i + 1]] != sb)
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "Cholesky candidate pivots must share a shell pair");
+    const DevShell& C = b->shells[sb];
+    const DevShell& D = b->shells[sa];
+    std::vector<Request> reqs;
+    reqs.reserve(npiv * (size_t)b->ns * b->ns);
+    for (size_t cand = 0; cand < npiv; ++cand) {
+        const long long cl = (long long)pivots[2 * cand + 1] - b->ao_off[sb];
+        const long long dl = (long long)pivots[2 * cand] - b->ao_off[sa];
+        const int xc = (int)(cl / C.ncart), cc = (int)(cl % C.ncart);
+        const int xd = (int)(dl / D.ncart), dc = (int)(dl % D.ncart);
+        const int comp = cc * D.ncart + dc;
+        for (int a = 0; a < b->ns; ++a)
+            for (int bb = 0; bb < b->ns; ++bb) {
+                const int q[4] = {a, bb, sb, sa};
+                const int st = check_quartet_supported(ctx, b, q);
+                if (st) return st;
+                Request r{};
+                for (int k = 0; k < 4; ++k) r.sh[k] = q[k];
+                r.xfix[0] = r.xfix[1] = -1;
+                r.xfix[2] = xc;
+                r.xfix[3] = xd;
+                r.cd0 = comp;
+                r.cd1 = comp + 1;
+                r.out_base = (long long)cand * n * n + b->ao_off[a] * n + b->ao_off[bb];
+                r.stride[0] = n;
+                r.stride[1] = 1;
+                r.stride[2] = 0;
+                r.stride[3] = 0;
+                reqs.push_back(r);
+            }
+    }
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const size_t ncol = npiv * (size_t)n * n;
+    HIP_TRY(ctx->blocks.ensure(ncol * sizeof(zc)), "hipMalloc(columns)");
+    int st = run_requests(ctx, b, field, reqs, ctx->blocks.as<zc>());
+    if (st) return st;
+    if (real_output) {
+        HIP_TRY(ctx->staging.ensure(ncol * sizeof(double)), "hipMalloc");
+        HIP_TRY(gh::launch_real_columns(ctx->blocks.as<zc>(), b->d_owner.as<int>(), n, (long long)npiv,
+                                        ctx->staging.as<double>(), s),
+                "real columns launch");
+        HIP_TRY(hipMemcpyAsync(out, ctx->staging.p, ncol * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    } else {
+        HIP_TRY(hipMemcpyAsync(out, ctx->blocks.p, ncol * sizeof(zc), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    }
+    HIP_TRY(hipStreamSynchronize(s), "columns execution");
+    return GIAOHIP_OK;
+}
+
+}  // extern "C"

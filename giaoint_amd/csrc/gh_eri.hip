@@ -1,0 +1,746 @@
+// gfx950 kernels of the London-orbital McMurchie-Davidson ERI path.
+//
+// Formulation. The reference (eri.cpp:283-368) builds, for every Cartesian
+// component quartet, a six-index complex table
+//   R^n_{h1,h2} = d_P^{h1} d_Q^{h2} [exp(-i q1.P - i q2.Q) G^n(P'-Q')]
+// and contracts it with real Hermite coefficients E_ab[h1] E_cd[h2]. Expanding
+// the derivatives of the phase binomially gives
+//   sum_{h1,h2} E_ab[h1] E_cd[h2] R_{h1,h2}
+//     = sum_{k,j} Et_ab[k] Et_cd[j] (-1)^{|j|} R0_{k+j},
+//   Et[k] = sum_{t>=k} E_t C(t,k) (-i q)^{t-k}   (per Cartesian axis),
+// where R0 is the ordinary three-index Hermite Coulomb table evaluated at the
+// complex displacement X = P'-Q' (R0^n_{t+1} = t R0^{n+1}_{t-1} + X R0^{n+1}_t)
+// seeded with common * (-2 rho)^n F_n(T). The London phase therefore lives in
+// complex one-dimensional Hermite coefficients and one R0 table serves every
+// component quartet of a primitive quartet.
+//
+// Kernels.
+//   seed_kernel:      one lane per primitive quartet: pair data
+//                     (overlap.cpp:60-91), Boys argument and scaling choice
+//                     (eri.cpp:162-210), complex Boys F_0..F_L, and the R0
+//                     seeds s_n = pref * common * (-2 rho)^n F_n plus X.
+//   transform_kernel: one 256-thread workgroup per work item (shell quartet x
+//                     contraction rows x ket-component range). Per primitive
+//                     quartet the R0 table is built level by level in LDS,
+//                     the ket transform W[k][cd] += sum_j Et_cd R0[k+j] is
+//                     accumulated in registers over ket primitive pairs, and
+//                     after each bra primitive pair the bra transform
+//                     out[ab][cd] += sum_k Et_ab[k] W[k][cd] runs from LDS.
+//   scatter kernels:  four symmetry images of canonical blocks
+//                     (eri.cpp:777-805) and the Cholesky diagonal extraction
+//                     (cholesky.cpp:305-312).
+#include "gh_boys.h"
+#include "gh_internal.h"
+
+namespace gh {
+
+__constant__ double c_gl16x[16];
+__constant__ double c_gl16w[16];
+__constant__ double c_gl32x[32];
+__constant__ double c_gl32w[32];
+// Cartesian components of l = 0..kMaxShellL, x-major (types.cpp:221-237),
+// packed x | y << 8 | z << 16, shell l starting at nherm(l - 1).
+__constant__ unsigned c_cart[nherm(kMaxShellL)];
+// Binomial coefficients C(t, k), t, k <= 2 * kMaxShellL.
+__constant__ double c_binom[2 * kMaxShellL + 1][2 * kMaxShellL + 1];
+
+__device__ inline unsigned cart_of(int l, int c) { return c_cart[l * (l + 1) * (l + 2) / 6 + c]; }
+
+// ---------------------------------------------------------------------------
+// Pair data (overlap.cpp:60-91, same operation order)
+// ---------------------------------------------------------------------------
+struct PairD {
+    double p;
+    double P[3];
+    double q[3];
+    zc Pc[3];
+    double K;
+    zc london;
+};
+
+__device__ inline void make_pair(double ea, const double* A, const double* ka, double eb, const double* B,
+                                 const double* kb, PairD& o) {
+    o.p = ea + eb;
+    const double mu = ea * eb / o.p;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o.P[i] = (ea * A[i] + eb * B[i]) / o.p;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o.q[i] = kb[i] - ka[i];
+    const double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
+    o.K = exp(-mu * (dx * dx + dy * dy + dz * dz));
+    const double phase = -(o.q[0] * o.P[0] + o.q[1] * o.P[1] + o.q[2] * o.P[2]);
+    const double damping = -(o.q[0] * o.q[0] + o.q[1] * o.q[1] + o.q[2] * o.q[2]) / (4.0 * o.p);
+    o.london = zexp(zc{damping, phase});
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o.Pc[i] = zc{o.P[i], -0.5 * o.q[i] / o.p};
+}
+
+__device__ inline void record_error(DevError* err, int code, long long item, zc z, int order) {
+    if (atomicCAS(&err->code, 0, code) == 0) {
+        err->item = item;
+        err->zr = z.x;
+        err->zi = z.y;
+        err->order = order;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Seed kernel
+// ---------------------------------------------------------------------------
+template <int NMAX>
+__global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict__ quartets, int nq,
+                                                   long long total, const DevShell* __restrict__ shells,
+                                                   const double* __restrict__ exps,
+                                                   const double* __restrict__ kappa, zc* __restrict__ seeds,
+                                                   DevError* err) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    int lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (quartets[mid].pq_begin <= gid) lo = mid;
+        else hi = mid - 1;
+    }
+    const SeedQuartet Q = quartets[lo];
+    const long long local = gid - Q.pq_begin;
+    const int ab = (int)(local / Q.kcd), cd = (int)(local % Q.kcd);
+    const DevShell& A = shells[Q.sh[0]];
+    const DevShell& B = shells[Q.sh[1]];
+    const DevShell& C = shells[Q.sh[2]];
+    const DevShell& D = shells[Q.sh[3]];
+    const int pa = ab / B.nprim, pb = ab % B.nprim, pc = cd / D.nprim, pd = cd % D.nprim;
+    PairD P1, P2;
+    make_pair(exps[A.prim_off + pa], A.center, kappa + 3 * Q.sh[0], exps[B.prim_off + pb], B.center,
+              kappa + 3 * Q.sh[1], P1);
+    make_pair(exps[C.prim_off + pc], C.center, kappa + 3 * Q.sh[2], exps[D.prim_off + pd], D.center,
+              kappa + 3 * Q.sh[3], P2);
+    const double p = P1.p, q = P2.p;
+    const double rho = p * q / (p + q);
+    const zc X[3] = {zsub(P1.Pc[0], P2.Pc[0]), zsub(P1.Pc[1], P2.Pc[1]), zsub(P1.Pc[2], P2.Pc[2])};
+    const zc T = zscale(rho, zadd(zadd(zmul(X[0], X[0]), zmul(X[1], X[1])), zmul(X[2], X[2])));
+    const bool scaled = T.x < 0.0;
+    zc F[NMAX + 1];
+    int region, segs;
+    double berr;
+    const int L = Q.L;
+    const int st = boys_eval<NMAX>(T, L, scaled, F, region, berr, segs);
+    zc* out = seeds + Q.seed_off + local * (L + 4);
+    if (st != kErrNone) {
+        record_error(err, st, lo, T, L);
+#pragma unroll
+        for (int n = 0; n <= NMAX; ++n) F[n] = zc{0.0, 0.0};
+    }
+    zc common = zmul(P1.london, P2.london);
+    if (scaled) {
+        double rd2 = 0.0, tw2 = 0.0, twc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double rd = P1.P[i] - P2.P[i];
+            const double tw = P1.q[i] + P2.q[i];
+            const double wc = (p * P1.P[i] + q * P2.P[i]) / (p + q);
+            rd2 += rd * rd;
+            tw2 += tw * tw;
+            twc += tw * wc;
+        }
+        common = zexp(zc{-rho * rd2 - tw2 / (4.0 * (p + q)), -twc});
+    }
+    const double pref = 2.0 * pow(kPi, 2.5) / (p * q * sqrt(p + q)) * P1.K * P2.K;
+    out[0] = X[0];
+    out[1] = X[1];
+    out[2] = X[2];
+    const zc base = zscale(pref, common);
+    double rf = 1.0;
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+        if (n <= L) out[3 + n] = zmul(zscale(rf, base), F[n]);
+        rf *= -2.0 * rho;
+    }
+}
+
+// Boys probe: one lane per argument (giaohip_boys).
+__global__ __launch_bounds__(64) void boys_probe_kernel(const zc* __restrict__ z, long long n, int nmax, int scaled,
+                                                        zc* __restrict__ out, int* __restrict__ region,
+                                                        double* __restrict__ errest, int* __restrict__ segments,
+                                                        DevError* err) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    zc F[kMaxBoysOrder + 1];
+    int reg = -1, segs = 0;
+    double e = 0.0;
+    const int st = boys_eval<kMaxBoysOrder>(z[i], nmax, scaled != 0, F, reg, e, segs);
+    if (st != kErrNone) record_error(err, st, i, z[i], nmax);
+#pragma unroll
+    for (int k = 0; k <= kMaxBoysOrder; ++k)
+        if (k <= nmax) out[i * (nmax + 1) + k] = st == kErrNone ? F[k] : zc{0.0, 0.0};
+    region[i] = reg;
+    errest[i] = e;
+    segments[i] = segs;
+}
+
+// ---------------------------------------------------------------------------
+// Transform kernel
+// ---------------------------------------------------------------------------
+constexpr int kNT = 256;
+constexpr int kMaxW = 8;  // W entries per thread
+constexpr int kMaxO = 8;  // output entries per thread
+constexpr int kMaxE = 2 * kMaxShellL + 1;
+
+// Complex 1-D Hermite table Et^{ij}[k] of one primitive pair on one axis:
+// E^{ij}_t from the field-free recurrence (hermite.cpp:8-40: raise the bra
+// power i times with P-A, then the ket power j times with P-B), then folded
+// with the pair's London wave number q: Et[k] = sum_t E_t C(t,k) (-iq)^(t-k).
+__device__ inline void build_phase_hermite(int i, int j, double PA, double PB, double p, double q, zc* dst) {
+    double cur[kMaxE + 1];
+#pragma unroll
+    for (int t = 0; t <= kMaxE; ++t) cur[t] = 0.0;
+    cur[0] = 1.0;
+    const double i2p = 0.5 / p;
+    int active = 0;
+    for (int step = 0; step < i + j; ++step) {
+        const double disp = step < i ? PA : PB;
+        double nxt[kMaxE + 1];
+#pragma unroll
+        for (int t = 0; t <= kMaxE; ++t) {
+            const double lower = t == 0 ? 0.0 : cur[t - 1];
+            const double same = t <= active ? cur[t] : 0.0;
+            const double upper = (t < active && t < kMaxE) ? cur[t < kMaxE ? t + 1 : kMaxE] : 0.0;
+            nxt[t] = t <= active + 1 ? i2p * lower + disp * same + (double)(t + 1) * upper : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t <= kMaxE; ++t) cur[t] = nxt[t];
+        ++active;
+    }
+    // powers of (-i q): (-i)^m q^m
+    const int top = i + j;
+    double qpow[kMaxE];
+    qpow[0] = 1.0;
+#pragma unroll
+    for (int m = 1; m < kMaxE; ++m) qpow[m] = qpow[m - 1] * q;
+#pragma unroll
+    for (int k = 0; k < kMaxE; ++k) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int m = 0; k + m < kMaxE; ++m) {
+            const double v = (k + m <= top) ? cur[k + m] * c_binom[k + m][k] * qpow[m] : 0.0;
+            if ((m & 3) == 0) re += v;
+            else if ((m & 3) == 1) im -= v;
+            else if ((m & 3) == 2) re -= v;
+            else im += v;
+        }
+        if (k <= top) dst[k] = zc{re, im};
+    }
+}
+
+struct LdsLayout {
+    int offL, listL, offAB, tuvAB;  // int offsets (in ints)
+    int seeds, eket, ebra, work;    // zc offsets (in zc) from zc base
+    int zc_base;                    // byte offset of the zc region
+};
+
+__device__ __host__ inline int align16(int bytes) { return (bytes + 15) & ~15; }
+
+__device__ __host__ inline LdsLayout lds_layout(int la, int lb, int lc, int ld, int tile, int* total_bytes) {
+    const int L = la + lb + lc + ld, Lab = la + lb, Lcd = lc + ld;
+    LdsLayout s;
+    int ints = 0;
+    s.offL = ints;
+    ints += (L + 1) * (L + 1);
+    s.listL = ints;
+    ints += nherm(L);
+    s.offAB = ints;
+    ints += (Lab + 1) * (Lab + 1);
+    s.tuvAB = ints;
+    ints += nherm(Lab);
+    s.zc_base = align16(ints * 4);
+    int z = 0;
+    s.seeds = z;
+    z += L + 4;
+    s.eket = z;
+    z += 3 * (lc + 1) * (ld + 1) * (Lcd + 1);
+    s.ebra = z;
+    z += 3 * (la + 1) * (lb + 1) * (Lab + 1);
+    s.work = z;
+    const int r0 = 2 * nherm(L), wz = nherm(Lab) * tile;
+    z += r0 > wz ? r0 : wz;
+    if (total_bytes) *total_bytes = s.zc_base + z * 16;
+    return s;
+}
+
+// Lexicographic (t, u, v) table of degree <= D: offsets off[t*(D+1)+u] so that
+// index(t,u,v) = off[t*(D+1)+u] + v.
+__device__ inline void build_lex_offsets(int D, int* off) {
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t <= D; ++t)
+            for (int u = 0; u <= D; ++u) {
+                if (t + u <= D) {
+                    off[t * (D + 1) + u] = run;
+                    run += D - t - u + 1;
+                } else {
+                    off[t * (D + 1) + u] = -1;
+                }
+            }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const WorkItem* __restrict__ items,
+                                                        const DevShell* __restrict__ shells,
+                                                        const double* __restrict__ exps,
+                                                        const double* __restrict__ coefs,
+                                                        const double* __restrict__ norms,
+                                                        const double* __restrict__ kappa,
+                                                        const zc* __restrict__ seeds, zc* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int la = ca.l[0], lb = ca.l[1], lc = ca.l[2], ld = ca.l[3];
+    const int L = la + lb + lc + ld, Lab = la + lb, Lcd = lc + ld;
+    const int tile = ca.tile;
+    const LdsLayout lay = lds_layout(la, lb, lc, ld, tile, nullptr);
+    int* ibase = reinterpret_cast<int*>(smem_raw);
+    zc* zbase = reinterpret_cast<zc*>(smem_raw + lay.zc_base);
+    int* offL = ibase + lay.offL;
+    int* listL = ibase + lay.listL;
+    int* offAB = ibase + lay.offAB;
+    int* tuvAB = ibase + lay.tuvAB;
+    zc* sseed = zbase + lay.seeds;
+    zc* eket = zbase + lay.eket;
+    zc* ebra = zbase + lay.ebra;
+    zc* work = zbase + lay.work;
+    const int tid = threadIdx.x;
+
+    // Index tables.
+    build_lex_offsets(L, offL);
+    if (tid == 32) {
+        int run = 0;
+        for (int t = 0; t <= Lab; ++t)
+            for (int u = 0; u <= Lab; ++u) {
+                if (t + u <= Lab) {
+                    offAB[t * (Lab + 1) + u] = run;
+                    for (int v = 0; v <= Lab - t - u; ++v) tuvAB[run + v] = t | (u << 6) | (v << 12);
+                    run += Lab - t - u + 1;
+                } else {
+                    offAB[t * (Lab + 1) + u] = -1;
+                }
+            }
+    }
+    if (tid == 64) {
+        // degree-sorted list of (t,u,v) with their lexicographic index for R0
+        int k = 0;
+        for (int d = 0; d <= L; ++d)
+            for (int t = d; t >= 0; --t)
+                for (int u = d - t; u >= 0; --u) {
+                    const int v = d - t - u;
+                    int lex = 0;
+                    for (int tt = 0; tt < t; ++tt) lex += (L - tt + 1) * (L - tt + 2) / 2;
+                    for (int uu = 0; uu < u; ++uu) lex += L - t - uu + 1;
+                    lex += v;
+                    listL[k++] = t | (u << 6) | (v << 12) | (lex << 18);
+                }
+    }
+    __syncthreads();
+
+    const WorkItem it = items[blockIdx.x];
+    const DevShell A = shells[it.sh[0]];
+    const DevShell B = shells[it.sh[1]];
+    const DevShell C = shells[it.sh[2]];
+    const DevShell D = shells[it.sh[3]];
+    const int nca = ncart(la), ncb = ncart(lb), ncc = ncart(lc), ncd = ncart(ld);
+    const int nab = nca * ncb;
+    const int NHab = nherm(Lab);
+    const int nsel = it.cd1 - it.cd0;
+    const int Kb = B.nprim, Kd = D.nprim;
+    const int Kab = A.nprim * Kb, Kcd = C.nprim * Kd;
+    const int stride_pq = L + 4;
+    const int ebra_i = (lb + 1) * (Lab + 1), ebra_a = (la + 1) * ebra_i;
+    const int eket_i = (ld + 1) * (Lcd + 1), eket_a = (lc + 1) * eket_i;
+
+    for (int t0 = 0; t0 < nsel; t0 += tile) {
+        const int tl = min(tile, nsel - t0);
+        zc acc[kMaxO];
+#pragma unroll
+        for (int r = 0; r < kMaxO; ++r) acc[r] = zc{0.0, 0.0};
+
+        for (int pab = 0; pab < Kab; ++pab) {
+            const int pa = pab / Kb, pb = pab % Kb;
+            const double ea = exps[A.prim_off + pa], eb = exps[B.prim_off + pb];
+            const double p = ea + eb;
+            // bra 1-D complex Hermite tables
+            for (int e = tid; e < 3 * (la + 1) * (lb + 1); e += kNT) {
+                const int axis = e / ((la + 1) * (lb + 1));
+                const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
+                const double Pc = (ea * A.center[axis] + eb * B.center[axis]) / p;
+                const double qv = kappa[3 * it.sh[1] + axis] - kappa[3 * it.sh[0] + axis];
+                build_phase_hermite(i, j, Pc - A.center[axis], Pc - B.center[axis], p, qv,
+                                    ebra + axis * ebra_a + i * ebra_i + j * (Lab + 1));
+            }
+            zc w[kMaxW];
+#pragma unroll
+            for (int r = 0; r < kMaxW; ++r) w[r] = zc{0.0, 0.0};
+
+            for (int pcd = 0; pcd < Kcd; ++pcd) {
+                const int pc = pcd / Kd, pd = pcd % Kd;
+                const double ec = exps[C.prim_off + pc], ed = exps[D.prim_off + pd];
+                const double q = ec + ed;
+                __syncthreads();  // previous primitive quartet finished with LDS
+                const zc* sp = seeds + it.seed_off + (long long)(pab * Kcd + pcd) * stride_pq;
+                if (tid < stride_pq) sseed[tid] = sp[tid];
+                for (int e = tid; e < 3 * (lc + 1) * (ld + 1); e += kNT) {
+                    const int axis = e / ((lc + 1) * (ld + 1));
+                    const int i = (e / (ld + 1)) % (lc + 1), j = e % (ld + 1);
+                    const double Qc = (ec * C.center[axis] + ed * D.center[axis]) / q;
+                    const double qv = kappa[3 * it.sh[3] + axis] - kappa[3 * it.sh[2] + axis];
+                    build_phase_hermite(i, j, Qc - C.center[axis], Qc - D.center[axis], q, qv,
+                                        eket + axis * eket_a + i * eket_i + j * (Lcd + 1));
+                }
+                __syncthreads();
+                // R0 table: level L holds s_L; level n from level n+1.
+                zc* buf0 = work;
+                zc* buf1 = work + nherm(L);
+                if (tid == 0) buf0[0] = sseed[3 + L];
+                const zc X0 = sseed[0], X1 = sseed[1], X2 = sseed[2];
+                for (int n = L - 1; n >= 0; --n) {
+                    __syncthreads();
+                    const zc* cur = ((L - n) & 1) ? buf0 : buf1;
+                    zc* nxt = ((L - n) & 1) ? buf1 : buf0;
+                    const int cnt = nherm(L - n);
+                    for (int e = tid; e < cnt; e += kNT) {
+                        const int pk = listL[e];
+                        const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = pk >> 18;
+                        zc val;
+                        if (t > 0) {
+                            const int i1 = offL[(t - 1) * (L + 1) + u] + v;
+                            val = zmul(X0, cur[i1]);
+                            if (t > 1) {
+                                const zc r2 = cur[offL[(t - 2) * (L + 1) + u] + v];
+                                val.x += (double)(t - 1) * r2.x;
+                                val.y += (double)(t - 1) * r2.y;
+                            }
+                        } else if (u > 0) {
+                            val = zmul(X1, cur[offL[u - 1] + v]);
+                            if (u > 1) {
+                                const zc r2 = cur[offL[u - 2] + v];
+                                val.x += (double)(u - 1) * r2.x;
+                                val.y += (double)(u - 1) * r2.y;
+                            }
+                        } else if (v > 0) {
+                            val = zmul(X2, cur[v - 1]);
+                            if (v > 1) {
+                                const zc r2 = cur[v - 2];
+                                val.x += (double)(v - 1) * r2.x;
+                                val.y += (double)(v - 1) * r2.y;
+                            }
+                        } else {
+                            val = sseed[3 + n];
+                        }
+                        nxt[lex] = val;
+                    }
+                }
+                __syncthreads();
+                const zc* R0 = (L & 1) ? buf1 : buf0;
+                // Ket transform into registers.
+                const int nW = NHab * tl;
+#pragma unroll
+                for (int r = 0; r < kMaxW; ++r) {
+                    const int e = tid + r * kNT;
+                    if (e < nW) {
+                        const int cdl = e / NHab, k = e % NHab;
+                        const int pk = tuvAB[k];
+                        const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63;
+                        const int cdcomp = it.cd0 + t0 + cdl;
+                        const int cc = cdcomp / ncd, dc = cdcomp % ncd;
+                        const unsigned pcp = cart_of(lc, cc), pdp = cart_of(ld, dc);
+                        const int cx = pcp & 255, cy = (pcp >> 8) & 255, cz = pcp >> 16;
+                        const int dx = pdp & 255, dy = (pdp >> 8) & 255, dz = pdp >> 16;
+                        const zc* Ex = eket + 0 * eket_a + cx * eket_i + dx * (Lcd + 1);
+                        const zc* Ey = eket + 1 * eket_a + cy * eket_i + dy * (Lcd + 1);
+                        const zc* Ez = eket + 2 * eket_a + cz * eket_i + dz * (Lcd + 1);
+                        zc s{0.0, 0.0};
+                        for (int tau = 0; tau <= cx + dx; ++tau) {
+                            zc ex = Ex[tau];
+                            if (tau & 1) ex = zneg(ex);
+                            for (int phi = 0; phi <= cy + dy; ++phi) {
+                                zc exy = zmul(ex, Ey[phi]);
+                                if (phi & 1) exy = zneg(exy);
+                                const zc* rr = R0 + offL[(t + tau) * (L + 1) + (u + phi)] + v;
+                                for (int chi = 0; chi <= cz + dz; ++chi) {
+                                    zc e3 = zmul(exy, Ez[chi]);
+                                    if (chi & 1) e3 = zneg(e3);
+                                    zmac(s, e3, rr[chi]);
+                                }
+                            }
+                        }
+                        const double sc = coefs[C.coef_off + it.x[2] * C.nprim + pc] *
+                                          norms[C.norm_off + cc * C.nprim + pc] *
+                                          coefs[D.coef_off + it.x[3] * D.nprim + pd] *
+                                          norms[D.norm_off + dc * D.nprim + pd];
+                        w[r].x = fma(sc, s.x, w[r].x);
+                        w[r].y = fma(sc, s.y, w[r].y);
+                    }
+                }
+            }
+            __syncthreads();
+            // W to LDS: work[k * tl + cdl]
+            {
+                const int nW = NHab * tl;
+#pragma unroll
+                for (int r = 0; r < kMaxW; ++r) {
+                    const int e = tid + r * kNT;
+                    if (e < nW) {
+                        const int cdl = e / NHab, k = e % NHab;
+                        work[k * tl + cdl] = w[r];
+                    }
+                }
+            }
+            __syncthreads();
+            // Bra transform.
+            const int nO = nab * tl;
+#pragma unroll
+            for (int r = 0; r < kMaxO; ++r) {
+                const int o = tid + r * kNT;
+                if (o < nO) {
+                    const int abcomp = o / tl, cdl = o % tl;
+                    const int cac = abcomp / ncb, cbc = abcomp % ncb;
+                    const unsigned pap = cart_of(la, cac), pbp = cart_of(lb, cbc);
+                    const int ax = pap & 255, ay = (pap >> 8) & 255, az = pap >> 16;
+                    const int bx = pbp & 255, by = (pbp >> 8) & 255, bz = pbp >> 16;
+                    const zc* Ex = ebra + 0 * ebra_a + ax * ebra_i + bx * (Lab + 1);
+                    const zc* Ey = ebra + 1 * ebra_a + ay * ebra_i + by * (Lab + 1);
+                    const zc* Ez = ebra + 2 * ebra_a + az * ebra_i + bz * (Lab + 1);
+                    zc s{0.0, 0.0};
+                    for (int t = 0; t <= ax + bx; ++t) {
+                        for (int u = 0; u <= ay + by; ++u) {
+                            const zc exy = zmul(Ex[t], Ey[u]);
+                            const zc* ww = work + (offAB[t * (Lab + 1) + u]) * tl + cdl;
+                            for (int v = 0; v <= az + bz; ++v) zmac(s, zmul(exy, Ez[v]), ww[v * tl]);
+                        }
+                    }
+                    const double sc = coefs[A.coef_off + it.x[0] * A.nprim + pa] *
+                                      norms[A.norm_off + cac * A.nprim + pa] *
+                                      coefs[B.coef_off + it.x[1] * B.nprim + pb] *
+                                      norms[B.norm_off + cbc * B.nprim + pb];
+                    acc[r].x = fma(sc, s.x, acc[r].x);
+                    acc[r].y = fma(sc, s.y, acc[r].y);
+                }
+            }
+            __syncthreads();
+        }
+        // Write the tile.
+        const int nO = nab * tl;
+#pragma unroll
+        for (int r = 0; r < kMaxO; ++r) {
+            const int o = tid + r * kNT;
+            if (o < nO) {
+                const int abcomp = o / tl, cdl = o % tl;
+                const int cac = abcomp / ncb, cbc = abcomp % ncb;
+                const int cdcomp = it.cd0 + t0 + cdl;
+                const int cc = cdcomp / ncd, dc = cdcomp % ncd;
+                const long long ia = (long long)it.x[0] * nca + cac, ib = (long long)it.x[1] * ncb + cbc;
+                const long long ic = (long long)it.x[2] * ncc + cc, id = (long long)it.x[3] * ncd + dc;
+                out[it.out_base + ia * it.stride[0] + ib * it.stride[1] + ic * it.stride[2] + id * it.stride[3]] =
+                    acc[r];
+            }
+        }
+    }
+}
+
+// Four symmetry images of canonical blocks into an n^4 tensor (eri.cpp:777-805).
+__global__ void tensor_scatter_kernel(const long long* __restrict__ block_off, const int* __restrict__ block_q,
+                                      int nblocks, long long total, const DevShell* __restrict__ shells,
+                                      const long long* __restrict__ ao_off, long long nao,
+                                      const zc* __restrict__ blocks, zc* __restrict__ tensor) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    int lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (block_off[mid] <= gid) lo = mid;
+        else hi = mid - 1;
+    }
+    const int* q = block_q + 4 * lo;
+    const long long na = shells[q[0]].ao_count, nb = shells[q[1]].ao_count;
+    const long long nc = shells[q[2]].ao_count, nd = shells[q[3]].ao_count;
+    long long k = gid - block_off[lo];
+    const long long id = k % nd;
+    k /= nd;
+    const long long ic = k % nc;
+    k /= nc;
+    const long long ib = k % nb;
+    const long long ia = k / nb;
+    const zc v = blocks[gid];
+    const zc vc{v.x, -v.y};
+    const long long ga = ao_off[q[0]] + ia, gb = ao_off[q[1]] + ib, gc = ao_off[q[2]] + ic, gd = ao_off[q[3]] + id;
+    const long long n1 = nao, n2 = nao * nao, n3 = n2 * nao;
+    tensor[ga * n3 + gb * n2 + gc * n1 + gd] = v;
+    tensor[gc * n3 + gd * n2 + ga * n1 + gb] = v;
+    tensor[gb * n3 + ga * n2 + gd * n1 + gc] = vc;
+    tensor[gd * n3 + gc * n2 + gb * n1 + ga] = vc;
+}
+
+// Diagonal D[nu,mu] = D[mu,nu] = Re (mu nu|nu mu) from (ab|ba) blocks
+// (cholesky.cpp:305-312).
+__global__ void diag_extract_kernel(const long long* __restrict__ block_off, const int* __restrict__ pairs,
+                                    int npairs, const DevShell* __restrict__ shells,
+                                    const long long* __restrict__ ao_off, long long nao,
+                                    const zc* __restrict__ blocks, double* __restrict__ diag) {
+    const int pr = blockIdx.x;
+    if (pr >= npairs) return;
+    const int a = pairs[2 * pr], b = pairs[2 * pr + 1];
+    const int na = shells[a].ao_count, nb = shells[b].ao_count;
+    const zc* blk = blocks + block_off[pr];
+    for (int e = threadIdx.x; e < na * nb; e += blockDim.x) {
+        const int i = e / nb, j = e % nb;
+        const double v = blk[(((long long)i * nb + j) * nb + j) * na + i].x;
+        const long long mu = ao_off[a] + i, nu = ao_off[b] + j;
+        diag[nu * nao + mu] = v;
+        if (a != b) diag[mu * nao + nu] = v;
+    }
+}
+
+// Real zero-field columns from complex columns over all shell pairs
+// (cholesky.cpp:250-258,271-279).
+__global__ void real_columns_kernel(const zc* __restrict__ cols, const int* __restrict__ owner, long long nao,
+                                    long long npiv, double* __restrict__ out) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n2 = nao * nao;
+    if (gid >= npiv * n2) return;
+    const long long cand = gid / n2, rest = gid % n2;
+    const long long mu = rest / nao, nu = rest % nao;
+    const int a = owner[mu], b = owner[nu];
+    const zc* col = cols + cand * n2;
+    double v;
+    if (a < b) v = col[mu * nao + nu].x;
+    else if (a > b) v = col[nu * nao + mu].x;
+    else v = (col[mu * nao + nu].x + col[nu * nao + mu].x) * 0.5;
+    out[gid] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch wrappers
+// ---------------------------------------------------------------------------
+int transform_lds_bytes(const ClassArgs& ca) {
+    int bytes = 0;
+    lds_layout(ca.l[0], ca.l[1], ca.l[2], ca.l[3], ca.tile, &bytes);
+    return bytes;
+}
+
+int transform_max_w() { return kMaxW * kNT; }
+int transform_max_o() { return kMaxO * kNT; }
+
+hipError_t upload_constants() {
+    // Gauss-Legendre nodes by Newton iteration on P_n, the same procedure as
+    // the reference's rule construction (boys.cpp:27-109).
+    auto rule = [](int n, double* x, double* w) {
+        for (int i = 0; i < n / 2; ++i) {
+            double root = std::cos(kPi * ((double)i + 0.75) / ((double)n + 0.5));
+            double derivative = 0.0;
+            for (int it = 0; it < 20; ++it) {
+                double prev = 1.0, cur = root;
+                for (int deg = 2; deg <= n; ++deg) {
+                    const double next = ((2.0 * (double)deg - 1.0) * root * cur - ((double)deg - 1.0) * prev) /
+                                        (double)deg;
+                    prev = cur;
+                    cur = next;
+                }
+                derivative = (double)n * (root * cur - prev) / (root * root - 1.0);
+                const double update = cur / derivative;
+                root -= update;
+                if (std::fabs(update) < 4.0 * 2.220446049250313e-16) break;
+            }
+            const double weight = 2.0 / ((1.0 - root * root) * derivative * derivative);
+            x[i] = -root;
+            x[n - 1 - i] = root;
+            w[i] = weight;
+            w[n - 1 - i] = weight;
+        }
+    };
+    double x16[16], w16[16], x32[32], w32[32];
+    rule(16, x16, w16);
+    rule(32, x32, w32);
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_gl16x), x16, sizeof x16)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_gl16w), w16, sizeof w16)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_gl32x), x32, sizeof x32)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_gl32w), w32, sizeof w32)) != hipSuccess) return e;
+    unsigned cart[nherm(kMaxShellL)];
+    int k = 0;
+    for (int l = 0; l <= kMaxShellL; ++l)
+        for (int x = l; x >= 0; --x)
+            for (int y = l - x; y >= 0; --y) cart[k++] = (unsigned)x | ((unsigned)y << 8) | ((unsigned)(l - x - y) << 16);
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cart), cart, sizeof cart)) != hipSuccess) return e;
+    double binom[2 * kMaxShellL + 1][2 * kMaxShellL + 1] = {};
+    for (int t = 0; t <= 2 * kMaxShellL; ++t) {
+        binom[t][0] = 1.0;
+        for (int kk = 1; kk <= t; ++kk) binom[t][kk] = binom[t - 1][kk - 1] + (kk <= t - 1 ? binom[t - 1][kk] : 0.0);
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_binom), binom, sizeof binom);
+}
+
+hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
+                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    const dim3 block(128);
+    const dim3 grid((unsigned)((total + 127) / 128));
+    switch (nmax_bucket) {
+        case 4: seed_kernel<4><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+        case 8: seed_kernel<8><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+        case 12: seed_kernel<12><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+        case 16: seed_kernel<16><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+        case 24: seed_kernel<24><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+        default: seed_kernel<32><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nitems, const DevShell* shells,
+                            const double* exps, const double* coefs, const double* norms, const double* kappa,
+                            const zc* seeds, zc* out, hipStream_t s) {
+    if (nitems == 0) return hipSuccess;
+    const int bytes = transform_lds_bytes(ca);
+    if (bytes > 64 * 1024) {
+        static bool raised = false;
+        if (!raised) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), bytes, s>>>(ca, items, shells, exps, coefs, norms, kappa,
+                                                                      seeds, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_boys_probe(const zc* z, long long n, int nmax, int scaled, zc* out, int* region, double* errest,
+                             int* segments, DevError* err, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    boys_probe_kernel<<<dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s>>>(z, n, nmax, scaled, out, region, errest,
+                                                                         segments, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_tensor_scatter(const long long* block_off, const int* block_q, int nblocks, long long total,
+                                 const DevShell* shells, const long long* ao_off, long long nao, const zc* blocks,
+                                 zc* tensor, hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    tensor_scatter_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(
+        block_off, block_q, nblocks, total, shells, ao_off, nao, blocks, tensor);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_extract(const long long* block_off, const int* pairs, int npairs, const DevShell* shells,
+                               const long long* ao_off, long long nao, const zc* blocks, double* diag,
+                               hipStream_t s) {
+    if (npairs == 0) return hipSuccess;
+    diag_extract_kernel<<<dim3((unsigned)npairs), dim3(256), 0, s>>>(block_off, pairs, npairs, shells, ao_off, nao,
+                                                                    blocks, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_real_columns(const zc* cols, const int* owner, long long nao, long long npiv, double* out,
+                               hipStream_t s) {
+    const long long total = npiv * nao * nao;
+    if (total == 0) return hipSuccess;
+    real_columns_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(cols, owner, nao, npiv, out);
+    return hipGetLastError();
+}
+
+}  // namespace gh

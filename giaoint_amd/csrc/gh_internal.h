@@ -1,0 +1,34 @@
+// Internal interface between the C ABI (gh_api.hip) and the kernel files.
+#pragma once
+
+#include "gh_common.h"
+
+namespace gh {
+
+struct ClassArgs {
+    int l[4];
+    int tile;  // ket components per pass
+};
+
+hipError_t upload_constants();
+int transform_lds_bytes(const ClassArgs& ca);
+int transform_max_w();
+int transform_max_o();
+
+hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
+                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s);
+hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nitems, const DevShell* shells,
+                            const double* exps, const double* coefs, const double* norms, const double* kappa,
+                            const zc* seeds, zc* out, hipStream_t s);
+hipError_t launch_boys_probe(const zc* z, long long n, int nmax, int scaled, zc* out, int* region, double* errest,
+                             int* segments, DevError* err, hipStream_t s);
+hipError_t launch_tensor_scatter(const long long* block_off, const int* block_q, int nblocks, long long total,
+                                 const DevShell* shells, const long long* ao_off, long long nao, const zc* blocks,
+                                 zc* tensor, hipStream_t s);
+hipError_t launch_diag_extract(const long long* block_off, const int* pairs, int npairs, const DevShell* shells,
+                               const long long* ao_off, long long nao, const zc* blocks, double* diag,
+                               hipStream_t s);
+hipError_t launch_real_columns(const zc* cols, const int* owner, long long nao, long long npiv, double* out,
+                               hipStream_t s);
+
+}  // namespace gh

@@ -1,0 +1,152 @@
+/*
+ * giaohip: C ABI of the MI355X (gfx950) London-orbital ERI library.
+ *
+ * Plain pointers and sizes only. Every entry point returns a status code
+ * (GIAOHIP_OK on success); the message of the last failure on a context is
+ * read with giaohip_last_error. Status codes map to the exception types of the
+ * reference C++ API (include/giao_integrals/eri.hpp, boys.hpp):
+ *   GIAOHIP_INVALID_ARGUMENT -> std::invalid_argument / ValueError
+ *   GIAOHIP_LENGTH_ERROR     -> std::length_error / ValueError
+ *   GIAOHIP_OUT_OF_RANGE     -> std::out_of_range / IndexError
+ *   GIAOHIP_BOYS_ERROR       -> giao::BoysNumericalError
+ *   GIAOHIP_DEVICE_ERROR     -> HIP runtime failure (RuntimeError)
+ *   GIAOHIP_UNSUPPORTED      -> a case the GPU kernels do not cover (RuntimeError)
+ *
+ * Basis layout (giaohip_basis_create): nshell shells; centers[3*nshell] in
+ * bohr; L[s] total angular momentum; nprim[s]; ncontr[s]; exps = all
+ * exponents shell by shell; coefs = per shell a row-major
+ * [contraction][primitive] matrix, already contraction-normalized as
+ * giao::Shell does (types.cpp:272-305). AOs of a shell are ordered
+ * contraction-major, then x-major Cartesian component (types.hpp:78-80,
+ * types.cpp:221-237); a basis orders its shells' AOs consecutively.
+ * Field layout: field[6] = {Bx, By, Bz, Ox, Oy, Oz} (atomic units, bohr).
+ * Complex values are interleaved (re, im) doubles (complex128 layout).
+ *
+ * A context owns one HIP stream on one device and serializes calls made on
+ * it with a mutex; use one context per thread for concurrency.
+ */
+#ifndef GIAOHIP_H
+#define GIAOHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GIAOHIP_OK = 0,
+    GIAOHIP_INVALID_ARGUMENT = 1,
+    GIAOHIP_LENGTH_ERROR = 2,
+    GIAOHIP_OUT_OF_RANGE = 3,
+    GIAOHIP_BOYS_ERROR = 4,
+    GIAOHIP_DEVICE_ERROR = 5,
+    GIAOHIP_UNSUPPORTED = 6
+};
+
+/* ERI kernel variants selectable per call (giaohip_set_option). */
+enum {
+    GIAOHIP_ERI_PLAIN = 0 /* one launch per angular-momentum class, FP64 */
+};
+
+typedef struct giaohip_ctx giaohip_ctx;
+typedef struct giaohip_basis giaohip_basis;
+
+/* Statistics of the last ERI call; times are device times from HIP events
+ * recorded on the context's stream around the launches. Mirrors
+ * giao::EriStatistics (eri.hpp:77-81) plus timing. */
+typedef struct {
+    uint64_t requested_quartets;
+    uint64_t computed_quartets;
+    uint64_t screened_quartets;
+    uint64_t primitive_quartets;
+    double seed_ms;      /* pair data + complex Boys + seeds kernel(s) */
+    double transform_ms; /* R recurrence + E.R.E contraction kernel(s) */
+    double total_ms;     /* everything on the stream, including copies */
+    uint64_t launches;
+} giaohip_stats;
+
+/* --- context ---------------------------------------------------------- */
+int giaohip_create(int device, giaohip_ctx** out);
+int giaohip_destroy(giaohip_ctx* ctx);
+/* Copies the last error message (NUL terminated) and returns its length. */
+size_t giaohip_last_error(const giaohip_ctx* ctx, char* buffer, size_t size);
+/* Stream (hipStream_t) the context launches on, for callers that time it. */
+void* giaohip_stream(giaohip_ctx* ctx);
+int giaohip_last_stats(const giaohip_ctx* ctx, giaohip_stats* out);
+/* Device memory owned by the context, for resident outputs (bench, chaining). */
+int giaohip_device_alloc(giaohip_ctx* ctx, size_t bytes, void** out);
+int giaohip_device_free(giaohip_ctx* ctx, void* ptr);
+int giaohip_memcpy_to_host(giaohip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int giaohip_synchronize(giaohip_ctx* ctx);
+
+/* --- basis (giao::Basis, types.hpp:108-124) ----------------------------- */
+int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, const int* L, const int* nprim,
+                         const int* ncontr, const double* exps, const double* coefs, giaohip_basis** out);
+int giaohip_basis_destroy(giaohip_ctx* ctx, giaohip_basis* basis);
+/* Number of AOs (Basis::ao_count, types.hpp:118). */
+size_t giaohip_basis_ao_count(const giaohip_basis* basis);
+
+/* --- host-side shell data (no GPU needed) ------------------------------
+ * Contraction normalization of giao::Shell (types.cpp:239-306, normalize
+ * policy) for n shells at once: exps/coefs_in/coefs_out follow the basis
+ * layout above; coefs_out may alias coefs_in. */
+int giaohip_normalize_shells(int n, const int* L, const int* nprim, const int* ncontr, const double* exps,
+                             const double* coefs_in, double* coefs_out, char* message, size_t message_size);
+/* primitive_normalization (types.cpp:207-219). */
+double giaohip_primitive_normalization(double exponent, int ax, int ay, int az);
+
+/* --- Boys function (replaces giao::compute_boys, boys.hpp:36-42) -------
+ * Evaluates F_0..F_nmax for n arguments z (interleaved complex). out holds
+ * n*(nmax+1) complex values; region (optional) holds giao::BoysRegion values
+ * (0 series, 1 adaptive quadrature, 2 scaled quadrature, 3 asymptotic);
+ * error_estimate and segments are optional per-argument diagnostics
+ * (BoysDiagnostics, boys.hpp:24-29). Host pointers. */
+int giaohip_boys(giaohip_ctx* ctx, const double* z, size_t n, int nmax, int scaled, double* out, int* region,
+                 double* error_estimate, int* segments);
+
+/* --- shell-quartet ERIs --------------------------------------------------
+ * Replaces evaluate_eri_shell_quartets (eri.cpp:644-767, decl. eri.hpp:131-135)
+ * with the packing consumer of _eri_batch (bindings.cpp:274-282,672-731):
+ * quartets[4*i..] are shell indices; block i (C-order complex128 of shape
+ * (ao_a, ao_b, ao_c, ao_d)) is written at out + 2*offsets[i] (offsets in
+ * complex elements; pass NULL for the packed layout offsets[i] = sum of the
+ * sizes of blocks 0..i-1). out is host memory unless out_on_device != 0, in
+ * which case it is device memory from giaohip_device_alloc. */
+int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32_t* quartets, size_t n,
+                      const double field[6], const uint64_t* offsets, double* out, int out_on_device);
+
+/* Size in complex elements of the packed output of a quartet list. */
+int giaohip_eri_batch_size(const giaohip_basis* basis, const uint32_t* quartets, size_t n, uint64_t* total);
+
+/* Replaces compute_eri_tensor (eri.cpp:809-847, decl. eri.hpp:137-143):
+ * the full (n,n,n,n) complex128 tensor from canonical quartets and their
+ * four symmetry images. */
+int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* basis, const double field[6], double* out,
+                       int out_on_device);
+
+/* Replaces primitive_eri (eri.cpp:438-452, decl. eri.hpp:114-121).
+ * prim[i] = {exponent, cx, cy, cz, coefficient, normalized}; ang = powers. */
+int giaohip_primitive_eri(giaohip_ctx* ctx, const double prim[4][6], const int ang[4][3], const double field[6],
+                          double out[2]);
+
+/* --- streamed Cholesky -------------------------------------------------
+ * Replaces detail::cholesky_diagonal (cholesky.cpp:285-314, decl.
+ * src/cholesky.hpp:9-11): pairs = npairs*(a,b) shell pairs with a <= b;
+ * out = n*n doubles (host). */
+int giaohip_cholesky_diagonal(giaohip_ctx* ctx, const giaohip_basis* basis, const uint64_t* pairs, size_t npairs,
+                              const double field[6], double* out);
+
+/* Replaces detail::cholesky_columns (cholesky.cpp:203-282,316-323, decl.
+ * src/cholesky.hpp:12-17): pivots = npiv*(mu,nu) AO pairs sharing one shell
+ * pair; out = npiv*n*n values, complex128 unless real_output (which requires
+ * an exact zero field and gives float64). Host memory. */
+int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* basis, const uint64_t* pivots, size_t npiv,
+                             const double field[6], double* out, int real_output);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
