@@ -14,7 +14,8 @@ import numpy as np
 
 LIBRARY = Path(__file__).resolve().parent / "libgiaohip.so"
 
-OK, INVALID_ARGUMENT, LENGTH_ERROR, OUT_OF_RANGE, BOYS_ERROR, DEVICE_ERROR, UNSUPPORTED = range(7)
+(OK, INVALID_ARGUMENT, LENGTH_ERROR, OUT_OF_RANGE, BOYS_ERROR, DEVICE_ERROR, UNSUPPORTED, OUT_OF_MEMORY,
+ RUNTIME_ERROR) = range(9)
 
 
 class BoysNumericalError(ArithmeticError):
@@ -32,6 +33,8 @@ _EXC = {
     BOYS_ERROR: BoysNumericalError,
     DEVICE_ERROR: RuntimeError,
     UNSUPPORTED: RuntimeError,
+    OUT_OF_MEMORY: MemoryError,
+    RUNTIME_ERROR: RuntimeError,
 }
 
 
@@ -80,6 +83,11 @@ _SIGNATURES = {
     "giaohip_primitive_eri": ([_vp, _dp, _ip, _dp, _dp], C.c_int),
     "giaohip_cholesky_diagonal": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp], C.c_int),
     "giaohip_cholesky_columns": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp, C.c_int], C.c_int),
+    "giaohip_cholesky_run": ([_vp, _vp, _dp, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(_vp)], C.c_int),
+    "giaohip_cholesky_info": ([_vp, _u64p, _dp, _dp, _ip, _dp, _dp, _dp], C.c_int),
+    "giaohip_cholesky_pivots": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+    "giaohip_cholesky_vectors": ([_vp, _vp, C.c_uint64, C.c_uint64, _dp], C.c_int),
+    "giaohip_cholesky_destroy": ([_vp, _vp], C.c_int),
 }
 
 EXPORTED = tuple(_SIGNATURES)
@@ -260,6 +268,8 @@ class DeviceBasis:
 
     def quartet_sizes(self, quartets) -> np.ndarray:
         q = np.asarray(quartets, dtype=np.int64).reshape(-1, 4)
+        if q.size and (q.min() < 0 or q.max() >= self.nshell):
+            raise IndexError("ERI shell-quartet index is out of range")
         return np.prod(self.ao_counts[q], axis=1) if len(q) else np.zeros(0, np.int64)
 
     def eri_batch(self, quartets, field, out=None, device_out: int | None = None):
@@ -317,6 +327,67 @@ class DeviceBasis:
             )
         )
         return out
+
+
+class DeviceCholesky:
+    """Factors of a device-resident Cholesky run (giaohip_cholesky)."""
+
+    def __init__(self, basis: "DeviceBasis", handle):
+        self.basis = basis
+        self.ctx = basis.ctx
+        self.handle = handle
+        rank = C.c_uint64()
+        res = C.c_double()
+        tol = C.c_double()
+        conv = C.c_int()
+        col = C.c_double()
+        upd = C.c_double()
+        dia = C.c_double()
+        self.ctx.check(
+            self.ctx.lib.giaohip_cholesky_info(handle, C.byref(rank), C.byref(res), C.byref(tol), C.byref(conv),
+                                               C.byref(col), C.byref(upd), C.byref(dia))
+        )
+        self.rank = int(rank.value)
+        self.max_residual = float(res.value)
+        self.tolerance = float(tol.value)
+        self.converged = bool(conv.value)
+        self.timings_ms = {"column": col.value, "update": upd.value, "diagonal": dia.value}
+
+    def pivots(self) -> np.ndarray:
+        out = np.zeros((self.rank, 2), dtype=np.int64)
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_pivots(self.handle, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def vectors(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        n = self.basis.nao
+        count = self.rank - first if count is None else count
+        out = np.empty((count, n, n), dtype=np.complex128)
+        self.ctx.check(
+            self.ctx.lib.giaohip_cholesky_vectors(self.ctx.handle, self.handle, C.c_uint64(first), C.c_uint64(count),
+                                                  out.ctypes.data_as(_dp))
+        )
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.giaohip_cholesky_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cholesky_run(basis: "DeviceBasis", field, threshold: float, max_rank: int = 0, stop_after: int = 0):
+    handle = _vp()
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    basis.ctx.check(
+        basis.ctx.lib.giaohip_cholesky_run(basis.ctx.handle, basis.handle, _p(f), C.c_double(float(threshold)),
+                                           C.c_uint64(int(max_rank)), C.c_uint64(int(stop_after)), C.byref(handle))
+    )
+    return DeviceCholesky(basis, handle)
 
 
 _default_ctx: Context | None = None

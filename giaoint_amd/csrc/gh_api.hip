@@ -90,6 +90,17 @@ struct giaohip_ctx {
     size_t staging_budget_bytes = size_t(1) << 30;
 };
 
+struct giaohip_cholesky {
+    long long n = 0;
+    int rows_per_chunk = 0;
+    std::vector<zc*> chunks;  // device rows [chunk][row][n*n]
+    std::vector<int64_t> pivots;
+    uint64_t rank = 0;
+    double max_residual = 0.0, tolerance = 0.0;
+    int converged = 0;
+    double column_ms = 0.0, update_ms = 0.0, diagonal_ms = 0.0;
+};
+
 namespace {
 
 int set_error(giaohip_ctx* ctx, int code, const std::string& msg) {
@@ -341,6 +352,68 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
 void reset_stats(giaohip_ctx* ctx) { ctx->stats = giaohip_stats{}; }
 
 long long ao_count(const giaohip_basis* b, int s) { return b->shells[s].ao_count; }
+
+// Cholesky diagonal D[nu,mu] = D[mu,nu] = Re (mu nu|nu mu) into device memory
+// (cholesky.cpp:285-314): (ab|ba) blocks, then the diagonal extraction.
+int diagonal_to_device(giaohip_ctx* ctx, const giaohip_basis* b, const uint64_t* pairs, size_t npairs,
+                       const double field[6], double* d_diag) {
+    const int ns = b->ns;
+    const long long nao = b->nao;
+    std::vector<char> seen((size_t)ns * ns + 1, 0);
+    std::vector<int> pr;
+    std::vector<Request> reqs;
+    std::vector<long long> boff;
+    long long total = 0;
+    for (size_t i = 0; i < npairs; ++i) {
+        const uint64_t a = pairs[2 * i], bb = pairs[2 * i + 1];
+        if (a >= (uint64_t)ns || bb >= (uint64_t)ns || a > bb)
+            return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky diagonal shell pair must be valid and ordered");
+        if (seen[a * ns + bb]) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "duplicate Cholesky diagonal shell pair");
+        seen[a * ns + bb] = 1;
+    }
+    for (size_t i = 0; i < npairs; ++i) {
+        const int a = (int)pairs[2 * i], bb = (int)pairs[2 * i + 1];
+        const int q[4] = {a, bb, bb, a};
+        const int st = check_quartet_supported(ctx, b, q);
+        if (st) return st;
+        Request r{};
+        for (int k = 0; k < 4; ++k) {
+            r.sh[k] = q[k];
+            r.xfix[k] = -1;
+        }
+        r.cd0 = 0;
+        r.cd1 = -1;
+        const long long na = ao_count(b, a), nb = ao_count(b, bb);
+        r.out_base = total;
+        r.stride[0] = nb * nb * na;
+        r.stride[1] = nb * na;
+        r.stride[2] = na;
+        r.stride[3] = 1;
+        reqs.push_back(r);
+        boff.push_back(total);
+        pr.push_back(a);
+        pr.push_back(bb);
+        total += na * nb * nb * na;
+    }
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx->blocks.ensure(std::max<long long>(total, 1) * sizeof(zc)), "hipMalloc(blocks)");
+    HIP_TRY(ctx->misc_a.ensure(boff.size() * sizeof(long long) + pr.size() * sizeof(int) + 64), "hipMalloc");
+    int st = run_requests(ctx, b, field, reqs, ctx->blocks.as<zc>());
+    if (st) return st;
+    long long* d_boff = ctx->misc_a.as<long long>();
+    int* d_pr = reinterpret_cast<int*>(d_boff + boff.size());
+    if (!boff.empty()) {
+        HIP_TRY(hipMemcpyAsync(d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        HIP_TRY(hipMemcpyAsync(d_pr, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    }
+    HIP_TRY(hipMemsetAsync(d_diag, 0, (size_t)nao * nao * sizeof(double), s), "hipMemsetAsync");
+    HIP_TRY(gh::launch_diag_extract(d_boff, d_pr, (int)npairs, b->d_shells.as<DevShell>(), b->d_aooff.as<long long>(), nao,
+                                    ctx->blocks.as<zc>(), d_diag, s),
+            "diagonal launch");
+    HIP_TRY(hipStreamSynchronize(s), "diagonal execution");
+    return GIAOHIP_OK;
+}
 
 }  // namespace
 
@@ -839,64 +912,13 @@ int giaohip_cholesky_diagonal(giaohip_ctx* ctx, const giaohip_basis* b, const ui
     if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
-    const int ns = b->ns;
-    const long long nao = b->nao;
-    std::vector<char> seen((size_t)ns * ns + 1, 0);
-    std::vector<int> pr;
-    std::vector<Request> reqs;
-    std::vector<long long> boff;
-    long long total = 0;
-    for (size_t i = 0; i < npairs; ++i) {
-        const uint64_t a = pairs[2 * i], bb = pairs[2 * i + 1];
-        if (a >= (uint64_t)ns || bb >= (uint64_t)ns || a > bb)
-            return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky diagonal shell pair must be valid and ordered");
-        if (seen[a * ns + bb]) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "duplicate Cholesky diagonal shell pair");
-        seen[a * ns + bb] = 1;
-    }
-    for (size_t i = 0; i < npairs; ++i) {
-        const int a = (int)pairs[2 * i], bb = (int)pairs[2 * i + 1];
-        const int q[4] = {a, bb, bb, a};
-        const int st = check_quartet_supported(ctx, b, q);
-        if (st) return st;
-        Request r{};
-        for (int k = 0; k < 4; ++k) {
-            r.sh[k] = q[k];
-            r.xfix[k] = -1;
-        }
-        r.cd0 = 0;
-        r.cd1 = -1;
-        const long long na = ao_count(b, a), nb = ao_count(b, bb);
-        r.out_base = total;
-        r.stride[0] = nb * nb * na;
-        r.stride[1] = nb * na;
-        r.stride[2] = na;
-        r.stride[3] = 1;
-        reqs.push_back(r);
-        boff.push_back(total);
-        pr.push_back(a);
-        pr.push_back(bb);
-        total += na * nb * nb * na;
-    }
     (void)hipSetDevice(ctx->device);
-    hipStream_t s = ctx->stream;
-    const size_t dbytes = (size_t)nao * nao * sizeof(double);
-    HIP_TRY(ctx->blocks.ensure(std::max<long long>(total, 1) * sizeof(zc)), "hipMalloc(blocks)");
-    HIP_TRY(ctx->misc_a.ensure(boff.size() * sizeof(long long) + pr.size() * sizeof(int) + 64), "hipMalloc");
+    const size_t dbytes = (size_t)b->nao * b->nao * sizeof(double);
     HIP_TRY(ctx->staging.ensure(std::max<size_t>(dbytes, 16)), "hipMalloc");
-    int st = run_requests(ctx, b, field, reqs, ctx->blocks.as<zc>());
+    int st = diagonal_to_device(ctx, b, pairs, npairs, field, ctx->staging.as<double>());
     if (st) return st;
-    long long* d_boff = ctx->misc_a.as<long long>();
-    int* d_pr = reinterpret_cast<int*>(d_boff + boff.size());
-    if (!boff.empty()) {
-        HIP_TRY(hipMemcpyAsync(d_boff, boff.data(), boff.size() * sizeof(long long), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-        HIP_TRY(hipMemcpyAsync(d_pr, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-    }
-    HIP_TRY(hipMemsetAsync(ctx->staging.p, 0, dbytes, s), "hipMemsetAsync");
-    HIP_TRY(gh::launch_diag_extract(d_boff, d_pr, (int)npairs, b->d_shells.as<DevShell>(), b->d_aooff.as<long long>(), nao,
-                                    ctx->blocks.as<zc>(), ctx->staging.as<double>(), s),
-            "diagonal launch");
-    HIP_TRY(hipMemcpyAsync(out, ctx->staging.p, dbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-    HIP_TRY(hipStreamSynchronize(s), "diagonal execution");
+    HIP_TRY(hipMemcpyAsync(out, ctx->staging.p, dbytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "diagonal execution");
     return GIAOHIP_OK;
 }
 
@@ -963,6 +985,284 @@ int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* b, const uin
         HIP_TRY(hipMemcpyAsync(out, ctx->blocks.p, ncol * sizeof(zc), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     }
     HIP_TRY(hipStreamSynchronize(s), "columns execution");
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double threshold,
+                         uint64_t max_rank, uint64_t stop_after, giaohip_cholesky** out) {
+    if (!ctx || !b || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    *out = nullptr;
+    if (!std::isfinite(threshold) || threshold < 0.0)
+        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "threshold must be finite and non-negative");
+    for (const auto& sh : b->shells)
+        if (sh.ncontr != 1)
+            return set_error(ctx, GIAOHIP_UNSUPPORTED,
+                             "device-resident Cholesky needs one contraction per shell; use the policy driver");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const long long n = b->nao, n2 = n * n;
+    const int ns = b->ns;
+    auto* chol = new giaohip_cholesky;
+    chol->n = n;
+    // Factor rows live in chunks of about 2 GiB (at least one row).
+    chol->rows_per_chunk = (int)std::max<long long>(1, (2LL << 30) / (n2 * (long long)sizeof(zc)));
+    auto cleanup = [&](int code) {
+        for (zc* c : chol->chunks) (void)hipFree(c);
+        delete chol;
+        return code;
+    };
+
+    // Static bra table: every ordered shell pair, grouped by (la, lb).
+    std::vector<gh::CholUnit> units;
+    std::vector<std::array<int, 4>> groups;  // la, lb, first, count
+    int max_lab = 0;
+    for (int la = 0; la <= gh::kMaxShellL; ++la)
+        for (int lb = 0; lb <= gh::kMaxShellL; ++lb) {
+            const int first = (int)units.size();
+            for (int a = 0; a < ns; ++a) {
+                if (b->shells[a].l != la) continue;
+                for (int bb = 0; bb < ns; ++bb) {
+                    if (b->shells[bb].l != lb) continue;
+                    gh::CholUnit u{};
+                    u.sh_a = a;
+                    u.sh_b = bb;
+                    u.lab = la + lb;
+                    u.kab = b->shells[a].nprim * b->shells[bb].nprim;
+                    u.out_base = b->ao_off[a] * n + b->ao_off[bb];
+                    units.push_back(u);
+                }
+            }
+            if ((int)units.size() > first) {
+                groups.push_back({la, lb, first, (int)units.size() - first});
+                max_lab = std::max(max_lab, la + lb);
+            }
+        }
+    long long p1 = 0, p2 = 0;
+    for (auto& u : units) {
+        u.p1 = p1;
+        u.p2 = p2;
+        p1 += u.kab;
+        p2 += (long long)u.kab * u.lab;
+    }
+    int max_l = 0, max_k = 1;
+    for (const auto& sh : b->shells) {
+        max_l = std::max(max_l, sh.l);
+        max_k = std::max(max_k, sh.nprim);
+    }
+    if (2 * max_l + max_lab > gh::kMaxTotalL || max_l > gh::kMaxShellL)
+        return cleanup(set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support combined angular order through 24"));
+    const int nunits = (int)units.size();
+    const long long seed_zc = (p2 + p1 * (long long)(2 * max_l + 4)) * (long long)max_k * max_k;
+    const int parts = gh::chol_partials(n2);
+
+    DevBuffer d_units, d_sq, d_wi, d_res, d_col, d_coef, d_pmax, d_pidx, d_state, d_flags, d_chunkptr;
+    auto free_all = [&]() {
+        for (DevBuffer* x : {&d_units, &d_sq, &d_wi, &d_res, &d_col, &d_coef, &d_pmax, &d_pidx, &d_state, &d_flags,
+                             &d_chunkptr})
+            x->release();
+    };
+#define CHOL_TRY(expr, what)                            \
+    do {                                                \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) {                         \
+            free_all();                                 \
+            return cleanup(hip_fail(ctx, _e, what));    \
+        }                                               \
+    } while (0)
+    CHOL_TRY(d_units.ensure(units.size() * sizeof(gh::CholUnit)), "hipMalloc");
+    CHOL_TRY(hipMemcpy(d_units.p, units.data(), units.size() * sizeof(gh::CholUnit), hipMemcpyHostToDevice), "hipMemcpy");
+    CHOL_TRY(d_sq.ensure(units.size() * sizeof(SeedQuartet)), "hipMalloc");
+    CHOL_TRY(d_wi.ensure(units.size() * sizeof(WorkItem)), "hipMalloc");
+    CHOL_TRY(d_res.ensure(n2 * sizeof(double)), "hipMalloc");
+    CHOL_TRY(d_col.ensure(n2 * sizeof(zc)), "hipMalloc");
+    CHOL_TRY(d_pmax.ensure(parts * sizeof(double)), "hipMalloc");
+    CHOL_TRY(d_pidx.ensure(parts * sizeof(long long)), "hipMalloc");
+    CHOL_TRY(d_state.ensure(sizeof(gh::CholState)), "hipMalloc");
+    CHOL_TRY(d_flags.ensure(sizeof(int)), "hipMalloc");
+    CHOL_TRY(ctx->seeds.ensure(std::max<long long>(seed_zc, 1) * sizeof(zc)), "hipMalloc(seeds)");
+    CHOL_TRY(hipMemset(d_flags.p, 0, sizeof(int)), "hipMemset");
+
+    // Diagonal over all ordered shell pairs a <= b (cholesky.py:143-169).
+    std::vector<uint64_t> pairs;
+    for (int a = 0; a < ns; ++a)
+        for (int bb = a; bb < ns; ++bb) {
+            pairs.push_back((uint64_t)a);
+            pairs.push_back((uint64_t)bb);
+        }
+    hipEvent_t e0, e1, e2;
+    CHOL_TRY(hipEventCreate(&e0), "hipEventCreate");
+    CHOL_TRY(hipEventCreate(&e1), "hipEventCreate");
+    CHOL_TRY(hipEventCreate(&e2), "hipEventCreate");
+    CHOL_TRY(hipEventRecord(e0, s), "hipEventRecord");
+    int st = diagonal_to_device(ctx, b, pairs.data(), pairs.size() / 2, field, d_res.as<double>());
+    if (st) {
+        free_all();
+        return cleanup(st);
+    }
+    CHOL_TRY(hipEventRecord(e1, s), "hipEventRecord");
+    CHOL_TRY(gh::launch_chol_argmax(d_res.as<double>(), n2, d_pmax.as<double>(), d_pidx.as<long long>(),
+                                    d_state.as<gh::CholState>(), true, s),
+             "argmax launch");
+    gh::CholState state{};
+    CHOL_TRY(hipMemcpyAsync(&state, d_state.p, sizeof state, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    CHOL_TRY(hipStreamSynchronize(s), "diagonal");
+    chol->diagonal_ms = elapsed(e0, e1);
+    const double floor = 64.0 * 2.220446049250313e-16 * std::max(state.value, 0.0);
+    const double tol = std::max(threshold, floor);
+    const double psd_tol = std::max(floor * 16.0, 1e-12);
+    chol->tolerance = tol;
+    const uint64_t limit = max_rank ? max_rank : (uint64_t)n2;
+    const std::vector<double> kap = wave_vectors(b, field);
+    CHOL_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
+    CHOL_TRY(hipMemcpy(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    CHOL_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
+    CHOL_TRY(hipMemset(ctx->errrec.p, 0, sizeof(DevError)), "hipMemset");
+    struct Host {
+        gh::CholState state;
+        int flags;
+    };
+    Host* host = nullptr;
+    CHOL_TRY(hipHostMalloc((void**)&host, sizeof(Host), hipHostMallocDefault), "hipHostMalloc");
+    int status = GIAOHIP_OK;
+    while (state.value > tol) {
+        if (stop_after && chol->rank >= stop_after) break;
+        if (chol->rank >= limit) {
+            std::ostringstream m;
+            m << "Cholesky ERI output requires at least " << (chol->rank + 1) * (uint64_t)(n2 * 16 + 16)
+              << " bytes, exceeding the requested rank limit " << limit;
+            status = set_error(ctx, GIAOHIP_OUT_OF_MEMORY, m.str());
+            break;
+        }
+        const long long p = state.pivot;
+        const long long mu = p / n, nu = p % n;
+        const int sa = b->owner[mu], sb = b->owner[nu];
+        const DevShell& C = b->shells[sb];
+        const DevShell& D = b->shells[sa];
+        gh::CholKet ket{};
+        ket.sh_c = sb;
+        ket.sh_d = sa;
+        ket.lcd = C.l + D.l;
+        ket.kcd = C.nprim * D.nprim;
+        const long long cl = nu - b->ao_off[sb], dl = mu - b->ao_off[sa];
+        ket.xc = (int)(cl / C.ncart);
+        ket.xd = (int)(dl / D.ncart);
+        ket.comp = (int)(cl % C.ncart) * D.ncart + (int)(dl % D.ncart);
+        ket.n = n;
+        // factor storage for row `rank`
+        const int chunk = (int)(chol->rank / chol->rows_per_chunk);
+        if (chunk >= (int)chol->chunks.size()) {
+            void* mem = nullptr;
+            CHOL_TRY(hipMalloc(&mem, (size_t)chol->rows_per_chunk * n2 * sizeof(zc)), "hipMalloc(factors)");
+            chol->chunks.push_back(static_cast<zc*>(mem));
+            CHOL_TRY(d_chunkptr.ensure(chol->chunks.size() * sizeof(zc*)), "hipMalloc");
+            CHOL_TRY(hipMemcpy(d_chunkptr.p, chol->chunks.data(), chol->chunks.size() * sizeof(zc*),
+                               hipMemcpyHostToDevice),
+                     "hipMemcpy");
+            CHOL_TRY(d_coef.ensure(chol->chunks.size() * chol->rows_per_chunk * sizeof(zc)), "hipMalloc");
+        }
+        zc* row = chol->chunks[chunk] + (long long)(chol->rank % chol->rows_per_chunk) * n2;
+        CHOL_TRY(hipEventRecord(e0, s), "hipEventRecord");
+        CHOL_TRY(gh::launch_chol_expand(d_units.as<gh::CholUnit>(), nunits, ket, d_sq.as<SeedQuartet>(),
+                                        d_wi.as<WorkItem>(), s),
+                 "expand launch");
+        int maxL = max_lab + ket.lcd;
+        CHOL_TRY(gh::launch_seed(nmax_bucket(maxL), d_sq.as<SeedQuartet>(), nunits, p1 * ket.kcd,
+                                 b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
+                                 ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), s),
+                 "seed launch");
+        for (const auto& g : groups) {
+            gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
+            CHOL_TRY(gh::launch_transform(ca, d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
+                                          b->d_exps.as<double>(), b->d_coefs.as<double>(), b->d_norms.as<double>(),
+                                          ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_col.as<zc>(), s),
+                     "transform launch");
+        }
+        CHOL_TRY(hipEventRecord(e1, s), "hipEventRecord");
+        CHOL_TRY(gh::launch_chol_gather(d_chunkptr.as<const zc*>(), chol->rows_per_chunk, n2, (int)chol->rank,
+                                        d_state.as<gh::CholState>(), d_coef.as<zc>(), s),
+                 "gather launch");
+        CHOL_TRY(gh::launch_chol_update(d_col.as<zc>(), d_chunkptr.as<const zc*>(), chol->rows_per_chunk, n2,
+                                        (int)chol->rank, d_coef.as<zc>(), d_state.as<gh::CholState>(),
+                                        d_res.as<double>(), row, psd_tol, d_flags.as<int>(), d_pmax.as<double>(),
+                                        d_pidx.as<long long>(), s),
+                 "update launch");
+        CHOL_TRY(gh::launch_chol_argmax(d_res.as<double>(), n2, d_pmax.as<double>(), d_pidx.as<long long>(),
+                                        d_state.as<gh::CholState>(), false, s),
+                 "argmax launch");
+        CHOL_TRY(hipEventRecord(e2, s), "hipEventRecord");
+        CHOL_TRY(hipMemcpyAsync(&host->state, d_state.p, sizeof(gh::CholState), hipMemcpyDeviceToHost, s), "copy");
+        CHOL_TRY(hipMemcpyAsync(&host->flags, d_flags.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+        CHOL_TRY(hipStreamSynchronize(s), "Cholesky step");
+        chol->column_ms += elapsed(e0, e1);
+        chol->update_ms += elapsed(e1, e2);
+        chol->pivots.push_back(mu);
+        chol->pivots.push_back(nu);
+        chol->rank += 1;
+        if (host->flags & 1) {
+            status = set_error(ctx, GIAOHIP_RUNTIME_ERROR, "Coulomb Cholesky residual lost positive semidefiniteness");
+            break;
+        }
+        state = host->state;
+    }
+    DevError de{};
+    (void)hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost);
+    (void)hipHostFree(host);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    free_all();
+#undef CHOL_TRY
+    if (de.code != 0) return cleanup(boys_error(ctx, de));
+    if (status != GIAOHIP_OK) return cleanup(status);
+    chol->max_residual = std::max(state.value, 0.0);
+    chol->converged = state.value <= tol ? 1 : 0;
+    *out = chol;
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max_residual, double* tolerance,
+                          int* converged, double* column_ms, double* update_ms, double* diagonal_ms) {
+    if (!c) return GIAOHIP_INVALID_ARGUMENT;
+    if (rank) *rank = c->rank;
+    if (max_residual) *max_residual = c->max_residual;
+    if (tolerance) *tolerance = c->tolerance;
+    if (converged) *converged = c->converged;
+    if (column_ms) *column_ms = c->column_ms;
+    if (update_ms) *update_ms = c->update_ms;
+    if (diagonal_ms) *diagonal_ms = c->diagonal_ms;
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_pivots(const giaohip_cholesky* c, int64_t* out) {
+    if (!c || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::copy(c->pivots.begin(), c->pivots.end(), out);
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_vectors(giaohip_ctx* ctx, const giaohip_cholesky* c, uint64_t first, uint64_t count, double* out) {
+    if (!ctx || !c || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (first + count > c->rank) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky vector range is out of range");
+    const long long n2 = c->n * c->n;
+    for (uint64_t k = 0; k < count; ++k) {
+        const uint64_t r = first + k;
+        const zc* row = c->chunks[r / c->rows_per_chunk] + (long long)(r % c->rows_per_chunk) * n2;
+        HIP_TRY(hipMemcpyAsync(out + 2 * k * n2, row, n2 * sizeof(zc), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "copy");
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_destroy(giaohip_ctx* ctx, giaohip_cholesky* c) {
+    if (!c) return GIAOHIP_OK;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    for (zc* p : c->chunks) (void)hipFree(p);
+    delete c;
     return GIAOHIP_OK;
 }
 

@@ -266,21 +266,12 @@ __device__ __host__ inline LdsLayout lds_layout(int la, int lb, int lc, int ld, 
     return s;
 }
 
-// Lexicographic (t, u, v) table of degree <= D: offsets off[t*(D+1)+u] so that
-// index(t,u,v) = off[t*(D+1)+u] + v.
-__device__ inline void build_lex_offsets(int D, int* off) {
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int t = 0; t <= D; ++t)
-            for (int u = 0; u <= D; ++u) {
-                if (t + u <= D) {
-                    off[t * (D + 1) + u] = run;
-                    run += D - t - u + 1;
-                } else {
-                    off[t * (D + 1) + u] = -1;
-                }
-            }
-    }
+// Lexicographic (t, u, v) layout of all Hermite indices of degree <= D:
+// index(t,u,v) = off(t,u) + v with off(t,u) = NH(D) - NH(D-t) + u(D-t+1) - u(u-1)/2
+// (entries with first index >= t form NH(D-t); within t, u' < u contribute
+// D-t-u'+1 each).
+__device__ __host__ inline int lex_off(int D, int t, int u) {
+    return nherm(D) - nherm(D - t) + u * (D - t + 1) - u * (u - 1) / 2;
 }
 
 __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const WorkItem* __restrict__ items,
@@ -307,34 +298,33 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
     zc* work = zbase + lay.work;
     const int tid = threadIdx.x;
 
-    // Index tables.
-    build_lex_offsets(L, offL);
-    if (tid == 32) {
-        int run = 0;
-        for (int t = 0; t <= Lab; ++t)
-            for (int u = 0; u <= Lab; ++u) {
-                if (t + u <= Lab) {
-                    offAB[t * (Lab + 1) + u] = run;
-                    for (int v = 0; v <= Lab - t - u; ++v) tuvAB[run + v] = t | (u << 6) | (v << 12);
-                    run += Lab - t - u + 1;
-                } else {
-                    offAB[t * (Lab + 1) + u] = -1;
-                }
-            }
+    // Index tables (parallel fills).
+    for (int e = tid; e < (L + 1) * (L + 1); e += kNT) {
+        const int t = e / (L + 1), u = e % (L + 1);
+        offL[e] = t + u <= L ? lex_off(L, t, u) : -1;
     }
-    if (tid == 64) {
-        // degree-sorted list of (t,u,v) with their lexicographic index for R0
-        int k = 0;
-        for (int d = 0; d <= L; ++d)
-            for (int t = d; t >= 0; --t)
-                for (int u = d - t; u >= 0; --u) {
-                    const int v = d - t - u;
-                    int lex = 0;
-                    for (int tt = 0; tt < t; ++tt) lex += (L - tt + 1) * (L - tt + 2) / 2;
-                    for (int uu = 0; uu < u; ++uu) lex += L - t - uu + 1;
-                    lex += v;
-                    listL[k++] = t | (u << 6) | (v << 12) | (lex << 18);
-                }
+    for (int e = tid; e < (Lab + 1) * (Lab + 1); e += kNT) {
+        const int t = e / (Lab + 1), u = e % (Lab + 1);
+        if (t + u <= Lab) {
+            const int o = lex_off(Lab, t, u);
+            offAB[e] = o;
+            for (int v = 0; v <= Lab - t - u; ++v) tuvAB[o + v] = t | (u << 6) | (v << 12);
+        } else {
+            offAB[e] = -1;
+        }
+    }
+    // R0 entries sorted by degree: degree d, then t descending, then u descending.
+    for (int e = tid; e < nherm(L); e += kNT) {
+        int d = 0;
+        while (nherm(d) <= e) ++d;
+        int r = e - (d > 0 ? nherm(d - 1) : 0);
+        int t = d;
+        while (r >= d - t + 1) {
+            r -= d - t + 1;
+            --t;
+        }
+        const int u = d - t - r, v = d - t - u;
+        listL[e] = t | (u << 6) | (v << 12) | ((lex_off(L, t, u) + v) << 18);
     }
     __syncthreads();
 

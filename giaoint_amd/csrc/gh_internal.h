@@ -10,6 +10,37 @@ struct ClassArgs {
     int tile;  // ket components per pass
 };
 
+// Device-resident Cholesky (gh_cholesky.hip).
+struct CholUnit {  // one bra shell pair of a pivot column
+    int sh_a, sh_b;
+    int lab, kab;
+    long long p1;        // sum of kab over earlier units
+    long long p2;        // sum of kab * lab over earlier units
+    long long out_base;  // off_a * n + off_b
+};
+struct CholKet {  // the pivot's ket (sb, sa) and component
+    int sh_c, sh_d;
+    int lcd, kcd;
+    int xc, xd;
+    int comp, pad;
+    long long n;
+};
+struct CholState {
+    double value;      // current maximum residual
+    long long pivot;   // its flat index (first maximum)
+};
+
+int chol_partials(long long n2);
+hipError_t launch_chol_expand(const CholUnit* units, int nunits, const CholKet& ket, SeedQuartet* sq, WorkItem* wi,
+                              hipStream_t s);
+hipError_t launch_chol_gather(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const CholState* st,
+                              zc* coef, hipStream_t s);
+hipError_t launch_chol_update(const zc* column, const zc* const* chunks, int rows_per_chunk, long long n2, int rank,
+                              const zc* coef, const CholState* st, double* res, zc* out_row, double psd_tol,
+                              int* flags, double* pmax, long long* pidx, hipStream_t s);
+hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
+                              bool partial_first, hipStream_t s);
+
 hipError_t upload_constants();
 int transform_lds_bytes(const ClassArgs& ca);
 int transform_max_w();

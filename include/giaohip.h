@@ -42,7 +42,9 @@ enum {
     GIAOHIP_OUT_OF_RANGE = 3,
     GIAOHIP_BOYS_ERROR = 4,
     GIAOHIP_DEVICE_ERROR = 5,
-    GIAOHIP_UNSUPPORTED = 6
+    GIAOHIP_UNSUPPORTED = 6,
+    GIAOHIP_OUT_OF_MEMORY = 7, /* MemoryError: an output cap would be exceeded */
+    GIAOHIP_RUNTIME_ERROR = 8  /* RuntimeError raised by the reference's driver */
 };
 
 /* ERI kernel variants selectable per call (giaohip_set_option). */
@@ -52,6 +54,7 @@ enum {
 
 typedef struct giaohip_ctx giaohip_ctx;
 typedef struct giaohip_basis giaohip_basis;
+typedef struct giaohip_cholesky giaohip_cholesky;
 
 /* Statistics of the last ERI call; times are device times from HIP events
  * recorded on the context's stream around the launches. Mirrors
@@ -144,6 +147,32 @@ int giaohip_cholesky_diagonal(giaohip_ctx* ctx, const giaohip_basis* basis, cons
  * an exact zero field and gives float64). Host memory. */
 int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* basis, const uint64_t* pivots, size_t npiv,
                              const double field[6], double* out, int real_output);
+
+/* Device-resident pivoted Cholesky of W[(mu,nu),(lam,sig)] = (nu mu|lam sig)
+ * with strict global pivoting: the loop of cholesky_eri
+ * (python/giao_integrals/cholesky.py:170-297) for the case where every
+ * candidate set is the pivot itself (segmented shells, complex factors;
+ * cholesky.py:193-247). Diagonal, argmax (first maximum in flat mu*n+nu
+ * order), column ERIs, projection, scaling, residual update, PSD check and
+ * clamp all run on the device; factors stay in device memory.
+ * Requires ncontr == 1 for every shell (GIAOHIP_UNSUPPORTED otherwise).
+ * Stops when the largest residual diagonal is <= max(threshold,
+ * 64*eps*max(diagonal)); max_rank caps the number of vectors
+ * (GIAOHIP_OUT_OF_MEMORY beyond it, as MemoryError in the reference);
+ * stop_after > 0 returns after that many pivots (timing of prefixes; the
+ * result then reports converged = 0). */
+int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* basis, const double field[6], double threshold,
+                         uint64_t max_rank, uint64_t stop_after, giaohip_cholesky** out);
+/* Rank, the measured largest residual diagonal, the stopping tolerance,
+ * whether the stopping test was met, and device time split (ms). */
+int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* max_residual, double* tolerance,
+                          int* converged, double* column_ms, double* update_ms, double* diagonal_ms);
+/* Pivot AO pairs (mu, nu), rank*2 int64 values. */
+int giaohip_cholesky_pivots(const giaohip_cholesky* chol, int64_t* out);
+/* Factor rows first..first+count-1 as complex128 (count*n*n values, host). */
+int giaohip_cholesky_vectors(giaohip_ctx* ctx, const giaohip_cholesky* chol, uint64_t first, uint64_t count,
+                             double* out);
+int giaohip_cholesky_destroy(giaohip_ctx* ctx, giaohip_cholesky* chol);
 
 #ifdef __cplusplus
 }

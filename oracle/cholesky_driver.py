@@ -1,0 +1,121 @@
+"""Checker restatement of the reference's Cholesky driver (TEST INFRASTRUCTURE).
+
+Restates python/giao_integrals/cholesky.py:60-297 in numpy with the
+selected-integral calls injected: ``diagonal(pairs) -> (n, n)`` and
+``columns(pivots) -> (len(pivots), n*n)`` (complex unless ``real``). With
+the C oracle's (or the reference-built library's) diagonal and columns this
+reproduces the reference's pivot policy: strict global argmax (first maximum
+in flat mu*n+nu order), the local-block override for complex generally
+contracted factors, the candidate sets and the bounded raw-column cache.
+Schwarz-screened diagonals (screening_threshold > 0) are not restated.
+"""
+
+from __future__ import annotations
+
+from collections import OrderedDict
+
+import numpy as np
+
+
+def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_center: bool,
+                 threshold: float = 1e-8, max_bytes=None, pivot_batch_size: int = 32, max_pivots=None,
+                 history=None):
+    """shells: list of (cartesian_count, contraction_count, primitive_count).
+
+    Returns (vectors (rank, n, n) complex128, pivots (rank, 2), max residual,
+    tolerance). ``max_pivots`` truncates the loop (prefix timing only);
+    ``history`` (a list) receives a copy of the residual before every step.
+    """
+    offsets = np.asarray(ao_offsets, dtype=np.int64)
+    ncart = np.array([s[0] for s in shells])
+    ncontr = np.array([s[1] for s in shells])
+    nprim = np.array([s[2] for s in shells])
+    ao = ncart * ncontr
+    n = int(ao.sum())
+    ns = len(shells)
+    spans = [slice(int(o), int(o + a)) for o, a in zip(offsets, ao)]
+    owners = np.repeat(np.arange(ns), ao)
+    pairs = [(a, b) for a in range(ns) for b in range(a, ns)]
+    measured = diagonal(pairs)
+    diag = np.zeros((n, n))
+    for a, b in pairs:
+        diag[spans[a], spans[b]] = measured[spans[a], spans[b]]
+        diag[spans[b], spans[a]] = measured[spans[b], spans[a]]
+    residual = diag.ravel().copy()
+    floor = 64 * np.finfo(float).eps * float(np.max(residual, initial=0.0))
+    tolerance = max(threshold, floor)
+    dtype = np.float64 if real else np.complex128
+    row_bytes = n * n * 16 + 16
+    limit = n * n if max_bytes is None else min(n * n, max_bytes // row_bytes)
+    factors = np.empty((0, n * n), dtype=dtype)
+    pivots = []
+    reuse = not single_center or bool(np.any(ncontr > 1) or np.any(nprim > 1))
+    candidate_count = 32 if reuse else 8
+    raw_cache = OrderedDict()
+    block = np.empty(0, dtype=int)
+    candidates = np.empty(0, dtype=int)
+    cols = np.empty((0, n * n), dtype=dtype)
+    while float(np.max(residual, initial=0.0)) > tolerance:
+        if max_pivots is not None and len(pivots) >= max_pivots:
+            break
+        if history is not None:
+            history.append(residual.copy())
+        pivot = int(np.argmax(residual))
+        if reuse and not real and pivot_batch_size > 1 and len(block):
+            local = int(block[np.argmax(residual[block])])
+            if residual[local] > tolerance and residual[local] >= 0.01 * residual[pivot]:
+                pivot = local
+        rank = len(pivots)
+        if rank >= limit:
+            raise MemoryError(f"Cholesky ERI output requires at least {(rank + 1) * row_bytes} bytes")
+        if rank == len(factors):
+            cap = min(limit, max(8, 2 * len(factors)))
+            grown = np.empty((cap, n * n), dtype=dtype)
+            grown[:rank] = factors[:rank]
+            factors = grown
+        if reuse and pivot in raw_cache:
+            candidates = np.array([pivot])
+            cols = raw_cache.pop(pivot)[None, :]
+        match = np.flatnonzero(candidates == pivot)
+        if not len(match):
+            mu, nu = divmod(pivot, n)
+            a, b = int(owners[mu]), int(owners[nu])
+            if reuse:
+                left = offsets[a] + (mu - offsets[a]) % ncart[a] + ncart[a] * np.arange(ncontr[a])
+                right = offsets[b] + (nu - offsets[b]) % ncart[b] + ncart[b] * np.arange(ncontr[b])
+                idx = (left[:, None] * n + right).ravel()
+            else:
+                idx = np.arange(n * n).reshape(n, n)[spans[a], spans[b]].ravel()
+            idx = idx[residual[idx] > tolerance]
+            candidates = idx[np.argsort(-residual[idx], kind="stable")[:candidate_count]]
+            if pivot not in candidates:
+                candidates[-1] = pivot
+            block = candidates[: min(32, pivot_batch_size)]
+            cmu, cnu = np.divmod(candidates, n)
+            cols = columns(list(zip(cmu.tolist(), cnu.tolist())))
+            if reuse:
+                for i, c in enumerate(candidates):
+                    if c != pivot:
+                        raw_cache[int(c)] = cols[i].copy()
+                cap = min(1024, max(8, rank + 8))
+                while len(raw_cache) > cap:
+                    raw_cache.popitem(last=False)
+            match = np.flatnonzero(candidates == pivot)
+        if reuse:
+            raw_cache.pop(pivot, None)
+        column = cols[int(match[0])].copy()
+        if rank:
+            column -= factors[:rank, pivot].conj() @ factors[:rank]
+        if real:
+            sq = column.reshape(n, n)
+            column = ((sq + sq.T) * 0.5).ravel()
+        column /= np.sqrt(residual[pivot])
+        factors[rank] = column
+        pivots.append(divmod(pivot, n))
+        residual -= np.abs(column) ** 2
+        if float(np.min(residual, initial=0.0)) < -max(floor * 16, 1e-12):
+            raise RuntimeError("Coulomb Cholesky residual lost positive semidefiniteness")
+        np.maximum(residual, 0, out=residual)
+        residual[pivot] = 0
+    vectors = np.array(factors[: len(pivots)], dtype=np.complex128).reshape(len(pivots), n, n)
+    return vectors, np.asarray(pivots, dtype=np.int64).reshape(-1, 2), float(np.max(residual, initial=0.0)), tolerance
