@@ -19,12 +19,19 @@ import numpy as np
 
 def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_center: bool,
                  threshold: float = 1e-8, max_bytes=None, pivot_batch_size: int = 32, max_pivots=None,
-                 history=None):
+                 history=None, follow=None, tie_rtol: float = 1e-12, stats=None):
     """shells: list of (cartesian_count, contraction_count, primitive_count).
 
     Returns (vectors (rank, n, n) complex128, pivots (rank, 2), max residual,
     tolerance). ``max_pivots`` truncates the loop (prefix timing only);
     ``history`` (a list) receives a copy of the residual before every step.
+
+    ``follow`` (strict pivoting only) replays another implementation's pivot
+    sequence: at every step the followed pivot must be a valid argmax up to
+    rounding, i.e. its residual within ``tie_rtol * max(initial diagonal)`` of
+    the largest residual (residuals D - sum |L|^2 carry absolute rounding of
+    order eps * max D per update); otherwise AssertionError. ``stats`` (a dict) receives the number
+    of steps where the followed pivot differs from this run's own argmax.
     """
     offsets = np.asarray(ao_offsets, dtype=np.int64)
     ncart = np.array([s[0] for s in shells])
@@ -42,7 +49,8 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
         diag[spans[a], spans[b]] = measured[spans[a], spans[b]]
         diag[spans[b], spans[a]] = measured[spans[b], spans[a]]
     residual = diag.ravel().copy()
-    floor = 64 * np.finfo(float).eps * float(np.max(residual, initial=0.0))
+    scale = float(np.max(residual, initial=0.0))
+    floor = 64 * np.finfo(float).eps * scale
     tolerance = max(threshold, floor)
     dtype = np.float64 if real else np.complex128
     row_bytes = n * n * 16 + 16
@@ -61,7 +69,21 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
         if history is not None:
             history.append(residual.copy())
         pivot = int(np.argmax(residual))
-        if reuse and not real and pivot_batch_size > 1 and len(block):
+        if follow is not None:
+            k = len(pivots)
+            if k >= len(follow):
+                raise AssertionError(f"followed run stopped after {k} pivots; residual {residual[pivot]:.3e} > {tolerance:.3e}")
+            fp = int(follow[k][0]) * n + int(follow[k][1])
+            top = residual[pivot]
+            if not residual[fp] >= top - tie_rtol * scale:
+                raise AssertionError(
+                    f"step {k}: followed pivot {divmod(fp, n)} residual {residual[fp]!r} is not a maximum "
+                    f"({divmod(pivot, n)} has {top!r})"
+                )
+            if fp != pivot and stats is not None:
+                stats["tie_flips"] = stats.get("tie_flips", 0) + 1
+            pivot = fp
+        elif reuse and not real and pivot_batch_size > 1 and len(block):
             local = int(block[np.argmax(residual[block])])
             if residual[local] > tolerance and residual[local] >= 0.01 * residual[pivot]:
                 pivot = local
@@ -117,5 +139,7 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
             raise RuntimeError("Coulomb Cholesky residual lost positive semidefiniteness")
         np.maximum(residual, 0, out=residual)
         residual[pivot] = 0
+    if follow is not None and len(follow) != len(pivots) and max_pivots is None:
+        raise AssertionError(f"followed run has {len(follow)} pivots, this run stopped after {len(pivots)}")
     vectors = np.array(factors[: len(pivots)], dtype=np.complex128).reshape(len(pivots), n, n)
     return vectors, np.asarray(pivots, dtype=np.int64).reshape(-1, 2), float(np.max(residual, initial=0.0)), tolerance

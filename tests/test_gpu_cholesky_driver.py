@@ -18,42 +18,37 @@ from oracle.pyoracle import make_basis
 pytestmark = pytest.mark.gpu
 
 
-def first_divergence(a, b):
-    for k, (x, y) in enumerate(zip(a, b)):
-        if tuple(x) != tuple(y):
-            return k
-    return None
-
-
-@pytest.mark.parametrize("natoms", [2])
+@pytest.mark.parametrize("natoms", [2, 3])
 def test_device_cholesky_matches_checker(gpu, port, natoms):
+    """Identical even-tempered atoms make many pair functions exactly equal
+    by symmetry, so the first maximum is decided by rounding. The checker
+    therefore replays the GPU's pivots and requires each to be a maximum up
+    to 1e-12 * max(initial diagonal); every factor must then agree to 1e-10."""
     sh, B, O = workloads.cholesky_atoms(natoms=natoms)
     b = make_basis(sh.shell_list(), port)
-    n = b.nao
     f = _native.field6(B, O)
-    hist = []
-    vec, piv, maxres, tol = cholesky_driver.cholesky_eri(
-        b.ao_offsets, [((l + 1) * (l + 2) // 2, 1, 1) for l in b.L],
-        lambda pairs: port.cholesky_diagonal(b, pairs, f, 16),
-        lambda pv: port.cholesky_columns(b, pv, f, 16),
-        real=False, single_center=False, threshold=1e-8, history=hist,
-    )
     db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
     res = _native.cholesky_run(db, f, 1e-8)
     assert res.converged
     assert res.max_residual <= res.tolerance
-    assert abs(res.tolerance - tol) <= 1e-20 + 1e-12 * tol
     gp = res.pivots()
-    k = first_divergence(gp, piv)
-    upto = min(len(gp), len(piv)) if k is None else k
-    print(f"rank gpu {res.rank} checker {len(piv)}; identical pivots for the first {upto}")
-    assert upto >= min(50, len(piv))
-    gv = res.vectors(0, upto)
-    assert np.max(np.abs(gv - vec[:upto])) <= 1e-10
-    if k is not None:
-        r = hist[k]
-        assert abs(r[gp[k][0] * n + gp[k][1]] - r[piv[k][0] * n + piv[k][1]]) <= 1e-12 * r.max()
-    assert abs(res.rank - len(piv)) <= max(2, len(piv) // 100)
+    stats = {}
+    vec, piv, maxres, tol = cholesky_driver.cholesky_eri(
+        b.ao_offsets, [((l + 1) * (l + 2) // 2, 1, 1) for l in b.L],
+        lambda pairs: port.cholesky_diagonal(b, pairs, f, 16),
+        lambda pv: port.cholesky_columns(b, pv, f, 16),
+        real=False, single_center=False, threshold=1e-8, follow=gp, stats=stats,
+    )
+    print(f"rank {res.rank}; pivots decided by rounding-level ties: {stats.get('tie_flips', 0)}")
+    assert abs(res.tolerance - tol) <= 1e-12 * tol
+    assert len(piv) == res.rank
+    worst = 0.0
+    for first in range(0, res.rank, 256):
+        cnt = min(256, res.rank - first)
+        worst = max(worst, float(np.max(np.abs(res.vectors(first, cnt) - vec[first:first + cnt]))))
+    print(f"max |L_gpu - L_checker| = {worst:.3e}")
+    assert worst <= 1e-10
+    assert abs(res.max_residual - maxres) <= 1e-12
 
 
 def test_device_cholesky_rank_cap_and_prefix(gpu, port):
