@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--quartets", type=int, default=200_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     return ap.parse_args()
 
 
@@ -108,6 +109,15 @@ def main():
         dbasis.eri_batch(batch.quartets, field, device_out=out)
         return ctx.stats()
 
+    # plain FP64 kernel (generic, one launch per class) on the same batch, for comparison
+    plain = None
+    if not args.no_plain:
+        ctx.set_variant("plain")
+        step()
+        plain = step()
+        ctx.set_variant("batched")
+        log(f"plain kernel: transform {plain['transform_ms']:.1f} ms seed {plain['seed_ms']:.1f} ms "
+            f"total {plain['total_ms']:.1f} ms")
     for i in range(args.warmup):
         s = step()
         log(f"warmup {i}: transform {s['transform_ms']:.1f} ms seed {s['seed_ms']:.1f} ms")
@@ -117,7 +127,7 @@ def main():
     for i in range(args.steps):
         stats.append(step())
         log(f"step {i}: transform {stats[-1]['transform_ms']:.1f} ms seed {stats[-1]['seed_ms']:.1f} ms "
-            f"total {stats[-1]['total_ms']:.1f} ms")
+            f"device {stats[-1]['total_ms']:.1f} ms call {stats[-1]['host_ms']:.1f} ms")
     ctx.synchronize()
     wall = time.perf_counter() - t
     ms_step = 1e3 * wall / args.steps
@@ -148,7 +158,13 @@ def main():
             "quartets": n,
             "output_bytes": total * 16,
             "algorithmic_flops": flops,
-            "kernel": "plain FP64, one launch per angular-momentum class",
+            "kernel": "batched FP64 (class-specialized transform, wave-uniform Hermite products)",
+        },
+        "plain_kernel": None if plain is None else {
+            "quartets_per_s": n / (plain["host_ms"] * 1e-3),
+            "transform_ms": plain["transform_ms"],
+            "seed_ms": plain["seed_ms"],
+            "speedup_batched_vs_plain": plain["host_ms"] / float(np.mean([s["host_ms"] for s in stats])),
         },
         "gflops": flops / (ms_step * 1e-3) / 1e9,
         "kernel_ms": {"seed": seed_ms, "transform": transform_ms, "launches_per_step": launches},
@@ -159,7 +175,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "transform_kernel (FP64 R recurrence + E.R.E)",
+            "kernel": "fast_transform_kernel<2,2,2,2> + pair_kernel (FP64 R recurrence + E.R.E)",
         },
         "checksum": float(np.abs(probe).sum()),
     }

@@ -48,6 +48,7 @@ class Stats(C.Structure):
         ("transform_ms", C.c_double),
         ("total_ms", C.c_double),
         ("launches", C.c_uint64),
+        ("host_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -67,6 +68,7 @@ _SIGNATURES = {
     "giaohip_last_error": ([_vp, C.c_char_p, C.c_size_t], C.c_size_t),
     "giaohip_stream": ([_vp], _vp),
     "giaohip_last_stats": ([_vp, C.POINTER(Stats)], C.c_int),
+    "giaohip_set_option": ([_vp, C.c_int, C.c_int], C.c_int),
     "giaohip_device_alloc": ([_vp, C.c_size_t, C.POINTER(_vp)], C.c_int),
     "giaohip_device_free": ([_vp, _vp], C.c_int),
     "giaohip_memcpy_to_host": ([_vp, _vp, _vp, C.c_size_t], C.c_int),
@@ -182,6 +184,11 @@ class Context:
         buf = C.create_string_buffer(1024)
         self.lib.giaohip_last_error(self.handle, buf, 1024)
         raise _EXC.get(status, RuntimeError)(buf.value.decode())
+
+    def set_variant(self, variant: str):
+        """'batched' (class-specialized kernels where available) or 'plain'."""
+        value = {"plain": 0, "batched": 1}[variant]
+        self.check(self.lib.giaohip_set_option(self.handle, 0, value))
 
     def stats(self) -> dict:
         s = Stats()

@@ -19,7 +19,7 @@ CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libgiaohip.so"
 ARCH = os.environ.get("GIAOHIP_ARCH", "gfx950")
 
-SOURCES = ["gh_api.hip", "gh_eri.hip", "gh_cholesky.hip"]
+SOURCES = ["gh_api.hip", "gh_eri.hip", "gh_eri_fast.hip", "gh_cholesky.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -33,6 +33,8 @@ HIPCC_FLAGS = [
     "-Xclang",
     "-packed-fp32-ops",
     "-Wno-pass-failed",
+    # compile-time plans of the class-specialized kernels
+    "-fconstexpr-steps=200000000",
 ]
 
 

@@ -9,6 +9,8 @@
 #include "gh_internal.h"
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -72,9 +74,9 @@ struct giaohip_basis {
     long long nao = 0;
     std::vector<DevShell> shells;
     std::vector<long long> ao_off;
-    std::vector<double> exps, coefs, norms;
+    std::vector<double> exps, coefs, norms, radial;
     std::vector<int> owner;
-    DevBuffer d_shells, d_exps, d_coefs, d_norms, d_aooff, d_owner;
+    DevBuffer d_shells, d_exps, d_coefs, d_norms, d_aooff, d_owner, d_radial;
 };
 
 struct giaohip_ctx {
@@ -83,7 +85,8 @@ struct giaohip_ctx {
     std::mutex mu;
     std::string err;
     giaohip_stats stats{};
-    DevBuffer seeds, squartets, items, kappa, errrec, staging, blocks, misc_a, misc_b, misc_c;
+    DevBuffer seeds, squartets, items, kappa, errrec, staging, blocks, misc_a, misc_b, misc_c, etab, fpq, fitems;
+    int eri_variant = GIAOHIP_ERI_BATCHED;
     hipEvent_t ev[4]{};
     std::set<void*> user_allocs;
     size_t seed_budget_bytes = size_t(4) << 30;
@@ -184,12 +187,25 @@ struct Request {
     long long stride[4];
 };
 
+// Per-class admission, memoized (the reference checks every component quartet).
+bool aux_within_limit(int la, int lb, int lc, int ld) {
+    constexpr int M = 33;
+    static std::mutex mu;
+    static std::vector<signed char> memo;  // -1 unknown, 0 over, 1 within
+    if (la >= M || lb >= M || lc >= M || ld >= M) return max_aux_entries(la, lb, lc, ld) <= kMaxAuxEntries;
+    std::lock_guard<std::mutex> lock(mu);
+    if (memo.empty()) memo.assign((size_t)M * M * M * M, -1);
+    signed char& v = memo[(((size_t)la * M + lb) * M + lc) * M + ld];
+    if (v < 0) v = max_aux_entries(la, lb, lc, ld) <= kMaxAuxEntries ? 1 : 0;
+    return v == 1;
+}
+
 int check_quartet_supported(giaohip_ctx* ctx, const giaohip_basis* b, const int sh[4]) {
     int l[4];
     for (int i = 0; i < 4; ++i) l[i] = b->shells[sh[i]].l;
     const int L = l[0] + l[1] + l[2] + l[3];
     if (L > 32) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "electron repulsion supports combined angular order through 32");
-    if (max_aux_entries(l[0], l[1], l[2], l[3]) > kMaxAuxEntries)
+    if (!aux_within_limit(l[0], l[1], l[2], l[3]))
         return set_error(ctx, GIAOHIP_LENGTH_ERROR, "ERI auxiliary exceeds the workspace limit");
     for (int i = 0; i < 4; ++i)
         if (l[i] > gh::kMaxShellL)
@@ -225,9 +241,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Runs a list of requests; outputs go to d_out (device). Accumulates stats.
-int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
-                 zc* d_out) {
+// Runs requests with the plain per-class transform kernel.
+int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
+              zc* d_out) {
     if (reqs.empty()) return GIAOHIP_OK;
     hipStream_t s = ctx->stream;
     const std::vector<double> kap = wave_vectors(b, field);
@@ -347,6 +363,199 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
         begin = end;
     }
     return GIAOHIP_OK;
+}
+
+
+// One request evaluated through a symmetry image of its quartet by the
+// class-specialized kernels: (ab|cd) = (cd|ab) and (ab|cd) = conj (ba|dc).
+struct FastReq {
+    int sh[4];    // oriented shells
+    int perm[4];  // oriented slot k holds original slot perm[k]
+    int conj;
+    const Request* r;
+    gh::FastClassInfo info;
+};
+
+bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
+    static const int perms[4][4] = {{0, 1, 2, 3}, {2, 3, 0, 1}, {1, 0, 3, 2}, {3, 2, 1, 0}};
+    for (int o = 0; o < 4; ++o) {
+        int l[4];
+        for (int k = 0; k < 4; ++k) l[k] = b->shells[r.sh[perms[o][k]]].l;
+        const gh::FastClassInfo info = gh::fast_class_info(l[0], l[1], l[2], l[3]);
+        if (!info.available) continue;
+        for (int k = 0; k < 4; ++k) {
+            f.sh[k] = r.sh[perms[o][k]];
+            f.perm[k] = perms[o][k];
+        }
+        f.conj = o >= 2 ? 1 : 0;
+        f.r = &r;
+        f.info = info;
+        return true;
+    }
+    return false;
+}
+
+int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq>& reqs, zc* d_out) {
+    if (reqs.empty()) return GIAOHIP_OK;
+    hipStream_t s = ctx->stream;
+    size_t begin = 0;
+    while (begin < reqs.size()) {
+        size_t end = begin;
+        size_t seed_zc = 0, tab_zc = 0;
+        long long pq_total = 0;
+        int maxL = 0;
+        std::vector<SeedQuartet> sq;
+        while (end < reqs.size()) {
+            const FastReq& f = reqs[end];
+            const DevShell &A = b->shells[f.sh[0]], &B = b->shells[f.sh[1]], &C = b->shells[f.sh[2]],
+                           &D = b->shells[f.sh[3]];
+            const int L = A.l + B.l + C.l + D.l;
+            const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
+            const size_t need_s = (size_t)(kab * kcd) * (size_t)(L + 4);
+            const size_t need_t = (size_t)(kab * f.info.bra_ne + kcd * f.info.ket_ne);
+            if (end > begin && (seed_zc + tab_zc + need_s + need_t) * sizeof(zc) > ctx->seed_budget_bytes) break;
+            SeedQuartet q{};
+            for (int i = 0; i < 4; ++i) q.sh[i] = f.sh[i];
+            q.L = L;
+            q.kab = (int)kab;
+            q.kcd = (int)kcd;
+            q.pq_begin = pq_total;
+            q.seed_off = (long long)seed_zc;
+            sq.push_back(q);
+            seed_zc += need_s;
+            tab_zc += need_t;
+            pq_total += kab * kcd;
+            maxL = std::max(maxL, L);
+            ++end;
+        }
+        HIP_TRY(ctx->seeds.ensure(seed_zc * sizeof(zc)), "hipMalloc(seeds)");
+        HIP_TRY(ctx->etab.ensure(std::max<size_t>(tab_zc, 1) * sizeof(zc)), "hipMalloc(pair tables)");
+        HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
+        HIP_TRY(hipMemcpyAsync(ctx->squartets.p, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+        // group by class
+        std::map<gh::FastLaunch, std::vector<size_t>> groups;
+        for (size_t i = begin; i < end; ++i) groups[reqs[i].info.launch].push_back(i);
+        std::vector<gh::FastPairQuartet> pqs;
+        std::vector<gh::FastItem> items;
+        struct Launch {
+            gh::FastLaunch fn;
+            size_t pq0, npq, it0, nit;
+            long long bra_units, ket_units;
+        };
+        std::vector<Launch> launches;
+        long long toff = 0;
+        for (auto& [fn, idx] : groups) {
+            Launch L{fn, pqs.size(), idx.size(), items.size(), 0, 0, 0};
+            long long ub = 0, uk = 0;
+            for (size_t i : idx) {
+                const FastReq& f = reqs[i];
+                const DevShell &A = b->shells[f.sh[0]], &B = b->shells[f.sh[1]], &C = b->shells[f.sh[2]],
+                               &D = b->shells[f.sh[3]];
+                const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
+                gh::FastPairQuartet pq{};
+                for (int k = 0; k < 4; ++k) pq.sh[k] = f.sh[k];
+                pq.unit_begin[0] = ub;
+                pq.unit_begin[1] = uk;
+                pq.table_off[0] = toff;
+                toff += kab * f.info.bra_ne;
+                pq.table_off[1] = toff;
+                toff += kcd * f.info.ket_ne;
+                ub += kab;
+                uk += kcd;
+                pqs.push_back(pq);
+                const DevShell* S[4] = {&A, &B, &C, &D};
+                int lo[4], hi[4];
+                for (int k = 0; k < 4; ++k) {
+                    const int orig = f.perm[k];
+                    lo[k] = f.r->xfix[orig] < 0 ? 0 : f.r->xfix[orig];
+                    hi[k] = f.r->xfix[orig] < 0 ? S[k]->ncontr : f.r->xfix[orig] + 1;
+                }
+                for (int xa = lo[0]; xa < hi[0]; ++xa)
+                    for (int xb = lo[1]; xb < hi[1]; ++xb)
+                        for (int xc = lo[2]; xc < hi[2]; ++xc)
+                            for (int xd = lo[3]; xd < hi[3]; ++xd) {
+                                gh::FastItem w{};
+                                for (int k = 0; k < 4; ++k) {
+                                    w.sh[k] = f.sh[k];
+                                    w.stride[k] = f.r->stride[f.perm[k]];
+                                }
+                                w.x[0] = xa;
+                                w.x[1] = xb;
+                                w.x[2] = xc;
+                                w.x[3] = xd;
+                                w.conj = f.conj;
+                                w.seed_off = sq[i - begin].seed_off;
+                                w.ebra_off = pq.table_off[0];
+                                w.eket_off = pq.table_off[1];
+                                w.out_base = f.r->out_base;
+                                items.push_back(w);
+                            }
+            }
+            L.nit = items.size() - L.it0;
+            L.bra_units = ub;
+            L.ket_units = uk;
+            launches.push_back(L);
+        }
+        HIP_TRY(ctx->fpq.ensure(pqs.size() * sizeof(gh::FastPairQuartet)), "hipMalloc");
+        HIP_TRY(hipMemcpyAsync(ctx->fpq.p, pqs.data(), pqs.size() * sizeof(gh::FastPairQuartet), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+        HIP_TRY(ctx->fitems.ensure(items.size() * sizeof(gh::FastItem)), "hipMalloc");
+        HIP_TRY(hipMemcpyAsync(ctx->fitems.p, items.data(), items.size() * sizeof(gh::FastItem), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+        HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
+        HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>(), (int)sq.size(), pq_total,
+                                b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
+                                ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), s, b->d_radial.as<double>()),
+                "seed kernel launch");
+        HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
+        for (const Launch& L : launches) {
+            HIP_TRY(L.fn(ctx->fpq.as<gh::FastPairQuartet>() + L.pq0, (int)L.npq, L.bra_units, L.ket_units,
+                         ctx->fitems.as<gh::FastItem>() + L.it0, (int)L.nit, b->d_shells.as<DevShell>(),
+                         b->d_exps.as<double>(), b->d_coefs.as<double>(), ctx->kappa.as<double>(), ctx->seeds.as<zc>(),
+                         ctx->etab.as<zc>(), d_out, s),
+                    "batched transform launch");
+            ctx->stats.launches += 3;
+        }
+        HIP_TRY(hipEventRecord(ctx->ev[2], s), "hipEventRecord");
+        HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
+        ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
+        ctx->stats.transform_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+        ctx->stats.launches += 1;
+        ctx->stats.primitive_quartets += (uint64_t)pq_total;
+        DevError de{};
+        HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (de.code != 0) return boys_error(ctx, de);
+        begin = end;
+    }
+    return GIAOHIP_OK;
+}
+
+// Runs a list of requests; outputs go to d_out (device). Full-block requests
+// of classes with a specialized kernel take the batched path (unless the
+// plain variant is selected); the rest the plain per-class kernel.
+int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
+                 zc* d_out) {
+    if (reqs.empty()) return GIAOHIP_OK;
+    std::vector<Request> plain;
+    std::vector<FastReq> fast;
+    for (const Request& r : reqs) {
+        FastReq f{};
+        const bool full = r.cd1 < 0;
+        if (ctx->eri_variant == GIAOHIP_ERI_BATCHED && full && orient(b, r, f)) fast.push_back(f);
+        else plain.push_back(r);
+    }
+    int st = run_plain(ctx, b, field, plain, d_out);
+    if (st || fast.empty()) return st;
+    // kappa and the error record were prepared by run_plain only if it ran
+    const std::vector<double> kap = wave_vectors(b, field);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
+    HIP_TRY(hipMemcpyAsync(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+    HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
+    HIP_TRY(hipMemsetAsync(ctx->errrec.p, 0, sizeof(DevError), s), "hipMemsetAsync");
+    return run_fast(ctx, b, fast, d_out);
 }
 
 void reset_stats(giaohip_ctx* ctx) { ctx->stats = giaohip_stats{}; }
@@ -516,6 +725,16 @@ size_t giaohip_last_error(const giaohip_ctx* ctx, char* buffer, size_t size) {
 
 void* giaohip_stream(giaohip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
+    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (option == GIAOHIP_OPTION_ERI_VARIANT && (value == GIAOHIP_ERI_PLAIN || value == GIAOHIP_ERI_BATCHED)) {
+        ctx->eri_variant = value;
+        return GIAOHIP_OK;
+    }
+    return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");
+}
+
 int giaohip_last_stats(const giaohip_ctx* ctx, giaohip_stats* out) {
     if (!ctx || !out) return GIAOHIP_INVALID_ARGUMENT;
     *out = ctx->stats;
@@ -587,6 +806,8 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
                 return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell exponents must be finite and positive");
             }
             b->exps.push_back(e);
+            // radial part of primitive_normalization (types.cpp:207-219)
+            b->radial.push_back(std::exp(0.75 * std::log(2.0 * e / kPi) + 0.5 * (double)d.l * std::log(4.0 * e)));
         }
         for (int k = 0; k < d.nprim * d.ncontr; ++k) b->coefs.push_back(coefs[co + k]);
         const auto comps = cartesian_components(d.l);
@@ -614,6 +835,7 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
     if (e == hipSuccess) e = up(b->d_norms, b->norms.data(), b->norms.size() * sizeof(double));
     if (e == hipSuccess) e = up(b->d_aooff, b->ao_off.data(), b->ao_off.size() * sizeof(long long));
     if (e == hipSuccess) e = up(b->d_owner, b->owner.data(), b->owner.size() * sizeof(int));
+    if (e == hipSuccess) e = up(b->d_radial, b->radial.data(), b->radial.size() * sizeof(double));
     if (e != hipSuccess) {
         delete b;
         return hip_fail(ctx, e, "basis upload");
@@ -629,7 +851,8 @@ int giaohip_basis_destroy(giaohip_ctx* ctx, giaohip_basis* b) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
     }
-    for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner}) d->release();
+    for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner, &b->d_radial})
+        d->release();
     delete b;
     return GIAOHIP_OK;
 }
@@ -699,6 +922,7 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
     ctx->stats.requested_quartets = n;
+    const auto host_t0 = std::chrono::steady_clock::now();
     for (size_t i = 0; i < 4 * n; ++i)
         if (quartets[i] >= (uint32_t)b->ns) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "ERI shell-quartet index is out of range");
     for (int k = 0; k < 6; ++k)
@@ -793,6 +1017,8 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     (void)hipEventRecord(t1, s);
     (void)hipEventSynchronize(t1);
     ctx->stats.total_ms = elapsed(t0, t1);
+    ctx->stats.host_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host_t0).count();
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return st;

@@ -30,6 +30,7 @@
 //                     (eri.cpp:777-805) and the Cholesky diagonal extraction
 //                     (cholesky.cpp:305-312).
 #include "gh_boys.h"
+#include "gh_hermite.h"
 #include "gh_internal.h"
 
 namespace gh {
@@ -41,8 +42,6 @@ __constant__ double c_gl32w[32];
 // Cartesian components of l = 0..kMaxShellL, x-major (types.cpp:221-237),
 // packed x | y << 8 | z << 16, shell l starting at nherm(l - 1).
 __constant__ unsigned c_cart[nherm(kMaxShellL)];
-// Binomial coefficients C(t, k), t, k <= 2 * kMaxShellL.
-__constant__ double c_binom[2 * kMaxShellL + 1][2 * kMaxShellL + 1];
 
 __device__ inline unsigned cart_of(int l, int c) { return c_cart[l * (l + 1) * (l + 2) / 6 + c]; }
 
@@ -92,7 +91,7 @@ __global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict
                                                    long long total, const DevShell* __restrict__ shells,
                                                    const double* __restrict__ exps,
                                                    const double* __restrict__ kappa, zc* __restrict__ seeds,
-                                                   DevError* err) {
+                                                   DevError* err, const double* __restrict__ radial) {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= total) return;
     int lo = 0, hi = nq - 1;
@@ -144,7 +143,10 @@ __global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict
         }
         common = zexp(zc{-rho * rd2 - tw2 / (4.0 * (p + q)), -twc});
     }
-    const double pref = 2.0 * pow(kPi, 2.5) / (p * q * sqrt(p + q)) * P1.K * P2.K;
+    double pref = 2.0 * pow(kPi, 2.5) / (p * q * sqrt(p + q)) * P1.K * P2.K;
+    if (radial)
+        pref *= radial[A.prim_off + pa] * radial[B.prim_off + pb] * radial[C.prim_off + pc] *
+                radial[D.prim_off + pd];
     out[0] = X[0];
     out[1] = X[1];
     out[2] = X[2];
@@ -183,53 +185,6 @@ __global__ __launch_bounds__(64) void boys_probe_kernel(const zc* __restrict__ z
 constexpr int kNT = 256;
 constexpr int kMaxW = 8;  // W entries per thread
 constexpr int kMaxO = 8;  // output entries per thread
-constexpr int kMaxE = 2 * kMaxShellL + 1;
-
-// Complex 1-D Hermite table Et^{ij}[k] of one primitive pair on one axis:
-// E^{ij}_t from the field-free recurrence (hermite.cpp:8-40: raise the bra
-// power i times with P-A, then the ket power j times with P-B), then folded
-// with the pair's London wave number q: Et[k] = sum_t E_t C(t,k) (-iq)^(t-k).
-__device__ inline void build_phase_hermite(int i, int j, double PA, double PB, double p, double q, zc* dst) {
-    double cur[kMaxE + 1];
-#pragma unroll
-    for (int t = 0; t <= kMaxE; ++t) cur[t] = 0.0;
-    cur[0] = 1.0;
-    const double i2p = 0.5 / p;
-    int active = 0;
-    for (int step = 0; step < i + j; ++step) {
-        const double disp = step < i ? PA : PB;
-        double nxt[kMaxE + 1];
-#pragma unroll
-        for (int t = 0; t <= kMaxE; ++t) {
-            const double lower = t == 0 ? 0.0 : cur[t - 1];
-            const double same = t <= active ? cur[t] : 0.0;
-            const double upper = (t < active && t < kMaxE) ? cur[t < kMaxE ? t + 1 : kMaxE] : 0.0;
-            nxt[t] = t <= active + 1 ? i2p * lower + disp * same + (double)(t + 1) * upper : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t <= kMaxE; ++t) cur[t] = nxt[t];
-        ++active;
-    }
-    // powers of (-i q): (-i)^m q^m
-    const int top = i + j;
-    double qpow[kMaxE];
-    qpow[0] = 1.0;
-#pragma unroll
-    for (int m = 1; m < kMaxE; ++m) qpow[m] = qpow[m - 1] * q;
-#pragma unroll
-    for (int k = 0; k < kMaxE; ++k) {
-        double re = 0.0, im = 0.0;
-#pragma unroll
-        for (int m = 0; k + m < kMaxE; ++m) {
-            const double v = (k + m <= top) ? cur[k + m] * c_binom[k + m][k] * qpow[m] : 0.0;
-            if ((m & 3) == 0) re += v;
-            else if ((m & 3) == 1) im -= v;
-            else if ((m & 3) == 2) re -= v;
-            else im += v;
-        }
-        if (k <= top) dst[k] = zc{re, im};
-    }
-}
 
 struct LdsLayout {
     int offL, listL, offAB, tuvAB;  // int offsets (in ints)
@@ -656,26 +611,22 @@ hipError_t upload_constants() {
         for (int x = l; x >= 0; --x)
             for (int y = l - x; y >= 0; --y) cart[k++] = (unsigned)x | ((unsigned)y << 8) | ((unsigned)(l - x - y) << 16);
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cart), cart, sizeof cart)) != hipSuccess) return e;
-    double binom[2 * kMaxShellL + 1][2 * kMaxShellL + 1] = {};
-    for (int t = 0; t <= 2 * kMaxShellL; ++t) {
-        binom[t][0] = 1.0;
-        for (int kk = 1; kk <= t; ++kk) binom[t][kk] = binom[t - 1][kk - 1] + (kk <= t - 1 ? binom[t - 1][kk] : 0.0);
-    }
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_binom), binom, sizeof binom);
+    return hipSuccess;
 }
 
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
-                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s) {
+                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s,
+                       const double* radial) {
     if (total == 0) return hipSuccess;
     const dim3 block(128);
     const dim3 grid((unsigned)((total + 127) / 128));
     switch (nmax_bucket) {
-        case 4: seed_kernel<4><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
-        case 8: seed_kernel<8><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
-        case 12: seed_kernel<12><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
-        case 16: seed_kernel<16><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
-        case 24: seed_kernel<24><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
-        default: seed_kernel<32><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err); break;
+        case 4: seed_kernel<4><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
+        case 8: seed_kernel<8><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
+        case 12: seed_kernel<12><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
+        case 16: seed_kernel<16><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
+        case 24: seed_kernel<24><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
+        default: seed_kernel<32><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
     }
     return hipGetLastError();
 }

@@ -41,13 +41,41 @@ hipError_t launch_chol_update(const zc* column, const zc* const* chunks, int row
 hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
                               bool partial_first, hipStream_t s);
 
+// Class-specialized batched transform (gh_eri_fast.hip).
+struct FastPairQuartet {  // one oriented shell quartet: pair-table bookkeeping
+    int sh[4];
+    long long unit_begin[2];  // first flat bra / ket primitive-pair unit
+    long long table_off[2];   // zc offsets of its bra / ket product tables
+};
+struct FastItem {  // one oriented quartet x contraction rows
+    int sh[4];
+    int x[4];
+    int conj, pad;
+    long long seed_off;
+    long long ebra_off, eket_off;
+    long long out_base;
+    long long stride[4];
+};
+using FastLaunch = hipError_t (*)(const FastPairQuartet* pq, int nq, long long bra_units, long long ket_units,
+                                  const FastItem* items, int nitems, const DevShell* shells, const double* exps,
+                                  const double* coefs, const double* kappa, const zc* seeds, zc* etab, zc* out,
+                                  hipStream_t s);
+struct FastClassInfo {
+    bool available;
+    int bra_ne, ket_ne;  // products per bra / ket primitive pair
+    int lds_bytes;
+    FastLaunch launch;
+};
+FastClassInfo fast_class_info(int la, int lb, int lc, int ld);
+
 hipError_t upload_constants();
 int transform_lds_bytes(const ClassArgs& ca);
 int transform_max_w();
 int transform_max_o();
 
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
-                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s);
+                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s,
+                       const double* radial = nullptr);
 hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nitems, const DevShell* shells,
                             const double* exps, const double* coefs, const double* norms, const double* kappa,
                             const zc* seeds, zc* out, hipStream_t s);

@@ -47,10 +47,12 @@ enum {
     GIAOHIP_RUNTIME_ERROR = 8  /* RuntimeError raised by the reference's driver */
 };
 
-/* ERI kernel variants selectable per call (giaohip_set_option). */
+/* ERI kernel variants (giaohip_set_option with GIAOHIP_OPTION_ERI_VARIANT). */
 enum {
-    GIAOHIP_ERI_PLAIN = 0 /* one launch per angular-momentum class, FP64 */
+    GIAOHIP_ERI_PLAIN = 0,  /* generic FP64 transform kernel, one launch per class */
+    GIAOHIP_ERI_BATCHED = 1 /* class-specialized FP64 kernels where available (default) */
 };
+enum { GIAOHIP_OPTION_ERI_VARIANT = 0 };
 
 typedef struct giaohip_ctx giaohip_ctx;
 typedef struct giaohip_basis giaohip_basis;
@@ -68,6 +70,7 @@ typedef struct {
     double transform_ms; /* R recurrence + E.R.E contraction kernel(s) */
     double total_ms;     /* everything on the stream, including copies */
     uint64_t launches;
+    double host_ms;      /* wall time of the whole call, validation included */
 } giaohip_stats;
 
 /* --- context ---------------------------------------------------------- */
@@ -78,6 +81,8 @@ size_t giaohip_last_error(const giaohip_ctx* ctx, char* buffer, size_t size);
 /* Stream (hipStream_t) the context launches on, for callers that time it. */
 void* giaohip_stream(giaohip_ctx* ctx);
 int giaohip_last_stats(const giaohip_ctx* ctx, giaohip_stats* out);
+/* Context options; see GIAOHIP_OPTION_*. No counterpart in the reference. */
+int giaohip_set_option(giaohip_ctx* ctx, int option, int value);
 /* Device memory owned by the context, for resident outputs (bench, chaining). */
 int giaohip_device_alloc(giaohip_ctx* ctx, size_t bytes, void** out);
 int giaohip_device_free(giaohip_ctx* ctx, void* ptr);

@@ -61,16 +61,25 @@ def test_primitive_eri_frozen_vectors(gpu):
             assert abs(v - complex(*expected)) <= 5e-14 + 5e-13 * abs(complex(*expected))
 
 
+@pytest.mark.parametrize("variant", ["batched", "plain"])
 @pytest.mark.parametrize("field", [FIELD, ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))], ids=["finite", "zero"])
-def test_blocks_all_classes_to_g(gpu, port, field):
+def test_blocks_all_classes_to_g(gpu, port, field, variant):
+    """Assorted classes; the batched variant reaches its specialized kernels
+    directly ((dd|dd), (pp|pp), (ss|ss)), through pair exchange ((ff|dd) ->
+    (dd|ff)) and through conjugate reversal ((sp|sp) -> (ps|ps)*)."""
     rng = np.random.default_rng(11)
-    ls = [0, 1, 2, 3, 4, 0, 1, 2, 3, 4]
+    ls = [0, 1, 2, 3, 4, 0, 1, 2, 3, 4, 2, 2]
     b = make_basis(random_shells(rng, ls, nprim=(1, 3), ncontr=(1, 3)), port)
     db = device_basis(gpu, b)
     qs = [(0, 0, 0, 0), (1, 0, 2, 3), (2, 2, 2, 2), (3, 2, 1, 0), (4, 3, 2, 1), (3, 3, 2, 2), (4, 4, 0, 0),
-          (0, 4, 1, 9), (5, 6, 7, 8), (9, 8, 7, 6), (2, 7, 2, 7), (1, 1, 1, 1), (3, 3, 3, 3)]
+          (0, 4, 1, 9), (5, 6, 7, 8), (9, 8, 7, 6), (2, 7, 2, 7), (1, 1, 1, 1), (3, 3, 3, 3), (0, 1, 5, 6),
+          (1, 0, 6, 5), (3, 8, 10, 11), (8, 3, 2, 7), (2, 11, 7, 10), (10, 11, 3, 8)]
     f = _native.field6(*field)
-    got = db.eri_batch(qs, f)
+    gpu.set_variant(variant)
+    try:
+        got = db.eri_batch(qs, f)
+    finally:
+        gpu.set_variant("batched")
     ref, _ = port.eri_batch(b, qs, f, threads=8)
     assert np.max(np.abs(got - ref)) <= ATOL
 
