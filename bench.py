@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
+    ap.add_argument("--chunk-mib", type=int, default=0, help="batched-path chunk size (default: library's)")
     return ap.parse_args()
 
 
@@ -98,12 +99,15 @@ def main():
     coefs = _native.normalize_shells(sh.L, sh.nprim, sh.ncontr, sh.exps, sh.coefs)
     log(f"workload generated in {time.perf_counter() - t0:.1f} s; field B={batch.B}")
     ctx = _native.Context(0)
+    if args.chunk_mib:
+        ctx.set_chunk_mib(args.chunk_mib)
     dbasis = _native.DeviceBasis(ctx, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
     n = len(batch.quartets)
     total = int(dbasis.quartet_sizes(batch.quartets).sum())
     out = ctx.alloc(total * 16)
     field = _native.field6(batch.B, batch.origin)
     flops = workloads.batch_flops(sh, batch.quartets)
+    executed = workloads.batch_executed_flops(sh, batch.quartets)
 
     def step():
         dbasis.eri_batch(batch.quartets, field, device_out=out)
@@ -175,7 +179,12 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "fast_transform_kernel<2,2,2,2> + pair_kernel (FP64 R recurrence + E.R.E)",
+            "kernel": "E.R.E stage: pair_kernel + r0_kernel + fast_transform_kernel<2,2,2,2>",
+            "flop_convention": "12 per R0 entry + 8 per complex MAC of the dense Hermite contraction (per quartet "
+                               f"{flops / n:.4g})",
+            "executed_tflops": executed / (transform_ms * 1e-3) / 1e12,
+            "executed_note": "FLOPs the kernels execute: structural zeros of the Hermite products skipped "
+                             f"(per quartet {executed / n:.4g})",
         },
         "checksum": float(np.abs(probe).sum()),
     }

@@ -190,6 +190,10 @@ class Context:
         value = {"plain": 0, "batched": 1}[variant]
         self.check(self.lib.giaohip_set_option(self.handle, 0, value))
 
+    def set_chunk_mib(self, mib: int):
+        """Work memory of one batched-path chunk in MiB (16..65536)."""
+        self.check(self.lib.giaohip_set_option(self.handle, 1, int(mib)))
+
     def stats(self) -> dict:
         s = Stats()
         self.check(self.lib.giaohip_last_stats(self.handle, C.byref(s)))

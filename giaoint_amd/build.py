@@ -11,22 +11,33 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libgiaohip.so"
+OBJDIR = PACKAGE / "_build"
 ARCH = os.environ.get("GIAOHIP_ARCH", "gfx950")
 
-SOURCES = ["gh_api.hip", "gh_eri.hip", "gh_eri_fast.hip", "gh_cholesky.hip"]
+SOURCES = [
+    "gh_api.hip",
+    "gh_eri.hip",
+    "gh_cholesky.hip",
+    "gh_eri_fast.hip",
+    "gh_fast_dddd.hip",
+    "gh_fast_ddff.hip",
+    "gh_fast_pppp.hip",
+    "gh_fast_psps.hip",
+    "gh_fast_ssss.hip",
+]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++20",
     "-fPIC",
-    "-shared",
     # Packed FP32 instructions stay disabled in every HIP object.
     "-Xclang",
     "-target-feature",
@@ -45,17 +56,40 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the gfx950 library cannot be built")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile the library; always rebuilds. Returns the library path."""
-    sources = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every source to an object (in parallel), then link the library.
+
+    Always rebuilds. Returns the library path.
+    """
+    missing = [s for s in SOURCES if not (CSRC / s).exists()]
+    if missing:
+        raise RuntimeError(f"missing HIP sources: {missing}")
+    OBJDIR.mkdir(exist_ok=True)
+    cc = hipcc()
+    include = [f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+    def compile_one(name: str) -> Path:
+        obj = OBJDIR / (Path(name).stem + ".o")
+        cmd = [cc, *HIPCC_FLAGS, *include, "-c", str(CSRC / name), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError(f"hipcc failed on {name} with status {proc.returncode}")
+        return obj
+
+    workers = jobs or min(len(SOURCES), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        objects = list(pool.map(compile_one, SOURCES))
     tmp = LIBRARY.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}", *sources, "-o", str(tmp)]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objects], "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed with status {proc.returncode}")
+        raise RuntimeError(f"hipcc link failed with status {proc.returncode}")
     os.replace(tmp, LIBRARY)
     return LIBRARY
 

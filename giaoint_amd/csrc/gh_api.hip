@@ -59,6 +59,35 @@ struct DevBuffer {
     }
 };
 
+// Page-locked host staging: copies from it are asynchronous with respect to
+// the host, so the host can plan the next chunk while the device works.
+struct PinnedBuffer {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
 struct Status {
     int code = GIAOHIP_OK;
     std::string msg;
@@ -85,9 +114,12 @@ struct giaohip_ctx {
     std::mutex mu;
     std::string err;
     giaohip_stats stats{};
-    DevBuffer seeds, squartets, items, kappa, errrec, staging, blocks, misc_a, misc_b, misc_c, etab, fpq, fitems;
+    DevBuffer seeds, squartets, items, kappa, errrec, staging, blocks, misc_a, misc_b, misc_c, fpq, fitems;
     int eri_variant = GIAOHIP_ERI_BATCHED;
     hipEvent_t ev[4]{};
+    std::vector<hipEvent_t> chunk_ev;  // per-chunk timing events of the batched path
+    PinnedBuffer h_sq, h_pq, h_items;  // host staging of the batched path's descriptors
+    DevBuffer work;                    // seeds, R0 tables and pair tables of one chunk
     std::set<void*> user_allocs;
     size_t seed_budget_bytes = size_t(4) << 30;
     size_t staging_budget_bytes = size_t(1) << 30;
@@ -395,87 +427,140 @@ bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
     return false;
 }
 
+// The batched path: requests are cut into chunks whose seeds, R0 tables and
+// pair tables fit the context budget. Each chunk is planned into page-locked
+// staging memory, copied asynchronously and launched right away, so the host
+// plans chunk c + 1 while the device runs chunk c; the chunks reuse one work
+// buffer in stream order and the host waits once, at the end.
 int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq>& reqs, zc* d_out) {
     if (reqs.empty()) return GIAOHIP_OK;
     hipStream_t s = ctx->stream;
+    struct Need {
+        size_t seed, r0, tab;
+        long long kab, kcd;
+        int L;
+        size_t items;
+    };
+    // pass 1: sizes
+    std::vector<Need> need(reqs.size());
+    size_t total_items = 0, max_one = 0;
+    for (size_t i = 0; i < reqs.size(); ++i) {
+        const FastReq& f = reqs[i];
+        const DevShell* S[4] = {&b->shells[f.sh[0]], &b->shells[f.sh[1]], &b->shells[f.sh[2]], &b->shells[f.sh[3]]};
+        Need& n = need[i];
+        n.L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
+        n.kab = (long long)S[0]->nprim * S[1]->nprim;
+        n.kcd = (long long)S[2]->nprim * S[3]->nprim;
+        n.seed = (size_t)(n.kab * n.kcd) * (size_t)(n.L + 4);
+        n.r0 = (size_t)(n.kab * n.kcd) * (size_t)f.info.nhl;
+        n.tab = (size_t)(n.kab * f.info.bra_ne + n.kcd * f.info.ket_ne);
+        n.items = 1;
+        for (int k = 0; k < 4; ++k)
+            if (f.r->xfix[f.perm[k]] < 0) n.items *= (size_t)S[k]->ncontr;
+        total_items += n.items;
+        max_one = std::max(max_one, n.seed + n.r0 + n.tab);
+    }
+    const size_t work_zc = std::max(max_one, ctx->seed_budget_bytes / sizeof(zc));
+    HIP_TRY(ctx->work.ensure(work_zc * sizeof(zc)), "hipMalloc(batched work buffer)");
+    HIP_TRY(ctx->squartets.ensure(reqs.size() * sizeof(SeedQuartet)), "hipMalloc");
+    HIP_TRY(ctx->fpq.ensure(reqs.size() * sizeof(gh::FastPairQuartet)), "hipMalloc");
+    HIP_TRY(ctx->fitems.ensure(total_items * sizeof(gh::FastItem)), "hipMalloc");
+    HIP_TRY(ctx->h_sq.ensure(reqs.size() * sizeof(SeedQuartet)), "hipHostMalloc");
+    HIP_TRY(ctx->h_pq.ensure(reqs.size() * sizeof(gh::FastPairQuartet)), "hipHostMalloc");
+    HIP_TRY(ctx->h_items.ensure(total_items * sizeof(gh::FastItem)), "hipHostMalloc");
+    SeedQuartet* hsq = ctx->h_sq.as<SeedQuartet>();
+    gh::FastPairQuartet* hpq = ctx->h_pq.as<gh::FastPairQuartet>();
+    gh::FastItem* hit = ctx->h_items.as<gh::FastItem>();
+    zc* work = ctx->work.as<zc>();
+
+    struct Group {
+        gh::FastLaunch fn;
+        int waves;
+        std::vector<size_t> idx;
+    };
+    std::vector<Group> groups;
+    std::vector<hipEvent_t> used;
+    size_t nsq = 0, npq = 0, nit = 0, nchunks = 0;
     size_t begin = 0;
     while (begin < reqs.size()) {
-        size_t end = begin;
-        size_t seed_zc = 0, tab_zc = 0;
+        // pass 2: one chunk within the budget
+        size_t end = begin, seed_zc = 0, r0_zc = 0, tab_zc = 0;
         long long pq_total = 0;
         int maxL = 0;
-        std::vector<SeedQuartet> sq;
+        const size_t sq0 = nsq;
         while (end < reqs.size()) {
-            const FastReq& f = reqs[end];
-            const DevShell &A = b->shells[f.sh[0]], &B = b->shells[f.sh[1]], &C = b->shells[f.sh[2]],
-                           &D = b->shells[f.sh[3]];
-            const int L = A.l + B.l + C.l + D.l;
-            const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
-            const size_t need_s = (size_t)(kab * kcd) * (size_t)(L + 4);
-            const size_t need_t = (size_t)(kab * f.info.bra_ne + kcd * f.info.ket_ne);
-            if (end > begin && (seed_zc + tab_zc + need_s + need_t) * sizeof(zc) > ctx->seed_budget_bytes) break;
-            SeedQuartet q{};
-            for (int i = 0; i < 4; ++i) q.sh[i] = f.sh[i];
-            q.L = L;
-            q.kab = (int)kab;
-            q.kcd = (int)kcd;
+            const Need& n = need[end];
+            if (end > begin && seed_zc + r0_zc + tab_zc + n.seed + n.r0 + n.tab > work_zc) break;
+            SeedQuartet& q = hsq[nsq++];
+            q = SeedQuartet{};
+            for (int k = 0; k < 4; ++k) q.sh[k] = reqs[end].sh[k];
+            q.L = n.L;
+            q.kab = (int)n.kab;
+            q.kcd = (int)n.kcd;
             q.pq_begin = pq_total;
             q.seed_off = (long long)seed_zc;
-            sq.push_back(q);
-            seed_zc += need_s;
-            tab_zc += need_t;
-            pq_total += kab * kcd;
-            maxL = std::max(maxL, L);
+            seed_zc += n.seed;
+            r0_zc += n.r0;
+            tab_zc += n.tab;
+            pq_total += n.kab * n.kcd;
+            maxL = std::max(maxL, n.L);
             ++end;
         }
-        HIP_TRY(ctx->seeds.ensure(seed_zc * sizeof(zc)), "hipMalloc(seeds)");
-        HIP_TRY(ctx->etab.ensure(std::max<size_t>(tab_zc, 1) * sizeof(zc)), "hipMalloc(pair tables)");
-        HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
-        HIP_TRY(hipMemcpyAsync(ctx->squartets.p, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
-                "hipMemcpyAsync");
-        // group by class
-        std::map<gh::FastLaunch, std::vector<size_t>> groups;
-        for (size_t i = begin; i < end; ++i) groups[reqs[i].info.launch].push_back(i);
-        std::vector<gh::FastPairQuartet> pqs;
-        std::vector<gh::FastItem> items;
+        zc* d_seeds = work;
+        zc* d_r0 = work + seed_zc;
+        zc* d_tab = d_r0 + r0_zc;
+        // group by class kernel and transform workgroup size
+        for (Group& g : groups) g.idx.clear();
+        for (size_t i = begin; i < end; ++i) {
+            const FastReq& f = reqs[i];
+            const int waves = gh::fast_waves(f.info.nhab, (int)need[i].kab);
+            Group* g = nullptr;
+            for (Group& x : groups)
+                if (x.fn == f.info.launch && x.waves == waves) g = &x;
+            if (!g) {
+                groups.push_back(Group{f.info.launch, waves, {}});
+                g = &groups.back();
+            }
+            g->idx.push_back(i);
+        }
         struct Launch {
             gh::FastLaunch fn;
+            int waves;
             size_t pq0, npq, it0, nit;
-            long long bra_units, ket_units;
         };
         std::vector<Launch> launches;
-        long long toff = 0;
-        for (auto& [fn, idx] : groups) {
-            Launch L{fn, pqs.size(), idx.size(), items.size(), 0, 0, 0};
-            long long ub = 0, uk = 0;
-            for (size_t i : idx) {
+        const size_t pq_chunk0 = npq, it_chunk0 = nit;
+        long long toff = 0, roff = 0;
+        for (const Group& g : groups) {
+            if (g.idx.empty()) continue;
+            Launch L{g.fn, g.waves, npq, g.idx.size(), nit, 0};
+            for (size_t i : g.idx) {
                 const FastReq& f = reqs[i];
-                const DevShell &A = b->shells[f.sh[0]], &B = b->shells[f.sh[1]], &C = b->shells[f.sh[2]],
-                               &D = b->shells[f.sh[3]];
-                const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
-                gh::FastPairQuartet pq{};
+                const Need& n = need[i];
+                gh::FastPairQuartet& pq = hpq[npq++];
+                pq = gh::FastPairQuartet{};
                 for (int k = 0; k < 4; ++k) pq.sh[k] = f.sh[k];
-                pq.unit_begin[0] = ub;
-                pq.unit_begin[1] = uk;
                 pq.table_off[0] = toff;
-                toff += kab * f.info.bra_ne;
+                toff += n.kab * f.info.bra_ne;
                 pq.table_off[1] = toff;
-                toff += kcd * f.info.ket_ne;
-                ub += kab;
-                uk += kcd;
-                pqs.push_back(pq);
-                const DevShell* S[4] = {&A, &B, &C, &D};
+                toff += n.kcd * f.info.ket_ne;
+                pq.seed_off = hsq[sq0 + (i - begin)].seed_off;
+                pq.r0_off = roff;
+                pq.kab = (int)n.kab;
+                pq.kcd = (int)n.kcd;
+                roff += n.kab * n.kcd * f.info.nhl;
                 int lo[4], hi[4];
                 for (int k = 0; k < 4; ++k) {
                     const int orig = f.perm[k];
                     lo[k] = f.r->xfix[orig] < 0 ? 0 : f.r->xfix[orig];
-                    hi[k] = f.r->xfix[orig] < 0 ? S[k]->ncontr : f.r->xfix[orig] + 1;
+                    hi[k] = f.r->xfix[orig] < 0 ? b->shells[f.sh[k]].ncontr : f.r->xfix[orig] + 1;
                 }
                 for (int xa = lo[0]; xa < hi[0]; ++xa)
                     for (int xb = lo[1]; xb < hi[1]; ++xb)
                         for (int xc = lo[2]; xc < hi[2]; ++xc)
                             for (int xd = lo[3]; xd < hi[3]; ++xd) {
-                                gh::FastItem w{};
+                                gh::FastItem& w = hit[nit++];
+                                w = gh::FastItem{};
                                 for (int k = 0; k < 4; ++k) {
                                     w.sh[k] = f.sh[k];
                                     w.stride[k] = f.r->stride[f.perm[k]];
@@ -485,49 +570,71 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
                                 w.x[2] = xc;
                                 w.x[3] = xd;
                                 w.conj = f.conj;
-                                w.seed_off = sq[i - begin].seed_off;
+                                w.r0_off = pq.r0_off;
                                 w.ebra_off = pq.table_off[0];
                                 w.eket_off = pq.table_off[1];
                                 w.out_base = f.r->out_base;
-                                items.push_back(w);
                             }
             }
-            L.nit = items.size() - L.it0;
-            L.bra_units = ub;
-            L.ket_units = uk;
+            L.nit = nit - L.it0;
             launches.push_back(L);
         }
-        HIP_TRY(ctx->fpq.ensure(pqs.size() * sizeof(gh::FastPairQuartet)), "hipMalloc");
-        HIP_TRY(hipMemcpyAsync(ctx->fpq.p, pqs.data(), pqs.size() * sizeof(gh::FastPairQuartet), hipMemcpyHostToDevice, s),
+        // copies of this chunk's descriptors, then its kernels
+        HIP_TRY(hipMemcpyAsync(ctx->squartets.as<SeedQuartet>() + sq0, hsq + sq0, (nsq - sq0) * sizeof(SeedQuartet),
+                               hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
-        HIP_TRY(ctx->fitems.ensure(items.size() * sizeof(gh::FastItem)), "hipMalloc");
-        HIP_TRY(hipMemcpyAsync(ctx->fitems.p, items.data(), items.size() * sizeof(gh::FastItem), hipMemcpyHostToDevice, s),
+        HIP_TRY(hipMemcpyAsync(ctx->fpq.as<gh::FastPairQuartet>() + pq_chunk0, hpq + pq_chunk0,
+                               (npq - pq_chunk0) * sizeof(gh::FastPairQuartet), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
-        HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
-        HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>(), (int)sq.size(), pq_total,
-                                b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
-                                ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), s, b->d_radial.as<double>()),
-                "seed kernel launch");
-        HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
-        for (const Launch& L : launches) {
-            HIP_TRY(L.fn(ctx->fpq.as<gh::FastPairQuartet>() + L.pq0, (int)L.npq, L.bra_units, L.ket_units,
-                         ctx->fitems.as<gh::FastItem>() + L.it0, (int)L.nit, b->d_shells.as<DevShell>(),
-                         b->d_exps.as<double>(), b->d_coefs.as<double>(), ctx->kappa.as<double>(), ctx->seeds.as<zc>(),
-                         ctx->etab.as<zc>(), d_out, s),
-                    "batched transform launch");
-            ctx->stats.launches += 3;
+        HIP_TRY(hipMemcpyAsync(ctx->fitems.as<gh::FastItem>() + it_chunk0, hit + it_chunk0,
+                               (nit - it_chunk0) * sizeof(gh::FastItem), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+        while (ctx->chunk_ev.size() < 3 * (nchunks + 1)) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+            ctx->chunk_ev.push_back(e);
         }
-        HIP_TRY(hipEventRecord(ctx->ev[2], s), "hipEventRecord");
-        HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
-        ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
-        ctx->stats.transform_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+        hipEvent_t* ev = ctx->chunk_ev.data() + 3 * nchunks;
+        HIP_TRY(hipEventRecord(ev[0], s), "hipEventRecord");
+        HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>() + sq0, (int)(nsq - sq0),
+                                pq_total, b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
+                                ctx->kappa.as<double>(), d_seeds, ctx->errrec.as<DevError>(), s,
+                                b->d_radial.as<double>()),
+                "seed kernel launch");
+        HIP_TRY(hipEventRecord(ev[1], s), "hipEventRecord");
+        for (const Launch& L : launches) {
+            gh::FastLaunchArgs a{};
+            a.pq = ctx->fpq.as<gh::FastPairQuartet>() + L.pq0;
+            a.npq = (int)L.npq;
+            a.items = ctx->fitems.as<gh::FastItem>() + L.it0;
+            a.nitems = (int)L.nit;
+            a.waves = L.waves;
+            a.shells = b->d_shells.as<DevShell>();
+            a.exps = b->d_exps.as<double>();
+            a.coefs = b->d_coefs.as<double>();
+            a.kappa = ctx->kappa.as<double>();
+            a.seeds = d_seeds;
+            a.r0 = d_r0;
+            a.etab = d_tab;
+            a.out = d_out;
+            HIP_TRY(L.fn(a, s), "batched transform launch");
+            ctx->stats.launches += 4;
+        }
+        HIP_TRY(hipEventRecord(ev[2], s), "hipEventRecord");
         ctx->stats.launches += 1;
         ctx->stats.primitive_quartets += (uint64_t)pq_total;
-        DevError de{};
-        HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
-        if (de.code != 0) return boys_error(ctx, de);
+        ++nchunks;
         begin = end;
     }
+    HIP_TRY(hipEventSynchronize(ctx->chunk_ev[3 * nchunks - 1]), "kernel execution");
+    for (size_t c = 0; c < nchunks; ++c) {
+        const hipEvent_t* ev = ctx->chunk_ev.data() + 3 * c;
+        ctx->stats.seed_ms += elapsed(ev[0], ev[1]);
+        ctx->stats.transform_ms += elapsed(ev[1], ev[2]);
+    }
+    DevError de{};
+    HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (de.code != 0) return boys_error(ctx, de);
     return GIAOHIP_OK;
 }
 
@@ -703,8 +810,12 @@ int giaohip_destroy(giaohip_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuffer* b : {&ctx->seeds, &ctx->squartets, &ctx->items, &ctx->kappa, &ctx->errrec, &ctx->staging,
-                         &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c})
+                         &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c, &ctx->fpq, &ctx->fitems})
         b->release();
+    for (auto& ev : ctx->chunk_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    ctx->work.release();
+    for (PinnedBuffer* b : {&ctx->h_sq, &ctx->h_pq, &ctx->h_items}) b->release();
     for (void* p : ctx->user_allocs) (void)hipFree(p);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -730,6 +841,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (option == GIAOHIP_OPTION_ERI_VARIANT && (value == GIAOHIP_ERI_PLAIN || value == GIAOHIP_ERI_BATCHED)) {
         ctx->eri_variant = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_CHUNK_MIB && value >= 16 && value <= 65536) {
+        ctx->seed_budget_bytes = (size_t)value << 20;
         return GIAOHIP_OK;
     }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");

@@ -41,32 +41,55 @@ hipError_t launch_chol_update(const zc* column, const zc* const* chunks, int row
 hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
                               bool partial_first, hipStream_t s);
 
-// Class-specialized batched transform (gh_eri_fast.hip).
-struct FastPairQuartet {  // one oriented shell quartet: pair-table bookkeeping
+// Class-specialized batched transform (gh_fast.h, gh_fast_*.hip, dispatch in gh_eri_fast.hip).
+constexpr int kFastBraGroups = 4;  // waves running the bra step, each with a quarter of the bra components
+constexpr int kFastMaxWaves = 8;   // largest transform workgroup
+struct FastPairQuartet {  // one oriented shell quartet: pair-table and R0 bookkeeping
     int sh[4];
     long long unit_begin[2];  // first flat bra / ket primitive-pair unit
     long long table_off[2];   // zc offsets of its bra / ket product tables
+    long long prim_begin;     // first flat primitive-quartet unit of the launch
+    long long seed_off;       // zc offset of its seeds (L + 4 per primitive quartet)
+    long long r0_off;         // zc offset of its R0 tables (nh(L) per primitive quartet)
+    int kab, kcd;             // primitive pairs on the bra / ket side
 };
 struct FastItem {  // one oriented quartet x contraction rows
     int sh[4];
     int x[4];
     int conj, pad;
-    long long seed_off;
+    long long r0_off;
     long long ebra_off, eket_off;
     long long out_base;
     long long stride[4];
 };
-using FastLaunch = hipError_t (*)(const FastPairQuartet* pq, int nq, long long bra_units, long long ket_units,
-                                  const FastItem* items, int nitems, const DevShell* shells, const double* exps,
-                                  const double* coefs, const double* kappa, const zc* seeds, zc* etab, zc* out,
-                                  hipStream_t s);
+struct FastLaunchArgs {
+    const FastPairQuartet* pq;
+    int npq;
+    long long bra_units, ket_units, prim_units;
+    const FastItem* items;
+    int nitems;
+    int waves;  // workgroup size of the transform in wavefronts
+    const DevShell* shells;
+    const double* exps;
+    const double* coefs;
+    const double* kappa;
+    const zc* seeds;
+    zc* r0;
+    zc* etab;
+    zc* out;
+};
+using FastLaunch = hipError_t (*)(const FastLaunchArgs& a, hipStream_t s);
 struct FastClassInfo {
     bool available;
     int bra_ne, ket_ne;  // products per bra / ket primitive pair
-    int lds_bytes;
+    int nhab;            // bra Hermite indices (rows per bra primitive pair)
+    int nhl;             // R0 entries per primitive quartet
     FastLaunch launch;
 };
 FastClassInfo fast_class_info(int la, int lb, int lc, int ld);
+// Transform workgroup size (wavefronts) for a bra side of nhab Hermite rows
+// and kab primitive pairs; the kernel derives the same pass split from it.
+int fast_waves(int nhab, int kab);
 
 hipError_t upload_constants();
 int transform_lds_bytes(const ClassArgs& ca);

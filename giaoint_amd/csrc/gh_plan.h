@@ -112,10 +112,27 @@ struct Side {
         return n;
     }
     static constexpr int NE = count();
+    // Stored tables pad every group to a multiple of kBatch entries, so that a
+    // wavefront loads one batch with one 16-byte load per lane.
+    static constexpr int kBatch = 16;
+    static constexpr int padded_count() {
+        int n = 0;
+        for (int g = 0; g < NG; ++g) {
+            int cnt = 0;
+            walk([&](int gg, int, int c) {
+                if (gg == g && c >= 0) ++cnt;
+            });
+            n += (cnt + kBatch - 1) / kBatch * kBatch;
+        }
+        return n;
+    }
+    static constexpr int NEP = padded_count();
     struct Tables {
         std::array<uint16_t, NE> comp{};
         std::array<uint16_t, NE> hidx{};
         std::array<std::array<uint16_t, NH + 1>, NG> start{};
+        std::array<uint16_t, NG> pbase{};  // padded start of each group
+        std::array<uint16_t, NEP> ent{};   // entry stored in a padded slot, 0xffff for padding
     };
     static constexpr Tables make() {
         Tables t{};
@@ -139,6 +156,14 @@ struct Side {
             }
             acc += cnt;
             t.start[g][NH] = (uint16_t)acc;
+        }
+        for (int p = 0; p < NEP; ++p) t.ent[p] = 0xffff;
+        int pb = 0;
+        for (int g = 0; g < NG; ++g) {
+            t.pbase[g] = (uint16_t)pb;
+            const int b = t.start[g][0], e = t.start[g][NH];
+            for (int i = b; i < e; ++i) t.ent[pb + (i - b)] = (uint16_t)i;
+            pb += (e - b + kBatch - 1) / kBatch * kBatch;
         }
         return t;
     }

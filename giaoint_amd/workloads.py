@@ -285,6 +285,44 @@ def quartet_flops(la, lb, lc, ld, ka, kb, kc, kd) -> float:
     return float(K * 12 * _r0(L) + min(a, b))
 
 
+def _cart(l):
+    return [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+
+
+def plan_entries(l1: int, l2: int) -> int:
+    """Nonzero complex Hermite products of a pair: sum over components of the box size."""
+    n = 0
+    for a in _cart(l1):
+        for b in _cart(l2):
+            n += (a[0] + b[0] + 1) * (a[1] + b[1] + 1) * (a[2] + b[2] + 1)
+    return n
+
+
+def quartet_executed_flops(la, lb, lc, ld, ka, kb, kc, kd) -> float:
+    """FLOPs the batched transform executes (8 per complex MAC, structural zeros skipped).
+
+    Ket step: every (bra pair, bra Hermite index) row meets every nonzero ket
+    product of every ket pair; bra step: every nonzero bra product meets every
+    ket component, once per bra pair. The pair with fewer components is the bra
+    side. The R0 tables are counted as in the convention (12 per entry).
+    """
+    if _nc(la) * _nc(lb) > _nc(lc) * _nc(ld):
+        la, lb, lc, ld, ka, kb, kc, kd = lc, ld, la, lb, kc, kd, ka, kb
+    K = ka * kb * kc * kd
+    ket = K * _nh(la + lb) * plan_entries(lc, ld)
+    bra = ka * kb * plan_entries(la, lb) * _nc(lc) * _nc(ld)
+    return float(K * 12 * _r0(la + lb + lc + ld) + 8 * (ket + bra))
+
+
+def batch_executed_flops(shells: ShellArrays, quartets: np.ndarray) -> float:
+    q = np.asarray(quartets, dtype=np.int64)
+    l = shells.L[q]
+    k = shells.nprim[q]
+    keys, inv = np.unique(np.concatenate([l, k], axis=1), axis=0, return_inverse=True)
+    counts = np.bincount(inv.reshape(-1), minlength=len(keys))
+    return float(sum(c * quartet_executed_flops(*map(int, key)) for key, c in zip(keys, counts)))
+
+
 def batch_flops(shells: ShellArrays, quartets: np.ndarray) -> float:
     q = np.asarray(quartets, dtype=np.int64)
     l = shells.L[q]

@@ -52,7 +52,13 @@ enum {
     GIAOHIP_ERI_PLAIN = 0,  /* generic FP64 transform kernel, one launch per class */
     GIAOHIP_ERI_BATCHED = 1 /* class-specialized FP64 kernels where available (default) */
 };
-enum { GIAOHIP_OPTION_ERI_VARIANT = 0 };
+enum {
+    GIAOHIP_OPTION_ERI_VARIANT = 0,
+    /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables), in
+       MiB, 16..65536; default 4096. Smaller chunks keep the R0 tables in the
+       device's caches, larger ones need fewer launches. */
+    GIAOHIP_OPTION_CHUNK_MIB = 1
+};
 
 typedef struct giaohip_ctx giaohip_ctx;
 typedef struct giaohip_basis giaohip_basis;
