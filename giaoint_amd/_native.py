@@ -286,11 +286,9 @@ class DeviceBasis:
     def eri_batch(self, quartets, field, out=None, device_out: int | None = None):
         """Packed complex128 blocks of the quartets (host array), or into device_out."""
         q = np.ascontiguousarray(np.asarray(quartets, dtype=np.uint32).reshape(-1, 4))
-        sizes = self.quartet_sizes(q)
-        total = int(sizes.sum())
         f = np.ascontiguousarray(field, dtype=np.float64)
         ctx = self.ctx
-        if device_out is not None:
+        if device_out is not None:  # the library validates the indices itself
             ctx.check(
                 ctx.lib.giaohip_eri_batch(
                     ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)), _p(f), None, _vp(device_out), 1
@@ -298,7 +296,7 @@ class DeviceBasis:
             )
             return None
         if out is None:
-            out = np.empty(total, dtype=np.complex128)
+            out = np.empty(int(self.quartet_sizes(q).sum()), dtype=np.complex128)
         ctx.check(
             ctx.lib.giaohip_eri_batch(
                 ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)), _p(f), None,

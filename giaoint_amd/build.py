@@ -56,10 +56,19 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the gfx950 library cannot be built")
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> Path:
+def _stale(obj: Path, src: Path) -> bool:
+    if not obj.exists():
+        return True
+    newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in CSRC.glob("*.h")] +
+                 [h.stat().st_mtime for h in (ROOT / "include").glob("*.h")])
+    return newest >= obj.stat().st_mtime
+
+
+def build(verbose: bool = False, jobs: int | None = None, incremental: bool = False) -> Path:
     """Compile every source to an object (in parallel), then link the library.
 
-    Always rebuilds. Returns the library path.
+    Rebuilds everything unless `incremental` (then only objects older than
+    their source or any header). Returns the library path.
     """
     missing = [s for s in SOURCES if not (CSRC / s).exists()]
     if missing:
@@ -70,6 +79,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
 
     def compile_one(name: str) -> Path:
         obj = OBJDIR / (Path(name).stem + ".o")
+        if incremental and not _stale(obj, CSRC / name):
+            return obj
         cmd = [cc, *HIPCC_FLAGS, *include, "-c", str(CSRC / name), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -95,4 +106,4 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -21,6 +22,7 @@
 #include <set>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -101,6 +103,11 @@ constexpr size_t kMaxAuxEntries = 4000000u;  // eri.cpp:24
 struct giaohip_basis {
     int ns = 0;
     long long nao = 0;
+    // Per shell-class (la, lb, lc, ld) tables, index ((la * lm + lb) * lm + lc) * lm + ld:
+    // admission status (see class_status) and the batched kernel's orientation.
+    int lm = 1;
+    std::vector<signed char> cls_status, cls_orient;
+    std::vector<gh::FastClassInfo> cls_info;
     std::vector<DevShell> shells;
     std::vector<long long> ao_off;
     std::vector<double> exps, coefs, norms, radial;
@@ -111,6 +118,7 @@ struct giaohip_basis {
 struct giaohip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // producer stream of the batched path (default priority, like `stream`)
     std::mutex mu;
     std::string err;
     giaohip_stats stats{};
@@ -119,9 +127,9 @@ struct giaohip_ctx {
     hipEvent_t ev[4]{};
     std::vector<hipEvent_t> chunk_ev;  // per-chunk timing events of the batched path
     PinnedBuffer h_sq, h_pq, h_items;  // host staging of the batched path's descriptors
-    DevBuffer work;                    // seeds, R0 tables and pair tables of one chunk
+    DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
     std::set<void*> user_allocs;
-    size_t seed_budget_bytes = size_t(4) << 30;
+    size_t seed_budget_bytes = size_t(2048) << 20;
     size_t staging_budget_bytes = size_t(1) << 30;
 };
 
@@ -220,31 +228,61 @@ struct Request {
 };
 
 // Per-class admission, memoized (the reference checks every component quartet).
-bool aux_within_limit(int la, int lb, int lc, int ld) {
-    constexpr int M = 33;
-    static std::mutex mu;
-    static std::vector<signed char> memo;  // -1 unknown, 0 over, 1 within
-    if (la >= M || lb >= M || lc >= M || ld >= M) return max_aux_entries(la, lb, lc, ld) <= kMaxAuxEntries;
-    std::lock_guard<std::mutex> lock(mu);
-    if (memo.empty()) memo.assign((size_t)M * M * M * M, -1);
-    signed char& v = memo[(((size_t)la * M + lb) * M + lc) * M + ld];
-    if (v < 0) v = max_aux_entries(la, lb, lc, ld) <= kMaxAuxEntries ? 1 : 0;
-    return v == 1;
+// Admission of a shell class, in the reference's check order.
+enum ClassStatus : signed char { kClsOk = 0, kClsOrder = 1, kClsAux = 2, kClsShellL = 3, kClsTotalL = 4 };
+
+ClassStatus class_admission(int la, int lb, int lc, int ld) {
+    const int L = la + lb + lc + ld;
+    if (L > 32) return kClsOrder;
+    if (max_aux_entries(la, lb, lc, ld) > kMaxAuxEntries) return kClsAux;
+    if (la > gh::kMaxShellL || lb > gh::kMaxShellL || lc > gh::kMaxShellL || ld > gh::kMaxShellL) return kClsShellL;
+    if (L > gh::kMaxTotalL) return kClsTotalL;
+    return kClsOk;
+}
+
+int class_error(giaohip_ctx* ctx, int status) {
+    switch (status) {
+        case kClsOrder:
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "electron repulsion supports combined angular order through 32");
+        case kClsAux: return set_error(ctx, GIAOHIP_LENGTH_ERROR, "ERI auxiliary exceeds the workspace limit");
+        case kClsShellL:
+            return set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support shell angular momentum through 8");
+        case kClsTotalL:
+            return set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support combined angular order through 24");
+        default: return GIAOHIP_OK;
+    }
+}
+
+inline size_t class_index(const giaohip_basis* b, const int sh[4]) {
+    const int lm = b->lm;
+    return (((size_t)b->shells[sh[0]].l * lm + b->shells[sh[1]].l) * lm + b->shells[sh[2]].l) * lm +
+           b->shells[sh[3]].l;
 }
 
 int check_quartet_supported(giaohip_ctx* ctx, const giaohip_basis* b, const int sh[4]) {
-    int l[4];
-    for (int i = 0; i < 4; ++i) l[i] = b->shells[sh[i]].l;
-    const int L = l[0] + l[1] + l[2] + l[3];
-    if (L > 32) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "electron repulsion supports combined angular order through 32");
-    if (!aux_within_limit(l[0], l[1], l[2], l[3]))
-        return set_error(ctx, GIAOHIP_LENGTH_ERROR, "ERI auxiliary exceeds the workspace limit");
-    for (int i = 0; i < 4; ++i)
-        if (l[i] > gh::kMaxShellL)
-            return set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support shell angular momentum through 8");
-    if (L > gh::kMaxTotalL)
-        return set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support combined angular order through 24");
-    return GIAOHIP_OK;
+    return class_error(ctx, b->cls_status[class_index(b, sh)]);
+}
+
+// Runs f(begin, end) over [0, n) split across up to 16 host threads.
+template <class F>
+void parallel_for(size_t n, F&& f) {
+    constexpr size_t kGrain = 8192;
+    size_t T = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    T = std::min(T, (n + kGrain - 1) / kGrain);
+    if (T <= 1) {
+        f((size_t)0, n);
+        return;
+    }
+    const size_t per = (n + T - 1) / T;
+    std::vector<std::thread> threads;
+    threads.reserve(T - 1);
+    for (size_t t = 1; t < T; ++t)
+        threads.emplace_back([&, t] {
+            const size_t lo = t * per, hi = std::min(n, lo + per);
+            if (lo < hi) f(lo, hi);
+        });
+    f((size_t)0, std::min(n, per));
+    for (auto& th : threads) th.join();
 }
 
 int boys_error(giaohip_ctx* ctx, const DevError& e) {
@@ -315,7 +353,9 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             maxL = std::max(maxL, L);
             ++end;
         }
-        HIP_TRY(ctx->seeds.ensure(seed_zc * sizeof(zc)), "hipMalloc(seeds)");
+        const size_t scr_zc = (gh::seed_scratch_ints(pq_total) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
+        HIP_TRY(ctx->seeds.ensure((seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
+        int* d_scratch = reinterpret_cast<int*>(ctx->seeds.as<zc>() + seed_zc);
         HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
         HIP_TRY(hipMemcpyAsync(ctx->squartets.p, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
@@ -373,7 +413,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
         HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>(), (int)sq.size(), pq_total,
                                 b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
-                                ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), s),
+                                ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), d_scratch, s),
                 "seed kernel launch");
         HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
         for (auto& [ca, range] : launches) {
@@ -408,60 +448,102 @@ struct FastReq {
     gh::FastClassInfo info;
 };
 
-bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
-    static const int perms[4][4] = {{0, 1, 2, 3}, {2, 3, 0, 1}, {1, 0, 3, 2}, {3, 2, 1, 0}};
+const int kPerms[4][4] = {{0, 1, 2, 3}, {2, 3, 0, 1}, {1, 0, 3, 2}, {3, 2, 1, 0}};
+
+// The batched kernel's orientation of a class: the first of the four images
+// (ab|cd), (cd|ab), conj (ba|dc), conj (dc|ba) with a specialized kernel, or -1.
+signed char class_orientation(int la, int lb, int lc, int ld, gh::FastClassInfo* info) {
+    const int l[4] = {la, lb, lc, ld};
     for (int o = 0; o < 4; ++o) {
-        int l[4];
-        for (int k = 0; k < 4; ++k) l[k] = b->shells[r.sh[perms[o][k]]].l;
-        const gh::FastClassInfo info = gh::fast_class_info(l[0], l[1], l[2], l[3]);
-        if (!info.available) continue;
-        for (int k = 0; k < 4; ++k) {
-            f.sh[k] = r.sh[perms[o][k]];
-            f.perm[k] = perms[o][k];
+        const gh::FastClassInfo f = gh::fast_class_info(l[kPerms[o][0]], l[kPerms[o][1]], l[kPerms[o][2]], l[kPerms[o][3]]);
+        if (f.available) {
+            *info = f;
+            return (signed char)o;
         }
-        f.conj = o >= 2 ? 1 : 0;
-        f.r = &r;
-        f.info = info;
-        return true;
     }
-    return false;
+    *info = gh::FastClassInfo{};
+    return -1;
 }
 
-// The batched path: requests are cut into chunks whose seeds, R0 tables and
-// pair tables fit the context budget. Each chunk is planned into page-locked
-// staging memory, copied asynchronously and launched right away, so the host
-// plans chunk c + 1 while the device runs chunk c; the chunks reuse one work
-// buffer in stream order and the host waits once, at the end.
+// Admission status and batched-kernel orientation of every class of the basis.
+void build_class_tables(giaohip_basis* b) {
+    int lmax = 0;
+    for (const DevShell& s : b->shells) lmax = std::max(lmax, s.l);
+    const int lm = lmax + 1;
+    b->lm = lm;
+    const size_t n = (size_t)lm * lm * lm * lm;
+    b->cls_status.assign(n, kClsOk);
+    b->cls_orient.assign(n, -1);
+    b->cls_info.assign(n, gh::FastClassInfo{});
+    for (int la = 0; la < lm; ++la)
+        for (int lb = 0; lb < lm; ++lb)
+            for (int lc = 0; lc < lm; ++lc)
+                for (int ld = 0; ld < lm; ++ld) {
+                    const size_t i = (((size_t)la * lm + lb) * lm + lc) * lm + ld;
+                    b->cls_status[i] = class_admission(la, lb, lc, ld);
+                    if (b->cls_status[i] == kClsOk) b->cls_orient[i] = class_orientation(la, lb, lc, ld, &b->cls_info[i]);
+                }
+}
+
+bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
+    const size_t ci = class_index(b, r.sh);
+    const int o = b->cls_orient[ci];
+    if (o < 0) return false;
+    for (int k = 0; k < 4; ++k) {
+        f.sh[k] = r.sh[kPerms[o][k]];
+        f.perm[k] = kPerms[o][k];
+    }
+    f.conj = o >= 2 ? 1 : 0;
+    f.r = &r;
+    f.info = b->cls_info[ci];
+    return true;
+}
+
+// The batched path. Requests are cut into chunks whose seeds, R0 tables, pair
+// tables and quadrature lists fit the chunk budget. Two streams pipeline the
+// chunks: the producer stream (`aux`) runs a chunk's seed, pair-table and R0
+// kernels while the consumer stream (`stream`) runs the previous chunk's
+// transform, so the latency-bound producer kernels share the CUs with the
+// FMA-bound transform. Chunks alternate between two work buffers; events order
+// a chunk's transform after its producer and a buffer's reuse after the
+// transform that read it. Descriptors are planned on the host into
+// page-locked staging while the device works, and the host waits once.
 int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq>& reqs, zc* d_out) {
     if (reqs.empty()) return GIAOHIP_OK;
-    hipStream_t s = ctx->stream;
+    hipStream_t sc = ctx->stream, sp = ctx->aux;
+    constexpr size_t kScratchHead = 4;  // zc holding the 16 counter ints of the quadrature lists
     struct Need {
-        size_t seed, r0, tab;
+        size_t seed, r0, tab, scr;
         long long kab, kcd;
         int L;
         size_t items;
     };
     // pass 1: sizes
     std::vector<Need> need(reqs.size());
+    parallel_for(reqs.size(), [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const FastReq& f = reqs[i];
+            const DevShell* S[4] = {&b->shells[f.sh[0]], &b->shells[f.sh[1]], &b->shells[f.sh[2]], &b->shells[f.sh[3]]};
+            Need& n = need[i];
+            n.L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
+            n.kab = (long long)S[0]->nprim * S[1]->nprim;
+            n.kcd = (long long)S[2]->nprim * S[3]->nprim;
+            n.seed = (size_t)(n.kab * n.kcd) * (size_t)(n.L + 4);
+            n.r0 = (size_t)(n.kab * n.kcd) * (size_t)f.info.nhl;
+            n.tab = (size_t)(n.kab * f.info.bra_ne + n.kcd * f.info.ket_ne);
+            n.scr = ((size_t)gh::kBoysBins * (size_t)(n.kab * n.kcd) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
+            n.items = 1;
+            for (int k = 0; k < 4; ++k)
+                if (f.r->xfix[f.perm[k]] < 0) n.items *= (size_t)S[k]->ncontr;
+        }
+    });
     size_t total_items = 0, max_one = 0;
-    for (size_t i = 0; i < reqs.size(); ++i) {
-        const FastReq& f = reqs[i];
-        const DevShell* S[4] = {&b->shells[f.sh[0]], &b->shells[f.sh[1]], &b->shells[f.sh[2]], &b->shells[f.sh[3]]};
-        Need& n = need[i];
-        n.L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
-        n.kab = (long long)S[0]->nprim * S[1]->nprim;
-        n.kcd = (long long)S[2]->nprim * S[3]->nprim;
-        n.seed = (size_t)(n.kab * n.kcd) * (size_t)(n.L + 4);
-        n.r0 = (size_t)(n.kab * n.kcd) * (size_t)f.info.nhl;
-        n.tab = (size_t)(n.kab * f.info.bra_ne + n.kcd * f.info.ket_ne);
-        n.items = 1;
-        for (int k = 0; k < 4; ++k)
-            if (f.r->xfix[f.perm[k]] < 0) n.items *= (size_t)S[k]->ncontr;
+    for (const Need& n : need) {
         total_items += n.items;
-        max_one = std::max(max_one, n.seed + n.r0 + n.tab);
+        max_one = std::max(max_one, n.seed + n.r0 + n.tab + n.scr + kScratchHead);
     }
     const size_t work_zc = std::max(max_one, ctx->seed_budget_bytes / sizeof(zc));
-    HIP_TRY(ctx->work.ensure(work_zc * sizeof(zc)), "hipMalloc(batched work buffer)");
+    for (DevBuffer& w : ctx->work) HIP_TRY(w.ensure(work_zc * sizeof(zc)), "hipMalloc(batched work buffer)");
     HIP_TRY(ctx->squartets.ensure(reqs.size() * sizeof(SeedQuartet)), "hipMalloc");
     HIP_TRY(ctx->fpq.ensure(reqs.size() * sizeof(gh::FastPairQuartet)), "hipMalloc");
     HIP_TRY(ctx->fitems.ensure(total_items * sizeof(gh::FastItem)), "hipMalloc");
@@ -471,26 +553,31 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
     SeedQuartet* hsq = ctx->h_sq.as<SeedQuartet>();
     gh::FastPairQuartet* hpq = ctx->h_pq.as<gh::FastPairQuartet>();
     gh::FastItem* hit = ctx->h_items.as<gh::FastItem>();
-    zc* work = ctx->work.as<zc>();
+    // events per chunk: 0 producer start, 1 seeds done, 2 producer done, 3 transform start, 4 transform done
+    constexpr int kEv = 5;
+    auto event = [&](size_t c, int k) -> hipEvent_t& { return ctx->chunk_ev[kEv * c + k]; };
+    // the producer starts after the work already queued on the main stream (field, error record)
+    hipEvent_t start_ev = ctx->ev[2];
+    HIP_TRY(hipEventRecord(start_ev, sc), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(sp, start_ev, 0), "hipStreamWaitEvent");
 
     struct Group {
-        gh::FastLaunch fn;
+        std::pair<gh::FastLaunch, gh::FastLaunch> fn;
         int waves;
         std::vector<size_t> idx;
     };
     std::vector<Group> groups;
-    std::vector<hipEvent_t> used;
     size_t nsq = 0, npq = 0, nit = 0, nchunks = 0;
     size_t begin = 0;
     while (begin < reqs.size()) {
         // pass 2: one chunk within the budget
-        size_t end = begin, seed_zc = 0, r0_zc = 0, tab_zc = 0;
+        size_t end = begin, seed_zc = 0, r0_zc = 0, tab_zc = 0, scr_zc = kScratchHead;
         long long pq_total = 0;
         int maxL = 0;
         const size_t sq0 = nsq;
         while (end < reqs.size()) {
             const Need& n = need[end];
-            if (end > begin && seed_zc + r0_zc + tab_zc + n.seed + n.r0 + n.tab > work_zc) break;
+            if (end > begin && seed_zc + r0_zc + tab_zc + scr_zc + n.seed + n.r0 + n.tab + n.scr > work_zc) break;
             SeedQuartet& q = hsq[nsq++];
             q = SeedQuartet{};
             for (int k = 0; k < 4; ++k) q.sh[k] = reqs[end].sh[k];
@@ -502,29 +589,33 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
             seed_zc += n.seed;
             r0_zc += n.r0;
             tab_zc += n.tab;
+            scr_zc += n.scr;
             pq_total += n.kab * n.kcd;
             maxL = std::max(maxL, n.L);
             ++end;
         }
+        zc* work = ctx->work[nchunks & 1].as<zc>();
         zc* d_seeds = work;
-        zc* d_r0 = work + seed_zc;
+        zc* d_r0 = d_seeds + seed_zc;
         zc* d_tab = d_r0 + r0_zc;
-        // group by class kernel and transform workgroup size
+        int* d_scratch = reinterpret_cast<int*>(d_tab + tab_zc);
+        // group by class kernels and transform workgroup size
         for (Group& g : groups) g.idx.clear();
         for (size_t i = begin; i < end; ++i) {
             const FastReq& f = reqs[i];
             const int waves = gh::fast_waves(f.info.nhab, (int)need[i].kab);
+            const std::pair<gh::FastLaunch, gh::FastLaunch> fn{f.info.produce, f.info.transform};
             Group* g = nullptr;
             for (Group& x : groups)
-                if (x.fn == f.info.launch && x.waves == waves) g = &x;
+                if (x.fn == fn && x.waves == waves) g = &x;
             if (!g) {
-                groups.push_back(Group{f.info.launch, waves, {}});
+                groups.push_back(Group{fn, waves, {}});
                 g = &groups.back();
             }
             g->idx.push_back(i);
         }
         struct Launch {
-            gh::FastLaunch fn;
+            std::pair<gh::FastLaunch, gh::FastLaunch> fn;
             int waves;
             size_t pq0, npq, it0, nit;
         };
@@ -579,30 +670,30 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
             L.nit = nit - L.it0;
             launches.push_back(L);
         }
-        // copies of this chunk's descriptors, then its kernels
-        HIP_TRY(hipMemcpyAsync(ctx->squartets.as<SeedQuartet>() + sq0, hsq + sq0, (nsq - sq0) * sizeof(SeedQuartet),
-                               hipMemcpyHostToDevice, s),
-                "hipMemcpyAsync");
-        HIP_TRY(hipMemcpyAsync(ctx->fpq.as<gh::FastPairQuartet>() + pq_chunk0, hpq + pq_chunk0,
-                               (npq - pq_chunk0) * sizeof(gh::FastPairQuartet), hipMemcpyHostToDevice, s),
-                "hipMemcpyAsync");
-        HIP_TRY(hipMemcpyAsync(ctx->fitems.as<gh::FastItem>() + it_chunk0, hit + it_chunk0,
-                               (nit - it_chunk0) * sizeof(gh::FastItem), hipMemcpyHostToDevice, s),
-                "hipMemcpyAsync");
-        while (ctx->chunk_ev.size() < 3 * (nchunks + 1)) {
+        while (ctx->chunk_ev.size() < kEv * (nchunks + 1)) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e), "hipEventCreate");
             ctx->chunk_ev.push_back(e);
         }
-        hipEvent_t* ev = ctx->chunk_ev.data() + 3 * nchunks;
-        HIP_TRY(hipEventRecord(ev[0], s), "hipEventRecord");
+        // producer: wait until the transform that last read this work buffer is done
+        if (nchunks >= 2) HIP_TRY(hipStreamWaitEvent(sp, event(nchunks - 2, 4), 0), "hipStreamWaitEvent");
+        HIP_TRY(hipMemcpyAsync(ctx->squartets.as<SeedQuartet>() + sq0, hsq + sq0, (nsq - sq0) * sizeof(SeedQuartet),
+                               hipMemcpyHostToDevice, sp),
+                "hipMemcpyAsync");
+        HIP_TRY(hipMemcpyAsync(ctx->fpq.as<gh::FastPairQuartet>() + pq_chunk0, hpq + pq_chunk0,
+                               (npq - pq_chunk0) * sizeof(gh::FastPairQuartet), hipMemcpyHostToDevice, sp),
+                "hipMemcpyAsync");
+        HIP_TRY(hipMemcpyAsync(ctx->fitems.as<gh::FastItem>() + it_chunk0, hit + it_chunk0,
+                               (nit - it_chunk0) * sizeof(gh::FastItem), hipMemcpyHostToDevice, sp),
+                "hipMemcpyAsync");
+        HIP_TRY(hipEventRecord(event(nchunks, 0), sp), "hipEventRecord");
         HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>() + sq0, (int)(nsq - sq0),
                                 pq_total, b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
-                                ctx->kappa.as<double>(), d_seeds, ctx->errrec.as<DevError>(), s,
+                                ctx->kappa.as<double>(), d_seeds, ctx->errrec.as<DevError>(), d_scratch, sp,
                                 b->d_radial.as<double>()),
                 "seed kernel launch");
-        HIP_TRY(hipEventRecord(ev[1], s), "hipEventRecord");
-        for (const Launch& L : launches) {
+        HIP_TRY(hipEventRecord(event(nchunks, 1), sp), "hipEventRecord");
+        auto args = [&](const Launch& L) {
             gh::FastLaunchArgs a{};
             a.pq = ctx->fpq.as<gh::FastPairQuartet>() + L.pq0;
             a.npq = (int)L.npq;
@@ -617,20 +708,30 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
             a.r0 = d_r0;
             a.etab = d_tab;
             a.out = d_out;
-            HIP_TRY(L.fn(a, s), "batched transform launch");
-            ctx->stats.launches += 4;
+            return a;
+        };
+        for (const Launch& L : launches) {
+            HIP_TRY(L.fn.first(args(L), sp), "batched pair/R0 launch");
+            ctx->stats.launches += 3;
         }
-        HIP_TRY(hipEventRecord(ev[2], s), "hipEventRecord");
-        ctx->stats.launches += 1;
+        HIP_TRY(hipEventRecord(event(nchunks, 2), sp), "hipEventRecord");
+        // consumer: the transform after this chunk's producer
+        HIP_TRY(hipStreamWaitEvent(sc, event(nchunks, 2), 0), "hipStreamWaitEvent");
+        HIP_TRY(hipEventRecord(event(nchunks, 3), sc), "hipEventRecord");
+        for (const Launch& L : launches) {
+            HIP_TRY(L.fn.second(args(L), sc), "batched transform launch");
+            ctx->stats.launches += 1;
+        }
+        HIP_TRY(hipEventRecord(event(nchunks, 4), sc), "hipEventRecord");
+        ctx->stats.launches += 2;
         ctx->stats.primitive_quartets += (uint64_t)pq_total;
         ++nchunks;
         begin = end;
     }
-    HIP_TRY(hipEventSynchronize(ctx->chunk_ev[3 * nchunks - 1]), "kernel execution");
+    HIP_TRY(hipEventSynchronize(event(nchunks - 1, 4)), "kernel execution");
     for (size_t c = 0; c < nchunks; ++c) {
-        const hipEvent_t* ev = ctx->chunk_ev.data() + 3 * c;
-        ctx->stats.seed_ms += elapsed(ev[0], ev[1]);
-        ctx->stats.transform_ms += elapsed(ev[1], ev[2]);
+        ctx->stats.seed_ms += elapsed(event(c, 0), event(c, 1));
+        ctx->stats.transform_ms += elapsed(event(c, 1), event(c, 2)) + elapsed(event(c, 3), event(c, 4));
     }
     DevError de{};
     HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -644,13 +745,23 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
 int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
                  zc* d_out) {
     if (reqs.empty()) return GIAOHIP_OK;
+    // split into batched and plain requests, in parallel and in input order
+    const size_t n = reqs.size();
+    const bool batched = ctx->eri_variant == GIAOHIP_ERI_BATCHED;
+    std::vector<FastReq> fast_all(n);
+    std::vector<char> is_fast(n);
+    parallel_for(n, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) is_fast[i] = batched && reqs[i].cd1 < 0 && orient(b, reqs[i], fast_all[i]);
+    });
     std::vector<Request> plain;
     std::vector<FastReq> fast;
-    for (const Request& r : reqs) {
-        FastReq f{};
-        const bool full = r.cd1 < 0;
-        if (ctx->eri_variant == GIAOHIP_ERI_BATCHED && full && orient(b, r, f)) fast.push_back(f);
-        else plain.push_back(r);
+    size_t nfast = 0;
+    for (size_t i = 0; i < n; ++i) nfast += is_fast[i] ? 1 : 0;
+    fast.reserve(nfast);
+    plain.reserve(n - nfast);
+    for (size_t i = 0; i < n; ++i) {
+        if (is_fast[i]) fast.push_back(fast_all[i]);
+        else plain.push_back(reqs[i]);
     }
     int st = run_plain(ctx, b, field, plain, d_out);
     if (st || fast.empty()) return st;
@@ -794,11 +905,12 @@ int giaohip_create(int device, giaohip_ctx** out) {
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
     for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
     if (e == hipSuccess) e = gh::upload_constants();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
-        delete ctx;
+        giaohip_destroy(ctx);
         return GIAOHIP_DEVICE_ERROR;
     }
     *out = ctx;
@@ -808,18 +920,20 @@ int giaohip_create(int device, giaohip_ctx** out) {
 int giaohip_destroy(giaohip_ctx* ctx) {
     if (!ctx) return GIAOHIP_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
     for (DevBuffer* b : {&ctx->seeds, &ctx->squartets, &ctx->items, &ctx->kappa, &ctx->errrec, &ctx->staging,
                          &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c, &ctx->fpq, &ctx->fitems})
         b->release();
     for (auto& ev : ctx->chunk_ev)
         if (ev) (void)hipEventDestroy(ev);
-    ctx->work.release();
+    for (DevBuffer& w : ctx->work) w.release();
     for (PinnedBuffer* b : {&ctx->h_sq, &ctx->h_pq, &ctx->h_items}) b->release();
     for (void* p : ctx->user_allocs) (void)hipFree(p);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     delete ctx;
     return GIAOHIP_OK;
 }
@@ -937,6 +1051,7 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
         no += (int)comps.size() * d.nprim;
     }
     b->nao = ao;
+    build_class_tables(b);
     (void)hipSetDevice(ctx->device);
     auto up = [&](DevBuffer& buf, const void* src, size_t bytes) -> hipError_t {
         hipError_t e = buf.ensure(bytes);
@@ -1038,18 +1153,45 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     reset_stats(ctx);
     ctx->stats.requested_quartets = n;
     const auto host_t0 = std::chrono::steady_clock::now();
-    for (size_t i = 0; i < 4 * n; ++i)
-        if (quartets[i] >= (uint32_t)b->ns) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "ERI shell-quartet index is out of range");
     for (int k = 0; k < 6; ++k)
         if (!std::isfinite(field[k])) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "field values must be finite");
+    // validation and block sizes in parallel; the first failing quartet decides the error
     std::vector<uint64_t> offs(n), sizes(n);
+    std::vector<size_t> first_bad(16, n);
+    std::vector<int> bad_status(16, 0);
+    std::atomic<size_t> slot{0};
+    parallel_for(n, [&](size_t lo, size_t hi) {
+        const size_t me = slot.fetch_add(1);
+        for (size_t i = lo; i < hi; ++i) {
+            int sh[4];
+            bool in_range = true;
+            for (int k = 0; k < 4; ++k) {
+                sh[k] = (int)quartets[4 * i + k];
+                in_range = in_range && quartets[4 * i + k] < (uint32_t)b->ns;
+            }
+            const int st = in_range ? b->cls_status[class_index(b, sh)] : -1;
+            if (st != 0) {
+                first_bad[me] = i;
+                bad_status[me] = st;
+                return;
+            }
+            sizes[i] = (uint64_t)b->shells[sh[0]].ao_count * b->shells[sh[1]].ao_count * b->shells[sh[2]].ao_count *
+                       b->shells[sh[3]].ao_count;
+        }
+    });
+    size_t bad = n;
+    int bad_st = 0;
+    for (size_t t = 0; t < first_bad.size(); ++t)
+        if (first_bad[t] < bad) {
+            bad = first_bad[t];
+            bad_st = bad_status[t];
+        }
+    if (bad < n) {
+        if (bad_st < 0) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "ERI shell-quartet index is out of range");
+        return class_error(ctx, bad_st);
+    }
     uint64_t run = 0;
     for (size_t i = 0; i < n; ++i) {
-        int sh[4];
-        for (int k = 0; k < 4; ++k) sh[k] = (int)quartets[4 * i + k];
-        const int st = check_quartet_supported(ctx, b, sh);
-        if (st) return st;
-        sizes[i] = (uint64_t)ao_count(b, sh[0]) * ao_count(b, sh[1]) * ao_count(b, sh[2]) * ao_count(b, sh[3]);
         offs[i] = offsets ? offsets[i] : run;
         run += sizes[i];
     }
@@ -1078,9 +1220,10 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     };
     int st = GIAOHIP_OK;
     if (out_on_device) {
-        std::vector<Request> reqs;
-        reqs.reserve(n);
-        for (size_t i = 0; i < n; ++i) reqs.push_back(make_req(i, (long long)offs[i]));
+        std::vector<Request> reqs(n);
+        parallel_for(n, [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) reqs[i] = make_req(i, (long long)offs[i]);
+        });
         st = run_requests(ctx, b, field, reqs, reinterpret_cast<zc*>(out));
     } else {
         // Stage groups of quartets through device memory, then copy back.
@@ -1422,7 +1565,10 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     CHOL_TRY(d_pidx.ensure(parts * sizeof(long long)), "hipMalloc");
     CHOL_TRY(d_state.ensure(sizeof(gh::CholState)), "hipMalloc");
     CHOL_TRY(d_flags.ensure(sizeof(int)), "hipMalloc");
-    CHOL_TRY(ctx->seeds.ensure(std::max<long long>(seed_zc, 1) * sizeof(zc)), "hipMalloc(seeds)");
+    const long long scr_zc =
+        (long long)((gh::seed_scratch_ints(p1 * (long long)max_k * max_k) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc));
+    CHOL_TRY(ctx->seeds.ensure((size_t)(seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
+    int* d_scratch = reinterpret_cast<int*>(ctx->seeds.as<zc>() + seed_zc);
     CHOL_TRY(hipMemset(d_flags.p, 0, sizeof(int)), "hipMemset");
 
     // Diagonal over all ordered shell pairs a <= b (cholesky.py:143-169).
@@ -1511,7 +1657,7 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
         int maxL = max_lab + ket.lcd;
         CHOL_TRY(gh::launch_seed(nmax_bucket(maxL), d_sq.as<SeedQuartet>(), nunits, p1 * ket.kcd,
                                  b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
-                                 ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), s),
+                                 ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), d_scratch, s),
                  "seed launch");
         for (const auto& g : groups) {
             gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};

@@ -84,24 +84,58 @@ __device__ inline void record_error(DevError* err, int code, long long item, zc 
 }
 
 // ---------------------------------------------------------------------------
-// Seed kernel
+// Seed kernels
+//
+// Phase A (seed_kernel): one lane per primitive quartet. Arguments in the
+// power-series and asymptotic regions (and domain errors) are finished here;
+// arguments that need the adaptive quadrature are appended to one of
+// kBoysBins lists by their starting segment count (wave-aggregated atomics).
+// Phase B (seed_quad_kernel): persistent wavefronts take 64 list entries at
+// a time, most expensive bin first, so a wavefront's lanes run quadratures of
+// similar length. Each argument is evaluated on its own, so the results do
+// not depend on the order of the lists.
+//
+// Scratch layout (ints): [0, kBoysBins) bin counts, [15] the work counter,
+// then bin b at 16 + b * total.
 // ---------------------------------------------------------------------------
+__device__ inline int boys_bin(zc z) {
+    const double d = fmax(fabs(z.x), fabs(z.y));
+    int s = (int)ceil(d / 8.0);  // the adaptive quadrature's starting segment count
+    if (s < 1) s = 1;
+    if (s <= 8) return s - 1;
+    if (s <= 10) return 8;
+    if (s <= 14) return 9;
+    return 10;
+}
+static_assert(kBoysBins == 11, "boys_bin covers 11 bins");
+
+// Series / asymptotic / domain error, as boys_eval decides them.
 template <int NMAX>
-__global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict__ quartets, int nq,
-                                                   long long total, const DevShell* __restrict__ shells,
-                                                   const double* __restrict__ exps,
-                                                   const double* __restrict__ kappa, zc* __restrict__ seeds,
-                                                   DevError* err, const double* __restrict__ radial) {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
-    int lo = 0, hi = nq - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (quartets[mid].pq_begin <= gid) lo = mid;
-        else hi = mid - 1;
+__device__ inline bool boys_direct(zc z, int nmax, bool scaled, zc (&F)[NMAX + 1], int& st) {
+    const double mag = zabs(z);
+    double err = 0.0;
+    if (mag <= 0.75) {
+        st = boys_series<NMAX>(z, nmax, scaled, F, err);
+        return true;
     }
-    const SeedQuartet Q = quartets[lo];
-    const long long local = gid - Q.pq_begin;
+    const double asym = 2.0 * (double)nmax + 60.0;
+    if (z.x >= asym && fabs(z.y) <= 0.5 * z.x) {
+        st = boys_asymptotic<NMAX>(z, nmax, scaled, F, err);
+        return true;
+    }
+    if (mag > 160.0) {
+        st = kErrBoysDomain;
+        return true;
+    }
+    return false;  // needs the adaptive quadrature
+}
+
+// Seeds of one primitive quartet. Phase A (QUAD false) returns false and sets
+// bin when the argument needs the quadrature; phase B (QUAD true) evaluates it.
+template <int NMAX, bool QUAD>
+__device__ inline bool seed_unit(const SeedQuartet& Q, int qidx, long long local, const DevShell* __restrict__ shells,
+                                 const double* __restrict__ exps, const double* __restrict__ kappa,
+                                 zc* __restrict__ seeds, DevError* err, const double* __restrict__ radial, int& bin) {
     const int ab = (int)(local / Q.kcd), cd = (int)(local % Q.kcd);
     const DevShell& A = shells[Q.sh[0]];
     const DevShell& B = shells[Q.sh[1]];
@@ -119,13 +153,21 @@ __global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict
     const zc T = zscale(rho, zadd(zadd(zmul(X[0], X[0]), zmul(X[1], X[1])), zmul(X[2], X[2])));
     const bool scaled = T.x < 0.0;
     zc F[NMAX + 1];
-    int region, segs;
-    double berr;
     const int L = Q.L;
-    const int st = boys_eval<NMAX>(T, L, scaled, F, region, berr, segs);
+    int st = kErrNone;
+    if constexpr (QUAD) {
+        int region, segs;
+        double berr;
+        st = boys_eval<NMAX>(T, L, scaled, F, region, berr, segs);
+    } else {
+        if (!boys_direct<NMAX>(T, L, scaled, F, st)) {
+            bin = boys_bin(T);
+            return false;
+        }
+    }
     zc* out = seeds + Q.seed_off + local * (L + 4);
     if (st != kErrNone) {
-        record_error(err, st, lo, T, L);
+        record_error(err, st, qidx, T, L);
 #pragma unroll
         for (int n = 0; n <= NMAX; ++n) F[n] = zc{0.0, 0.0};
     }
@@ -156,6 +198,86 @@ __global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict
     for (int n = 0; n <= NMAX; ++n) {
         if (n <= L) out[3 + n] = zmul(zscale(rf, base), F[n]);
         rf *= -2.0 * rho;
+    }
+    return true;
+}
+
+__device__ inline int find_quartet(const SeedQuartet* __restrict__ quartets, int nq, long long gid) {
+    int lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (quartets[mid].pq_begin <= gid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict__ quartets, int nq,
+                                                   long long total, const DevShell* __restrict__ shells,
+                                                   const double* __restrict__ exps,
+                                                   const double* __restrict__ kappa, zc* __restrict__ seeds,
+                                                   DevError* err, const double* __restrict__ radial,
+                                                   int* __restrict__ scratch) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int bin = -1;
+    if (gid < total) {
+        const int qi = find_quartet(quartets, nq, gid);
+        const SeedQuartet Q = quartets[qi];
+        seed_unit<NMAX, false>(Q, qi, gid - Q.pq_begin, shells, exps, kappa, seeds, err, radial, bin);
+    }
+    // append the deferred arguments, one atomic per (wavefront, bin)
+    const int lane = threadIdx.x % 64;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+    for (int bb = 0; bb < kBoysBins; ++bb) {
+        const unsigned long long mask = __ballot(bin == bb);
+        if (mask == 0) continue;
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(scratch + bb, __popcll(mask));
+        base = __shfl(base, leader);
+        if (bin == bb) scratch[16 + (long long)bb * total + base + __popcll(mask & below)] = (int)gid;
+    }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(64) void seed_quad_kernel(const SeedQuartet* __restrict__ quartets, int nq,
+                                                       long long total, const DevShell* __restrict__ shells,
+                                                       const double* __restrict__ exps,
+                                                       const double* __restrict__ kappa, zc* __restrict__ seeds,
+                                                       DevError* err, const double* __restrict__ radial,
+                                                       int* __restrict__ scratch) {
+    int counts[kBoysBins];
+    int all = 0;
+#pragma unroll
+    for (int bb = 0; bb < kBoysBins; ++bb) {
+        counts[bb] = scratch[bb];
+        all += counts[bb];
+    }
+    const int lane = threadIdx.x;
+    for (;;) {
+        int start = 0;
+        if (lane == 0) start = atomicAdd(scratch + 15, 64);
+        start = __shfl(start, 0);
+        if (start >= all) break;  // every wavefront reaches this once the lists are taken
+        const int idx = start + lane;
+        int off = 0, bin = -1, pos = 0;
+#pragma unroll
+        for (int bb = kBoysBins - 1; bb >= 0; --bb) {  // most expensive bin first
+            if (bin < 0 && idx < off + counts[bb]) {
+                bin = bb;
+                pos = idx - off;
+            }
+            off += counts[bb];
+        }
+        if (bin >= 0) {
+            const long long gid = scratch[16 + (long long)bin * total + pos];
+            const int qi = find_quartet(quartets, nq, gid);
+            const SeedQuartet Q = quartets[qi];
+            int unused;
+            seed_unit<NMAX, true>(Q, qi, gid - Q.pq_begin, shells, exps, kappa, seeds, err, radial, unused);
+        }
     }
 }
 
@@ -614,20 +736,31 @@ hipError_t upload_constants() {
     return hipSuccess;
 }
 
+size_t seed_scratch_ints(long long total) { return 16 + (size_t)kBoysBins * (size_t)total; }
+
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
-                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s,
-                       const double* radial) {
+                       const double* exps, const double* kappa, zc* seeds, DevError* err, int* scratch,
+                       hipStream_t s, const double* radial) {
     if (total == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(scratch, 0, 16 * sizeof(int), s);
+    if (e != hipSuccess) return e;
     const dim3 block(128);
     const dim3 grid((unsigned)((total + 127) / 128));
+    const long long qwaves = (total + 63) / 64;
+    const dim3 qgrid((unsigned)(qwaves < 4096 ? qwaves : 4096));
+#define GH_SEED(N)                                                                                              \
+    seed_kernel<N><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial, scratch);     \
+    seed_quad_kernel<N><<<qgrid, dim3(64), 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial, scratch); \
+    break;
     switch (nmax_bucket) {
-        case 4: seed_kernel<4><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
-        case 8: seed_kernel<8><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
-        case 12: seed_kernel<12><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
-        case 16: seed_kernel<16><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
-        case 24: seed_kernel<24><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
-        default: seed_kernel<32><<<grid, block, 0, s>>>(q, nq, total, shells, exps, kappa, seeds, err, radial); break;
+        case 4: GH_SEED(4)
+        case 8: GH_SEED(8)
+        case 12: GH_SEED(12)
+        case 16: GH_SEED(16)
+        case 24: GH_SEED(24)
+        default: GH_SEED(32)
     }
+#undef GH_SEED
     return hipGetLastError();
 }
 

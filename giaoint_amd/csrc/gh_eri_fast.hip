@@ -28,7 +28,7 @@ FastClassInfo fast_class_info(int la, int lb, int lc, int ld) {
     GH_FAST(1, 0, 1, 0)
     GH_FAST(0, 0, 0, 0)
 #undef GH_FAST
-    return FastClassInfo{false, 0, 0, 0, 0, nullptr};
+    return FastClassInfo{false, 0, 0, 0, 0, nullptr, nullptr};
 }
 
 }  // namespace gh

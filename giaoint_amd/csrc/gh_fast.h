@@ -589,9 +589,8 @@ __global__ __launch_bounds__(kMaxThreads) void fast_transform_kernel(const FastI
 // Launch table
 // ---------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD>
-hipError_t launch_class(const FastLaunchArgs& a, hipStream_t s) {
+hipError_t launch_produce(const FastLaunchArgs& a, hipStream_t s) {
     using C = Cls<LA, LB, LC, LD>;
-    if (a.waves < kG || a.waves > kMaxWaves || 64 * a.waves < C::NHAB) return hipErrorInvalidValue;
     if (a.npq) {
         pair_kernel<LA, LB, kG, 1, false><<<dim3((unsigned)a.npq), dim3(64), 0, s>>>(a.pq, 0, a.shells, a.exps,
                                                                                     a.kappa, a.etab);
@@ -599,6 +598,13 @@ hipError_t launch_class(const FastLaunchArgs& a, hipStream_t s) {
                                                                                           a.kappa, a.etab);
         r0_kernel<C::L><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, a.seeds, a.r0);
     }
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t launch_transform(const FastLaunchArgs& a, hipStream_t s) {
+    using C = Cls<LA, LB, LC, LD>;
+    if (a.waves < kG || a.waves > kMaxWaves || 64 * a.waves < C::NHAB) return hipErrorInvalidValue;
     if (a.nitems)
         fast_transform_kernel<LA, LB, LC, LD><<<dim3((unsigned)a.nitems), dim3(64 * a.waves), 0, s>>>(
             a.items, a.shells, a.coefs, a.r0, a.etab, a.out);
@@ -608,7 +614,15 @@ hipError_t launch_class(const FastLaunchArgs& a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 FastClassInfo info() {
     using C = Cls<LA, LB, LC, LD>;
-    return FastClassInfo{true, C::Bra::NEP, C::Ket::NEP, C::NHAB, C::NHL, &launch_class<LA, LB, LC, LD>};
+    FastClassInfo f{};
+    f.available = true;
+    f.bra_ne = C::Bra::NEP;
+    f.ket_ne = C::Ket::NEP;
+    f.nhab = C::NHAB;
+    f.nhl = C::NHL;
+    f.produce = &launch_produce<LA, LB, LC, LD>;
+    f.transform = &launch_transform<LA, LB, LC, LD>;
+    return f;
 }
 
 }  // namespace fast

@@ -84,7 +84,8 @@ struct FastClassInfo {
     int bra_ne, ket_ne;  // products per bra / ket primitive pair
     int nhab;            // bra Hermite indices (rows per bra primitive pair)
     int nhl;             // R0 entries per primitive quartet
-    FastLaunch launch;
+    FastLaunch produce;  // pair tables and R0 tables (after the seeds)
+    FastLaunch transform;  // the E.R.E transform (after produce)
 };
 FastClassInfo fast_class_info(int la, int lb, int lc, int ld);
 // Transform workgroup size (wavefronts) for a bra side of nhab Hermite rows
@@ -96,9 +97,13 @@ int transform_lds_bytes(const ClassArgs& ca);
 int transform_max_w();
 int transform_max_o();
 
+// Seeds of `total` primitive quartets. `scratch` holds seed_scratch_ints(total)
+// ints (quadrature work lists); it is reused by the next launch in stream order.
+constexpr int kBoysBins = 11;
+size_t seed_scratch_ints(long long total);
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
-                       const double* exps, const double* kappa, zc* seeds, DevError* err, hipStream_t s,
-                       const double* radial = nullptr);
+                       const double* exps, const double* kappa, zc* seeds, DevError* err, int* scratch,
+                       hipStream_t s, const double* radial = nullptr);
 hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nitems, const DevShell* shells,
                             const double* exps, const double* coefs, const double* norms, const double* kappa,
                             const zc* seeds, zc* out, hipStream_t s);

@@ -54,9 +54,9 @@ enum {
 };
 enum {
     GIAOHIP_OPTION_ERI_VARIANT = 0,
-    /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables), in
-       MiB, 16..65536; default 4096. Smaller chunks keep the R0 tables in the
-       device's caches, larger ones need fewer launches. */
+    /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables),
+       in MiB, 16..65536; default 2048. Two chunks are in flight at once (one
+       being produced, one transformed), so the path holds twice this much. */
     GIAOHIP_OPTION_CHUNK_MIB = 1
 };
 
