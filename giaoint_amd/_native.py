@@ -89,6 +89,7 @@ _SIGNATURES = {
     "giaohip_cholesky_info": ([_vp, _u64p, _dp, _dp, _ip, _dp, _dp, _dp], C.c_int),
     "giaohip_cholesky_pivots": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "giaohip_cholesky_vectors": ([_vp, _vp, C.c_uint64, C.c_uint64, _dp], C.c_int),
+    "giaohip_cholesky_resume": ([_vp, _vp, C.c_uint64], C.c_int),
     "giaohip_cholesky_destroy": ([_vp, _vp], C.c_int),
 }
 
@@ -345,6 +346,9 @@ class DeviceCholesky:
         self.basis = basis
         self.ctx = basis.ctx
         self.handle = handle
+        self._refresh()
+
+    def _refresh(self):
         rank = C.c_uint64()
         res = C.c_double()
         tol = C.c_double()
@@ -353,7 +357,7 @@ class DeviceCholesky:
         upd = C.c_double()
         dia = C.c_double()
         self.ctx.check(
-            self.ctx.lib.giaohip_cholesky_info(handle, C.byref(rank), C.byref(res), C.byref(tol), C.byref(conv),
+            self.ctx.lib.giaohip_cholesky_info(self.handle, C.byref(rank), C.byref(res), C.byref(tol), C.byref(conv),
                                                C.byref(col), C.byref(upd), C.byref(dia))
         )
         self.rank = int(rank.value)
@@ -361,6 +365,12 @@ class DeviceCholesky:
         self.tolerance = float(tol.value)
         self.converged = bool(conv.value)
         self.timings_ms = {"column": col.value, "update": upd.value, "diagonal": dia.value}
+
+    def resume(self, stop_after: int = 0):
+        """Continue the run until converged or the rank reaches stop_after (0: no limit)."""
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_resume(self.ctx.handle, self.handle, C.c_uint64(int(stop_after))))
+        self._refresh()
+        return self
 
     def pivots(self) -> np.ndarray:
         out = np.zeros((self.rank, 2), dtype=np.int64)

@@ -133,6 +133,8 @@ struct giaohip_ctx {
     size_t staging_budget_bytes = size_t(1) << 30;
 };
 
+// A device-resident Cholesky: factors, residual and every buffer a pivot step
+// needs, so that a run can be continued (giaohip_cholesky_resume).
 struct giaohip_cholesky {
     long long n = 0;
     int rows_per_chunk = 0;
@@ -142,6 +144,31 @@ struct giaohip_cholesky {
     double max_residual = 0.0, tolerance = 0.0;
     int converged = 0;
     double column_ms = 0.0, update_ms = 0.0, diagonal_ms = 0.0;
+    // state carried between steps
+    const giaohip_basis* basis = nullptr;
+    int failed = 0;         // status that ended the run early (the factors stop there)
+    double psd_tol = 0.0;
+    uint64_t limit = 0;     // rank limit (max_rank, or n*n)
+    gh::CholState state{};  // current maximum residual and its pivot
+    std::vector<std::array<int, 4>> groups;  // bra (la, lb, first unit, count)
+    int nunits = 0, max_lab = 0;
+    long long p1 = 0;
+    DevBuffer d_units, d_sq, d_wi, d_res, d_col, d_coef, d_pmax, d_pidx, d_state, d_flags, d_chunkptr, d_seeds,
+        d_kappa, d_err;
+    int* d_scratch = nullptr;
+    hipEvent_t ev[3]{};
+    void* host = nullptr;  // page-locked copy of the state and the PSD flag
+    void release() {
+        for (zc* c : chunks) (void)hipFree(c);
+        chunks.clear();
+        for (DevBuffer* x : {&d_units, &d_sq, &d_wi, &d_res, &d_col, &d_coef, &d_pmax, &d_pidx, &d_state, &d_flags,
+                             &d_chunkptr, &d_seeds, &d_kappa, &d_err})
+            x->release();
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (host) (void)hipHostFree(host);
+        host = nullptr;
+    }
 };
 
 namespace {
@@ -1472,6 +1499,136 @@ int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* b, const uin
     return GIAOHIP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+struct CholHost {
+    gh::CholState state;
+    int flags;
+};
+
+// Pivot steps of a run until the residual is below the tolerance or the rank
+// reaches `target` (0: no target). One column of ERIs, the projection against
+// the earlier factors, the residual update and the next argmax per step; the
+// host reads the pivot back to choose the column's ket class.
+int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
+    const giaohip_basis* b = c->basis;
+    hipStream_t s = ctx->stream;
+    const long long n = c->n, n2 = n * n;
+#define CHOL_STEP_TRY(expr, what)                        \
+    do {                                                 \
+        hipError_t _e = (expr);                          \
+        if (_e != hipSuccess) {                          \
+            c->failed = hip_fail(ctx, _e, what);         \
+            return c->failed;                            \
+        }                                                \
+    } while (0)
+    CholHost* host = static_cast<CholHost*>(c->host);
+    while (c->state.value > c->tolerance) {
+        if (target && c->rank >= target) break;
+        if (c->rank >= c->limit) {
+            std::ostringstream m;
+            m << "Cholesky ERI output requires at least " << (c->rank + 1) * (uint64_t)(n2 * 16 + 16)
+              << " bytes, exceeding the requested rank limit " << c->limit;
+            c->failed = set_error(ctx, GIAOHIP_OUT_OF_MEMORY, m.str());
+            return c->failed;
+        }
+        const long long p = c->state.pivot;
+        const long long mu = p / n, nu = p % n;
+        const int sa = b->owner[mu], sb = b->owner[nu];
+        const DevShell& C = b->shells[sb];
+        const DevShell& D = b->shells[sa];
+        gh::CholKet ket{};
+        ket.sh_c = sb;
+        ket.sh_d = sa;
+        ket.lcd = C.l + D.l;
+        ket.kcd = C.nprim * D.nprim;
+        const long long cl = nu - b->ao_off[sb], dl = mu - b->ao_off[sa];
+        ket.xc = (int)(cl / C.ncart);
+        ket.xd = (int)(dl / D.ncart);
+        ket.comp = (int)(cl % C.ncart) * D.ncart + (int)(dl % D.ncart);
+        ket.n = n;
+        // factor storage for row `rank`
+        const int chunk = (int)(c->rank / c->rows_per_chunk);
+        if (chunk >= (int)c->chunks.size()) {
+            void* mem = nullptr;
+            CHOL_STEP_TRY(hipMalloc(&mem, (size_t)c->rows_per_chunk * n2 * sizeof(zc)), "hipMalloc(factors)");
+            c->chunks.push_back(static_cast<zc*>(mem));
+            CHOL_STEP_TRY(c->d_chunkptr.ensure(c->chunks.size() * sizeof(zc*)), "hipMalloc");
+            CHOL_STEP_TRY(hipMemcpy(c->d_chunkptr.p, c->chunks.data(), c->chunks.size() * sizeof(zc*),
+                                    hipMemcpyHostToDevice),
+                          "hipMemcpy");
+            CHOL_STEP_TRY(c->d_coef.ensure(c->chunks.size() * c->rows_per_chunk * sizeof(zc)), "hipMalloc");
+        }
+        zc* row = c->chunks[chunk] + (long long)(c->rank % c->rows_per_chunk) * n2;
+        CHOL_STEP_TRY(hipEventRecord(c->ev[0], s), "hipEventRecord");
+        CHOL_STEP_TRY(gh::launch_chol_expand(c->d_units.as<gh::CholUnit>(), c->nunits, ket, c->d_sq.as<SeedQuartet>(),
+                                             c->d_wi.as<WorkItem>(), s),
+                      "expand launch");
+        const int maxL = c->max_lab + ket.lcd;
+        CHOL_STEP_TRY(gh::launch_seed(nmax_bucket(maxL), c->d_sq.as<SeedQuartet>(), c->nunits, c->p1 * ket.kcd,
+                                      b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
+                                      c->d_seeds.as<zc>(), c->d_err.as<DevError>(), c->d_scratch, s),
+                      "seed launch");
+        for (const auto& g : c->groups) {
+            gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
+            CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
+                                               b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                                               b->d_norms.as<double>(), c->d_kappa.as<double>(),
+                                               c->d_seeds.as<zc>(), c->d_col.as<zc>(), s),
+                          "transform launch");
+        }
+        CHOL_STEP_TRY(hipEventRecord(c->ev[1], s), "hipEventRecord");
+        CHOL_STEP_TRY(gh::launch_chol_gather(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                             c->d_state.as<gh::CholState>(), c->d_coef.as<zc>(), s),
+                      "gather launch");
+        CHOL_STEP_TRY(gh::launch_chol_update(c->d_col.as<zc>(), c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
+                                             (int)c->rank, c->d_coef.as<zc>(), c->d_state.as<gh::CholState>(),
+                                             c->d_res.as<double>(), row, c->psd_tol, c->d_flags.as<int>(),
+                                             c->d_pmax.as<double>(), c->d_pidx.as<long long>(), s),
+                      "update launch");
+        CHOL_STEP_TRY(gh::launch_chol_argmax(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
+                                             c->d_pidx.as<long long>(), c->d_state.as<gh::CholState>(), false, s),
+                      "argmax launch");
+        CHOL_STEP_TRY(hipEventRecord(c->ev[2], s), "hipEventRecord");
+        CHOL_STEP_TRY(hipMemcpyAsync(&host->state, c->d_state.p, sizeof(gh::CholState), hipMemcpyDeviceToHost, s),
+                      "copy");
+        CHOL_STEP_TRY(hipMemcpyAsync(&host->flags, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+        CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky step");
+        c->column_ms += elapsed(c->ev[0], c->ev[1]);
+        c->update_ms += elapsed(c->ev[1], c->ev[2]);
+        c->pivots.push_back(mu);
+        c->pivots.push_back(nu);
+        c->rank += 1;
+        if (host->flags & 1) {
+            c->failed = set_error(ctx, GIAOHIP_RUNTIME_ERROR, "Coulomb Cholesky residual lost positive semidefiniteness");
+            return c->failed;
+        }
+        c->state = host->state;
+        DevError de{};
+        if (c->rank % 64 == 0 || c->state.value <= c->tolerance) {
+            CHOL_STEP_TRY(hipMemcpy(&de, c->d_err.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+            if (de.code != 0) {
+                c->failed = boys_error(ctx, de);
+                return c->failed;
+            }
+        }
+    }
+#undef CHOL_STEP_TRY
+    DevError de{};
+    if (hipMemcpy(&de, c->d_err.p, sizeof de, hipMemcpyDeviceToHost) != hipSuccess)
+        return c->failed = set_error(ctx, GIAOHIP_DEVICE_ERROR, "hipMemcpy");
+    if (de.code != 0) return c->failed = boys_error(ctx, de);
+    c->max_residual = std::max(c->state.value, 0.0);
+    c->converged = c->state.value <= c->tolerance ? 1 : 0;
+    return GIAOHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double threshold,
                          uint64_t max_rank, uint64_t stop_after, giaohip_cholesky** out) {
     if (!ctx || !b || !out) return GIAOHIP_INVALID_ARGUMENT;
@@ -1488,20 +1645,19 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     hipStream_t s = ctx->stream;
     const long long n = b->nao, n2 = n * n;
     const int ns = b->ns;
-    auto* chol = new giaohip_cholesky;
-    chol->n = n;
+    auto* c = new giaohip_cholesky;
+    c->n = n;
+    c->basis = b;
     // Factor rows live in chunks of about 2 GiB (at least one row).
-    chol->rows_per_chunk = (int)std::max<long long>(1, (2LL << 30) / (n2 * (long long)sizeof(zc)));
+    c->rows_per_chunk = (int)std::max<long long>(1, (2LL << 30) / (n2 * (long long)sizeof(zc)));
     auto cleanup = [&](int code) {
-        for (zc* c : chol->chunks) (void)hipFree(c);
-        delete chol;
+        c->release();
+        delete c;
         return code;
     };
 
     // Static bra table: every ordered shell pair, grouped by (la, lb).
     std::vector<gh::CholUnit> units;
-    std::vector<std::array<int, 4>> groups;  // la, lb, first, count
-    int max_lab = 0;
     for (int la = 0; la <= gh::kMaxShellL; ++la)
         for (int lb = 0; lb <= gh::kMaxShellL; ++lb) {
             const int first = (int)units.size();
@@ -1519,8 +1675,8 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
                 }
             }
             if ((int)units.size() > first) {
-                groups.push_back({la, lb, first, (int)units.size() - first});
-                max_lab = std::max(max_lab, la + lb);
+                c->groups.push_back({la, lb, first, (int)units.size() - first});
+                c->max_lab = std::max(c->max_lab, la + lb);
             }
         }
     long long p1 = 0, p2 = 0;
@@ -1530,46 +1686,40 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
         p1 += u.kab;
         p2 += (long long)u.kab * u.lab;
     }
+    c->p1 = p1;
     int max_l = 0, max_k = 1;
     for (const auto& sh : b->shells) {
         max_l = std::max(max_l, sh.l);
         max_k = std::max(max_k, sh.nprim);
     }
-    if (2 * max_l + max_lab > gh::kMaxTotalL || max_l > gh::kMaxShellL)
+    if (2 * max_l + c->max_lab > gh::kMaxTotalL || max_l > gh::kMaxShellL)
         return cleanup(set_error(ctx, GIAOHIP_UNSUPPORTED, "GPU ERI kernels support combined angular order through 24"));
-    const int nunits = (int)units.size();
+    c->nunits = (int)units.size();
     const long long seed_zc = (p2 + p1 * (long long)(2 * max_l + 4)) * (long long)max_k * max_k;
-    const int parts = gh::chol_partials(n2);
-
-    DevBuffer d_units, d_sq, d_wi, d_res, d_col, d_coef, d_pmax, d_pidx, d_state, d_flags, d_chunkptr;
-    auto free_all = [&]() {
-        for (DevBuffer* x : {&d_units, &d_sq, &d_wi, &d_res, &d_col, &d_coef, &d_pmax, &d_pidx, &d_state, &d_flags,
-                             &d_chunkptr})
-            x->release();
-    };
-#define CHOL_TRY(expr, what)                            \
-    do {                                                \
-        hipError_t _e = (expr);                         \
-        if (_e != hipSuccess) {                         \
-            free_all();                                 \
-            return cleanup(hip_fail(ctx, _e, what));    \
-        }                                               \
-    } while (0)
-    CHOL_TRY(d_units.ensure(units.size() * sizeof(gh::CholUnit)), "hipMalloc");
-    CHOL_TRY(hipMemcpy(d_units.p, units.data(), units.size() * sizeof(gh::CholUnit), hipMemcpyHostToDevice), "hipMemcpy");
-    CHOL_TRY(d_sq.ensure(units.size() * sizeof(SeedQuartet)), "hipMalloc");
-    CHOL_TRY(d_wi.ensure(units.size() * sizeof(WorkItem)), "hipMalloc");
-    CHOL_TRY(d_res.ensure(n2 * sizeof(double)), "hipMalloc");
-    CHOL_TRY(d_col.ensure(n2 * sizeof(zc)), "hipMalloc");
-    CHOL_TRY(d_pmax.ensure(parts * sizeof(double)), "hipMalloc");
-    CHOL_TRY(d_pidx.ensure(parts * sizeof(long long)), "hipMalloc");
-    CHOL_TRY(d_state.ensure(sizeof(gh::CholState)), "hipMalloc");
-    CHOL_TRY(d_flags.ensure(sizeof(int)), "hipMalloc");
     const long long scr_zc =
         (long long)((gh::seed_scratch_ints(p1 * (long long)max_k * max_k) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc));
-    CHOL_TRY(ctx->seeds.ensure((size_t)(seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
-    int* d_scratch = reinterpret_cast<int*>(ctx->seeds.as<zc>() + seed_zc);
-    CHOL_TRY(hipMemset(d_flags.p, 0, sizeof(int)), "hipMemset");
+    const int parts = gh::chol_partials(n2);
+#define CHOL_TRY(expr, what)                             \
+    do {                                                 \
+        hipError_t _e = (expr);                          \
+        if (_e != hipSuccess) return cleanup(hip_fail(ctx, _e, what)); \
+    } while (0)
+    CHOL_TRY(c->d_units.ensure(units.size() * sizeof(gh::CholUnit)), "hipMalloc");
+    CHOL_TRY(hipMemcpy(c->d_units.p, units.data(), units.size() * sizeof(gh::CholUnit), hipMemcpyHostToDevice),
+             "hipMemcpy");
+    CHOL_TRY(c->d_sq.ensure(units.size() * sizeof(SeedQuartet)), "hipMalloc");
+    CHOL_TRY(c->d_wi.ensure(units.size() * sizeof(WorkItem)), "hipMalloc");
+    CHOL_TRY(c->d_res.ensure(n2 * sizeof(double)), "hipMalloc");
+    CHOL_TRY(c->d_col.ensure(n2 * sizeof(zc)), "hipMalloc");
+    CHOL_TRY(c->d_pmax.ensure(parts * sizeof(double)), "hipMalloc");
+    CHOL_TRY(c->d_pidx.ensure(parts * sizeof(long long)), "hipMalloc");
+    CHOL_TRY(c->d_state.ensure(sizeof(gh::CholState)), "hipMalloc");
+    CHOL_TRY(c->d_flags.ensure(sizeof(int)), "hipMalloc");
+    CHOL_TRY(c->d_seeds.ensure((size_t)(seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
+    c->d_scratch = reinterpret_cast<int*>(c->d_seeds.as<zc>() + seed_zc);
+    CHOL_TRY(hipMemset(c->d_flags.p, 0, sizeof(int)), "hipMemset");
+    for (auto& e : c->ev) CHOL_TRY(hipEventCreate(&e), "hipEventCreate");
+    CHOL_TRY(hipHostMalloc(&c->host, sizeof(CholHost), hipHostMallocDefault), "hipHostMalloc");
 
     // Diagonal over all ordered shell pairs a <= b (cholesky.py:143-169).
     std::vector<uint64_t> pairs;
@@ -1578,135 +1728,40 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
             pairs.push_back((uint64_t)a);
             pairs.push_back((uint64_t)bb);
         }
-    hipEvent_t e0, e1, e2;
-    CHOL_TRY(hipEventCreate(&e0), "hipEventCreate");
-    CHOL_TRY(hipEventCreate(&e1), "hipEventCreate");
-    CHOL_TRY(hipEventCreate(&e2), "hipEventCreate");
-    CHOL_TRY(hipEventRecord(e0, s), "hipEventRecord");
-    int st = diagonal_to_device(ctx, b, pairs.data(), pairs.size() / 2, field, d_res.as<double>());
-    if (st) {
-        free_all();
-        return cleanup(st);
-    }
-    CHOL_TRY(hipEventRecord(e1, s), "hipEventRecord");
-    CHOL_TRY(gh::launch_chol_argmax(d_res.as<double>(), n2, d_pmax.as<double>(), d_pidx.as<long long>(),
-                                    d_state.as<gh::CholState>(), true, s),
+    CHOL_TRY(hipEventRecord(c->ev[0], s), "hipEventRecord");
+    const int st = diagonal_to_device(ctx, b, pairs.data(), pairs.size() / 2, field, c->d_res.as<double>());
+    if (st) return cleanup(st);
+    CHOL_TRY(hipEventRecord(c->ev[1], s), "hipEventRecord");
+    CHOL_TRY(gh::launch_chol_argmax(c->d_res.as<double>(), n2, c->d_pmax.as<double>(), c->d_pidx.as<long long>(),
+                                    c->d_state.as<gh::CholState>(), true, s),
              "argmax launch");
-    gh::CholState state{};
-    CHOL_TRY(hipMemcpyAsync(&state, d_state.p, sizeof state, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    CHOL_TRY(hipMemcpyAsync(&c->state, c->d_state.p, sizeof c->state, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     CHOL_TRY(hipStreamSynchronize(s), "diagonal");
-    chol->diagonal_ms = elapsed(e0, e1);
-    const double floor = 64.0 * 2.220446049250313e-16 * std::max(state.value, 0.0);
-    const double tol = std::max(threshold, floor);
-    const double psd_tol = std::max(floor * 16.0, 1e-12);
-    chol->tolerance = tol;
-    const uint64_t limit = max_rank ? max_rank : (uint64_t)n2;
+    c->diagonal_ms = elapsed(c->ev[0], c->ev[1]);
+    const double floor = 64.0 * 2.220446049250313e-16 * std::max(c->state.value, 0.0);
+    c->tolerance = std::max(threshold, floor);
+    c->psd_tol = std::max(floor * 16.0, 1e-12);
+    c->limit = max_rank ? max_rank : (uint64_t)n2;
     const std::vector<double> kap = wave_vectors(b, field);
-    CHOL_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
-    CHOL_TRY(hipMemcpy(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
-    CHOL_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
-    CHOL_TRY(hipMemset(ctx->errrec.p, 0, sizeof(DevError)), "hipMemset");
-    struct Host {
-        gh::CholState state;
-        int flags;
-    };
-    Host* host = nullptr;
-    CHOL_TRY(hipHostMalloc((void**)&host, sizeof(Host), hipHostMallocDefault), "hipHostMalloc");
-    int status = GIAOHIP_OK;
-    while (state.value > tol) {
-        if (stop_after && chol->rank >= stop_after) break;
-        if (chol->rank >= limit) {
-            std::ostringstream m;
-            m << "Cholesky ERI output requires at least " << (chol->rank + 1) * (uint64_t)(n2 * 16 + 16)
-              << " bytes, exceeding the requested rank limit " << limit;
-            status = set_error(ctx, GIAOHIP_OUT_OF_MEMORY, m.str());
-            break;
-        }
-        const long long p = state.pivot;
-        const long long mu = p / n, nu = p % n;
-        const int sa = b->owner[mu], sb = b->owner[nu];
-        const DevShell& C = b->shells[sb];
-        const DevShell& D = b->shells[sa];
-        gh::CholKet ket{};
-        ket.sh_c = sb;
-        ket.sh_d = sa;
-        ket.lcd = C.l + D.l;
-        ket.kcd = C.nprim * D.nprim;
-        const long long cl = nu - b->ao_off[sb], dl = mu - b->ao_off[sa];
-        ket.xc = (int)(cl / C.ncart);
-        ket.xd = (int)(dl / D.ncart);
-        ket.comp = (int)(cl % C.ncart) * D.ncart + (int)(dl % D.ncart);
-        ket.n = n;
-        // factor storage for row `rank`
-        const int chunk = (int)(chol->rank / chol->rows_per_chunk);
-        if (chunk >= (int)chol->chunks.size()) {
-            void* mem = nullptr;
-            CHOL_TRY(hipMalloc(&mem, (size_t)chol->rows_per_chunk * n2 * sizeof(zc)), "hipMalloc(factors)");
-            chol->chunks.push_back(static_cast<zc*>(mem));
-            CHOL_TRY(d_chunkptr.ensure(chol->chunks.size() * sizeof(zc*)), "hipMalloc");
-            CHOL_TRY(hipMemcpy(d_chunkptr.p, chol->chunks.data(), chol->chunks.size() * sizeof(zc*),
-                               hipMemcpyHostToDevice),
-                     "hipMemcpy");
-            CHOL_TRY(d_coef.ensure(chol->chunks.size() * chol->rows_per_chunk * sizeof(zc)), "hipMalloc");
-        }
-        zc* row = chol->chunks[chunk] + (long long)(chol->rank % chol->rows_per_chunk) * n2;
-        CHOL_TRY(hipEventRecord(e0, s), "hipEventRecord");
-        CHOL_TRY(gh::launch_chol_expand(d_units.as<gh::CholUnit>(), nunits, ket, d_sq.as<SeedQuartet>(),
-                                        d_wi.as<WorkItem>(), s),
-                 "expand launch");
-        int maxL = max_lab + ket.lcd;
-        CHOL_TRY(gh::launch_seed(nmax_bucket(maxL), d_sq.as<SeedQuartet>(), nunits, p1 * ket.kcd,
-                                 b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
-                                 ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), d_scratch, s),
-                 "seed launch");
-        for (const auto& g : groups) {
-            gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
-            CHOL_TRY(gh::launch_transform(ca, d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
-                                          b->d_exps.as<double>(), b->d_coefs.as<double>(), b->d_norms.as<double>(),
-                                          ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_col.as<zc>(), s),
-                     "transform launch");
-        }
-        CHOL_TRY(hipEventRecord(e1, s), "hipEventRecord");
-        CHOL_TRY(gh::launch_chol_gather(d_chunkptr.as<const zc*>(), chol->rows_per_chunk, n2, (int)chol->rank,
-                                        d_state.as<gh::CholState>(), d_coef.as<zc>(), s),
-                 "gather launch");
-        CHOL_TRY(gh::launch_chol_update(d_col.as<zc>(), d_chunkptr.as<const zc*>(), chol->rows_per_chunk, n2,
-                                        (int)chol->rank, d_coef.as<zc>(), d_state.as<gh::CholState>(),
-                                        d_res.as<double>(), row, psd_tol, d_flags.as<int>(), d_pmax.as<double>(),
-                                        d_pidx.as<long long>(), s),
-                 "update launch");
-        CHOL_TRY(gh::launch_chol_argmax(d_res.as<double>(), n2, d_pmax.as<double>(), d_pidx.as<long long>(),
-                                        d_state.as<gh::CholState>(), false, s),
-                 "argmax launch");
-        CHOL_TRY(hipEventRecord(e2, s), "hipEventRecord");
-        CHOL_TRY(hipMemcpyAsync(&host->state, d_state.p, sizeof(gh::CholState), hipMemcpyDeviceToHost, s), "copy");
-        CHOL_TRY(hipMemcpyAsync(&host->flags, d_flags.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-        CHOL_TRY(hipStreamSynchronize(s), "Cholesky step");
-        chol->column_ms += elapsed(e0, e1);
-        chol->update_ms += elapsed(e1, e2);
-        chol->pivots.push_back(mu);
-        chol->pivots.push_back(nu);
-        chol->rank += 1;
-        if (host->flags & 1) {
-            status = set_error(ctx, GIAOHIP_RUNTIME_ERROR, "Coulomb Cholesky residual lost positive semidefiniteness");
-            break;
-        }
-        state = host->state;
-    }
-    DevError de{};
-    (void)hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost);
-    (void)hipHostFree(host);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
-    free_all();
+    CHOL_TRY(c->d_kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
+    CHOL_TRY(hipMemcpy(c->d_kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    CHOL_TRY(c->d_err.ensure(sizeof(DevError)), "hipMalloc");
+    CHOL_TRY(hipMemset(c->d_err.p, 0, sizeof(DevError)), "hipMemset");
 #undef CHOL_TRY
-    if (de.code != 0) return cleanup(boys_error(ctx, de));
+    const int status = chol_steps(ctx, c, stop_after);
     if (status != GIAOHIP_OK) return cleanup(status);
-    chol->max_residual = std::max(state.value, 0.0);
-    chol->converged = state.value <= tol ? 1 : 0;
-    *out = chol;
+    *out = c;
     return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t stop_after) {
+    if (!ctx || !c) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    if (c->failed) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "this Cholesky run ended with an error");
+    if (c->converged) return GIAOHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    return chol_steps(ctx, c, stop_after);
 }
 
 int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max_residual, double* tolerance,
@@ -1748,7 +1803,7 @@ int giaohip_cholesky_destroy(giaohip_ctx* ctx, giaohip_cholesky* c) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         (void)hipStreamSynchronize(ctx->stream);
     }
-    for (zc* p : c->chunks) (void)hipFree(p);
+    c->release();
     delete c;
     return GIAOHIP_OK;
 }

@@ -174,6 +174,13 @@ int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* basis, const
  * result then reports converged = 0). */
 int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* basis, const double field[6], double threshold,
                          uint64_t max_rank, uint64_t stop_after, giaohip_cholesky** out);
+/* Continues a run stopped by stop_after from its device state until the
+ * stopping test is met or the rank reaches stop_after (0: no limit). The
+ * basis given to giaohip_cholesky_run must still exist. A run that ended with
+ * an error cannot be continued. Replaces the reference driver's pivot loop
+ * (python/giao_integrals/cholesky.py:183-297) continued in steps; it has no
+ * counterpart call in the reference. */
+int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* chol, uint64_t stop_after);
 /* Rank, the measured largest residual diagonal, the stopping tolerance,
  * whether the stopping test was met, and device time split (ms). */
 int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* max_residual, double* tolerance,

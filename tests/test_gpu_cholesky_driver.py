@@ -60,3 +60,25 @@ def test_device_cholesky_rank_cap_and_prefix(gpu, port):
         _native.cholesky_run(db, f, 1e-8, max_rank=5)
     pre = _native.cholesky_run(db, f, 1e-8, stop_after=7)
     assert pre.rank == 7 and not pre.converged
+
+
+def test_device_cholesky_resume_matches_single_run(gpu, port):
+    """A run stopped after a prefix and continued in steps from its device
+    state gives exactly the pivots and factors of an uninterrupted run."""
+    sh, B, O = workloads.cholesky_atoms(natoms=2)
+    b = make_basis(sh.shell_list(), port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    f = _native.field6(B, O)
+    full = _native.cholesky_run(db, f, 1e-8)
+    part = _native.cholesky_run(db, f, 1e-8, stop_after=7)
+    assert part.rank == 7 and not part.converged
+    target = 7
+    while not part.converged:
+        target += 25
+        part.resume(target)
+        assert part.rank == min(target, full.rank)
+    part.resume(target + 10)  # a converged run stays as it is
+    assert part.rank == full.rank and part.converged
+    np.testing.assert_array_equal(part.pivots(), full.pivots())
+    np.testing.assert_array_equal(part.vectors(0, part.rank), full.vectors(0, full.rank))
+    assert part.max_residual == full.max_residual
