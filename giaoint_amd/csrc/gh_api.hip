@@ -1573,11 +1573,21 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                       "seed launch");
         for (const auto& g : c->groups) {
             gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
-            CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
-                                               b->d_exps.as<double>(), b->d_coefs.as<double>(),
-                                               b->d_norms.as<double>(), c->d_kappa.as<double>(),
-                                               c->d_seeds.as<zc>(), c->d_col.as<zc>(), s),
-                          "transform launch");
+            const bool column_kernel = g[0] + g[1] + C.l + D.l <= gh::kColumnMaxL &&
+                                       gh::chol_column_lds_bytes(g[0], g[1], C.l, D.l, ket.comp) <= 64 * 1024;
+            if (column_kernel)
+                CHOL_STEP_TRY(gh::launch_chol_column(ca, ket.comp, c->d_wi.as<WorkItem>() + g[2], g[3],
+                                                     b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
+                                                     b->d_coefs.as<double>(), b->d_norms.as<double>(),
+                                                     c->d_kappa.as<double>(), c->d_seeds.as<zc>(), c->d_col.as<zc>(),
+                                                     s),
+                              "column launch");
+            else
+                CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
+                                                   b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                                                   b->d_norms.as<double>(), c->d_kappa.as<double>(),
+                                                   c->d_seeds.as<zc>(), c->d_col.as<zc>(), s),
+                              "transform launch");
         }
         CHOL_STEP_TRY(hipEventRecord(c->ev[1], s), "hipEventRecord");
         CHOL_STEP_TRY(gh::launch_chol_gather(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,

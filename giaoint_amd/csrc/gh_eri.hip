@@ -45,6 +45,13 @@ __constant__ unsigned c_cart[nherm(kMaxShellL)];
 
 __device__ inline unsigned cart_of(int l, int c) { return c_cart[l * (l + 1) * (l + 2) / 6 + c]; }
 
+// R0 entries of degree <= L sorted by degree (t | u << 6 | v << 12 | lexicographic index << 18),
+// for L = 0..kColumnMaxL, the list of L at offset C(L + 3, 4).
+constexpr int kColMaxL = kColumnMaxL;
+constexpr int kColListSize = (kColMaxL + 1) * (kColMaxL + 2) * (kColMaxL + 3) * (kColMaxL + 4) / 24;
+__constant__ unsigned c_rlist_all[kColListSize];
+__device__ __host__ inline int rlist_offset(int L) { return L * (L + 1) * (L + 2) * (L + 3) / 24; }
+
 // ---------------------------------------------------------------------------
 // Pair data (overlap.cpp:60-91, same operation order)
 // ---------------------------------------------------------------------------
@@ -733,6 +740,20 @@ hipError_t upload_constants() {
         for (int x = l; x >= 0; --x)
             for (int y = l - x; y >= 0; --y) cart[k++] = (unsigned)x | ((unsigned)y << 8) | ((unsigned)(l - x - y) << 16);
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cart), cart, sizeof cart)) != hipSuccess) return e;
+    std::vector<unsigned> rlist(kColListSize);
+    for (int L = 0; L <= kColMaxL; ++L) {
+        unsigned* dst = rlist.data() + rlist_offset(L);
+        int e = 0;
+        for (int d = 0; d <= L; ++d)
+            for (int t = d; t >= 0; --t)
+                for (int u = d - t; u >= 0; --u) {
+                    const int v = d - t - u;
+                    dst[e++] = (unsigned)t | ((unsigned)u << 6) | ((unsigned)v << 12) |
+                               ((unsigned)(lex_off(L, t, u) + v) << 18);
+                }
+    }
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_rlist_all), rlist.data(), rlist.size() * sizeof(unsigned))) != hipSuccess)
+        return e;
     return hipSuccess;
 }
 
@@ -780,6 +801,216 @@ hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nite
     }
     transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), bytes, s>>>(ca, items, shells, exps, coefs, norms, kappa,
                                                                       seeds, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky column kernel: one wavefront per bra shell pair of a column, for
+// the pivot's single ket component (cholesky.cpp:203-282 evaluates the same
+// elements). The ket side needs only that component's 1-D tables and the
+// products over its box; R0 is built level by level in the wavefront's LDS;
+// lanes take bra Hermite indices for V[k] = sum_j E_cd[j] (-1)^|j| R0[k+j],
+// then bra components for out[ab] = sum_k E_ab[k] V[k]. Normalization and
+// contraction coefficients enter as in transform_kernel.
+// LDS (complex): 2 nh(L) + 3 (la+1)(lb+1)(Lab+1) + 3 (Lcd+1) + box + nh(Lab) + nab + L + 4.
+// ---------------------------------------------------------------------------
+__device__ inline void wave_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const WorkItem* __restrict__ items,
+                                                         const DevShell* __restrict__ shells,
+                                                         const double* __restrict__ exps,
+                                                         const double* __restrict__ coefs,
+                                                         const double* __restrict__ norms,
+                                                         const double* __restrict__ kappa,
+                                                         const zc* __restrict__ seeds, zc* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int la = ca.l[0], lb = ca.l[1], lc = ca.l[2], ld = ca.l[3];
+    const int L = la + lb + lc + ld, Lab = la + lb, Lcd = lc + ld;
+    const int NHL = nherm(L), NHab = nherm(Lab);
+    const int lane = threadIdx.x;
+    const WorkItem it = items[blockIdx.x];
+    const DevShell A = shells[it.sh[0]];
+    const DevShell B = shells[it.sh[1]];
+    const DevShell C = shells[it.sh[2]];
+    const DevShell D = shells[it.sh[3]];
+    const int nca = ncart(la), ncb = ncart(lb), ncd = ncart(ld);
+    const int nab = nca * ncb;
+    const int cdcomp = it.cd0;
+    const int cc = cdcomp / ncd, dc = cdcomp % ncd;
+    const unsigned pcp = cart_of(lc, cc), pdp = cart_of(ld, dc);
+    const int cpow[3] = {(int)(pcp & 255), (int)((pcp >> 8) & 255), (int)(pcp >> 16)};
+    const int dpow[3] = {(int)(pdp & 255), (int)((pdp >> 8) & 255), (int)(pdp >> 16)};
+    const int bx = cpow[0] + dpow[0] + 1, by = cpow[1] + dpow[1] + 1, bz = cpow[2] + dpow[2] + 1;
+    const int nbox = bx * by * bz;
+    const int ebra_i = (lb + 1) * (Lab + 1), ebra_a = (la + 1) * ebra_i;
+    zc* buf0 = reinterpret_cast<zc*>(smem_raw);
+    zc* buf1 = buf0 + NHL;
+    zc* ebra = buf1 + NHL;
+    zc* e1d = ebra + 3 * ebra_a;
+    zc* eket = e1d + 3 * (Lcd + 1);
+    zc* vv = eket + nbox;
+    zc* acc = vv + NHab;
+    zc* sseed = acc + nab;
+    const unsigned* rl = c_rlist_all + rlist_offset(L);
+    for (int o = lane; o < nab; o += 64) acc[o] = zc{0.0, 0.0};
+    const int Kb = B.nprim, Kd = D.nprim;
+    const int Kab = A.nprim * Kb, Kcd = C.nprim * Kd;
+    for (int pab = 0; pab < Kab; ++pab) {
+        const int pa = pab / Kb, pb = pab % Kb;
+        const double ea = exps[A.prim_off + pa], eb = exps[B.prim_off + pb];
+        const double p = ea + eb;
+        wave_barrier();
+        for (int e = lane; e < 3 * (la + 1) * (lb + 1); e += 64) {
+            const int axis = e / ((la + 1) * (lb + 1));
+            const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
+            const double Pc = (ea * A.center[axis] + eb * B.center[axis]) / p;
+            const double qv = kappa[3 * it.sh[1] + axis] - kappa[3 * it.sh[0] + axis];
+            build_phase_hermite(i, j, Pc - A.center[axis], Pc - B.center[axis], p, qv,
+                                ebra + axis * ebra_a + i * ebra_i + j * (Lab + 1));
+        }
+        for (int k = lane; k < NHab; k += 64) vv[k] = zc{0.0, 0.0};
+        for (int pcd = 0; pcd < Kcd; ++pcd) {
+            const int pc = pcd / Kd, pd = pcd % Kd;
+            const double ec = exps[C.prim_off + pc], ed = exps[D.prim_off + pd];
+            const double q = ec + ed;
+            wave_barrier();  // the previous primitive quartet is done with the buffers
+            const zc* sp = seeds + it.seed_off + (long long)(pab * Kcd + pcd) * (L + 4);
+            if (lane < L + 4) sseed[lane] = sp[lane];
+            if (lane < 3) {
+                const int axis = lane;
+                const double Qc = (ec * C.center[axis] + ed * D.center[axis]) / q;
+                const double qv = kappa[3 * it.sh[3] + axis] - kappa[3 * it.sh[2] + axis];
+                build_phase_hermite(cpow[axis], dpow[axis], Qc - C.center[axis], Qc - D.center[axis], q, qv,
+                                    e1d + axis * (Lcd + 1));
+            }
+            wave_barrier();
+            for (int e = lane; e < nbox; e += 64) {
+                const int jt = e / (by * bz), ju = (e / bz) % by, jv = e % bz;
+                zc v = zmul(zmul(e1d[jt], e1d[(Lcd + 1) + ju]), e1d[2 * (Lcd + 1) + jv]);
+                if ((jt + ju + jv) & 1) v = zneg(v);
+                eket[e] = v;
+            }
+            // R0 table: level L holds s_L; level n from level n + 1.
+            if (lane == 0) buf0[0] = sseed[3 + L];
+            const zc X0 = sseed[0], X1 = sseed[1], X2 = sseed[2];
+            for (int n = L - 1; n >= 0; --n) {
+                wave_barrier();
+                const zc* cur = ((L - n) & 1) ? buf0 : buf1;
+                zc* nxt = ((L - n) & 1) ? buf1 : buf0;
+                const int cnt = nherm(L - n);
+                for (int e = lane; e < cnt; e += 64) {
+                    const unsigned pk = rl[e];
+                    const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = (int)(pk >> 18);
+                    zc val;
+                    if (t > 0) {
+                        val = zmul(X0, cur[lex_off(L, t - 1, u) + v]);
+                        if (t > 1) {
+                            const zc r2 = cur[lex_off(L, t - 2, u) + v];
+                            val.x += (double)(t - 1) * r2.x;
+                            val.y += (double)(t - 1) * r2.y;
+                        }
+                    } else if (u > 0) {
+                        val = zmul(X1, cur[lex_off(L, 0, u - 1) + v]);
+                        if (u > 1) {
+                            const zc r2 = cur[lex_off(L, 0, u - 2) + v];
+                            val.x += (double)(u - 1) * r2.x;
+                            val.y += (double)(u - 1) * r2.y;
+                        }
+                    } else if (v > 0) {
+                        val = zmul(X2, cur[v - 1]);
+                        if (v > 1) {
+                            const zc r2 = cur[v - 2];
+                            val.x += (double)(v - 1) * r2.x;
+                            val.y += (double)(v - 1) * r2.y;
+                        }
+                    } else {
+                        val = sseed[3 + n];
+                    }
+                    nxt[lex] = val;
+                }
+            }
+            wave_barrier();
+            const zc* R0 = (L & 1) ? buf1 : buf0;
+            const double sc = coefs[C.coef_off + it.x[2] * C.nprim + pc] * norms[C.norm_off + cc * C.nprim + pc] *
+                              coefs[D.coef_off + it.x[3] * D.nprim + pd] * norms[D.norm_off + dc * D.nprim + pd];
+            for (int e = lane; e < NHab; e += 64) {
+                const unsigned pk = c_rlist_all[rlist_offset(Lab) + e];
+                const int kt = pk & 63, ku = (pk >> 6) & 63, kv = (pk >> 12) & 63, k = (int)(pk >> 18);
+                zc s{0.0, 0.0};
+                for (int jt = 0; jt < bx; ++jt)
+                    for (int ju = 0; ju < by; ++ju) {
+                        const zc* rr = R0 + lex_off(L, kt + jt, ku + ju) + kv;
+                        const zc* ee = eket + (jt * by + ju) * bz;
+                        for (int jv = 0; jv < bz; ++jv) zmac(s, ee[jv], rr[jv]);
+                    }
+                vv[k].x = fma(sc, s.x, vv[k].x);
+                vv[k].y = fma(sc, s.y, vv[k].y);
+            }
+        }
+        wave_barrier();
+        for (int o = lane; o < nab; o += 64) {
+            const int cac = o / ncb, cbc = o % ncb;
+            const unsigned pap = cart_of(la, cac), pbp = cart_of(lb, cbc);
+            const int ax = pap & 255, ay = (pap >> 8) & 255, az = pap >> 16;
+            const int bxp = pbp & 255, byp = (pbp >> 8) & 255, bzp = pbp >> 16;
+            const zc* Ex = ebra + 0 * ebra_a + ax * ebra_i + bxp * (Lab + 1);
+            const zc* Ey = ebra + 1 * ebra_a + ay * ebra_i + byp * (Lab + 1);
+            const zc* Ez = ebra + 2 * ebra_a + az * ebra_i + bzp * (Lab + 1);
+            zc s{0.0, 0.0};
+            for (int t = 0; t <= ax + bxp; ++t)
+                for (int u = 0; u <= ay + byp; ++u) {
+                    const zc exy = zmul(Ex[t], Ey[u]);
+                    const zc* ww = vv + lex_off(Lab, t, u);
+                    for (int v = 0; v <= az + bzp; ++v) zmac(s, zmul(exy, Ez[v]), ww[v]);
+                }
+            const double sc = coefs[A.coef_off + it.x[0] * A.nprim + pa] * norms[A.norm_off + cac * A.nprim + pa] *
+                              coefs[B.coef_off + it.x[1] * B.nprim + pb] * norms[B.norm_off + cbc * B.nprim + pb];
+            acc[o].x = fma(sc, s.x, acc[o].x);
+            acc[o].y = fma(sc, s.y, acc[o].y);
+        }
+    }
+    const long long ic = (long long)it.x[2] * ncart(lc) + cc, id = (long long)it.x[3] * ncd + dc;
+    for (int o = lane; o < nab; o += 64) {
+        const int cac = o / ncb, cbc = o % ncb;
+        const long long ia = (long long)it.x[0] * nca + cac, ib = (long long)it.x[1] * ncb + cbc;
+        out[it.out_base + ia * it.stride[0] + ib * it.stride[1] + ic * it.stride[2] + id * it.stride[3]] = acc[o];
+    }
+}
+
+int chol_column_lds_bytes(int la, int lb, int lc, int ld, int comp) {
+    const int L = la + lb + lc + ld, Lab = la + lb, Lcd = lc + ld;
+    const int ncd = ncart(ld), cc = comp / ncd, dc = comp % ncd;
+    int cpow[3], dpow[3];
+    {
+        int k = 0;
+        for (int x = lc; x >= 0; --x)
+            for (int y = lc - x; y >= 0; --y, ++k)
+                if (k == cc) cpow[0] = x, cpow[1] = y, cpow[2] = lc - x - y;
+        k = 0;
+        for (int x = ld; x >= 0; --x)
+            for (int y = ld - x; y >= 0; --y, ++k)
+                if (k == dc) dpow[0] = x, dpow[1] = y, dpow[2] = ld - x - y;
+    }
+    const int nbox = (cpow[0] + dpow[0] + 1) * (cpow[1] + dpow[1] + 1) * (cpow[2] + dpow[2] + 1);
+    const int zcs = 2 * nherm(L) + 3 * (la + 1) * (lb + 1) * (Lab + 1) + 3 * (Lcd + 1) + nbox + nherm(Lab) +
+                    ncart(la) * ncart(lb) + L + 4;
+    return zcs * (int)sizeof(zc);
+}
+
+hipError_t launch_chol_column(const ClassArgs& ca, int comp, const WorkItem* items, int nitems, const DevShell* shells,
+                              const double* exps, const double* coefs, const double* norms, const double* kappa,
+                              const zc* seeds, zc* out, hipStream_t s) {
+    if (nitems == 0) return hipSuccess;
+    const int L = ca.l[0] + ca.l[1] + ca.l[2] + ca.l[3];
+    if (L > kColMaxL) return hipErrorInvalidValue;
+    const int bytes = chol_column_lds_bytes(ca.l[0], ca.l[1], ca.l[2], ca.l[3], comp);
+    if (bytes > 64 * 1024) return hipErrorInvalidValue;
+    chol_column_kernel<<<dim3((unsigned)nitems), dim3(64), bytes, s>>>(ca, items, shells, exps, coefs, norms, kappa,
+                                                                        seeds, out);
     return hipGetLastError();
 }
 

@@ -107,6 +107,15 @@ hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long 
 hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nitems, const DevShell* shells,
                             const double* exps, const double* coefs, const double* norms, const double* kappa,
                             const zc* seeds, zc* out, hipStream_t s);
+// Cholesky column ERIs for one (la, lb) group of bra shell pairs and the
+// pivot's ket class (lc, ld) and component `comp`: one wavefront per pair.
+// Needs la + lb + lc + ld <= kColumnMaxL; `items` as for launch_transform
+// with a one-component ket range.
+constexpr int kColumnMaxL = 16;
+int chol_column_lds_bytes(int la, int lb, int lc, int ld, int comp);
+hipError_t launch_chol_column(const ClassArgs& ca, int comp, const WorkItem* items, int nitems, const DevShell* shells,
+                              const double* exps, const double* coefs, const double* norms, const double* kappa,
+                              const zc* seeds, zc* out, hipStream_t s);
 hipError_t launch_boys_probe(const zc* z, long long n, int nmax, int scaled, zc* out, int* region, double* errest,
                              int* segments, DevError* err, hipStream_t s);
 hipError_t launch_tensor_scatter(const long long* block_off, const int* block_q, int nblocks, long long total,
