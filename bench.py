@@ -4,7 +4,9 @@ One step = one evaluation of the whole synthetic batch of 200,000 (dd|dd)
 Cartesian shell quartets (four private shells per quartet, contraction degree
 3 => 81 primitive quartets each, complex128 output of 4.15 GB kept in HBM).
 Inputs (shell tables) are resident on the device before timing; the output
-is written to device memory. Prints one JSON line.
+is written to device memory. Prints one JSON line. With --cholesky-rank
+the line also carries the streamed Cholesky of the 30-atom set (wall time,
+rank, device time split).
 
 The CPU baseline times GIAOINT's own OpenMP path compiled from its sources
 (oracle/_ref, kind "reference"), or the C restatement when that library is
@@ -27,6 +29,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix (dense)
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per (dd|dd) quartet of fast_transform_kernel<2,2,2,2>, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a copy-kernel
+# calibration (profiles/fast_transform_dddd_200k_pmc.md).
+TRANSFORM_HBM_BYTES_PER_QUARTET = 369_357
+TRANSFORM_TRAFFIC_NOTE = ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), "
+                          "factors from a 256 MiB copy; 369,357 B per (dd|dd) quartet x quartets per launch")
 METRIC = "shell quartets/s + FP64 GFLOP/s vs peak; Cholesky wall time, 1 MI355X"
 
 
@@ -40,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--chunk-mib", type=int, default=0, help="batched-path chunk size (default: library's)")
+    ap.add_argument("--cholesky-rank", type=int, default=-1,
+                    help="Cholesky leg: stop after this many pivots (0: run to convergence, -1: skip the leg)")
+    ap.add_argument("--cholesky-block", type=int, default=0, help="candidates per pivot block (default: library's)")
     return ap.parse_args()
 
 
@@ -85,6 +97,50 @@ def cpu_baseline(batch, B, origin, target_s: float):
     }
 
 
+def cholesky_leg(rank_cap: int, block: int = 0):
+    """Streamed pivoted Cholesky of the 30-atom 4s3p2d1f set (1050 AOs), tau = 1e-8.
+
+    Runs the device driver in steps of 1000 pivots (a progress line after
+    each) until convergence, or until rank_cap pivots when rank_cap > 0.
+    """
+    from giaoint_amd import _native, workloads
+
+    sh, B, O = workloads.cholesky_atoms()
+    coefs = _native.normalize_shells(sh.L, sh.nprim, sh.ncontr, sh.exps, sh.coefs)
+    ctx = _native.Context(0)
+    if block:
+        ctx.set_cholesky_block(block)
+    db = _native.DeviceBasis(ctx, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
+    field = _native.field6(B, O)
+    step = 1000
+    t0 = time.perf_counter()
+    first = step if rank_cap <= 0 else min(step, rank_cap)
+    ch = _native.cholesky_run(db, field, 1e-8, 0, first)
+    while not ch.converged and (rank_cap <= 0 or ch.rank < rank_cap):
+        log(f"cholesky: rank {ch.rank} max residual {ch.max_residual:.3e} {time.perf_counter() - t0:.1f} s")
+        ch.resume(ch.rank + step if rank_cap <= 0 else min(ch.rank + step, rank_cap))
+    wall = time.perf_counter() - t0
+    n = db.nao
+    m = ch.rank
+    t = ch.timings_ms
+    # projection: every pivot reads the earlier factors once (16 B per complex element)
+    proj_bytes = 16.0 * n * n * m * (m - 1) / 2
+    result = {
+        "workload": "30 atoms, min spacing 1.8 bohr in a 14-bohr box (seed 5), uncontracted 4s3p2d1f "
+                    f"(alpha_i = 0.05*3^i), {n} Cartesian AOs, B=(0.05,0,0.1) a.u., tau=1e-8, complex128 factors",
+        "rank": m,
+        "converged": ch.converged,
+        "max_residual": ch.max_residual,
+        "tolerance": ch.tolerance,
+        "wall_s": wall,
+        "device_ms": t,
+        "projection_gbs": proj_bytes / (t["update"] * 1e-3) / 1e9,
+        "projection_frac_hbm": proj_bytes / (t["update"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    ch.close()
+    return result
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -107,6 +163,7 @@ def main():
     out = ctx.alloc(total * 16)
     field = _native.field6(batch.B, batch.origin)
     flops = workloads.batch_flops(sh, batch.quartets)
+    contract_flops = workloads.batch_flops(sh, batch.quartets, part="contract")
     executed = workloads.batch_executed_flops(sh, batch.quartets)
 
     def step():
@@ -137,11 +194,19 @@ def main():
     ms_step = 1e3 * wall / args.steps
     transform_ms = float(np.mean([s["transform_ms"] for s in stats]))
     seed_ms = float(np.mean([s["seed_ms"] for s in stats]))
+    produce_ms = float(np.mean([s["produce_ms"] for s in stats]))
+    contract_ms = float(np.mean([s["contract_ms"] for s in stats]))
+    contract_launches = stats[-1]["contract_launches"]
     launches = stats[-1]["launches"]
     # checksum of the resident output for run-to-run comparison
     probe = np.empty(min(total, 1 << 20), dtype=np.complex128)
     ctx.to_host(out, probe)
-    achieved = flops / (transform_ms * 1e-3) / 1e12
+    # dominant kernel: the E.R.E contraction, timed with events on the stream it runs on
+    avg_launch_ms = contract_ms / contract_launches
+    achieved = contract_flops / contract_launches / (avg_launch_ms * 1e-3) / 1e12
+    traffic = None
+    if TRANSFORM_HBM_BYTES_PER_QUARTET is not None:
+        traffic = TRANSFORM_HBM_BYTES_PER_QUARTET * n / contract_launches
     result = {
         "metric": METRIC,
         "value": n / (ms_step * 1e-3),
@@ -171,20 +236,26 @@ def main():
             "speedup_batched_vs_plain": plain["host_ms"] / float(np.mean([s["host_ms"] for s in stats])),
         },
         "gflops": flops / (ms_step * 1e-3) / 1e9,
-        "kernel_ms": {"seed": seed_ms, "transform": transform_ms, "launches_per_step": launches},
+        "kernel_ms": {"seed": seed_ms, "pair_and_r0": produce_ms, "contract": contract_ms,
+                      "launches_per_step": launches, "contract_launches_per_step": contract_launches},
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": "E.R.E stage: pair_kernel + r0_kernel + fast_transform_kernel<2,2,2,2>",
-            "flop_convention": "12 per R0 entry + 8 per complex MAC of the dense Hermite contraction (per quartet "
-                               f"{flops / n:.4g})",
-            "executed_tflops": executed / (transform_ms * 1e-3) / 1e12,
-            "executed_note": "FLOPs the kernels execute: structural zeros of the Hermite products skipped "
-                             f"(per quartet {executed / n:.4g})",
+            "traffic": traffic,
+            "kernel": "gh::fast::fast_transform_kernel<2,2,2,2> (E.R.E contraction)",
+            "avg_launch_ms": avg_launch_ms,
+            "flops_per_launch": contract_flops / contract_launches,
+            "flop_convention": "8 per complex MAC of the cheaper dense Hermite-to-Cartesian contraction (per quartet "
+                               f"{contract_flops / n:.4g}); the R0 part (12 per entry, {(flops - contract_flops) / n:.4g}"
+                               " per quartet) runs in r0_kernel",
+            "whole_path_tflops": flops / (ms_step * 1e-3) / 1e12,
+            "executed_tflops": (executed - (flops - contract_flops)) / (contract_ms * 1e-3) / 1e12,
+            "executed_note": "complex MACs the contraction executes (structural zeros of the Hermite products "
+                             "skipped), 8 FLOP each",
+            "traffic_note": None if traffic is None else TRANSFORM_TRAFFIC_NOTE,
         },
         "checksum": float(np.abs(probe).sum()),
     }
@@ -195,6 +266,9 @@ def main():
         except Exception as exc:  # the baseline is reported, never required
             result["cpu_baseline"] = {"error": str(exc)}
     ctx.free(out)
+    del dbasis, ctx
+    if args.cholesky_rank >= 0:
+        result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block)
     print(json.dumps(result), flush=True)
 
 

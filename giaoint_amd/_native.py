@@ -49,6 +49,9 @@ class Stats(C.Structure):
         ("total_ms", C.c_double),
         ("launches", C.c_uint64),
         ("host_ms", C.c_double),
+        ("produce_ms", C.c_double),
+        ("contract_ms", C.c_double),
+        ("contract_launches", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -194,6 +197,10 @@ class Context:
     def set_chunk_mib(self, mib: int):
         """Work memory of one batched-path chunk in MiB (16..65536)."""
         self.check(self.lib.giaohip_set_option(self.handle, 1, int(mib)))
+
+    def set_cholesky_block(self, candidates: int):
+        """Candidates per pivot block of later Cholesky runs (1..32; results do not depend on it)."""
+        self.check(self.lib.giaohip_set_option(self.handle, 2, int(candidates)))
 
     def stats(self) -> dict:
         s = Stats()

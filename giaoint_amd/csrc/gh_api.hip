@@ -130,6 +130,7 @@ struct giaohip_ctx {
     DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
     std::set<void*> user_allocs;
     size_t seed_budget_bytes = size_t(2048) << 20;
+    int chol_block = 16;  // candidates per Cholesky pivot block
     size_t staging_budget_bytes = size_t(1) << 30;
 };
 
@@ -154,15 +155,21 @@ struct giaohip_cholesky {
     int nunits = 0, max_lab = 0;
     long long p1 = 0;
     DevBuffer d_units, d_sq, d_wi, d_res, d_col, d_coef, d_pmax, d_pidx, d_state, d_flags, d_chunkptr, d_seeds,
-        d_kappa, d_err;
+        d_kappa, d_err, d_proj, d_cand;
     int* d_scratch = nullptr;
-    hipEvent_t ev[3]{};
+    // pivot block: candidates (host copy), slots, slots not yet taken
+    int block = 1, slots = 1;
+    int ncand = 0;
+    long long cand[gh::kCholMaxBlock]{};
+    unsigned open = 0;
+    uint64_t blocks = 0;
+    hipEvent_t ev[5]{};
     void* host = nullptr;  // page-locked copy of the state and the PSD flag
     void release() {
         for (zc* c : chunks) (void)hipFree(c);
         chunks.clear();
         for (DevBuffer* x : {&d_units, &d_sq, &d_wi, &d_res, &d_col, &d_coef, &d_pmax, &d_pidx, &d_state, &d_flags,
-                             &d_chunkptr, &d_seeds, &d_kappa, &d_err})
+                             &d_chunkptr, &d_seeds, &d_kappa, &d_err, &d_proj, &d_cand})
             x->release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -454,6 +461,8 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
         ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
         ctx->stats.transform_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+        ctx->stats.contract_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+        ctx->stats.contract_launches += launches.size();
         ctx->stats.launches += 1;
         ctx->stats.primitive_quartets += (uint64_t)pq_total;
         DevError de{};
@@ -748,6 +757,7 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
         for (const Launch& L : launches) {
             HIP_TRY(L.fn.second(args(L), sc), "batched transform launch");
             ctx->stats.launches += 1;
+            ctx->stats.contract_launches += 1;
         }
         HIP_TRY(hipEventRecord(event(nchunks, 4), sc), "hipEventRecord");
         ctx->stats.launches += 2;
@@ -758,7 +768,10 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
     HIP_TRY(hipEventSynchronize(event(nchunks - 1, 4)), "kernel execution");
     for (size_t c = 0; c < nchunks; ++c) {
         ctx->stats.seed_ms += elapsed(event(c, 0), event(c, 1));
-        ctx->stats.transform_ms += elapsed(event(c, 1), event(c, 2)) + elapsed(event(c, 3), event(c, 4));
+        const double produce = elapsed(event(c, 1), event(c, 2)), contract = elapsed(event(c, 3), event(c, 4));
+        ctx->stats.transform_ms += produce + contract;
+        ctx->stats.produce_ms += produce;
+        ctx->stats.contract_ms += contract;
     }
     DevError de{};
     HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -986,6 +999,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_CHUNK_MIB && value >= 16 && value <= 65536) {
         ctx->seed_budget_bytes = (size_t)value << 20;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_CHOL_BLOCK && value >= 1 && value <= gh::kCholMaxBlock) {
+        ctx->chol_block = value;
         return GIAOHIP_OK;
     }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");
@@ -1506,12 +1523,17 @@ namespace {
 struct CholHost {
     gh::CholState state;
     int flags;
+    int ncand;
+    long long cand[gh::kCholMaxBlock];
 };
 
 // Pivot steps of a run until the residual is below the tolerance or the rank
-// reaches `target` (0: no target). One column of ERIs, the projection against
-// the earlier factors, the residual update and the next argmax per step; the
-// host reads the pivot back to choose the column's ket class.
+// reaches `target` (0: no target). Per step: when the pivot is not an open
+// candidate of the current block, a new block (top-B candidates and their
+// projections in one pass over the factors); then the pivot's column of ERIs,
+// the accept step (factor row, residual update, projections of the other
+// open candidates) and the next argmax. The host reads the pivot back to
+// choose the column's ket class and to test block membership.
 int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
     const giaohip_basis* b = c->basis;
     hipStream_t s = ctx->stream;
@@ -1535,6 +1557,44 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             return c->failed;
         }
         const long long p = c->state.pivot;
+        int slot = -1;
+        for (int m = 0; m < c->ncand; ++m)
+            if (((c->open >> m) & 1u) && c->cand[m] == p) slot = m;
+        if (slot < 0) {
+            // new block: candidates and their projections onto the rank factors so far
+            CHOL_STEP_TRY(hipEventRecord(c->ev[3], s), "hipEventRecord");
+            CHOL_STEP_TRY(gh::launch_chol_topk(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
+                                               c->d_pidx.as<long long>(), c->block, c->d_cand.as<long long>(),
+                                               reinterpret_cast<int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock),
+                                               s),
+                          "top-k launch");
+            const int* d_ncand = reinterpret_cast<const int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock);
+            CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(c->rank, 1) * c->slots * sizeof(zc)),
+                          "hipMalloc");
+            CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                               c->d_cand.as<long long>(), d_ncand, c->slots, c->d_coef.as<zc>(), s),
+                          "coefficient launch");
+            CHOL_STEP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                                  c->d_coef.as<zc>(), d_ncand, c->slots, c->d_proj.as<zc>(), s),
+                          "projection launch");
+            CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
+            CHOL_STEP_TRY(hipMemcpyAsync(host->cand, c->d_cand.p, sizeof host->cand, hipMemcpyDeviceToHost, s),
+                          "copy");
+            CHOL_STEP_TRY(hipMemcpyAsync(&host->ncand, d_ncand, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+            CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky block");
+            c->update_ms += elapsed(c->ev[3], c->ev[4]);
+            c->ncand = host->ncand;
+            std::copy(host->cand, host->cand + gh::kCholMaxBlock, c->cand);
+            c->open = c->ncand >= 32 ? 0xffffffffu : ((1u << c->ncand) - 1u);
+            c->blocks += 1;
+            for (int m = 0; m < c->ncand; ++m)
+                if (c->cand[m] == p) slot = m;
+            if (slot != 0) {
+                c->failed = set_error(ctx, GIAOHIP_RUNTIME_ERROR,
+                                      "Cholesky pivot block does not start at the maximum residual");
+                return c->failed;
+            }
+        }
         const long long mu = p / n, nu = p % n;
         const int sa = b->owner[mu], sb = b->owner[nu];
         const DevShell& C = b->shells[sb];
@@ -1559,7 +1619,6 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             CHOL_STEP_TRY(hipMemcpy(c->d_chunkptr.p, c->chunks.data(), c->chunks.size() * sizeof(zc*),
                                     hipMemcpyHostToDevice),
                           "hipMemcpy");
-            CHOL_STEP_TRY(c->d_coef.ensure(c->chunks.size() * c->rows_per_chunk * sizeof(zc)), "hipMalloc");
         }
         zc* row = c->chunks[chunk] + (long long)(c->rank % c->rows_per_chunk) * n2;
         CHOL_STEP_TRY(hipEventRecord(c->ev[0], s), "hipEventRecord");
@@ -1590,14 +1649,12 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                               "transform launch");
         }
         CHOL_STEP_TRY(hipEventRecord(c->ev[1], s), "hipEventRecord");
-        CHOL_STEP_TRY(gh::launch_chol_gather(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
-                                             c->d_state.as<gh::CholState>(), c->d_coef.as<zc>(), s),
-                      "gather launch");
-        CHOL_STEP_TRY(gh::launch_chol_update(c->d_col.as<zc>(), c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
-                                             (int)c->rank, c->d_coef.as<zc>(), c->d_state.as<gh::CholState>(),
-                                             c->d_res.as<double>(), row, c->psd_tol, c->d_flags.as<int>(),
-                                             c->d_pmax.as<double>(), c->d_pidx.as<long long>(), s),
-                      "update launch");
+        const unsigned rest = c->open & ~(1u << slot);
+        CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2,
+                                             c->d_proj.as<zc>(), c->d_cand.as<long long>(), rest, n2,
+                                             c->d_state.as<gh::CholState>(), c->d_res.as<double>(), row, c->psd_tol,
+                                             c->d_flags.as<int>(), c->d_pmax.as<double>(), c->d_pidx.as<long long>(), s),
+                      "accept launch");
         CHOL_STEP_TRY(gh::launch_chol_argmax(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
                                              c->d_pidx.as<long long>(), c->d_state.as<gh::CholState>(), false, s),
                       "argmax launch");
@@ -1611,6 +1668,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         c->pivots.push_back(mu);
         c->pivots.push_back(nu);
         c->rank += 1;
+        c->open &= ~(1u << slot);
         if (host->flags & 1) {
             c->failed = set_error(ctx, GIAOHIP_RUNTIME_ERROR, "Coulomb Cholesky residual lost positive semidefiniteness");
             return c->failed;
@@ -1725,6 +1783,10 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     CHOL_TRY(c->d_pidx.ensure(parts * sizeof(long long)), "hipMalloc");
     CHOL_TRY(c->d_state.ensure(sizeof(gh::CholState)), "hipMalloc");
     CHOL_TRY(c->d_flags.ensure(sizeof(int)), "hipMalloc");
+    c->block = ctx->chol_block;
+    c->slots = gh::chol_block_slots(c->block);
+    CHOL_TRY(c->d_proj.ensure((size_t)c->slots * n2 * sizeof(zc)), "hipMalloc(projections)");
+    CHOL_TRY(c->d_cand.ensure(gh::kCholMaxBlock * sizeof(long long) + sizeof(int)), "hipMalloc");
     CHOL_TRY(c->d_seeds.ensure((size_t)(seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
     c->d_scratch = reinterpret_cast<int*>(c->d_seeds.as<zc>() + seed_zc);
     CHOL_TRY(hipMemset(c->d_flags.p, 0, sizeof(int)), "hipMemset");

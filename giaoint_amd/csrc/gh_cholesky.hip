@@ -2,18 +2,32 @@
 // Coulomb matrix W[(mu,nu),(lam,sig)] = (nu mu|lam sig)
 // (python/giao_integrals/cholesky.py:60-297, strict global pivoting).
 //
-// Per pivot p = (mu_p, nu_p):
-//   expand_kernel   writes the seed quartets and transform work items of the
-//                   raw column (mu' nu'|nu_p mu_p) for every bra shell pair from
-//                   a static bra table and the pivot's ket (cholesky.cpp:203-282),
-//                   so no per-pivot host planning is needed;
-//   seed/transform  the ERI kernels of gh_eri.hip (single ket component);
-//   gather_kernel   c_k = conj(L[k][p]) for the k < rank existing factors;
-//   update_kernel   column -= sum_k c_k L[k] (memory-bound GEMV over the
-//                   factor rows), L[rank] = column / sqrt(res[p]),
-//                   res -= |L[rank]|^2 with the PSD check, clamp at 0,
-//                   res[p] = 0, and per-block (max, first index) partials;
-//   argmax kernels  first maximum in flat index order (np.argmax semantics).
+// Pivots are taken in blocks. A block starts at the current maximum of the
+// residual diagonal and names as candidates the B largest residuals in
+// (value desc, index asc) order, which is np.argmax's order, so the first
+// candidate is the pivot itself:
+//   topk_kernel     the B candidates, from the per-block maxima of the last
+//                   residual update (only the blocks holding a top-B maximum
+//                   can hold a top-B entry);
+//   coef_kernel     c[k][m] = conj(L[k][p_m]) for the k < rank factors;
+//   project_kernel  proj[m] = sum_k c[k][m] L[k] for all candidates in one
+//                   pass over the factors (the GEMV of cholesky.py:268 for B
+//                   columns at once: B times fewer factor reads).
+// Every accepted pivot then needs its raw ERI column (expand_kernel writes
+// its seed quartets and work items from a static bra table and the pivot's
+// ket (cholesky.cpp:203-282), the seed and column kernels of gh_eri.hip
+// evaluate it) and accept_kernel: L[rank] = (column - proj) / sqrt(res[p]),
+// res -= |L[rank]|^2 with the PSD check, clamp at 0, res[p] = 0, per-block
+// (max, first index) partials, and proj[m] += conj(L[rank][p_m]) L[rank] for
+// the candidates not yet taken. The argmax kernels finish the first maximum.
+// When that maximum is a candidate not yet taken, its projection is already
+// complete and the next pivot needs no pass over the factors; otherwise a new
+// block starts.
+//
+// Every projection accumulates conj(L[k][p]) L[k] in increasing k with the
+// same fused multiply-adds (cmac), starting from zero, and subtracts the sum
+// from the raw column once. Blocked and one-at-a-time runs therefore produce
+// bitwise identical factors and pivots for every block size.
 #include "gh_internal.h"
 
 namespace gh {
@@ -62,18 +76,9 @@ __global__ void chol_expand_kernel(const CholUnit* __restrict__ units, int nunit
 // ---------------------------------------------------------------------------
 constexpr int kUpdNT = 256;
 constexpr int kUpdPerThread = 2;
-constexpr int kCoefChunk = 1024;
 
 __device__ inline const zc* factor_row(const zc* const* chunks, int rows_per_chunk, long long n2, int k) {
     return chunks[k / rows_per_chunk] + (long long)(k % rows_per_chunk) * n2;
-}
-
-__global__ void chol_gather_kernel(const zc* const* __restrict__ chunks, int rows_per_chunk, long long n2, int rank,
-                                   const CholState* __restrict__ st, zc* __restrict__ coef) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= rank) return;
-    const zc v = factor_row(chunks, rows_per_chunk, n2, k)[st->pivot];
-    coef[k] = zc{v.x, -v.y};
 }
 
 // Block reduction of (value, index): larger value wins, lower index on ties.
@@ -106,77 +111,257 @@ __device__ inline void block_argmax(double v, long long i, double* sv, long long
     }
 }
 
-__global__ __launch_bounds__(kUpdNT) void chol_update_kernel(const zc* __restrict__ column,
-                                                             const zc* const* __restrict__ chunks, int rows_per_chunk,
-                                                             long long n2, int rank, const zc* __restrict__ coef,
-                                                             const CholState* __restrict__ st,
+// acc += c * v with a fixed operation order (shared by every projection).
+__device__ __forceinline__ void cmac(zc& acc, const zc c, const zc v) {
+    acc.x = fma(c.x, v.x, acc.x);
+    acc.x = fma(-c.y, v.y, acc.x);
+    acc.y = fma(c.x, v.y, acc.y);
+    acc.y = fma(c.y, v.x, acc.y);
+}
+
+constexpr int kPartElems = kUpdNT * kUpdPerThread;  // residual entries per partial maximum
+constexpr int kTopNT = 1024;
+
+// (v, i) follows (pv, pi) in (value desc, index asc) order
+__device__ inline bool follows(double v, long long i, double pv, long long pi) {
+    return v < pv || (v == pv && i > pi);
+}
+
+// First maximum over the workgroup; every thread receives it.
+template <int NT>
+__device__ inline void block_first_max(double& v, long long& i, double* sv, long long* si) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const long long i2 = __shfl_xor(i, off);
+        argmax_combine(v, i, v2, i2);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sv[wid] = v;
+        si[wid] = i;
+    }
+    __syncthreads();
+    v = sv[0];
+    i = si[0];
+    for (int w = 1; w < NT / 64; ++w) argmax_combine(v, i, sv[w], si[w]);
+}
+
+// The nb (<= kCholMaxBlock) largest positive residuals in (value desc, index
+// asc) order. An entry among the top nb lies in a partial block whose
+// (max, first index) is among the top nb partials: each partial ranked above
+// it holds a distinct larger entry.
+__global__ __launch_bounds__(kTopNT) void chol_topk_kernel(const double* __restrict__ res, long long n2,
+                                                           const double* __restrict__ pmax,
+                                                           const long long* __restrict__ pidx, int nparts, int nb,
+                                                           long long* __restrict__ cand, int* __restrict__ ncand) {
+    __shared__ double sv[kTopNT / 64];
+    __shared__ long long si[kTopNT / 64];
+    __shared__ long long blocks[kCholMaxBlock];
+    double pv = INFINITY;
+    long long pi = -1;
+    int nblk = 0;
+    for (int t = 0; t < nb; ++t) {
+        double v = -1.0;
+        long long i = 0x7fffffffffffffffLL;
+        for (int b = threadIdx.x; b < nparts; b += kTopNT) {
+            const double bv = pmax[b];
+            const long long bi = pidx[b];
+            if (follows(bv, bi, pv, pi)) argmax_combine(v, i, bv, bi);
+        }
+        block_first_max<kTopNT>(v, i, sv, si);
+        if (!(v > 0.0)) break;
+        if (threadIdx.x == 0) blocks[t] = i / kPartElems;
+        pv = v;
+        pi = i;
+        nblk = t + 1;
+    }
+    __syncthreads();
+    constexpr int kPer = kCholMaxBlock * kPartElems / kTopNT;
+    double ev[kPer];
+    long long ei[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int slot = e * kTopNT + threadIdx.x;
+        const int blk = slot / kPartElems;
+        ev[e] = -1.0;
+        ei[e] = 0x7fffffffffffffffLL;
+        if (blk < nblk) {
+            const long long j = blocks[blk] * kPartElems + slot % kPartElems;
+            if (j < n2) {
+                ev[e] = res[j];
+                ei[e] = j;
+            }
+        }
+    }
+    pv = INFINITY;
+    pi = -1;
+    int count = 0;
+    for (int t = 0; t < nb; ++t) {
+        double v = -1.0;
+        long long i = 0x7fffffffffffffffLL;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e)
+            if (follows(ev[e], ei[e], pv, pi)) argmax_combine(v, i, ev[e], ei[e]);
+        block_first_max<kTopNT>(v, i, sv, si);
+        if (!(v > 0.0)) break;
+        if (threadIdx.x == 0) cand[t] = i;
+        pv = v;
+        pi = i;
+        count = t + 1;
+    }
+    if (threadIdx.x == 0) *ncand = count;
+}
+
+// coef[k][m] = conj(L[k][cand[m]]) for k < rank, m < ncand; zero for the
+// padding slots ncand <= m < nbt.
+__global__ void chol_coef_kernel(const zc* const* __restrict__ chunks, int rows_per_chunk, long long n2, int rank,
+                                 const long long* __restrict__ cand, const int* __restrict__ ncand, int nbt,
+                                 zc* __restrict__ coef) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)rank * nbt) return;
+    const int k = (int)(t / nbt), m = (int)(t % nbt);
+    zc v{0.0, 0.0};
+    if (m < *ncand) {
+        const zc x = factor_row(chunks, rows_per_chunk, n2, k)[cand[m]];
+        v = zc{x.x, -x.y};
+    }
+    coef[t] = v;
+}
+
+// Load through a global-address-space pointer (the factor rows are reached
+// through a table of chunk pointers, which the compiler cannot place).
+typedef __attribute__((address_space(1))) const double global_double;
+__device__ __forceinline__ zc gload(const zc* p) {
+    const global_double* g = (const global_double*)p;
+    return zc{g[0], g[1]};
+}
+
+// proj[m][j] = sum_{k < rank} coef[k][m] L[k][j] for the NB candidate slots;
+// NE entries per thread. The factor rows are read once for all slots, U rows
+// per group with the next group's rows in flight during the current FMAs.
+// Lanes past n2 read entry n2 - 1 and store nothing.
+template <int NB, int NE, int U>
+__global__ __launch_bounds__(256) void chol_project_kernel(const zc* const* __restrict__ chunks, int rows_per_chunk,
+                                                           long long n2, int rank, const zc* __restrict__ coef,
+                                                           const int* __restrict__ ncand, zc* __restrict__ proj) {
+    const long long base = (long long)blockIdx.x * 256 * NE + threadIdx.x;
+    long long jc[NE];
+    zc acc[NB][NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+        const long long j = base + e * 256;
+        jc[e] = j < n2 ? j : n2 - 1;
+#pragma unroll
+        for (int m = 0; m < NB; ++m) acc[m][e] = zc{0.0, 0.0};
+    }
+    auto step = [&](const zc* ck, const zc (&v)[NE]) {
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const zc c = ck[m];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) cmac(acc[m][e], c, v[e]);
+        }
+    };
+    for (int k0 = 0; k0 < rank; k0 += rows_per_chunk) {
+        const zc* chunk = chunks[k0 / rows_per_chunk];
+        const int rows = min(rows_per_chunk, rank - k0);
+        const int full = rows - rows % (2 * U);  // rows taken 2U at a time
+        const zc* ck = coef + (long long)k0 * NB;
+        // rows r..r+U-1 (clamped to the chunk: the loads past the last group
+        // repeat row rows-1 and are not used)
+        auto load = [&](zc (&v)[U][NE], int r) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int rr = min(r + u, rows - 1);
+#pragma unroll
+                for (int e = 0; e < NE; ++e) v[u][e] = gload(chunk + (long long)rr * n2 + jc[e]);
+            }
+        };
+        zc a[U][NE], b[U][NE];
+        load(a, 0);
+        // the scheduling barriers keep each group's loads ahead of the
+        // previous group's FMAs (the scheduler would otherwise sink them)
+        for (int r = 0; r < full; r += 2 * U) {
+            load(b, r + U);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) step(ck + (long long)(r + u) * NB, a[u]);
+            __builtin_amdgcn_sched_barrier(0);
+            load(a, r + 2 * U);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) step(ck + (long long)(r + U + u) * NB, b[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (int r = full; r < rows; ++r) {
+            zc x[NE];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) x[e] = gload(chunk + (long long)r * n2 + jc[e]);
+            step(ck + (long long)r * NB, x);
+        }
+    }
+    const int nc = *ncand;
+#pragma unroll
+    for (int m = 0; m < NB; ++m)
+        if (m < nc)
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+                if (base + e * 256 < n2) proj[(long long)m * n2 + base + e * 256] = acc[m][e];
+}
+
+// Accepts the pivot st->pivot, whose raw column is `column` and whose
+// projection onto the earlier factors is `pproj`: writes the new factor row,
+// updates the residual diagonal (PSD check, clamp, res[p] = 0), the
+// per-block (max, first index) partials and the projections of the candidate
+// slots in `rest` (bit m: slot m at proj + m * n2, pivot cand[m]).
+__global__ __launch_bounds__(kUpdNT) void chol_accept_kernel(const zc* __restrict__ column,
+                                                             const zc* __restrict__ pproj, zc* proj,
+                                                             const long long* __restrict__ cand, unsigned rest,
+                                                             long long n2, const CholState* __restrict__ st,
                                                              double* __restrict__ res, zc* __restrict__ out_row,
                                                              double psd_tol, int* __restrict__ flags,
                                                              double* __restrict__ pmax, long long* __restrict__ pidx) {
-    __shared__ zc sc[kCoefChunk];
     __shared__ double sv[kUpdNT];
     __shared__ long long si[kUpdNT];
     const int tid = threadIdx.x;
-    const long long base = (long long)blockIdx.x * kUpdNT * kUpdPerThread;
-    zc acc[kUpdPerThread];
-    long long j[kUpdPerThread];
-#pragma unroll
-    for (int e = 0; e < kUpdPerThread; ++e) {
-        j[e] = base + e * kUpdNT + tid;
-        acc[e] = j[e] < n2 ? column[j[e]] : zc{0.0, 0.0};
-    }
-    for (int k0 = 0; k0 < rank; k0 += kCoefChunk) {
-        const int cnt = min(kCoefChunk, rank - k0);
-        __syncthreads();
-        for (int k = tid; k < cnt; k += kUpdNT) sc[k] = coef[k0 + k];
-        __syncthreads();
-        int k = 0;
-        for (; k + 4 <= cnt; k += 4) {
-            zc v[4][kUpdPerThread];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const zc* row = factor_row(chunks, rows_per_chunk, n2, k0 + k + kk);
-#pragma unroll
-                for (int e = 0; e < kUpdPerThread; ++e) v[kk][e] = j[e] < n2 ? row[j[e]] : zc{0.0, 0.0};
-            }
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                const zc c = sc[k + kk];
-#pragma unroll
-                for (int e = 0; e < kUpdPerThread; ++e) {
-                    acc[e].x -= c.x * v[kk][e].x - c.y * v[kk][e].y;
-                    acc[e].y -= c.x * v[kk][e].y + c.y * v[kk][e].x;
-                }
-            }
-        }
-        for (; k < cnt; ++k) {
-            const zc* row = factor_row(chunks, rows_per_chunk, n2, k0 + k);
-            const zc c = sc[k];
-#pragma unroll
-            for (int e = 0; e < kUpdPerThread; ++e) {
-                if (j[e] < n2) {
-                    const zc v = row[j[e]];
-                    acc[e].x -= c.x * v.x - c.y * v.y;
-                    acc[e].y -= c.x * v.y + c.y * v.x;
-                }
-            }
-        }
-    }
+    const long long base = (long long)blockIdx.x * kPartElems;
     const double root = sqrt(st->value);
     const long long p = st->pivot;
+    long long j[kUpdPerThread];
+    zc val[kUpdPerThread];
     double best = -1.0;
     long long bidx = 0x7fffffffffffffffLL;
 #pragma unroll
     for (int e = 0; e < kUpdPerThread; ++e) {
+        j[e] = base + e * kUpdNT + tid;
+        val[e] = zc{0.0, 0.0};
         if (j[e] < n2) {
-            const zc val{acc[e].x / root, acc[e].y / root};
-            out_row[j[e]] = val;
-            double nr = res[j[e]] - (val.x * val.x + val.y * val.y);
+            const zc a = column[j[e]], q = pproj[j[e]];
+            val[e] = zc{(a.x - q.x) / root, (a.y - q.y) / root};
+            out_row[j[e]] = val[e];
+            double nr = res[j[e]] - (val[e].x * val[e].x + val[e].y * val[e].y);
             if (nr < -psd_tol) atomicOr(flags, 1);
             nr = nr > 0.0 ? nr : 0.0;
             if (j[e] == p) nr = 0.0;
             res[j[e]] = nr;
             argmax_combine(best, bidx, nr, j[e]);
+        }
+    }
+    for (unsigned mask = rest; mask; mask &= mask - 1) {
+        const int m = __builtin_ctz(mask);
+        const long long pm = cand[m];
+        const zc a = column[pm], q = pproj[pm];
+        const zc c{(a.x - q.x) / root, -((a.y - q.y) / root)};
+        zc* pr = proj + (long long)m * n2;
+#pragma unroll
+        for (int e = 0; e < kUpdPerThread; ++e) {
+            if (j[e] < n2) {
+                zc acc = pr[j[e]];
+                cmac(acc, c, val[e]);
+                pr[j[e]] = acc;
+            }
         }
     }
     block_argmax<kUpdNT>(best, bidx, sv, si, pmax + blockIdx.x, pidx + blockIdx.x);
@@ -226,19 +411,55 @@ hipError_t launch_chol_expand(const CholUnit* units, int nunits, const CholKet& 
     return hipGetLastError();
 }
 
-hipError_t launch_chol_gather(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const CholState* st,
-                              zc* coef, hipStream_t s) {
-    if (rank == 0) return hipSuccess;
-    chol_gather_kernel<<<dim3((unsigned)((rank + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rank, st,
-                                                                                 coef);
+hipError_t launch_chol_topk(const double* res, long long n2, const double* pmax, const long long* pidx, int nb,
+                            long long* cand, int* ncand, hipStream_t s) {
+    if (nb < 1 || nb > kCholMaxBlock) return hipErrorInvalidValue;
+    chol_topk_kernel<<<dim3(1), dim3(kTopNT), 0, s>>>(res, n2, pmax, pidx, chol_partials(n2), nb, cand, ncand);
     return hipGetLastError();
 }
 
-hipError_t launch_chol_update(const zc* column, const zc* const* chunks, int rows_per_chunk, long long n2, int rank,
-                              const zc* coef, const CholState* st, double* res, zc* out_row, double psd_tol,
-                              int* flags, double* pmax, long long* pidx, hipStream_t s) {
-    chol_update_kernel<<<dim3((unsigned)chol_partials(n2)), dim3(kUpdNT), 0, s>>>(
-        column, chunks, rows_per_chunk, n2, rank, coef, st, res, out_row, psd_tol, flags, pmax, pidx);
+hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const long long* cand,
+                            const int* ncand, int nbt, zc* coef, hipStream_t s) {
+    if (rank == 0) return hipSuccess;
+    const long long total = (long long)rank * nbt;
+    chol_coef_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rank, cand,
+                                                                                ncand, nbt, coef);
+    return hipGetLastError();
+}
+
+template <int NB, int NE, int U>
+static hipError_t project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
+                          const int* ncand, zc* proj, hipStream_t s) {
+    const long long per = 256LL * NE;
+    chol_project_kernel<NB, NE, U><<<dim3((unsigned)((n2 + per - 1) / per)), dim3(256), 0, s>>>(
+        chunks, rows_per_chunk, n2, rank, coef, ncand, proj);
+    return hipGetLastError();
+}
+
+int chol_block_slots(int nb) {
+    int t = 1;
+    while (t < nb) t *= 2;
+    return t;
+}
+
+hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
+                               const int* ncand, int nbt, zc* proj, hipStream_t s) {
+    switch (nbt) {
+        case 1: return project<1, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 2: return project<2, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 4: return project<4, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 8: return project<8, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 16: return project<16, 1, 4>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 32: return project<32, 1, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
+                              long long n2, const CholState* st, double* res, zc* out_row, double psd_tol, int* flags,
+                              double* pmax, long long* pidx, hipStream_t s) {
+    chol_accept_kernel<<<dim3((unsigned)chol_partials(n2)), dim3(kUpdNT), 0, s>>>(
+        column, pproj, proj, cand, rest, n2, st, res, out_row, psd_tol, flags, pmax, pidx);
     return hipGetLastError();
 }
 

@@ -33,11 +33,19 @@ struct CholState {
 int chol_partials(long long n2);
 hipError_t launch_chol_expand(const CholUnit* units, int nunits, const CholKet& ket, SeedQuartet* sq, WorkItem* wi,
                               hipStream_t s);
-hipError_t launch_chol_gather(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const CholState* st,
-                              zc* coef, hipStream_t s);
-hipError_t launch_chol_update(const zc* column, const zc* const* chunks, int rows_per_chunk, long long n2, int rank,
-                              const zc* coef, const CholState* st, double* res, zc* out_row, double psd_tol,
-                              int* flags, double* pmax, long long* pidx, hipStream_t s);
+// Pivot blocks (gh_cholesky.hip): at most kCholMaxBlock candidates.
+constexpr int kCholMaxBlock = 32;
+// Candidate slots of a block of nb candidates (nb rounded up to a power of two).
+int chol_block_slots(int nb);
+hipError_t launch_chol_topk(const double* res, long long n2, const double* pmax, const long long* pidx, int nb,
+                            long long* cand, int* ncand, hipStream_t s);
+hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const long long* cand,
+                            const int* ncand, int nbt, zc* coef, hipStream_t s);
+hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
+                               const int* ncand, int nbt, zc* proj, hipStream_t s);
+hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
+                              long long n2, const CholState* st, double* res, zc* out_row, double psd_tol, int* flags,
+                              double* pmax, long long* pidx, hipStream_t s);
 hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
                               bool partial_first, hipStream_t s);
 

@@ -274,15 +274,25 @@ def _r0(L):
     return sum((d + 1) * (d + 2) // 2 * (L - d + 1) for d in range(L + 1))
 
 
-def quartet_flops(la, lb, lc, ld, ka, kb, kc, kd) -> float:
-    """F = 12 K R0(L) + min(bra-deferred, ket-deferred) complex-MAC cost."""
+def quartet_flop_parts(la, lb, lc, ld, ka, kb, kc, kd) -> tuple:
+    """(R0 part, contraction part) of the FLOP convention for one shell quartet.
+
+    R0 part: 12 per R0 entry per primitive quartet. Contraction part: 8 per
+    complex MAC of the cheaper of the bra-deferred and ket-deferred dense
+    Hermite-to-Cartesian contractions.
+    """
     K = ka * kb * kc * kd
     L = la + lb + lc + ld
     nab, ncd = _nc(la) * _nc(lb), _nc(lc) * _nc(ld)
     hab, hcd = _nh(la + lb), _nh(lc + ld)
     a = K * 8 * hab * hcd * ncd + ka * kb * 8 * nab * hab * ncd
     b = K * 8 * hab * hcd * nab + kc * kd * 8 * nab * hcd * ncd
-    return float(K * 12 * _r0(L) + min(a, b))
+    return float(K * 12 * _r0(L)), float(min(a, b))
+
+
+def quartet_flops(la, lb, lc, ld, ka, kb, kc, kd) -> float:
+    """F = 12 K R0(L) + min(bra-deferred, ket-deferred) complex-MAC cost."""
+    return sum(quartet_flop_parts(la, lb, lc, ld, ka, kb, kc, kd))
 
 
 def _cart(l):
@@ -323,10 +333,12 @@ def batch_executed_flops(shells: ShellArrays, quartets: np.ndarray) -> float:
     return float(sum(c * quartet_executed_flops(*map(int, key)) for key, c in zip(keys, counts)))
 
 
-def batch_flops(shells: ShellArrays, quartets: np.ndarray) -> float:
+def batch_flops(shells: ShellArrays, quartets: np.ndarray, part: str = "all") -> float:
+    """Convention FLOPs of a batch; part "all", "r0" or "contract"."""
     q = np.asarray(quartets, dtype=np.int64)
     l = shells.L[q]
     k = shells.nprim[q]
     keys, inv = np.unique(np.concatenate([l, k], axis=1), axis=0, return_inverse=True)
     counts = np.bincount(inv.reshape(-1), minlength=len(keys))
-    return float(sum(c * quartet_flops(*map(int, key)) for key, c in zip(keys, counts)))
+    pick = {"all": lambda r, c: r + c, "r0": lambda r, c: r, "contract": lambda r, c: c}[part]
+    return float(sum(n * pick(*quartet_flop_parts(*map(int, key))) for key, n in zip(keys, counts)))

@@ -57,7 +57,14 @@ enum {
     /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables),
        in MiB, 16..65536; default 2048. Two chunks are in flight at once (one
        being produced, one transformed), so the path holds twice this much. */
-    GIAOHIP_OPTION_CHUNK_MIB = 1
+    GIAOHIP_OPTION_CHUNK_MIB = 1,
+    /* Candidates per pivot block of giaohip_cholesky_run, 1..32; default 16.
+       A block projects its candidates (the largest residuals) onto the
+       earlier factors in one pass; consecutive pivots that are candidates
+       need no further pass. Factors and pivots are bitwise identical for
+       every value; it only trades passes over the factors against the
+       projection work per pass. Read when a run starts. */
+    GIAOHIP_OPTION_CHOL_BLOCK = 2
 };
 
 typedef struct giaohip_ctx giaohip_ctx;
@@ -77,6 +84,14 @@ typedef struct {
     double total_ms;     /* everything on the stream, including copies */
     uint64_t launches;
     double host_ms;      /* wall time of the whole call, validation included */
+    /* Split of transform_ms by kernel. Batched path: produce_ms = pair-table
+     * and R0 kernels on the producer stream, contract_ms = E.R.E contraction
+     * kernels on the context stream (giaohip_stream). Plain path: all of
+     * transform_ms is contract_ms. contract_launches counts the contraction
+     * launches those times cover. */
+    double produce_ms;
+    double contract_ms;
+    uint64_t contract_launches;
 } giaohip_stats;
 
 /* --- context ---------------------------------------------------------- */

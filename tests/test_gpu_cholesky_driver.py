@@ -82,3 +82,51 @@ def test_device_cholesky_resume_matches_single_run(gpu, port):
     np.testing.assert_array_equal(part.pivots(), full.pivots())
     np.testing.assert_array_equal(part.vectors(0, part.rank), full.vectors(0, full.rank))
     assert part.max_residual == full.max_residual
+
+
+def _run_with_block(ctx, db, f, nb, **kw):
+    ctx.set_cholesky_block(nb)
+    try:
+        return _native.cholesky_run(db, f, 1e-8, **kw)
+    finally:
+        ctx.set_cholesky_block(16)
+
+
+def test_device_cholesky_block_sizes_bitwise(gpu, port):
+    """Pivot blocks only decide which candidate projections share a pass over
+    the factors; every projection sums the same terms in the same order.
+    Runs with 1, 3, 16 and 32 candidates per block agree bitwise."""
+    sh, B, O = workloads.cholesky_atoms(natoms=3)
+    b = make_basis(sh.shell_list(), port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    f = _native.field6(B, O)
+    ref = _run_with_block(gpu, db, f, 1)
+    assert ref.converged
+    ref_piv, ref_vec = ref.pivots(), ref.vectors()
+    for nb in (3, 16, 32):
+        got = _run_with_block(gpu, db, f, nb)
+        print(f"block {nb}: rank {got.rank}")
+        assert got.rank == ref.rank and got.converged
+        np.testing.assert_array_equal(got.pivots(), ref_piv)
+        np.testing.assert_array_equal(got.vectors(), ref_vec)
+        assert got.max_residual == ref.max_residual
+    for bad in (0, 33):
+        with pytest.raises(ValueError):
+            gpu.set_cholesky_block(bad)
+
+
+def test_device_cholesky_blocks_across_factor_chunks(gpu, port):
+    """Ten atoms (350 AOs): the factors span two device chunks of 1095 rows,
+    so projections cross a chunk boundary. Blocked (16) and one-at-a-time
+    runs stopped at rank 1400 agree bitwise."""
+    sh, B, O = workloads.cholesky_atoms(natoms=10)
+    b = make_basis(sh.shell_list(), port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    f = _native.field6(B, O)
+    one = _run_with_block(gpu, db, f, 1, stop_after=1400)
+    blk = _run_with_block(gpu, db, f, 16, stop_after=1400)
+    assert one.rank == blk.rank == 1400
+    np.testing.assert_array_equal(blk.pivots(), one.pivots())
+    for first in range(0, 1400, 200):
+        np.testing.assert_array_equal(blk.vectors(first, 200), one.vectors(first, 200))
+    assert blk.max_residual == one.max_residual
