@@ -71,6 +71,8 @@ _SIGNATURES = {
     "giaohip_last_error": ([_vp, C.c_char_p, C.c_size_t], C.c_size_t),
     "giaohip_stream": ([_vp], _vp),
     "giaohip_last_stats": ([_vp, C.POINTER(Stats)], C.c_int),
+    "giaohip_schwarz_bounds": ([_vp, _vp, _dp, _dp], C.c_int),
+    "giaohip_screen_quartets": ([_vp, _vp, _u32p, C.c_size_t, _dp, C.c_double, _u32p, C.POINTER(C.c_size_t)], C.c_int),
     "giaohip_set_option": ([_vp, C.c_int, C.c_int], C.c_int),
     "giaohip_device_alloc": ([_vp, C.c_size_t, C.POINTER(_vp)], C.c_int),
     "giaohip_device_free": ([_vp, _vp], C.c_int),
@@ -312,6 +314,27 @@ class DeviceBasis:
             )
         )
         return out
+
+    def schwarz_bounds(self, field) -> np.ndarray:
+        """Shell-pair Schwarz factors (EriSchwarzBounds values), nshell x nshell."""
+        out = np.empty((self.nshell, self.nshell))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.giaohip_schwarz_bounds(self.ctx.handle, self.handle, _p(f), _p(out)))
+        return out
+
+    def screen(self, quartets, bounds, threshold: float) -> np.ndarray:
+        """Quartets kept by Schwarz screening, in input order (indices validated)."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.uint32).reshape(-1, 4))
+        kept = np.empty_like(q)
+        nkept = C.c_size_t()
+        bp = None if bounds is None else np.ascontiguousarray(bounds, dtype=np.float64)
+        self.ctx.check(
+            self.ctx.lib.giaohip_screen_quartets(
+                self.ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)),
+                None if bp is None else _p(bp), C.c_double(threshold), _p(kept, C.c_uint32), C.byref(nkept),
+            )
+        )
+        return kept[: nkept.value]
 
     def eri_tensor(self, field):
         n = self.nao

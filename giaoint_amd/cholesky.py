@@ -1,0 +1,109 @@
+"""Pivoted Cholesky of the ordered-pair Coulomb matrix, over the HIP library.
+
+Mirrors python/giao_integrals/cholesky.py:16-297 (CholeskyERI, cholesky_eri)
+for the bases whose reference pivot policy is strict global pivoting with one
+column per pivot: multi-centre bases with one contraction per shell in a
+nonzero field (cholesky.py:175-247: every candidate set is the pivot
+itself). Those runs stay on the device end to end (giaohip_cholesky_run).
+Zero-field / single-centre runs (real arithmetic, cholesky.py:175-176) and
+general contractions (contraction-variant candidate blocks and the column
+LRU, cholesky.py:193-262) follow other pivot policies and raise
+NotImplementedError here.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from .core import Basis, MagneticField, _field6, _threads
+
+
+@dataclass(frozen=True)
+class CholeskyERI:
+    """Coulomb factors: (mu nu|lam sig) ~ sum_P L[P,mu,nu] conj(L[P,sig,lam]).
+
+    vectors complex128 (rank, nao, nao); pivots int64 (rank, 2) AO pairs;
+    threshold as requested; max_residual_diagonal the measured largest
+    remaining diagonal of W[(mu,nu),(lam,sig)] = (nu mu|lam sig).
+    """
+
+    vectors: np.ndarray
+    pivots: np.ndarray
+    threshold: float
+    max_residual_diagonal: float
+
+    def reconstruct(self, slices=None) -> np.ndarray:
+        """Four AO slices (mu, nu, lam, sig) of the reconstructed ERIs (cholesky.py:39-57)."""
+        if slices is None:
+            slices = (slice(None),) * 4
+        if len(slices) != 4 or not all(isinstance(s, slice) for s in slices):
+            raise ValueError("slices must contain four slices")
+        a, b, c, d = slices
+        left = self.vectors[:, a, b]
+        right = self.vectors[:, d, c].conj()
+        m, n = left.shape[1:], right.shape[1:]
+        out = left.reshape(len(left), -1).T @ right.reshape(len(right), -1)
+        # right[P, s, l] -> (mu, nu, l, s) order
+        return out.reshape(*m, *n).transpose(0, 1, 3, 2)
+
+
+def _nonnegative(name: str, value) -> float:
+    v = float(value)
+    if not np.isfinite(v) or v < 0:
+        raise ValueError(f"{name} must be finite and non-negative")
+    return v
+
+
+def cholesky_eri(basis: Basis, *, field: MagneticField | None = None, threshold: float = 1e-8,
+                 screening_threshold: float = 0.0, threads: int | None = None, max_bytes: int | None = None,
+                 pivot_batch_size: int = 32) -> CholeskyERI:
+    """Factor the ERIs without forming the AO tensor (cholesky.py:60-297).
+
+    Stops when the largest residual diagonal is <= max(threshold,
+    64 eps max(initial diagonal)). screening_threshold only lets the
+    reference skip initial diagonal blocks whose bound is already below the
+    threshold; such entries can never be pivots, so the device run (which
+    measures every diagonal block) takes the same pivots. max_bytes caps the
+    returned vectors and pivot indices (MemoryError beyond it).
+    """
+    threshold = _nonnegative("threshold", threshold)
+    _nonnegative("screening_threshold", screening_threshold)
+    _threads(threads)
+    if isinstance(pivot_batch_size, (bool, np.bool_)) or not isinstance(pivot_batch_size, (int, np.integer)) \
+            or pivot_batch_size < 1:
+        raise ValueError("pivot_batch_size must be a positive integer")
+    if max_bytes is not None and (isinstance(max_bytes, (bool, np.bool_))
+                                  or not isinstance(max_bytes, (int, np.integer)) or max_bytes < 0):
+        raise ValueError("max_bytes must be a non-negative integer or None")
+    shells = basis.shells
+    n = basis.ao_count
+    single_center = all(s.center == shells[0].center for s in shells)
+    real = field is None or all(x == 0 for x in field.B) or single_center
+    if real:
+        raise NotImplementedError("the device Cholesky covers complex (nonzero-field, multi-centre) factors")
+    if any(s.contraction_count != 1 for s in shells):
+        raise NotImplementedError("the device Cholesky covers bases with one contraction per shell")
+    row_bytes = n * n * 16 + 16
+    limit = n * n if max_bytes is None else min(n * n, int(max_bytes) // row_bytes)
+    dev = basis._dev()
+    f = _field6(field)
+
+    def over(rank):
+        return MemoryError(f"Cholesky ERI output requires at least {(rank + 1) * row_bytes} bytes, "
+                           f"exceeding max_bytes={max_bytes}")
+
+    if limit == 0:
+        probe = _native.cholesky_run(dev, f, threshold, 0, 1)
+        if probe.rank:
+            raise over(0)
+        run = probe
+    else:
+        try:
+            run = _native.cholesky_run(dev, f, threshold, limit if max_bytes is not None else 0)
+        except MemoryError:
+            raise over(limit) from None
+    vectors = run.vectors() if run.rank else np.zeros((0, n, n), dtype=np.complex128)
+    return CholeskyERI(vectors, run.pivots().astype(np.int64).reshape(-1, 2), threshold, float(run.max_residual))

@@ -13,6 +13,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -1324,6 +1325,64 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return st;
+}
+
+int giaohip_schwarz_bounds(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out) {
+    if (!ctx || !b || !out) return GIAOHIP_INVALID_ARGUMENT;
+    const int ns = b->ns;
+    std::vector<uint32_t> q;
+    q.reserve((size_t)ns * (ns + 1) * 2);
+    for (int a = 0; a < ns; ++a)
+        for (int c = a; c < ns; ++c) q.insert(q.end(), {(uint32_t)a, (uint32_t)c, (uint32_t)c, (uint32_t)a});
+    const size_t nq = q.size() / 4;
+    std::vector<uint64_t> offs(nq + 1, 0);
+    for (size_t i = 0; i < nq; ++i) {
+        const uint64_t na = b->shells[q[4 * i]].ao_count, nb = b->shells[q[4 * i + 1]].ao_count;
+        offs[i + 1] = offs[i] + na * nb * nb * na;
+    }
+    std::vector<double> blocks(2 * std::max<uint64_t>(offs[nq], 1));
+    const int st = giaohip_eri_batch(ctx, b, q.data(), nq, field, offs.data(), blocks.data(), 0);
+    if (st != GIAOHIP_OK) return st;
+    constexpr double safety = 1.0 + 64.0 * 2.220446049250313e-16;
+    parallel_for(nq, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const int a = (int)q[4 * i], c = (int)q[4 * i + 1];
+            const uint64_t na = b->shells[a].ao_count, nb = b->shells[c].ao_count;
+            const double* blk = blocks.data() + 2 * offs[i];
+            double m = 0.0;
+            for (uint64_t ia = 0; ia < na; ++ia)
+                for (uint64_t ib = 0; ib < nb; ++ib) {
+                    const uint64_t e = ((ia * nb + ib) * nb + ib) * na + ia;
+                    m = std::max(m, std::hypot(blk[2 * e], blk[2 * e + 1]));
+                }
+            const double bound = std::nextafter(std::sqrt(m) * safety, std::numeric_limits<double>::infinity());
+            out[(size_t)a * ns + c] = bound;
+            out[(size_t)c * ns + a] = bound;
+        }
+    });
+    return GIAOHIP_OK;
+}
+
+int giaohip_screen_quartets(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
+                            const double* bounds, double threshold, uint32_t* kept, size_t* nkept) {
+    if (!ctx || !b || !nkept || (n && (!quartets || !kept))) return GIAOHIP_INVALID_ARGUMENT;
+    if (!std::isfinite(threshold) || threshold < 0.0)
+        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "ERI screening threshold must be finite and non-negative");
+    if (threshold > 0.0 && !bounds)
+        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "positive ERI screening requires Schwarz bounds");
+    const uint32_t ns = (uint32_t)b->ns;
+    for (size_t i = 0; i < 4 * n; ++i)
+        if (quartets[i] >= ns) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "ERI shell-quartet index is out of range");
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t* x = quartets + 4 * i;
+        if (threshold > 0.0 && bounds[(size_t)x[0] * ns + x[1]] * bounds[(size_t)x[2] * ns + x[3]] < threshold)
+            continue;
+        if (kept + 4 * m != x) std::copy(x, x + 4, kept + 4 * m);
+        ++m;
+    }
+    *nkept = m;
+    return GIAOHIP_OK;
 }
 
 int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out, int out_on_device) {

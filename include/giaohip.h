@@ -149,6 +149,18 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32
 /* Size in complex elements of the packed output of a quartet list. */
 int giaohip_eri_batch_size(const giaohip_basis* basis, const uint32_t* quartets, size_t n, uint64_t* total);
 
+/* Replaces EriSchwarzBounds (eri.cpp:587-620, decl. eri.hpp:83-101): out =
+ * nshell*nshell doubles (host), out[a*nshell+b] = out[b*nshell+a] = the next
+ * double above sqrt(max |(a_i b_j|b_j a_i)|) * (1 + 64 eps). */
+int giaohip_schwarz_bounds(giaohip_ctx* ctx, const giaohip_basis* basis, const double field[6], double* out);
+
+/* Screening step of evaluate_eri_shell_quartets (eri.cpp:652-688): checks the
+ * threshold (finite, >= 0; > 0 needs bounds) and every shell index, then
+ * copies the quartets with bounds[a,b]*bounds[c,d] >= threshold to kept (in
+ * input order; kept may alias quartets) and their count to nkept. Host only. */
+int giaohip_screen_quartets(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32_t* quartets, size_t n,
+                            const double* bounds, double threshold, uint32_t* kept, size_t* nkept);
+
 /* Replaces compute_eri_tensor (eri.cpp:809-847, decl. eri.hpp:137-143):
  * the full (n,n,n,n) complex128 tensor from canonical quartets and their
  * four symmetry images. */

@@ -808,6 +808,45 @@ int or_eri_batch(const or_basis* b, const uint32_t* quartets, size_t n, const do
     return OR_OK;
 }
 
+/* EriSchwarzBounds (eri.cpp:587-620): Q[a][b] = Q[b][a] = the next double
+ * above sqrt(max |(a_i b_j | b_j a_i)|) * (1 + 64 eps), shell pairs a <= b. */
+int or_schwarz_bounds(const or_basis* b, const double field[6], double* out) {
+    const int ns = b->nshell;
+    shell_view* sv;
+    int st = make_all_shells(b, &sv);
+    if (st) return st;
+    workspace w;
+    memset(&w, 0, sizeof w);
+    cplx* block = NULL;
+    size_t cap = 0;
+    const double safety = 1.0 + 64.0 * 2.220446049250313e-16;
+    for (int a = 0; a < ns && !st; ++a) {
+        for (int c = a; c < ns; ++c) {
+            const uint32_t q[4] = {(uint32_t)a, (uint32_t)c, (uint32_t)c, (uint32_t)a};
+            const size_t size = or_quartet_size(b, q);
+            block = (cplx*)grow(block, &cap, size, sizeof(cplx));
+            const shell_view* sh[4] = {&sv[a], &sv[c], &sv[c], &sv[a]};
+            st = compute_eri_shell(sh, field, block, &w);
+            if (st) break;
+            const size_t na = shell_ao_count(b, a), nb = shell_ao_count(b, c);
+            double m = 0.0;
+            for (size_t i = 0; i < na; ++i)
+                for (size_t j = 0; j < nb; ++j) {
+                    const cplx v = block[((i * nb + j) * nb + j) * na + i];
+                    const double r = cabs(v);
+                    if (r > m) m = r;
+                }
+            const double bound = nextafter(sqrt(m) * safety, INFINITY);
+            out[(size_t)a * ns + c] = bound;
+            out[(size_t)c * ns + a] = bound;
+        }
+    }
+    free(block);
+    ws_free(&w);
+    free_all_shells(b, sv);
+    return st;
+}
+
 /* eri.cpp:414-429 */
 int or_canonicalize(const uint32_t in[4], uint32_t out[4]) {
     const uint32_t cand[4][4] = {{in[0], in[1], in[2], in[3]},

@@ -93,6 +93,9 @@ int or_eri_tensor(const or_basis* basis, const double field[6], double* out,
 /* canonicalize_shell_quartet (eri.cpp:414-429). Returns conjugate flag. */
 int or_canonicalize(const uint32_t in[4], uint32_t out[4]);
 
+/* EriSchwarzBounds (eri.cpp:587-620): out = nshell*nshell doubles. */
+int or_schwarz_bounds(const or_basis* basis, const double field[6], double* out);
+
 /* cholesky.cpp:285-314. pairs: npairs*(a,b) with a<=b. out: n*n doubles. */
 int or_cholesky_diagonal(const or_basis* basis, const uint64_t* pairs,
                          size_t npairs, const double field[6], double* out,

@@ -173,6 +173,13 @@ class Checker:
         )
         return out[: 2 * int(offsets[-1])].view(np.complex128), offsets
 
+    def schwarz_bounds(self, basis: BasisArrays, field):
+        ns = len(basis.L)
+        out = np.zeros((ns, ns))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self._check(self._fn("schwarz_bounds")(C.byref(basis.struct), _ptr(f), _ptr(out)))
+        return out
+
     def eri_tensor(self, basis: BasisArrays, field, threads=1):
         n = basis.nao
         out = np.zeros(2 * n**4)

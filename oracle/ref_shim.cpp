@@ -143,6 +143,15 @@ int ref_eri_batch(const RefBasis* b, const std::uint32_t* quartets, std::size_t 
     });
 }
 
+int ref_schwarz_bounds(const RefBasis* b, const double field[6], double* out) {
+    return guarded([&] {
+        const auto basis = make_basis(b);
+        const giao::EriSchwarzBounds bounds(basis, make_field(field));
+        const auto v = bounds.values();
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
 int ref_eri_tensor(const RefBasis* b, const double field[6], double* out, int threads) {
     return guarded([&] {
         const auto basis = make_basis(b);

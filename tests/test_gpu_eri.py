@@ -151,3 +151,38 @@ def test_dddd_batch_sample_and_symmetries():
     b = make_basis(sub.shell_list(), port)
     ref, _ = port.eri_batch(b, np.arange(4 * len(pick)).reshape(-1, 4), f, threads=16)
     assert np.max(np.abs(a[pick].reshape(-1) - ref)) <= ATOL
+
+
+def test_schwarz_bounds_and_screening(gpu, port):
+    """Device Schwarz factors against the oracle's (eri.cpp:587-620), and the
+    screening step of evaluate_eri_shell_quartets (eri.cpp:652-688): kept
+    quartets in input order, validation before screening."""
+    rng = np.random.default_rng(11)
+    shells = random_shells(rng, [0, 1, 2, 3, 1, 0, 2])
+    b = make_basis(shells, port)
+    db = device_basis(gpu, b)
+    for f in (port.field6(*FIELD), port.field6()):
+        Q = db.schwarz_bounds(f)
+        Qp = port.schwarz_bounds(b, f)
+        np.testing.assert_allclose(Q, Qp, rtol=1e-12, atol=0)
+        assert np.array_equal(Q, Q.T)
+    ns = len(shells)
+    qs = np.array([(a, c, d, e) for a in range(ns) for c in range(ns) for d in range(ns) for e in range(ns)],
+                  dtype=np.uint32)
+    prod = Qp[qs[:, 0], qs[:, 1]] * Qp[qs[:, 2], qs[:, 3]]
+    thr = float(np.median(prod))
+    kept = db.screen(qs, Qp, thr)
+    np.testing.assert_array_equal(kept, qs[prod >= thr])
+    assert 0 < len(kept) < len(qs)
+    np.testing.assert_array_equal(db.screen(qs, None, 0.0), qs)
+    with pytest.raises(IndexError):
+        db.screen(np.array([[0, 0, 0, ns]]), Qp, thr)
+    with pytest.raises(ValueError, match="finite and non-negative"):
+        db.screen(qs, Qp, -1.0)
+    with pytest.raises(ValueError, match="requires Schwarz bounds"):
+        db.screen(qs, None, 1e-8)
+    # the kept quartets evaluate as usual
+    f = port.field6(*FIELD)
+    got = db.eri_batch(kept[:40], f)
+    want, _ = port.eri_batch(b, kept[:40], f, 4)
+    assert np.max(np.abs(got - want)) <= ATOL

@@ -110,3 +110,24 @@ def test_tensor_symmetries_and_cholesky_slices(port, rng):
     piv = [(int(off[2]) + 1, int(off[1]))]
     col = port.cholesky_columns(basis, piv, f, threads=2)[0].reshape(n, n)
     np.testing.assert_allclose(col, T[:, :, piv[0][1], piv[0][0]], atol=1e-13)
+
+
+def test_schwarz_bounds_bound_every_block(port, rng):
+    """|(ab|cd)| <= Q_ab Q_cd elementwise (the Cauchy-Schwarz inequality for
+    the Coulomb form), Q symmetric, and Q_ab^2 at least the largest
+    |(a_i b_j|b_j a_i)| (eri.cpp:587-620; test_eri_api.py:176-185)."""
+    from tests.conftest import random_shells
+
+    shells = random_shells(rng, [0, 1, 2, 1, 0])
+    basis = make_basis(shells, port)
+    f = port.field6((0.1, -0.3, 0.25), (0.2, 0.1, -0.1))
+    Q = port.schwarz_bounds(basis, f)
+    ns = len(shells)
+    assert np.array_equal(Q, Q.T)
+    qs = [(a, b, c, d) for a in range(ns) for b in range(ns) for c in range(ns) for d in range(ns)]
+    vals, offs = port.eri_batch(basis, qs, f, 4)
+    for i, (a, b, c, d) in enumerate(qs):
+        blk = np.abs(vals[offs[i]:offs[i + 1]])
+        assert blk.max() <= Q[a, b] * Q[c, d]
+        if c == b and d == a:
+            assert Q[a, b] ** 2 >= blk.max()

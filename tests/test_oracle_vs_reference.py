@@ -45,3 +45,10 @@ def test_blocks_tensor_cholesky_bitwise(port, reference, rng):
     assert np.array_equal(
         port.cholesky_columns(small, piv, z, 4, real=True), reference.cholesky_columns(small, piv, z, 4, real=True)
     )
+
+
+def test_schwarz_bounds_bitwise(port, reference, rng):
+    shells = random_shells(rng, [0, 1, 2, 3, 1, 0])
+    basis = make_basis(shells, port)
+    for f in (port.field6(*F), port.field6()):
+        assert np.array_equal(port.schwarz_bounds(basis, f), reference.schwarz_bounds(basis, f))
