@@ -1,0 +1,163 @@
+// Driver for the C++ host layer tests (include/giao_hip.hpp).
+//   api_check host <dir>   checks that need no GPU
+//   api_check gpu <dir>    evaluates the integrals of <dir>/shells.txt and
+//                          writes raw complex128/float64 outputs to <dir>
+// shells.txt: one shell per line: cx cy cz l nprim ncontr exps... coefs...
+// Prints one "name value" line per result; exceptions print their type.
+#include "giao_hip.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <typeinfo>
+
+using namespace giao;
+
+template <class F>
+static void expect(const char* name, F&& f) {
+    try {
+        f();
+        std::printf("%s ok\n", name);
+    } catch (const BoysNumericalError& e) {
+        std::printf("%s BoysNumericalError %s\n", name, e.what());
+    } catch (const std::invalid_argument& e) {
+        std::printf("%s invalid_argument %s\n", name, e.what());
+    } catch (const std::length_error& e) {
+        std::printf("%s length_error %s\n", name, e.what());
+    } catch (const std::out_of_range& e) {
+        std::printf("%s out_of_range %s\n", name, e.what());
+    } catch (const std::runtime_error& e) {
+        std::printf("%s runtime_error %s\n", name, e.what());
+    }
+}
+
+static std::vector<Shell> read_shells(const std::string& path) {
+    std::ifstream in(path);
+    std::vector<Shell> out;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream s(line);
+        double cx, cy, cz;
+        int l, np, nc;
+        if (!(s >> cx >> cy >> cz >> l >> np >> nc)) continue;
+        std::vector<double> e(np), c(np * nc);
+        for (auto& x : e) s >> x;
+        for (auto& x : c) s >> x;
+        out.emplace_back(Vec3{cx, cy, cz}, static_cast<std::uint16_t>(l), e, c, static_cast<std::size_t>(nc));
+    }
+    return out;
+}
+
+template <class T>
+static void dump(const std::string& path, const std::vector<T>& v) {
+    std::ofstream(path, std::ios::binary).write(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(T));
+}
+
+static void consume(const ShellQuartetBlockView& v, void* user) {
+    auto* out = static_cast<std::vector<Complex>*>(user);
+    out->insert(out->end(), v.values.begin(), v.values.end());
+}
+
+int main(int argc, char** argv) {
+    if (argc < 3) return 2;
+    const std::string mode = argv[1], dir = argv[2];
+    if (mode == "host") {
+        expect("shell_count", [] { Shell(Vec3{}, 1, {1.0, 2.0}, {1.0}); });
+        expect("shell_exponent", [] { Shell(Vec3{}, 0, {-1.0}, {1.0}); });
+        expect("primitive_exponent", [] { PrimitiveGaussian(0.0, Vec3{}); });
+        const auto k = canonicalize_shell_quartet({3, 1, 0, 2});
+        std::printf("canonical %u %u %u %u %d\n", k.shells.a, k.shells.b, k.shells.c, k.shells.d, int(k.conjugate));
+        const Shell d(Vec3{}, 2, {1.0, 0.5}, {0.3, 0.7, 1.0, -1.0}, 2);
+        std::printf("shell %zu %zu %zu %zu\n", d.primitive_count(), d.contraction_count(), d.cartesian_count(),
+                    d.ao_count());
+        const Vec3 kappa = MagneticField({0.1, -0.2, 0.3}, {1.0, 0.0, -1.0}).london_wave_vector({0.5, 2.0, -3.0});
+        std::printf("kappa %.17g %.17g %.17g\n", kappa.x, kappa.y, kappa.z);
+        expect("boys_without_gpu", [] { (void)boys_values({1.0, 0.5}, 4); });
+        return 0;
+    }
+    const Basis basis(read_shells(dir + "/shells.txt"));
+    const MagneticField field({0.1, -0.3, 0.25}, {0.2, 0.1, -0.1});
+    const auto sh = basis.shells();
+    // one block through compute_eri
+    std::vector<Complex> block(shell_quartet_size(sh[0], sh[1], sh[2], sh[3]));
+    EriWorkspace ws;
+    compute_eri(sh[0], sh[1], sh[2], sh[3], field, block, ws);
+    dump(dir + "/block.bin", block);
+    // every quartet through evaluate_eri_shell_quartets, with and without screening
+    const std::uint32_t ns = static_cast<std::uint32_t>(sh.size());
+    std::vector<ShellQuartetIndex> all;
+    for (std::uint32_t a = 0; a < ns; ++a)
+        for (std::uint32_t b = 0; b < ns; ++b)
+            for (std::uint32_t c = 0; c < ns; ++c)
+                for (std::uint32_t d = 0; d < ns; ++d) all.push_back({a, b, c, d});
+    std::vector<Complex> packed;
+    auto st = evaluate_eri_shell_quartets(basis, all, field, {0.0, 4}, nullptr, consume, &packed);
+    std::printf("stats %zu %zu %zu\n", st.requested_quartets, st.computed_quartets, st.screened_quartets);
+    dump(dir + "/packed.bin", packed);
+    const EriSchwarzBounds bounds(basis, field);
+    dump(dir + "/bounds.bin", std::vector<double>(bounds.values().begin(), bounds.values().end()));
+    double thr = 0.0;
+    {
+        std::vector<double> prods;
+        for (const auto& q : all) prods.push_back(bounds(q.a, q.b) * bounds(q.c, q.d));
+        std::nth_element(prods.begin(), prods.begin() + prods.size() / 2, prods.end());
+        thr = prods[prods.size() / 2];
+    }
+    std::vector<Complex> screened;
+    st = evaluate_eri_shell_quartets(basis, all, field, {thr, 1}, &bounds, consume, &screened);
+    std::printf("screened %.17g %zu %zu %zu %zu\n", thr, st.requested_quartets, st.computed_quartets,
+                st.screened_quartets, screened.size());
+    // tensors
+    const std::size_t n = basis.ao_count();
+    std::vector<Complex> tensor(n * n * n * n), tensor2(n * n * n * n);
+    compute_eri_tensor(basis, field, tensor);
+    dump(dir + "/tensor.bin", tensor);
+    st = compute_eri_tensor(basis, field, {0.0, 1}, nullptr, tensor2);
+    dump(dir + "/tensor2.bin", tensor2);
+    std::printf("tensor_stats %zu %zu\n", st.requested_quartets, st.computed_quartets);
+    // Boys and one primitive integral
+    std::vector<Complex> f(9);
+    const auto diag = compute_boys({20.0, -35.0}, f);
+    dump(dir + "/boys.bin", f);
+    std::printf("boys_region %d\n", int(diag.region));
+    const Complex p = primitive_eri(PrimitiveGaussian(0.8, {0.0, -0.2, 0.0}, {0, 0, 1}),
+                                    PrimitiveGaussian(0.9, {0.1, -0.2, 0.3}, {1, 0, 1}),
+                                    PrimitiveGaussian(1.0, {0.2, -0.2, 0.6}, {0, 0, 1}),
+                                    PrimitiveGaussian(1.1, {0.3, -0.2, 0.9}, {1, 0, 1}), field);
+    std::printf("primitive %.17g %.17g\n", p.real(), p.imag());
+    // selected Cholesky integrals
+    std::vector<std::array<std::size_t, 2>> pairs;
+    for (std::size_t a = 0; a < ns; ++a)
+        for (std::size_t b = a; b < ns; ++b) pairs.push_back({a, b});
+    std::vector<double> d(n * n);
+    detail::cholesky_diagonal(basis, pairs, field, d, 4);
+    dump(dir + "/diag.bin", d);
+    const std::array<std::size_t, 2> piv{1, 0};
+    std::vector<Complex> col(n * n);
+    detail::cholesky_columns(basis, std::span(&piv, 1), field, col, 4);
+    dump(dir + "/column.bin", col);
+    // error mapping
+    expect("bad_output_size", [&] {
+        std::vector<Complex> small(3);
+        compute_eri(sh[0], sh[1], sh[2], sh[3], field, small, ws);
+    });
+    expect("bad_index", [&] {
+        const ShellQuartetIndex q{0, 0, 0, ns};
+        (void)evaluate_eri_shell_quartets(basis, std::span(&q, 1), field, {}, nullptr, consume, &packed);
+    });
+    expect("null_consumer", [&] { (void)evaluate_eri_shell_quartets(basis, all, field, {}, nullptr, nullptr, nullptr); });
+    expect("screening_without_bounds",
+           [&] { (void)evaluate_eri_shell_quartets(basis, all, field, {1e-8, 1}, nullptr, consume, &packed); });
+    expect("bounds_other_field", [&] {
+        (void)evaluate_eri_shell_quartets(basis, all, MagneticField{}, {1e-8, 1}, &bounds, consume, &packed);
+    });
+    expect("zero_threads", [&] { (void)evaluate_eri_shell_quartets(basis, all, field, {0.0, 0}, nullptr, consume, &packed); });
+    expect("boys_outside_domain", [] { (void)boys_values({-400.0, 10.0}, 4); });
+    expect("boys_order", [] { (void)boys_values({1.0, 0.0}, 33); });
+    expect("real_columns_in_field", [&] {
+        std::vector<double> r(n * n);
+        detail::cholesky_columns(basis, std::span(&piv, 1), field, r, 1);
+    });
+    return 0;
+}
