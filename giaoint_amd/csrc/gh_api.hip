@@ -131,7 +131,7 @@ struct giaohip_ctx {
     DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
     std::set<void*> user_allocs;
     size_t seed_budget_bytes = size_t(2048) << 20;
-    int chol_block = 16;  // candidates per Cholesky pivot block
+    int chol_block = 32;  // candidates per Cholesky pivot block
     size_t staging_budget_bytes = size_t(1) << 30;
 };
 
@@ -164,7 +164,9 @@ struct giaohip_cholesky {
     long long cand[gh::kCholMaxBlock]{};
     unsigned open = 0;
     uint64_t blocks = 0;
-    hipEvent_t ev[5]{};
+    uint64_t rows_read = 0;  // factor rows read by the block projections
+    double block_ms = 0.0;   // candidate selection + projection
+    hipEvent_t ev[6]{};
     void* host = nullptr;  // page-locked copy of the state and the PSD flag
     void release() {
         for (zc* c : chunks) (void)hipFree(c);
@@ -1619,41 +1621,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         int slot = -1;
         for (int m = 0; m < c->ncand; ++m)
             if (((c->open >> m) & 1u) && c->cand[m] == p) slot = m;
-        if (slot < 0) {
-            // new block: candidates and their projections onto the rank factors so far
-            CHOL_STEP_TRY(hipEventRecord(c->ev[3], s), "hipEventRecord");
-            CHOL_STEP_TRY(gh::launch_chol_topk(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
-                                               c->d_pidx.as<long long>(), c->block, c->d_cand.as<long long>(),
-                                               reinterpret_cast<int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock),
-                                               s),
-                          "top-k launch");
-            const int* d_ncand = reinterpret_cast<const int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock);
-            CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(c->rank, 1) * c->slots * sizeof(zc)),
-                          "hipMalloc");
-            CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
-                                               c->d_cand.as<long long>(), d_ncand, c->slots, c->d_coef.as<zc>(), s),
-                          "coefficient launch");
-            CHOL_STEP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
-                                                  c->d_coef.as<zc>(), d_ncand, c->slots, c->d_proj.as<zc>(), s),
-                          "projection launch");
-            CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
-            CHOL_STEP_TRY(hipMemcpyAsync(host->cand, c->d_cand.p, sizeof host->cand, hipMemcpyDeviceToHost, s),
-                          "copy");
-            CHOL_STEP_TRY(hipMemcpyAsync(&host->ncand, d_ncand, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-            CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky block");
-            c->update_ms += elapsed(c->ev[3], c->ev[4]);
-            c->ncand = host->ncand;
-            std::copy(host->cand, host->cand + gh::kCholMaxBlock, c->cand);
-            c->open = c->ncand >= 32 ? 0xffffffffu : ((1u << c->ncand) - 1u);
-            c->blocks += 1;
-            for (int m = 0; m < c->ncand; ++m)
-                if (c->cand[m] == p) slot = m;
-            if (slot != 0) {
-                c->failed = set_error(ctx, GIAOHIP_RUNTIME_ERROR,
-                                      "Cholesky pivot block does not start at the maximum residual");
-                return c->failed;
-            }
-        }
+        const bool new_block = slot < 0;
         const long long mu = p / n, nu = p % n;
         const int sa = b->owner[mu], sb = b->owner[nu];
         const DevShell& C = b->shells[sb];
@@ -1680,14 +1648,17 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                           "hipMemcpy");
         }
         zc* row = c->chunks[chunk] + (long long)(c->rank % c->rows_per_chunk) * n2;
-        CHOL_STEP_TRY(hipEventRecord(c->ev[0], s), "hipEventRecord");
+        // The pivot's column ERIs; at a block start they run on the second
+        // stream, beside the block's candidate selection and projection.
+        hipStream_t sc = new_block ? ctx->aux : s;
+        CHOL_STEP_TRY(hipEventRecord(c->ev[0], sc), "hipEventRecord");
         CHOL_STEP_TRY(gh::launch_chol_expand(c->d_units.as<gh::CholUnit>(), c->nunits, ket, c->d_sq.as<SeedQuartet>(),
-                                             c->d_wi.as<WorkItem>(), s),
+                                             c->d_wi.as<WorkItem>(), sc),
                       "expand launch");
         const int maxL = c->max_lab + ket.lcd;
         CHOL_STEP_TRY(gh::launch_seed(nmax_bucket(maxL), c->d_sq.as<SeedQuartet>(), c->nunits, c->p1 * ket.kcd,
                                       b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
-                                      c->d_seeds.as<zc>(), c->d_err.as<DevError>(), c->d_scratch, s),
+                                      c->d_seeds.as<zc>(), c->d_err.as<DevError>(), c->d_scratch, sc),
                       "seed launch");
         for (const auto& g : c->groups) {
             gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
@@ -1698,17 +1669,58 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                                                      b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
                                                      b->d_coefs.as<double>(), b->d_norms.as<double>(),
                                                      c->d_kappa.as<double>(), c->d_seeds.as<zc>(), c->d_col.as<zc>(),
-                                                     s),
+                                                     sc),
                               "column launch");
             else
                 CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
                                                    b->d_exps.as<double>(), b->d_coefs.as<double>(),
                                                    b->d_norms.as<double>(), c->d_kappa.as<double>(),
-                                                   c->d_seeds.as<zc>(), c->d_col.as<zc>(), s),
+                                                   c->d_seeds.as<zc>(), c->d_col.as<zc>(), sc),
                               "transform launch");
         }
-        CHOL_STEP_TRY(hipEventRecord(c->ev[1], s), "hipEventRecord");
+        CHOL_STEP_TRY(hipEventRecord(c->ev[1], sc), "hipEventRecord");
+        if (new_block) {
+            // new block: candidates and their projections onto the rank factors so far
+            CHOL_STEP_TRY(hipEventRecord(c->ev[3], s), "hipEventRecord");
+            CHOL_STEP_TRY(gh::launch_chol_topk(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
+                                               c->d_pidx.as<long long>(), c->block, c->d_cand.as<long long>(),
+                                               reinterpret_cast<int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock),
+                                               s),
+                          "top-k launch");
+            const int* d_ncand = reinterpret_cast<const int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock);
+            CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(c->rank, 1) * c->slots * sizeof(zc)),
+                          "hipMalloc");
+            CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                               c->d_cand.as<long long>(), d_ncand, c->slots, c->d_coef.as<zc>(), s),
+                          "coefficient launch");
+            CHOL_STEP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                                  c->d_coef.as<zc>(), d_ncand, c->slots, c->d_proj.as<zc>(), s),
+                          "projection launch");
+            CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
+            CHOL_STEP_TRY(hipMemcpyAsync(host->cand, c->d_cand.p, sizeof host->cand, hipMemcpyDeviceToHost, s),
+                          "copy");
+            CHOL_STEP_TRY(hipMemcpyAsync(&host->ncand, d_ncand, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+            CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky block");
+            const double bms = elapsed(c->ev[3], c->ev[4]);
+            c->update_ms += bms;
+            c->block_ms += bms;
+            c->rows_read += c->rank;
+            c->ncand = host->ncand;
+            std::copy(host->cand, host->cand + gh::kCholMaxBlock, c->cand);
+            c->open = c->ncand >= 32 ? 0xffffffffu : ((1u << c->ncand) - 1u);
+            c->blocks += 1;
+            for (int m = 0; m < c->ncand; ++m)
+                if (c->cand[m] == p) slot = m;
+            if (slot != 0) {
+                (void)hipStreamSynchronize(sc);
+                c->failed = set_error(ctx, GIAOHIP_RUNTIME_ERROR,
+                                      "Cholesky pivot block does not start at the maximum residual");
+                return c->failed;
+            }
+            CHOL_STEP_TRY(hipStreamWaitEvent(s, c->ev[1], 0), "hipStreamWaitEvent");
+        }
         const unsigned rest = c->open & ~(1u << slot);
+        CHOL_STEP_TRY(hipEventRecord(c->ev[5], s), "hipEventRecord");
         CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2,
                                              c->d_proj.as<zc>(), c->d_cand.as<long long>(), rest, n2,
                                              c->d_state.as<gh::CholState>(), c->d_res.as<double>(), row, c->psd_tol,
@@ -1723,7 +1735,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         CHOL_STEP_TRY(hipMemcpyAsync(&host->flags, c->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
         CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky step");
         c->column_ms += elapsed(c->ev[0], c->ev[1]);
-        c->update_ms += elapsed(c->ev[1], c->ev[2]);
+        c->update_ms += elapsed(c->ev[5], c->ev[2]);
         c->pivots.push_back(mu);
         c->pivots.push_back(nu);
         c->rank += 1;
@@ -1908,6 +1920,15 @@ int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max
     return GIAOHIP_OK;
 }
 
+int giaohip_cholesky_blocks(const giaohip_cholesky* c, uint64_t* blocks, uint64_t* factor_rows_read,
+                            double* block_ms) {
+    if (!c) return GIAOHIP_INVALID_ARGUMENT;
+    if (blocks) *blocks = c->blocks;
+    if (factor_rows_read) *factor_rows_read = c->rows_read;
+    if (block_ms) *block_ms = c->block_ms;
+    return GIAOHIP_OK;
+}
+
 int giaohip_cholesky_pivots(const giaohip_cholesky* c, int64_t* out) {
     if (!c || !out) return GIAOHIP_INVALID_ARGUMENT;
     std::copy(c->pivots.begin(), c->pivots.end(), out);
@@ -1933,6 +1954,7 @@ int giaohip_cholesky_destroy(giaohip_ctx* ctx, giaohip_cholesky* c) {
     if (ctx) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->aux);
     }
     c->release();
     delete c;

@@ -311,6 +311,112 @@ __global__ __launch_bounds__(256) void chol_project_kernel(const zc* const* __re
                 if (base + e * 256 < n2) proj[(long long)m * n2 + base + e * 256] = acc[m][e];
 }
 
+// acc[M0 + j] += c(M0 + j) * v for j < 4, where c(m) is the coefficient that
+// lane m of the lane's 16-lane row holds (DPP row broadcast of src0). Each
+// accumulator component sees cmac's order (x += c.x v.x, x += -c.y v.y;
+// y += c.x v.y, y += c.y v.x); the four slots are interleaved so that
+// dependent FMAs are eight instructions apart. The leading s_nop provides the
+// wait states a DPP read needs after a VALU write of its source.
+template <int M0, int NB>
+__device__ __forceinline__ void cmac4_bcast(zc (&acc)[NB], const zc& c, const zc& v) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %8, %10 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %2, %8, %10 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %4, %8, %10 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %6, %8, %10 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %8, %11 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %3, %8, %11 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %5, %8, %11 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %7, %8, %11 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %0, -%9, %11 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %2, -%9, %11 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %4, -%9, %11 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %6, -%9, %11 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %9, %10 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %3, %9, %10 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %5, %9, %10 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %7, %9, %10 row_newbcast:%c15 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc[M0].x), "+v"(acc[M0].y), "+v"(acc[M0 + 1].x), "+v"(acc[M0 + 1].y), "+v"(acc[M0 + 2].x),
+          "+v"(acc[M0 + 2].y), "+v"(acc[M0 + 3].x), "+v"(acc[M0 + 3].y)
+        : "v"(c.x), "v"(c.y), "v"(v.x), "v"(v.y), "n"(M0 % 16), "n"(M0 % 16 + 1), "n"(M0 % 16 + 2),
+          "n"(M0 % 16 + 3));
+}
+
+// The same projection as chol_project_kernel for NB = 16 or 32 slots, one
+// entry per thread: the coefficient row of factor k reaches the lanes as one
+// 16-byte load per lane (slot lane % 16, and 16 + lane % 16 for NB = 32) and
+// the FMAs take slot m's coefficient from lane m of the row by DPP broadcast,
+// instead of every wave fetching the row through the scalar cache.
+template <int NB, int U>
+__global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const* __restrict__ chunks,
+                                                                 int rows_per_chunk, long long n2, int rank,
+                                                                 const zc* __restrict__ coef,
+                                                                 const int* __restrict__ ncand,
+                                                                 zc* __restrict__ proj) {
+    static_assert(NB == 16 || NB == 32, "one or two coefficients per lane");
+    constexpr int NL = NB / 16;  // coefficients per lane and row
+    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long jc = j < n2 ? j : n2 - 1;
+    const int l16 = threadIdx.x & 15;
+    zc acc[NB];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) acc[m] = zc{0.0, 0.0};
+    auto step = [&](const zc (&c)[NL], const zc& v) {
+        cmac4_bcast<0>(acc, c[0], v);
+        cmac4_bcast<4>(acc, c[0], v);
+        cmac4_bcast<8>(acc, c[0], v);
+        cmac4_bcast<12>(acc, c[0], v);
+        if constexpr (NL == 2) {
+            cmac4_bcast<16>(acc, c[NL - 1], v);
+            cmac4_bcast<20>(acc, c[NL - 1], v);
+            cmac4_bcast<24>(acc, c[NL - 1], v);
+            cmac4_bcast<28>(acc, c[NL - 1], v);
+        }
+    };
+    for (int k0 = 0; k0 < rank; k0 += rows_per_chunk) {
+        const zc* chunk = chunks[k0 / rows_per_chunk];
+        const int rows = min(rows_per_chunk, rank - k0);
+        const int full = rows - rows % (2 * U);
+        const zc* ck = coef + (long long)k0 * NB + l16;
+        // rows r..r+U-1, clamped to the chunk (loads past the last group repeat row rows-1, unused)
+        auto load = [&](zc (&v)[U], zc (&c)[U][NL], int r) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int rr = min(r + u, rows - 1);
+                v[u] = gload(chunk + (long long)rr * n2 + jc);
+#pragma unroll
+                for (int h = 0; h < NL; ++h) c[u][h] = ck[(long long)rr * NB + 16 * h];
+            }
+        };
+        zc va[U], vb[U], ca[U][NL], cb[U][NL];
+        load(va, ca, 0);
+        for (int r = 0; r < full; r += 2 * U) {
+            load(vb, cb, r + U);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) step(ca[u], va[u]);
+            __builtin_amdgcn_sched_barrier(0);
+            load(va, ca, r + 2 * U);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) step(cb[u], vb[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (int r = full; r < rows; ++r) {
+            zc c[NL];
+            const zc v = gload(chunk + (long long)r * n2 + jc);
+#pragma unroll
+            for (int h = 0; h < NL; ++h) c[h] = ck[(long long)r * NB + 16 * h];
+            step(c, v);
+        }
+    }
+    const int nc = *ncand;
+    if (j < n2)
+#pragma unroll
+        for (int m = 0; m < NB; ++m)
+            if (m < nc) proj[(long long)m * n2 + j] = acc[m];
+}
+
 // Accepts the pivot st->pivot, whose raw column is `column` and whose
 // projection onto the earlier factors is `pproj`: writes the new factor row,
 // updates the residual diagonal (PSD check, clamp, res[p] = 0), the
@@ -449,8 +555,14 @@ hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long
         case 2: return project<2, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
         case 4: return project<4, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
         case 8: return project<8, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
-        case 16: return project<16, 1, 4>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
-        case 32: return project<32, 1, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 16:
+            chol_project_bcast_kernel<16, 4><<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(
+                chunks, rows_per_chunk, n2, rank, coef, ncand, proj);
+            return hipGetLastError();
+        case 32:
+            chol_project_bcast_kernel<32, 2><<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(
+                chunks, rows_per_chunk, n2, rank, coef, ncand, proj);
+            return hipGetLastError();
         default: return hipErrorInvalidValue;
     }
 }

@@ -16,14 +16,16 @@
 //    lanes are the rows (bra primitive pair, bra Hermite index k), so a dd bra
 //    side with three-term contractions (9 x 35 rows) fills 5 wavefronts.
 //    W[row][cd] for a chunk of ket components lives in registers. Per ket
-//    primitive pair each lane gathers R0[k + j] with vector loads while the
-//    products Et_cd[j], identical in every lane, arrive by wave-uniform scalar
-//    loads: the inner loop is FMAs with one scalar operand, and its two load
-//    streams wait on separate counters;
+//    primitive pair each lane gathers R0[k + j] with vector loads; the
+//    products Et_cd[j], identical for every lane, come 16 plan entries at a
+//    time: each lane of a 16-lane row loads one entry (one 16-byte load) and
+//    the FMAs read entry n from lane n of the row by DPP row broadcast, so the
+//    inner loop is FMAs with no separate operand traffic, software-pipelined
+//    a few batches ahead (Pipe);
 //  * bra step: the bra primitive pairs go one at a time through a
 //    double-buffered LDS tile W[k][cd]; waves 0..3 each take a quarter of the
 //    bra components (round robin), lane = ket component, products again by
-//    scalar loads;
+//    row broadcast;
 //  * contraction coefficients: ket pair coefficients scale the gathered R0
 //    values, bra pair coefficients scale W on its way into LDS. The primitive
 //    normalization is split into a radial factor (folded into the seeds by the

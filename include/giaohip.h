@@ -58,7 +58,7 @@ enum {
        in MiB, 16..65536; default 2048. Two chunks are in flight at once (one
        being produced, one transformed), so the path holds twice this much. */
     GIAOHIP_OPTION_CHUNK_MIB = 1,
-    /* Candidates per pivot block of giaohip_cholesky_run, 1..32; default 16.
+    /* Candidates per pivot block of giaohip_cholesky_run, 1..32; default 32.
        A block projects its candidates (the largest residuals) onto the
        earlier factors in one pass; consecutive pivots that are candidates
        need no further pass. Factors and pivots are bitwise identical for
@@ -212,6 +212,13 @@ int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* chol, uint64_t s
  * whether the stopping test was met, and device time split (ms). */
 int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* max_residual, double* tolerance,
                           int* converged, double* column_ms, double* update_ms, double* diagonal_ms);
+/* Pivot-block statistics of a run: blocks started, factor rows read by their
+ * projections (sum of the rank at each block start; each row is n*n complex),
+ * and device time of candidate selection + projection (ms, included in
+ * update_ms). No counterpart in the reference (its projection is one GEMV
+ * per pivot). */
+int giaohip_cholesky_blocks(const giaohip_cholesky* chol, uint64_t* blocks, uint64_t* factor_rows_read,
+                            double* block_ms);
 /* Pivot AO pairs (mu, nu), rank*2 int64 values. */
 int giaohip_cholesky_pivots(const giaohip_cholesky* chol, int64_t* out);
 /* Factor rows first..first+count-1 as complex128 (count*n*n values, host). */

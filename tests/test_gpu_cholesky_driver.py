@@ -89,7 +89,7 @@ def _run_with_block(ctx, db, f, nb, **kw):
     try:
         return _native.cholesky_run(db, f, 1e-8, **kw)
     finally:
-        ctx.set_cholesky_block(16)
+        ctx.set_cholesky_block(32)
 
 
 def test_device_cholesky_block_sizes_bitwise(gpu, port):
@@ -102,10 +102,14 @@ def test_device_cholesky_block_sizes_bitwise(gpu, port):
     f = _native.field6(B, O)
     ref = _run_with_block(gpu, db, f, 1)
     assert ref.converged
+    # one candidate per block: one pass over all earlier factors per pivot
+    assert ref.blocks["blocks"] == ref.rank
+    assert ref.blocks["factor_rows_read"] == ref.rank * (ref.rank - 1) // 2
     ref_piv, ref_vec = ref.pivots(), ref.vectors()
     for nb in (3, 16, 32):
         got = _run_with_block(gpu, db, f, nb)
-        print(f"block {nb}: rank {got.rank}")
+        print(f"block {nb}: rank {got.rank}, {got.blocks['blocks']} blocks")
+        assert got.blocks["blocks"] < ref.blocks["blocks"]
         assert got.rank == ref.rank and got.converged
         np.testing.assert_array_equal(got.pivots(), ref_piv)
         np.testing.assert_array_equal(got.vectors(), ref_vec)
