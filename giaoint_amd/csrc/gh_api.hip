@@ -125,7 +125,7 @@ struct giaohip_ctx {
     giaohip_stats stats{};
     DevBuffer seeds, squartets, items, kappa, errrec, staging, blocks, misc_a, misc_b, misc_c, fpq, fitems;
     int eri_variant = GIAOHIP_ERI_BATCHED;
-    hipEvent_t ev[4]{};
+    hipEvent_t ev[5]{};
     std::vector<hipEvent_t> chunk_ev;  // per-chunk timing events of the batched path
     PinnedBuffer h_sq, h_pq, h_items;  // host staging of the batched path's descriptors
     DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
@@ -397,8 +397,16 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         HIP_TRY(hipMemcpyAsync(ctx->squartets.p, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
 
-        // Work items grouped by (class, tile).
-        std::map<std::tuple<int, int, int, int, int>, std::vector<WorkItem>> groups;
+        // Work items of every class, one per pass of `tile` ket components,
+        // each carrying its class; launched in a few LDS-size buckets, the
+        // most expensive items first (a mixed-class batch would otherwise
+        // leave the GPU idle behind launches of a handful of workgroups).
+        struct Planned {
+            WorkItem w;
+            int lds;
+            double cost;
+        };
+        std::vector<Planned> planned;
         for (size_t i = begin; i < end; ++i) {
             const Request& r = reqs[i];
             const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
@@ -410,9 +418,12 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             if (tile < 1) return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class too large for the GPU transform kernel");
             gh::ClassArgs ca{{S[0]->l, S[1]->l, S[2]->l, S[3]->l}, tile};
             while (tile > 1 && gh::transform_lds_bytes(ca) > 160 * 1024) ca.tile = --tile;
-            if (gh::transform_lds_bytes(ca) > 160 * 1024)
+            const int lds = gh::transform_lds_bytes(ca);
+            if (lds > 160 * 1024)
                 return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class exceeds the LDS capacity of the transform kernel");
-            auto& vec = groups[{S[0]->l, S[1]->l, S[2]->l, S[3]->l, tile}];
+            const int L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
+            const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
+            const int cls = S[0]->l | (S[1]->l << 4) | (S[2]->l << 8) | (S[3]->l << 12) | (ca.tile << 16);
             int lo[4], hi[4];
             for (int k = 0; k < 4; ++k) {
                 lo[k] = r.xfix[k] < 0 ? 0 : r.xfix[k];
@@ -421,27 +432,48 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             for (int xa = lo[0]; xa < hi[0]; ++xa)
                 for (int xb = lo[1]; xb < hi[1]; ++xb)
                     for (int xc = lo[2]; xc < hi[2]; ++xc)
-                        for (int xd = lo[3]; xd < hi[3]; ++xd) {
-                            WorkItem w{};
-                            for (int k = 0; k < 4; ++k) w.sh[k] = r.sh[k];
-                            w.x[0] = xa;
-                            w.x[1] = xb;
-                            w.x[2] = xc;
-                            w.x[3] = xd;
-                            w.cd0 = cd0;
-                            w.cd1 = cd1;
-                            w.seed_off = sq[i - begin].seed_off;
-                            w.out_base = r.out_base;
-                            for (int k = 0; k < 4; ++k) w.stride[k] = r.stride[k];
-                            vec.push_back(w);
-                        }
+                        for (int xd = lo[3]; xd < hi[3]; ++xd)
+                            for (int t0 = cd0; t0 < cd1; t0 += ca.tile) {
+                                Planned pl{};
+                                WorkItem& w = pl.w;
+                                for (int k = 0; k < 4; ++k) w.sh[k] = r.sh[k];
+                                w.x[0] = xa;
+                                w.x[1] = xb;
+                                w.x[2] = xc;
+                                w.x[3] = xd;
+                                w.cd0 = t0;
+                                w.cd1 = std::min(t0 + ca.tile, cd1);
+                                w.seed_off = sq[i - begin].seed_off;
+                                w.out_base = r.out_base;
+                                for (int k = 0; k < 4; ++k) w.stride[k] = r.stride[k];
+                                w.cls = cls;
+                                pl.lds = lds;
+                                pl.cost = kq * (gh::nherm(L) + 0.5 * NHab * gh::nherm(S[2]->l + S[3]->l) *
+                                                                   (w.cd1 - w.cd0));
+                                planned.push_back(pl);
+                            }
         }
+        constexpr int kBuckets[] = {16 * 1024, 32 * 1024, 64 * 1024, 96 * 1024, 160 * 1024};
         std::vector<WorkItem> all;
-        std::vector<std::pair<gh::ClassArgs, std::pair<size_t, size_t>>> launches;
-        for (auto& [key, vec] : groups) {
-            gh::ClassArgs ca{{std::get<0>(key), std::get<1>(key), std::get<2>(key), std::get<3>(key)}, std::get<4>(key)};
-            launches.push_back({ca, {all.size(), vec.size()}});
-            all.insert(all.end(), vec.begin(), vec.end());
+        all.reserve(planned.size());
+        struct MixedLaunch {
+            int lds;
+            size_t first, count;
+        };
+        std::vector<MixedLaunch> launches;
+        for (size_t bk = 0; bk < std::size(kBuckets); ++bk) {
+            const int lo_b = bk ? kBuckets[bk - 1] : 0, hi_b = kBuckets[bk];
+            std::vector<const Planned*> sel;
+            int lds_max = 0;
+            for (const Planned& pl : planned)
+                if (pl.lds > lo_b && pl.lds <= hi_b) {
+                    sel.push_back(&pl);
+                    lds_max = std::max(lds_max, pl.lds);
+                }
+            if (sel.empty()) continue;
+            std::stable_sort(sel.begin(), sel.end(), [](const Planned* x, const Planned* y) { return x->cost > y->cost; });
+            launches.push_back({lds_max, all.size(), sel.size()});
+            for (const Planned* pl : sel) all.push_back(pl->w);
         }
         HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
         HIP_TRY(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
@@ -453,13 +485,21 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                                 ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), d_scratch, s),
                 "seed kernel launch");
         HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
-        for (auto& [ca, range] : launches) {
-            HIP_TRY(gh::launch_transform(ca, ctx->items.as<WorkItem>() + range.first, (int)range.second,
-                                         b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
-                                         b->d_norms.as<double>(), ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_out, s),
+        // class launches alternate between the two streams (disjoint outputs),
+        // so that a launch with few workgroups does not leave the GPU idle
+        HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev[1], 0), "hipStreamWaitEvent");
+        size_t nl = 0;
+        for (const MixedLaunch& L : launches) {
+            hipStream_t ls = (nl++ & 1) ? ctx->aux : s;
+            HIP_TRY(gh::launch_transform_mixed(L.lds, ctx->items.as<WorkItem>() + L.first, (int)L.count,
+                                               b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
+                                               b->d_coefs.as<double>(), b->d_norms.as<double>(),
+                                               ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_out, ls),
                     "transform kernel launch");
             ctx->stats.launches += 1;
         }
+        HIP_TRY(hipEventRecord(ctx->ev[4], ctx->aux), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(s, ctx->ev[4], 0), "hipStreamWaitEvent");
         HIP_TRY(hipEventRecord(ctx->ev[2], s), "hipEventRecord");
         HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
         ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
@@ -949,7 +989,7 @@ int giaohip_create(int device, giaohip_ctx** out) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
-    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
     if (e == hipSuccess) e = gh::upload_constants();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {

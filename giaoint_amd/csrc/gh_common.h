@@ -89,6 +89,8 @@ struct WorkItem {
     long long seed_off;
     long long out_base;     // complex element offset of component (0,0,0,0)
     long long stride[4];    // complex element strides of the four AO indices
+    int cls;                // 0: the launch's class; else la | lb<<4 | lc<<8 | ld<<12 | tile<<16
+    int pad;
 };
 
 // Device error record: first failing lane wins.

@@ -366,9 +366,13 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
                                                         const double* __restrict__ kappa,
                                                         const zc* __restrict__ seeds, zc* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int la = ca.l[0], lb = ca.l[1], lc = ca.l[2], ld = ca.l[3];
+    const WorkItem it = items[blockIdx.x];
+    // the item's own class (mixed-class launches) or the launch's
+    const int cls = it.cls;
+    const int la = cls ? (cls & 15) : ca.l[0], lb = cls ? ((cls >> 4) & 15) : ca.l[1];
+    const int lc = cls ? ((cls >> 8) & 15) : ca.l[2], ld = cls ? ((cls >> 12) & 15) : ca.l[3];
     const int L = la + lb + lc + ld, Lab = la + lb, Lcd = lc + ld;
-    const int tile = ca.tile;
+    const int tile = cls ? (cls >> 16) : ca.tile;
     const LdsLayout lay = lds_layout(la, lb, lc, ld, tile, nullptr);
     int* ibase = reinterpret_cast<int*>(smem_raw);
     zc* zbase = reinterpret_cast<zc*>(smem_raw + lay.zc_base);
@@ -412,7 +416,6 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
     }
     __syncthreads();
 
-    const WorkItem it = items[blockIdx.x];
     const DevShell A = shells[it.sh[0]];
     const DevShell B = shells[it.sh[1]];
     const DevShell C = shells[it.sh[2]];
@@ -801,6 +804,26 @@ hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nite
     }
     transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), bytes, s>>>(ca, items, shells, exps, coefs, norms, kappa,
                                                                       seeds, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
+                                  const double* exps, const double* coefs, const double* norms, const double* kappa,
+                                  const zc* seeds, zc* out, hipStream_t s) {
+    if (nitems == 0) return hipSuccess;
+    if (lds_bytes > 160 * 1024) return hipErrorInvalidValue;
+    if (lds_bytes > 64 * 1024) {
+        static bool raised = false;
+        if (!raised) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            raised = true;
+        }
+    }
+    const ClassArgs none{{0, 0, 0, 0}, 1};  // every item carries its class
+    transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), lds_bytes, s>>>(none, items, shells, exps, coefs, norms,
+                                                                          kappa, seeds, out);
     return hipGetLastError();
 }
 

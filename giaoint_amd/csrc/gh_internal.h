@@ -115,6 +115,11 @@ hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long 
 hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nitems, const DevShell* shells,
                             const double* exps, const double* coefs, const double* norms, const double* kappa,
                             const zc* seeds, zc* out, hipStream_t s);
+// Transform of items of several classes in one launch: every item carries
+// its class and tile in WorkItem::cls; lds_bytes covers the largest.
+hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
+                                  const double* exps, const double* coefs, const double* norms, const double* kappa,
+                                  const zc* seeds, zc* out, hipStream_t s);
 // Cholesky column ERIs for one (la, lb) group of bra shell pairs and the
 // pivot's ket class (lc, ld) and component `comp`: one wavefront per pair.
 // Needs la + lb + lc + ld <= kColumnMaxL; `items` as for launch_transform

@@ -182,17 +182,21 @@ def random_quartets(classes, degrees, *, seed, box=6.0, log_exp=(-1.0, 4.0), B=N
         m = len(todo)
         centers[todo] = rng.uniform(0.0, box, (m, 4, 3))
         exps[todo] = 10.0 ** rng.uniform(log_exp[0], log_exp[1], (m, 4, kmax))
-        # Boys-domain check per (class, degree) group
+        # Boys-domain check per contraction-degree group (the check is per
+        # quartet; grouping only lets it run vectorized)
         bad = []
-        keys = np.unique(np.concatenate([classes[todo], degrees[todo]], axis=1), axis=0)
-        for key in keys:
-            sel = todo[np.all(np.concatenate([classes[todo], degrees[todo]], axis=1) == key, axis=1)]
-            deg = key[4:]
-            L = np.full(len(sel), int(key[:4].sum()))
+        uniq, inv = np.unique(degrees[todo], axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        order = np.argsort(inv, kind="stable")
+        edges = np.searchsorted(inv[order], np.arange(len(uniq) + 1))
+        for g, deg in enumerate(uniq):
+            sel = todo[order[edges[g]:edges[g + 1]]]
+            L = classes[sel].sum(axis=1)
             c = centers[sel]
             k = [_wave(B, c[:, i]) for i in range(4)]
-            for start in range(0, len(sel), 4096):
-                s = slice(start, start + 4096)
+            step = max(1, 4096 // int(np.prod(deg)) * 16)
+            for start in range(0, len(sel), step):
+                s = slice(start, start + step)
                 ok = _boys_domain_ok(
                     exps[sel[s], 0, : deg[0]], c[s, 0], k[0][s], exps[sel[s], 1, : deg[1]], c[s, 1], k[1][s],
                     exps[sel[s], 2, : deg[2]], c[s, 2], k[2][s], exps[sel[s], 3, : deg[3]], c[s, 3], k[3][s], L[s],
