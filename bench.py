@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cholesky-rank", type=int, default=0,
                     help="Cholesky leg: stop after this many pivots (0: run to convergence, -1: skip the leg)")
     ap.add_argument("--cholesky-block", type=int, default=0, help="candidates per pivot block (default: library's)")
+    ap.add_argument("--mixed", type=int, default=200_000,
+                    help="mixed-class leg: quartets of the l<=3, degree 1..8 workload (0 skips it)")
     return ap.parse_args()
 
 
@@ -94,6 +96,40 @@ def cpu_baseline(batch, B, origin, target_s: float):
         "cores": threads,
         "kind": kind,
         "sample": f"first {n} of the same (dd|dd) batch, OpenMP schedule(dynamic,1), {dt:.1f} s",
+    }
+
+
+def mixed_leg(n: int):
+    """Mixed classes l <= 3 (weights (1 + sum l)^-2), contraction degree U{1..8}:
+    the generator of the 2M-quartet workload at n quartets. One warmup call,
+    one timed call, inputs and output resident on the device."""
+    from giaoint_amd import _native, workloads
+
+    batch = workloads.mixed_batch(n)
+    sh = batch.shells
+    coefs = _native.normalize_shells(sh.L, sh.nprim, sh.ncontr, sh.exps, sh.coefs)
+    ctx = _native.Context(0)
+    db = _native.DeviceBasis(ctx, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
+    total = int(db.quartet_sizes(batch.quartets).sum())
+    out = ctx.alloc(total * 16)
+    f = _native.field6(batch.B, batch.origin)
+    flops = workloads.batch_flops(sh, batch.quartets)
+    db.eri_batch(batch.quartets, f, device_out=out)
+    ctx.synchronize()
+    t = time.perf_counter()
+    db.eri_batch(batch.quartets, f, device_out=out)
+    ctx.synchronize()
+    dt = time.perf_counter() - t
+    st = ctx.stats()
+    ctx.free(out)
+    log(f"mixed: {n} quartets in {dt:.2f} s")
+    return {
+        "workload": f"{n} quartets, classes l<=3 weighted (1+sum l)^-2, degree U{{1..8}}, seed 3, "
+                    f"{st['primitive_quartets']} primitive quartets",
+        "quartets_per_s": n / dt,
+        "seconds": dt,
+        "tflops_convention": flops / dt / 1e12,
+        "launches": st["launches"],
     }
 
 
@@ -289,6 +325,8 @@ def main():
             result["cpu_baseline"] = {"error": str(exc)}
     ctx.free(out)
     del dbasis, ctx
+    if args.mixed > 0:
+        result["mixed_class"] = mixed_leg(args.mixed)
     if args.cholesky_rank >= 0:
         result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block)
     print(json.dumps(result), flush=True)

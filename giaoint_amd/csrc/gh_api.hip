@@ -376,7 +376,8 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const int L = A.l + B.l + C.l + D.l;
             const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
             const size_t need = (size_t)(kab * kcd) * (size_t)(L + 4);
-            if (end > begin && (seed_zc + need) * sizeof(zc) > ctx->seed_budget_bytes) break;
+            // seeds are this path's only large buffer: four chunk budgets
+            if (end > begin && (seed_zc + need) * sizeof(zc) > 4 * ctx->seed_budget_bytes) break;
             SeedQuartet q{};
             for (int i = 0; i < 4; ++i) q.sh[i] = r.sh[i];
             q.L = L;
@@ -459,6 +460,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         struct MixedLaunch {
             int lds;
             size_t first, count;
+            double cost;
         };
         std::vector<MixedLaunch> launches;
         for (size_t bk = 0; bk < std::size(kBuckets); ++bk) {
@@ -472,7 +474,9 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                 }
             if (sel.empty()) continue;
             std::stable_sort(sel.begin(), sel.end(), [](const Planned* x, const Planned* y) { return x->cost > y->cost; });
-            launches.push_back({lds_max, all.size(), sel.size()});
+            double cost = 0.0;
+            for (const Planned* pl : sel) cost += pl->cost;
+            launches.push_back({lds_max, all.size(), sel.size(), cost});
             for (const Planned* pl : sel) all.push_back(pl->w);
         }
         HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
@@ -488,9 +492,17 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         // class launches alternate between the two streams (disjoint outputs),
         // so that a launch with few workgroups does not leave the GPU idle
         HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev[1], 0), "hipStreamWaitEvent");
-        size_t nl = 0;
-        for (const MixedLaunch& L : launches) {
-            hipStream_t ls = (nl++ & 1) ? ctx->aux : s;
+        // greedy: the next launch goes to the stream with less work queued
+        double queued[2] = {0.0, 0.0};
+        std::vector<size_t> order(launches.size());
+        std::iota(order.begin(), order.end(), size_t{0});
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return launches[x].cost > launches[y].cost; });
+        for (size_t li : order) {
+            const MixedLaunch& L = launches[li];
+            const int side = queued[1] < queued[0] ? 1 : 0;
+            queued[side] += L.cost;
+            hipStream_t ls = side ? ctx->aux : s;
             HIP_TRY(gh::launch_transform_mixed(L.lds, ctx->items.as<WorkItem>() + L.first, (int)L.count,
                                                b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
                                                b->d_coefs.as<double>(), b->d_norms.as<double>(),
