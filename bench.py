@@ -174,6 +174,7 @@ def cholesky_leg(rank_cap: int, block: int = 0):
         "device_ms": t,
         "pivot_blocks": blk["blocks"],
         "pivots_per_block": m / max(blk["blocks"], 1),
+        "candidates_projected_per_block": blk["projected"] / max(blk["blocks"], 1),
         "projection": {
             "bound": "hbm",
             "achieved": proj_bytes / (blk["block_ms"] * 1e-3) / 1e9,

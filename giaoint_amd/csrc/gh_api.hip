@@ -165,8 +165,9 @@ struct giaohip_cholesky {
     unsigned open = 0;
     uint64_t blocks = 0;
     uint64_t rows_read = 0;  // factor rows read by the block projections
+    uint64_t projected = 0;  // candidates projected (the others carried over)
     double block_ms = 0.0;   // candidate selection + projection
-    hipEvent_t ev[6]{};
+    hipEvent_t ev[8]{};
     void* host = nullptr;  // page-locked copy of the state and the PSD flag
     void release() {
         for (zc* c : chunks) (void)hipFree(c);
@@ -1636,9 +1637,16 @@ namespace {
 struct CholHost {
     gh::CholState state;
     int flags;
-    int ncand;
-    long long cand[gh::kCholMaxBlock];
+    int ntop;
+    long long top[gh::kCholMaxBlock];  // a block's candidates, largest residual first
+    // uploads of a block start: slot table, new candidates and their slots
+    long long slot_piv[gh::kCholMaxBlock];
+    long long new_piv[gh::kCholMaxBlock];
+    int new_slot[gh::kCholMaxBlock];
 };
+// Device layout of giaohip_cholesky::d_cand (bytes).
+constexpr size_t kCandSlots = 0, kCandTop = 256, kCandNewPiv = 512, kCandNewSlot = 768, kCandCount = 896,
+                 kCandBytes = 1024;
 
 // Pivot steps of a run until the residual is below the tolerance or the rank
 // reaches `target` (0: no target). Per step: when the pivot is not an open
@@ -1674,6 +1682,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         for (int m = 0; m < c->ncand; ++m)
             if (((c->open >> m) & 1u) && c->cand[m] == p) slot = m;
         const bool new_block = slot < 0;
+        bool block_pending = false;
         const long long mu = p / n, nu = p % n;
         const int sa = b->owner[mu], sb = b->owner[nu];
         const DevShell& C = b->shells[sb];
@@ -1732,44 +1741,84 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         }
         CHOL_STEP_TRY(hipEventRecord(c->ev[1], sc), "hipEventRecord");
         if (new_block) {
-            // new block: candidates and their projections onto the rank factors so far
+            // New block: the largest residuals as candidates. Candidates still
+            // open from the previous block keep their slots and projections
+            // (complete up to this rank); only the others are projected, in
+            // one pass over the factors.
+            char* dc = static_cast<char*>(c->d_cand.p);
             CHOL_STEP_TRY(hipEventRecord(c->ev[3], s), "hipEventRecord");
             CHOL_STEP_TRY(gh::launch_chol_topk(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
-                                               c->d_pidx.as<long long>(), c->block, c->d_cand.as<long long>(),
-                                               reinterpret_cast<int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock),
-                                               s),
+                                               c->d_pidx.as<long long>(), c->block,
+                                               reinterpret_cast<long long*>(dc + kCandTop),
+                                               reinterpret_cast<int*>(dc + kCandCount), s),
                           "top-k launch");
-            const int* d_ncand = reinterpret_cast<const int*>(c->d_cand.as<long long>() + gh::kCholMaxBlock);
-            CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(c->rank, 1) * c->slots * sizeof(zc)),
-                          "hipMalloc");
-            CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
-                                               c->d_cand.as<long long>(), d_ncand, c->slots, c->d_coef.as<zc>(), s),
-                          "coefficient launch");
-            CHOL_STEP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
-                                                  c->d_coef.as<zc>(), d_ncand, c->slots, c->d_proj.as<zc>(), s),
-                          "projection launch");
-            CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
-            CHOL_STEP_TRY(hipMemcpyAsync(host->cand, c->d_cand.p, sizeof host->cand, hipMemcpyDeviceToHost, s),
+            CHOL_STEP_TRY(hipEventRecord(c->ev[6], s), "hipEventRecord");
+            CHOL_STEP_TRY(hipMemcpyAsync(host->top, dc + kCandTop, sizeof host->top, hipMemcpyDeviceToHost, s),
                           "copy");
-            CHOL_STEP_TRY(hipMemcpyAsync(&host->ncand, d_ncand, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+            CHOL_STEP_TRY(hipMemcpyAsync(&host->ntop, dc + kCandCount, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
             CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky block");
-            const double bms = elapsed(c->ev[3], c->ev[4]);
-            c->update_ms += bms;
-            c->block_ms += bms;
-            c->rows_read += c->rank;
-            c->ncand = host->ncand;
-            std::copy(host->cand, host->cand + gh::kCholMaxBlock, c->cand);
-            c->open = c->ncand >= 32 ? 0xffffffffu : ((1u << c->ncand) - 1u);
-            c->blocks += 1;
-            for (int m = 0; m < c->ncand; ++m)
-                if (c->cand[m] == p) slot = m;
-            if (slot != 0) {
+            const int ntop = host->ntop;
+            if (ntop < 1 || host->top[0] != p) {
                 (void)hipStreamSynchronize(sc);
                 c->failed = set_error(ctx, GIAOHIP_RUNTIME_ERROR,
                                       "Cholesky pivot block does not start at the maximum residual");
                 return c->failed;
             }
+            unsigned keep = 0;
+            int nnew = 0;
+            for (int m = 0; m < ntop; ++m) {
+                int found = -1;
+                for (int q = 0; q < gh::kCholMaxBlock; ++q)
+                    if (((c->open >> q) & 1u) && c->cand[q] == host->top[m]) found = q;
+                if (found >= 0) {
+                    keep |= 1u << found;
+                } else {
+                    host->new_piv[nnew] = host->top[m];
+                    ++nnew;
+                }
+            }
+            unsigned used = keep;
+            for (int m = 0, q = 0; m < nnew; ++m) {
+                while ((used >> q) & 1u) ++q;  // a free slot: at most ntop <= slots are in use
+                host->new_slot[m] = q;
+                c->cand[q] = host->new_piv[m];
+                used |= 1u << q;
+            }
+            c->open = used;
+            std::copy(c->cand, c->cand + gh::kCholMaxBlock, host->slot_piv);
+            CHOL_STEP_TRY(hipMemcpyAsync(dc + kCandSlots, host->slot_piv, sizeof host->slot_piv,
+                                         hipMemcpyHostToDevice, s),
+                          "copy");
+            CHOL_STEP_TRY(hipMemcpyAsync(dc + kCandNewPiv, host->new_piv, sizeof host->new_piv, hipMemcpyHostToDevice,
+                                         s),
+                          "copy");
+            CHOL_STEP_TRY(hipMemcpyAsync(dc + kCandNewSlot, host->new_slot, sizeof host->new_slot,
+                                         hipMemcpyHostToDevice, s),
+                          "copy");
+            CHOL_STEP_TRY(hipEventRecord(c->ev[7], s), "hipEventRecord");
+            if (nnew > 0) {
+                const int nbt = gh::chol_block_slots(nnew);
+                CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(c->rank, 1) * nbt * sizeof(zc)),
+                              "hipMalloc");
+                CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
+                                                   (int)c->rank, reinterpret_cast<const long long*>(dc + kCandNewPiv),
+                                                   nnew, nbt, c->d_coef.as<zc>(), s),
+                              "coefficient launch");
+                CHOL_STEP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
+                                                      (int)c->rank, c->d_coef.as<zc>(), nnew, nbt,
+                                                      reinterpret_cast<const int*>(dc + kCandNewSlot),
+                                                      c->d_proj.as<zc>(), s),
+                              "projection launch");
+                c->rows_read += c->rank;
+            }
+            CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
+            c->blocks += 1;
+            c->projected += nnew;
+            c->ncand = gh::kCholMaxBlock;  // slot table is searched by the open mask
+            for (int q = 0; q < gh::kCholMaxBlock; ++q)
+                if (((c->open >> q) & 1u) && c->cand[q] == p) slot = q;
             CHOL_STEP_TRY(hipStreamWaitEvent(s, c->ev[1], 0), "hipStreamWaitEvent");
+            block_pending = true;
         }
         const unsigned rest = c->open & ~(1u << slot);
         CHOL_STEP_TRY(hipEventRecord(c->ev[5], s), "hipEventRecord");
@@ -1788,6 +1837,12 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky step");
         c->column_ms += elapsed(c->ev[0], c->ev[1]);
         c->update_ms += elapsed(c->ev[5], c->ev[2]);
+        if (block_pending) {
+            const double bms = elapsed(c->ev[3], c->ev[6]) + elapsed(c->ev[7], c->ev[4]);
+            c->update_ms += bms;
+            c->block_ms += bms;
+            block_pending = false;
+        }
         c->pivots.push_back(mu);
         c->pivots.push_back(nu);
         c->rank += 1;
@@ -1909,7 +1964,7 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     c->block = ctx->chol_block;
     c->slots = gh::chol_block_slots(c->block);
     CHOL_TRY(c->d_proj.ensure((size_t)c->slots * n2 * sizeof(zc)), "hipMalloc(projections)");
-    CHOL_TRY(c->d_cand.ensure(gh::kCholMaxBlock * sizeof(long long) + sizeof(int)), "hipMalloc");
+    CHOL_TRY(c->d_cand.ensure(kCandBytes), "hipMalloc");
     CHOL_TRY(c->d_seeds.ensure((size_t)(seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
     c->d_scratch = reinterpret_cast<int*>(c->d_seeds.as<zc>() + seed_zc);
     CHOL_TRY(hipMemset(c->d_flags.p, 0, sizeof(int)), "hipMemset");
@@ -1973,9 +2028,10 @@ int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max
 }
 
 int giaohip_cholesky_blocks(const giaohip_cholesky* c, uint64_t* blocks, uint64_t* factor_rows_read,
-                            double* block_ms) {
+                            double* block_ms, uint64_t* projected) {
     if (!c) return GIAOHIP_INVALID_ARGUMENT;
     if (blocks) *blocks = c->blocks;
+    if (projected) *projected = c->projected;
     if (factor_rows_read) *factor_rows_read = c->rows_read;
     if (block_ms) *block_ms = c->block_ms;
     return GIAOHIP_OK;

@@ -214,17 +214,16 @@ __global__ __launch_bounds__(kTopNT) void chol_topk_kernel(const double* __restr
     if (threadIdx.x == 0) *ncand = count;
 }
 
-// coef[k][m] = conj(L[k][cand[m]]) for k < rank, m < ncand; zero for the
-// padding slots ncand <= m < nbt.
+// coef[k][m] = conj(L[k][piv[m]]) for k < rank, m < nnew; zero for the
+// padding slots nnew <= m < nbt.
 __global__ void chol_coef_kernel(const zc* const* __restrict__ chunks, int rows_per_chunk, long long n2, int rank,
-                                 const long long* __restrict__ cand, const int* __restrict__ ncand, int nbt,
-                                 zc* __restrict__ coef) {
+                                 const long long* __restrict__ piv, int nnew, int nbt, zc* __restrict__ coef) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)rank * nbt) return;
     const int k = (int)(t / nbt), m = (int)(t % nbt);
     zc v{0.0, 0.0};
-    if (m < *ncand) {
-        const zc x = factor_row(chunks, rows_per_chunk, n2, k)[cand[m]];
+    if (m < nnew) {
+        const zc x = factor_row(chunks, rows_per_chunk, n2, k)[piv[m]];
         v = zc{x.x, -x.y};
     }
     coef[t] = v;
@@ -238,14 +237,16 @@ __device__ __forceinline__ zc gload(const zc* p) {
     return zc{g[0], g[1]};
 }
 
-// proj[m][j] = sum_{k < rank} coef[k][m] L[k][j] for the NB candidate slots;
+// proj[slots[m]][j] = sum_{k < rank} coef[k][m] L[k][j] for the m < nnew new
+// candidates (NB >= nnew coefficient columns);
 // NE entries per thread. The factor rows are read once for all slots, U rows
 // per group with the next group's rows in flight during the current FMAs.
 // Lanes past n2 read entry n2 - 1 and store nothing.
 template <int NB, int NE, int U>
 __global__ __launch_bounds__(256) void chol_project_kernel(const zc* const* __restrict__ chunks, int rows_per_chunk,
                                                            long long n2, int rank, const zc* __restrict__ coef,
-                                                           const int* __restrict__ ncand, zc* __restrict__ proj) {
+                                                           int nnew, const int* __restrict__ slots,
+                                                           zc* __restrict__ proj) {
     const long long base = (long long)blockIdx.x * 256 * NE + threadIdx.x;
     long long jc[NE];
     zc acc[NB][NE];
@@ -302,13 +303,12 @@ __global__ __launch_bounds__(256) void chol_project_kernel(const zc* const* __re
             step(ck + (long long)r * NB, x);
         }
     }
-    const int nc = *ncand;
 #pragma unroll
     for (int m = 0; m < NB; ++m)
-        if (m < nc)
+        if (m < nnew)
 #pragma unroll
             for (int e = 0; e < NE; ++e)
-                if (base + e * 256 < n2) proj[(long long)m * n2 + base + e * 256] = acc[m][e];
+                if (base + e * 256 < n2) proj[(long long)slots[m] * n2 + base + e * 256] = acc[m][e];
 }
 
 // acc[M0 + j] += c(M0 + j) * v for j < 4, where c(m) is the coefficient that
@@ -350,8 +350,8 @@ __device__ __forceinline__ void cmac4_bcast(zc (&acc)[NB], const zc& c, const zc
 template <int NB, int U>
 __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const* __restrict__ chunks,
                                                                  int rows_per_chunk, long long n2, int rank,
-                                                                 const zc* __restrict__ coef,
-                                                                 const int* __restrict__ ncand,
+                                                                 const zc* __restrict__ coef, int nnew,
+                                                                 const int* __restrict__ slots,
                                                                  zc* __restrict__ proj) {
     static_assert(NB == 16 || NB == 32, "one or two coefficients per lane");
     constexpr int NL = NB / 16;  // coefficients per lane and row
@@ -410,11 +410,10 @@ __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const
             step(c, v);
         }
     }
-    const int nc = *ncand;
     if (j < n2)
 #pragma unroll
         for (int m = 0; m < NB; ++m)
-            if (m < nc) proj[(long long)m * n2 + j] = acc[m];
+            if (m < nnew) proj[(long long)slots[m] * n2 + j] = acc[m];
 }
 
 // Accepts the pivot st->pivot, whose raw column is `column` and whose
@@ -524,21 +523,21 @@ hipError_t launch_chol_topk(const double* res, long long n2, const double* pmax,
     return hipGetLastError();
 }
 
-hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const long long* cand,
-                            const int* ncand, int nbt, zc* coef, hipStream_t s) {
+hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const long long* piv,
+                            int nnew, int nbt, zc* coef, hipStream_t s) {
     if (rank == 0) return hipSuccess;
     const long long total = (long long)rank * nbt;
-    chol_coef_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rank, cand,
-                                                                                ncand, nbt, coef);
+    chol_coef_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rank, piv,
+                                                                                nnew, nbt, coef);
     return hipGetLastError();
 }
 
 template <int NB, int NE, int U>
-static hipError_t project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
-                          const int* ncand, zc* proj, hipStream_t s) {
+static hipError_t project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef, int nnew,
+                          const int* slots, zc* proj, hipStream_t s) {
     const long long per = 256LL * NE;
     chol_project_kernel<NB, NE, U><<<dim3((unsigned)((n2 + per - 1) / per)), dim3(256), 0, s>>>(
-        chunks, rows_per_chunk, n2, rank, coef, ncand, proj);
+        chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj);
     return hipGetLastError();
 }
 
@@ -549,19 +548,20 @@ int chol_block_slots(int nb) {
 }
 
 hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
-                               const int* ncand, int nbt, zc* proj, hipStream_t s) {
+                               int nnew, int nbt, const int* slots, zc* proj, hipStream_t s) {
+    if (nnew == 0) return hipSuccess;
     switch (nbt) {
-        case 1: return project<1, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
-        case 2: return project<2, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
-        case 4: return project<4, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
-        case 8: return project<8, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, ncand, proj, s);
+        case 1: return project<1, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
+        case 2: return project<2, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
+        case 4: return project<4, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
+        case 8: return project<8, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
         case 16:
             chol_project_bcast_kernel<16, 4><<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(
-                chunks, rows_per_chunk, n2, rank, coef, ncand, proj);
+                chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj);
             return hipGetLastError();
         case 32:
             chol_project_bcast_kernel<32, 2><<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(
-                chunks, rows_per_chunk, n2, rank, coef, ncand, proj);
+                chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj);
             return hipGetLastError();
         default: return hipErrorInvalidValue;
     }

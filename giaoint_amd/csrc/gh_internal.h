@@ -39,10 +39,12 @@ constexpr int kCholMaxBlock = 32;
 int chol_block_slots(int nb);
 hipError_t launch_chol_topk(const double* res, long long n2, const double* pmax, const long long* pidx, int nb,
                             long long* cand, int* ncand, hipStream_t s);
-hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const long long* cand,
-                            const int* ncand, int nbt, zc* coef, hipStream_t s);
+hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const long long* piv,
+                            int nnew, int nbt, zc* coef, hipStream_t s);
+// Projections of the nnew new candidates (coefficient columns padded to
+// nbt, a power of two <= kCholMaxBlock) into candidate slots slots[m].
 hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
-                               const int* ncand, int nbt, zc* proj, hipStream_t s);
+                               int nnew, int nbt, const int* slots, zc* proj, hipStream_t s);
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
                               long long n2, const CholState* st, double* res, zc* out_row, double psd_tol, int* flags,
                               double* pmax, long long* pidx, hipStream_t s);

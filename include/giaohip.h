@@ -213,12 +213,14 @@ int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* chol, uint64_t s
 int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* max_residual, double* tolerance,
                           int* converged, double* column_ms, double* update_ms, double* diagonal_ms);
 /* Pivot-block statistics of a run: blocks started, factor rows read by their
- * projections (sum of the rank at each block start; each row is n*n complex),
- * and device time of candidate selection + projection (ms, included in
- * update_ms). No counterpart in the reference (its projection is one GEMV
+ * projections (sum of the rank at each block start that projected any
+ * candidate; each row is n*n complex), device time of candidate selection +
+ * projection (ms, included in update_ms), and the number of candidates
+ * projected (candidates still open from the previous block keep their
+ * projections). No counterpart in the reference (its projection is one GEMV
  * per pivot). */
 int giaohip_cholesky_blocks(const giaohip_cholesky* chol, uint64_t* blocks, uint64_t* factor_rows_read,
-                            double* block_ms);
+                            double* block_ms, uint64_t* projected);
 /* Pivot AO pairs (mu, nu), rank*2 int64 values. */
 int giaohip_cholesky_pivots(const giaohip_cholesky* chol, int64_t* out);
 /* Factor rows first..first+count-1 as complex128 (count*n*n values, host). */
