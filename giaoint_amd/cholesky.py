@@ -1,14 +1,14 @@
 """Pivoted Cholesky of the ordered-pair Coulomb matrix, over the HIP library.
 
 Mirrors python/giao_integrals/cholesky.py:16-297 (CholeskyERI, cholesky_eri)
-for the bases whose reference pivot policy is strict global pivoting with one
-column per pivot: multi-centre bases with one contraction per shell in a
-nonzero field (cholesky.py:175-247: every candidate set is the pivot
-itself). Those runs stay on the device end to end (giaohip_cholesky_run).
-Zero-field / single-centre runs (real arithmetic, cholesky.py:175-176) and
-general contractions (contraction-variant candidate blocks and the column
-LRU, cholesky.py:193-262) follow other pivot policies and raise
-NotImplementedError here.
+for bases with one contraction per shell, where the reference's pivot policy
+is strict global pivoting (cholesky.py:175-247: complex multi-centre runs
+take each candidate set as the pivot itself; zero-field and one-centre runs
+use real arithmetic with symmetrized columns and never override the global
+maximum). Those runs stay on the device end to end (giaohip_cholesky_run).
+General contractions (contraction-variant candidate blocks, the local pivot
+override and the column LRU, cholesky.py:193-262) follow another policy and
+raise NotImplementedError here.
 """
 
 from __future__ import annotations
@@ -80,10 +80,6 @@ def cholesky_eri(basis: Basis, *, field: MagneticField | None = None, threshold:
         raise ValueError("max_bytes must be a non-negative integer or None")
     shells = basis.shells
     n = basis.ao_count
-    single_center = all(s.center == shells[0].center for s in shells)
-    real = field is None or all(x == 0 for x in field.B) or single_center
-    if real:
-        raise NotImplementedError("the device Cholesky covers complex (nonzero-field, multi-centre) factors")
     if any(s.contraction_count != 1 for s in shells):
         raise NotImplementedError("the device Cholesky covers bases with one contraction per shell")
     row_bytes = n * n * 16 + 16

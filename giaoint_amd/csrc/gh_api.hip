@@ -166,6 +166,7 @@ struct giaohip_cholesky {
     uint64_t blocks = 0;
     uint64_t rows_read = 0;  // factor rows read by the block projections
     uint64_t projected = 0;  // candidates projected (the others carried over)
+    int real = 0;            // real arithmetic: zero field or one centre (symmetrized columns)
     double block_ms = 0.0;   // candidate selection + projection
     hipEvent_t ev[8]{};
     void* host = nullptr;  // page-locked copy of the state and the PSD flag
@@ -1823,7 +1824,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         const unsigned rest = c->open & ~(1u << slot);
         CHOL_STEP_TRY(hipEventRecord(c->ev[5], s), "hipEventRecord");
         CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2,
-                                             c->d_proj.as<zc>(), c->d_cand.as<long long>(), rest, n2,
+                                             c->d_proj.as<zc>(), c->d_cand.as<long long>(), rest, c->real, n, n2,
                                              c->d_state.as<gh::CholState>(), c->d_res.as<double>(), row, c->psd_tol,
                                              c->d_flags.as<int>(), c->d_pmax.as<double>(), c->d_pidx.as<long long>(), s),
                       "accept launch");
@@ -1891,9 +1892,20 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     hipStream_t s = ctx->stream;
     const long long n = b->nao, n2 = n * n;
     const int ns = b->ns;
+    // cholesky.py:128-133,175-176: on one centre the London phase cancels and
+    // the integrals are taken at zero field; one centre or zero field runs in
+    // real arithmetic (symmetrized columns)
+    bool single_center = true;
+    for (const auto& sh : b->shells)
+        for (int k = 0; k < 3; ++k) single_center = single_center && sh.center[k] == b->shells[0].center[k];
+    double ifield_store[6];
+    for (int k = 0; k < 6; ++k) ifield_store[k] = single_center ? 0.0 : field[k];
+    const double* ifield = ifield_store;
+    const bool real = single_center || (field[0] == 0.0 && field[1] == 0.0 && field[2] == 0.0);
     auto* c = new giaohip_cholesky;
     c->n = n;
     c->basis = b;
+    c->real = real ? 1 : 0;
     // Factor rows live in chunks of about 2 GiB (at least one row).
     c->rows_per_chunk = (int)std::max<long long>(1, (2LL << 30) / (n2 * (long long)sizeof(zc)));
     auto cleanup = [&](int code) {
@@ -1979,7 +1991,7 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
             pairs.push_back((uint64_t)bb);
         }
     CHOL_TRY(hipEventRecord(c->ev[0], s), "hipEventRecord");
-    const int st = diagonal_to_device(ctx, b, pairs.data(), pairs.size() / 2, field, c->d_res.as<double>());
+    const int st = diagonal_to_device(ctx, b, pairs.data(), pairs.size() / 2, ifield, c->d_res.as<double>());
     if (st) return cleanup(st);
     CHOL_TRY(hipEventRecord(c->ev[1], s), "hipEventRecord");
     CHOL_TRY(gh::launch_chol_argmax(c->d_res.as<double>(), n2, c->d_pmax.as<double>(), c->d_pidx.as<long long>(),
@@ -1992,7 +2004,7 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     c->tolerance = std::max(threshold, floor);
     c->psd_tol = std::max(floor * 16.0, 1e-12);
     c->limit = max_rank ? max_rank : (uint64_t)n2;
-    const std::vector<double> kap = wave_vectors(b, field);
+    const std::vector<double> kap = wave_vectors(b, ifield);
     CHOL_TRY(c->d_kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
     CHOL_TRY(hipMemcpy(c->d_kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
     CHOL_TRY(c->d_err.ensure(sizeof(DevError)), "hipMalloc");

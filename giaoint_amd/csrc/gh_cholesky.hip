@@ -421,9 +421,25 @@ __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const
 // updates the residual diagonal (PSD check, clamp, res[p] = 0), the
 // per-block (max, first index) partials and the projections of the candidate
 // slots in `rest` (bit m: slot m at proj + m * n2, pivot cand[m]).
+// With `real` set (zero-field or one-centre bases, real arithmetic in
+// cholesky.py:175-176,269-271) the projected column is first symmetrized:
+// (c[mu,nu] + c[nu,mu]) / 2, imaginary part dropped; n is the AO count.
+__device__ inline zc chol_column_value(const zc* __restrict__ column, const zc* __restrict__ pproj, long long j,
+                                       int real, long long n) {
+    const zc a = column[j], q = pproj[j];
+    zc v{a.x - q.x, a.y - q.y};
+    if (real) {
+        const long long jt = (j % n) * n + j / n;
+        const zc at = column[jt], qt = pproj[jt];
+        v = zc{(v.x + (at.x - qt.x)) * 0.5, 0.0};
+    }
+    return v;
+}
+
 __global__ __launch_bounds__(kUpdNT) void chol_accept_kernel(const zc* __restrict__ column,
                                                              const zc* __restrict__ pproj, zc* proj,
                                                              const long long* __restrict__ cand, unsigned rest,
+                                                             int real, long long n,
                                                              long long n2, const CholState* __restrict__ st,
                                                              double* __restrict__ res, zc* __restrict__ out_row,
                                                              double psd_tol, int* __restrict__ flags,
@@ -443,8 +459,8 @@ __global__ __launch_bounds__(kUpdNT) void chol_accept_kernel(const zc* __restric
         j[e] = base + e * kUpdNT + tid;
         val[e] = zc{0.0, 0.0};
         if (j[e] < n2) {
-            const zc a = column[j[e]], q = pproj[j[e]];
-            val[e] = zc{(a.x - q.x) / root, (a.y - q.y) / root};
+            const zc cv = chol_column_value(column, pproj, j[e], real, n);
+            val[e] = zc{cv.x / root, cv.y / root};
             out_row[j[e]] = val[e];
             double nr = res[j[e]] - (val[e].x * val[e].x + val[e].y * val[e].y);
             if (nr < -psd_tol) atomicOr(flags, 1);
@@ -457,8 +473,8 @@ __global__ __launch_bounds__(kUpdNT) void chol_accept_kernel(const zc* __restric
     for (unsigned mask = rest; mask; mask &= mask - 1) {
         const int m = __builtin_ctz(mask);
         const long long pm = cand[m];
-        const zc a = column[pm], q = pproj[pm];
-        const zc c{(a.x - q.x) / root, -((a.y - q.y) / root)};
+        const zc cv = chol_column_value(column, pproj, pm, real, n);
+        const zc c{cv.x / root, -(cv.y / root)};
         zc* pr = proj + (long long)m * n2;
 #pragma unroll
         for (int e = 0; e < kUpdPerThread; ++e) {
@@ -568,10 +584,10 @@ hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long
 }
 
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
-                              long long n2, const CholState* st, double* res, zc* out_row, double psd_tol, int* flags,
-                              double* pmax, long long* pidx, hipStream_t s) {
+                              int real, long long n, long long n2, const CholState* st, double* res, zc* out_row,
+                              double psd_tol, int* flags, double* pmax, long long* pidx, hipStream_t s) {
     chol_accept_kernel<<<dim3((unsigned)chol_partials(n2)), dim3(kUpdNT), 0, s>>>(
-        column, pproj, proj, cand, rest, n2, st, res, out_row, psd_tol, flags, pmax, pidx);
+        column, pproj, proj, cand, rest, real, n, n2, st, res, out_row, psd_tol, flags, pmax, pidx);
     return hipGetLastError();
 }
 

@@ -46,8 +46,8 @@ hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long lo
 hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
                                int nnew, int nbt, const int* slots, zc* proj, hipStream_t s);
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
-                              long long n2, const CholState* st, double* res, zc* out_row, double psd_tol, int* flags,
-                              double* pmax, long long* pidx, hipStream_t s);
+                              int real, long long n, long long n2, const CholState* st, double* res, zc* out_row,
+                              double psd_tol, int* flags, double* pmax, long long* pidx, hipStream_t s);
 hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
                               bool partial_first, hipStream_t s);
 

@@ -194,6 +194,11 @@ int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* basis, const
  * order), column ERIs, projection, scaling, residual update, PSD check and
  * clamp all run on the device; factors stay in device memory.
  * Requires ncontr == 1 for every shell (GIAOHIP_UNSUPPORTED otherwise).
+ * As in cholesky.py:128-133,175-176,269-271, a basis on one centre is
+ * factored at zero field (the London phase cancels), and zero-field or
+ * one-centre runs use real arithmetic: each projected column is
+ * symmetrized, (c[mu,nu] + c[nu,mu]) / 2, and the factors are real
+ * (stored as complex128 with zero imaginary parts).
  * Stops when the largest residual diagonal is <= max(threshold,
  * 64*eps*max(diagonal)); max_rank caps the number of vectors
  * (GIAOHIP_OUT_OF_MEMORY beyond it, as MemoryError in the reference);

@@ -134,3 +134,37 @@ def test_device_cholesky_blocks_across_factor_chunks(gpu, port):
     for first in range(0, 1400, 200):
         np.testing.assert_array_equal(blk.vectors(first, 200), one.vectors(first, 200))
     assert blk.max_residual == one.max_residual
+
+
+@pytest.mark.parametrize("case", ["zero-field", "one-centre"])
+def test_device_cholesky_real_arithmetic(gpu, port, case):
+    """Zero-field and one-centre bases run in real arithmetic with
+    symmetrized columns (cholesky.py:128-133,175-176,269-271); on one centre
+    the integrals are taken at zero field whatever B is. Checker replay as
+    for the complex runs; the factors are real and symmetric."""
+    if case == "zero-field":
+        sh, _, _ = workloads.cholesky_atoms(natoms=2)
+        B, O = (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)
+    else:
+        sh, B, O = workloads.cholesky_atoms(natoms=1)
+    b = make_basis(sh.shell_list(), port)
+    f = _native.field6(B, O)
+    fint = f if case == "zero-field" else _native.field6()
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    res = _native.cholesky_run(db, f, 1e-8)
+    assert res.converged and res.max_residual <= res.tolerance
+    gp = res.pivots()
+    stats = {}
+    vec, piv, maxres, tol = cholesky_driver.cholesky_eri(
+        b.ao_offsets, [((l + 1) * (l + 2) // 2, 1, 1) for l in b.L],
+        lambda pairs: port.cholesky_diagonal(b, pairs, fint, 16),
+        lambda pv: port.cholesky_columns(b, pv, fint, 16, real=True),
+        real=True, single_center=(case == "one-centre"), threshold=1e-8, follow=gp, stats=stats,
+    )
+    print(f"{case}: rank {res.rank}; rounding-level tie decisions {stats.get('tie_flips', 0)}")
+    assert len(piv) == res.rank
+    got = res.vectors()
+    assert np.all(got.imag == 0.0)
+    np.testing.assert_array_equal(got, got.transpose(0, 2, 1))
+    assert np.max(np.abs(got - vec)) <= 1e-10
+    assert abs(res.max_residual - maxres) <= 1e-12
