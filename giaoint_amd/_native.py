@@ -96,6 +96,8 @@ _SIGNATURES = {
     "giaohip_cholesky_vectors": ([_vp, _vp, C.c_uint64, C.c_uint64, _dp], C.c_int),
     "giaohip_cholesky_resume": ([_vp, _vp, C.c_uint64], C.c_int),
     "giaohip_cholesky_blocks": ([_vp, _u64p, _u64p, _dp, _u64p], C.c_int),
+    "giaohip_cholesky_reconstruct": ([_vp, _vp, C.POINTER(C.c_int64), _dp], C.c_int),
+    "giaohip_cholesky_coulomb": ([_vp, _vp, _dp, _dp], C.c_int),
     "giaohip_cholesky_destroy": ([_vp, _vp], C.c_int),
 }
 
@@ -424,6 +426,29 @@ class DeviceCholesky:
             self.ctx.lib.giaohip_cholesky_vectors(self.ctx.handle, self.handle, C.c_uint64(first), C.c_uint64(count),
                                                   out.ctypes.data_as(_dp))
         )
+        return out
+
+    def reconstruct(self, mu=(0, None), nu=(0, None), lam=(0, None), sig=(0, None)) -> np.ndarray:
+        """(mu nu|lam sig) ~ sum_P L_P[mu,nu] conj(L_P[sig,lam]) on AO ranges (start, stop), on the device."""
+        n = self.basis.nao
+        r = []
+        for a, b in (mu, nu, lam, sig):
+            r += [int(a), n if b is None else int(b)]
+        rr = np.ascontiguousarray(r, dtype=np.int64)
+        shape = tuple(max(0, r[2 * k + 1] - r[2 * k]) for k in range(4))
+        out = np.zeros(shape, dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_reconstruct(self.ctx.handle, self.handle,
+                                                                 rr.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                                 out.ctypes.data_as(_dp)))
+        return out
+
+    def coulomb(self, density) -> np.ndarray:
+        """J[mu,nu] = sum_{lam,sig} (mu nu|lam sig) D[lam,sig] from the device factors."""
+        n = self.basis.nao
+        d = np.ascontiguousarray(density, dtype=np.complex128).reshape(n, n)
+        out = np.empty((n, n), dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_coulomb(self.ctx.handle, self.handle, d.ctypes.data_as(_dp),
+                                                             out.ctypes.data_as(_dp)))
         return out
 
     def close(self):

@@ -62,6 +62,14 @@ struct DevBuffer {
     }
 };
 
+// A DevBuffer released when it goes out of scope (temporaries of one call).
+struct ScopedBuffer : DevBuffer {
+    ScopedBuffer() = default;
+    ScopedBuffer(const ScopedBuffer&) = delete;
+    ScopedBuffer& operator=(const ScopedBuffer&) = delete;
+    ~ScopedBuffer() { release(); }
+};
+
 // Page-locked host staging: copies from it are asynchronous with respect to
 // the host, so the host can plan the next chunk while the device works.
 struct PinnedBuffer {
@@ -2046,6 +2054,59 @@ int giaohip_cholesky_blocks(const giaohip_cholesky* c, uint64_t* blocks, uint64_
     if (projected) *projected = c->projected;
     if (factor_rows_read) *factor_rows_read = c->rows_read;
     if (block_ms) *block_ms = c->block_ms;
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_reconstruct(giaohip_ctx* ctx, const giaohip_cholesky* c, const int64_t ranges[8], double* out) {
+    if (!ctx || !c || !ranges || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    long long r[8];
+    long long total = 1;
+    for (int k = 0; k < 4; ++k) {
+        r[2 * k] = ranges[2 * k];
+        r[2 * k + 1] = ranges[2 * k + 1];
+        if (r[2 * k] < 0 || r[2 * k] > r[2 * k + 1] || r[2 * k + 1] > c->n)
+            return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky reconstruction range is out of range");
+        total *= r[2 * k + 1] - r[2 * k];
+    }
+    if (total == 0) return GIAOHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    ScopedBuffer d_r, d_out;
+    HIP_TRY(d_r.ensure(sizeof r), "hipMalloc");
+    HIP_TRY(d_out.ensure((size_t)total * sizeof(zc)), "hipMalloc");
+    HIP_TRY(hipMemcpyAsync(d_r.p, r, sizeof r, hipMemcpyHostToDevice, s), "copy");
+    HIP_TRY(gh::launch_chol_reconstruct(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, c->n, (int)c->rank,
+                                        d_r.as<long long>(), total, d_out.as<zc>(), s),
+            "reconstruct launch");
+    HIP_TRY(hipMemcpyAsync(out, d_out.p, (size_t)total * sizeof(zc), hipMemcpyDeviceToHost, s), "copy");
+    HIP_TRY(hipStreamSynchronize(s), "reconstruct");
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_coulomb(giaohip_ctx* ctx, const giaohip_cholesky* c, const double* density, double* out) {
+    if (!ctx || !c || !density || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const long long n = c->n, n2 = n * n;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    ScopedBuffer d_dens, d_gamma, d_j, d_slot;
+    HIP_TRY(d_dens.ensure((size_t)n2 * sizeof(zc)), "hipMalloc");
+    HIP_TRY(d_gamma.ensure((size_t)std::max<uint64_t>(c->rank, 1) * sizeof(zc)), "hipMalloc");
+    HIP_TRY(d_j.ensure((size_t)n2 * sizeof(zc)), "hipMalloc");
+    HIP_TRY(d_slot.ensure(sizeof(int)), "hipMalloc");
+    const int zero = 0;
+    HIP_TRY(hipMemcpyAsync(d_slot.p, &zero, sizeof zero, hipMemcpyHostToDevice, s), "copy");
+    HIP_TRY(hipMemcpyAsync(d_dens.p, density, (size_t)n2 * sizeof(zc), hipMemcpyHostToDevice, s), "copy");
+    // gamma_P = sum conj(L_P[s,l]) D[l,s]; J = sum_P gamma_P L_P (one pass over the factors each)
+    HIP_TRY(gh::launch_chol_gamma(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n, (int)c->rank,
+                                  d_dens.as<zc>(), d_gamma.as<zc>(), s),
+            "gamma launch");
+    HIP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                    d_gamma.as<zc>(), 1, 1, d_slot.as<int>(), d_j.as<zc>(), s),
+            "coulomb launch");
+    HIP_TRY(hipMemcpyAsync(out, d_j.p, (size_t)n2 * sizeof(zc), hipMemcpyDeviceToHost, s), "copy");
+    HIP_TRY(hipStreamSynchronize(s), "coulomb");
     return GIAOHIP_OK;
 }
 

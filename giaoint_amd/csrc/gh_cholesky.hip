@@ -521,6 +521,64 @@ __global__ __launch_bounds__(kUpdNT) void chol_argmax_final_kernel(const double*
 }
 
 // ---------------------------------------------------------------------------
+// Consumers of the device factors (no copy of the factors to the host)
+// ---------------------------------------------------------------------------
+// gamma[k] = sum_{s,l} conj(L[k][s*n + l]) D[l*n + s]: one workgroup per
+// factor row, fixed-order reduction.
+__global__ __launch_bounds__(256) void chol_gamma_kernel(const zc* const* __restrict__ chunks, int rows_per_chunk,
+                                                         long long n, const zc* __restrict__ dens,
+                                                         zc* __restrict__ gamma) {
+    __shared__ zc red[256];
+    const int k = blockIdx.x;
+    const zc* row = factor_row(chunks, rows_per_chunk, n * n, k);
+    zc acc{0.0, 0.0};
+    for (long long x = threadIdx.x; x < n * n; x += 256) {
+        const long long s = x / n, l = x - s * n;
+        const zc v = row[x], d = dens[l * n + s];
+        // conj(v) * d
+        acc.x = fma(v.x, d.x, acc.x);
+        acc.x = fma(v.y, d.y, acc.x);
+        acc.y = fma(v.x, d.y, acc.y);
+        acc.y = fma(-v.y, d.x, acc.y);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x].x += red[threadIdx.x + w].x;
+            red[threadIdx.x].y += red[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gamma[k] = red[0];
+}
+
+// out[m,a,l,s] (C order over the four ranges) = sum_k L[k][mu,nu] conj(L[k][sig,lam])
+// with mu = r[0] + m, nu = r[2] + a, lam = r[4] + l, sig = r[6] + s.
+__global__ __launch_bounds__(256) void chol_reconstruct_kernel(const zc* const* __restrict__ chunks,
+                                                               int rows_per_chunk, long long n, int rank,
+                                                               const long long* __restrict__ r,
+                                                               zc* __restrict__ out) {
+    const long long nm = r[1] - r[0], na = r[3] - r[2], nl = r[5] - r[4], ns = r[7] - r[6];
+    const long long total = nm * na * nl * ns;
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const long long s = t % ns, l = (t / ns) % nl, a = (t / (ns * nl)) % na, m = t / (ns * nl * na);
+    const long long left = (r[0] + m) * n + (r[2] + a), right = (r[6] + s) * n + (r[4] + l);
+    zc acc{0.0, 0.0};
+    for (int k = 0; k < rank; ++k) {
+        const zc* row = factor_row(chunks, rows_per_chunk, n * n, k);
+        const zc u = row[left], v = row[right];
+        // u * conj(v)
+        acc.x = fma(u.x, v.x, acc.x);
+        acc.x = fma(u.y, v.y, acc.x);
+        acc.y = fma(u.y, v.x, acc.y);
+        acc.y = fma(-u.x, v.y, acc.y);
+    }
+    out[t] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------
 int chol_partials(long long n2) { return (int)((n2 + kUpdNT * kUpdPerThread - 1) / (kUpdNT * kUpdPerThread)); }
@@ -581,6 +639,21 @@ hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long
             return hipGetLastError();
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_chol_gamma(const zc* const* chunks, int rows_per_chunk, long long n, int rank, const zc* dens,
+                             zc* gamma, hipStream_t s) {
+    if (rank == 0) return hipSuccess;
+    chol_gamma_kernel<<<dim3((unsigned)rank), dim3(256), 0, s>>>(chunks, rows_per_chunk, n, dens, gamma);
+    return hipGetLastError();
+}
+
+hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, long long n, int rank,
+                                   const long long* ranges, long long total, zc* out, hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    chol_reconstruct_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n,
+                                                                                       rank, ranges, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,

@@ -45,6 +45,12 @@ hipError_t launch_chol_coef(const zc* const* chunks, int rows_per_chunk, long lo
 // nbt, a power of two <= kCholMaxBlock) into candidate slots slots[m].
 hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long long n2, int rank, const zc* coef,
                                int nnew, int nbt, const int* slots, zc* proj, hipStream_t s);
+// Consumers of the device factors: gamma[k] = sum conj(L[k][s,l]) D[l,s]
+// (one workgroup per factor row) and reconstruction of AO blocks.
+hipError_t launch_chol_gamma(const zc* const* chunks, int rows_per_chunk, long long n, int rank, const zc* dens,
+                             zc* gamma, hipStream_t s);
+hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, long long n, int rank,
+                                   const long long* ranges, long long total, zc* out, hipStream_t s);
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
                               int real, long long n, long long n2, const CholState* st, double* res, zc* out_row,
                               double psd_tol, int* flags, double* pmax, long long* pidx, hipStream_t s);

@@ -226,6 +226,20 @@ int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* 
  * per pivot). */
 int giaohip_cholesky_blocks(const giaohip_cholesky* chol, uint64_t* blocks, uint64_t* factor_rows_read,
                             double* block_ms, uint64_t* projected);
+/* Reconstructed ERIs from the device factors, (mu nu|lam sig) ~
+ * sum_P L_P[mu,nu] conj(L_P[sig,lam]), on the AO ranges mu in
+ * [ranges[0], ranges[1]), nu in [ranges[2], ranges[3]), lam in [ranges[4],
+ * ranges[5]), sig in [ranges[6], ranges[7]); out is the C-order complex128
+ * block (host). Replaces CholeskyERI.reconstruct for contiguous slices
+ * (python/giao_integrals/cholesky.py:39-57) without copying the factors. */
+int giaohip_cholesky_reconstruct(giaohip_ctx* ctx, const giaohip_cholesky* chol, const int64_t ranges[8],
+                                 double* out);
+/* Coulomb-type contraction with the device factors:
+ * J[mu,nu] = sum_{lam,sig} (mu nu|lam sig) D[lam,sig] ~ sum_P L_P[mu,nu] gamma_P,
+ * gamma_P = sum_{lam,sig} conj(L_P[sig,lam]) D[lam,sig]. density and out are
+ * n*n complex128 (host). Two passes over the factors. No counterpart in the
+ * reference (a device-resident consumer of its CholeskyERI factors). */
+int giaohip_cholesky_coulomb(giaohip_ctx* ctx, const giaohip_cholesky* chol, const double* density, double* out);
 /* Pivot AO pairs (mu, nu), rank*2 int64 values. */
 int giaohip_cholesky_pivots(const giaohip_cholesky* chol, int64_t* out);
 /* Factor rows first..first+count-1 as complex128 (count*n*n values, host). */

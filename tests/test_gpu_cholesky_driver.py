@@ -168,3 +168,34 @@ def test_device_cholesky_real_arithmetic(gpu, port, case):
     np.testing.assert_array_equal(got, got.transpose(0, 2, 1))
     assert np.max(np.abs(got - vec)) <= 1e-10
     assert abs(res.max_residual - maxres) <= 1e-12
+
+
+def test_device_factor_consumers(gpu, port):
+    """Reconstruction of AO blocks and the Coulomb contraction
+    J[mu,nu] = sum (mu nu|lam sig) D[lam,sig] computed from the device factors,
+    against numpy on the fetched factors and against the ERI tensor itself
+    (reconstruction bound: every element within the threshold)."""
+    sh, B, O = workloads.cholesky_atoms(natoms=2)
+    b = make_basis(sh.shell_list(), port)
+    f = _native.field6(B, O)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    res = _native.cholesky_run(db, f, 1e-8)
+    L = res.vectors()
+    n = db.nao
+    got = res.reconstruct((3, 20), (0, 15), (5, 30), (10, 25))
+    want = np.einsum("Pmn,Psl->mnls", L[:, 3:20, 0:15], L[:, 10:25, 5:30].conj())
+    assert got.shape == (17, 15, 25, 15)
+    assert np.max(np.abs(got - want)) <= 1e-13 * max(1.0, np.abs(want).max())
+    exact = db.eri_tensor(f)
+    full = res.reconstruct()
+    assert np.max(np.abs(full - exact)) <= 1e-8 + 1e-12
+    rng = np.random.default_rng(7)
+    D = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    J = res.coulomb(D)
+    gamma = np.einsum("Psl,ls->P", L.conj(), D)
+    Jw = np.einsum("Pmn,P->mn", L, gamma)
+    assert np.max(np.abs(J - Jw)) <= 1e-11 * np.abs(Jw).max()
+    Jx = np.einsum("mnls,ls->mn", exact, D)
+    assert np.max(np.abs(J - Jx)) <= (1e-8 + 1e-12) * np.abs(D).sum()
+    with pytest.raises(IndexError):
+        res.reconstruct((0, n + 1))
