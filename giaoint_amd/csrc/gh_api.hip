@@ -176,7 +176,8 @@ struct giaohip_cholesky {
     uint64_t projected = 0;  // candidates projected (the others carried over)
     int real = 0;            // real arithmetic: zero field or one centre (symmetrized columns)
     double block_ms = 0.0;   // candidate selection + projection
-    hipEvent_t ev[8]{};
+    hipEvent_t ev[10]{};
+    hipStream_t s3 = nullptr;  // second stream of the column launches
     void* host = nullptr;  // page-locked copy of the state and the PSD flag
     void release() {
         for (zc* c : chunks) (void)hipFree(c);
@@ -186,6 +187,8 @@ struct giaohip_cholesky {
             x->release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (s3) (void)hipStreamDestroy(s3);
+        s3 = nullptr;
         if (host) (void)hipHostFree(host);
         host = nullptr;
     }
@@ -1730,7 +1733,22 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                                       b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
                                       c->d_seeds.as<zc>(), c->d_err.as<DevError>(), c->d_scratch, sc),
                       "seed launch");
-        for (const auto& g : c->groups) {
+        // the (la, lb) groups of bra pairs write disjoint rows of the column:
+        // the larger ones first, spread over two streams
+        CHOL_STEP_TRY(hipEventRecord(c->ev[8], sc), "hipEventRecord");
+        CHOL_STEP_TRY(hipStreamWaitEvent(c->s3, c->ev[8], 0), "hipStreamWaitEvent");
+        std::vector<std::pair<double, size_t>> gorder;
+        for (size_t gi = 0; gi < c->groups.size(); ++gi) {
+            const auto& g = c->groups[gi];
+            gorder.push_back({(double)g[3] * gh::nherm(g[0] + g[1] + C.l + D.l), gi});
+        }
+        std::sort(gorder.begin(), gorder.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        double gq[2] = {0.0, 0.0};
+        for (const auto& [gcost, gi] : gorder) {
+            const auto& g = c->groups[gi];
+            const int side = gq[1] < gq[0] ? 1 : 0;
+            gq[side] += gcost;
+            hipStream_t gs = side ? c->s3 : sc;
             gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
             const bool column_kernel = g[0] + g[1] + C.l + D.l <= gh::kColumnMaxL &&
                                        gh::chol_column_lds_bytes(g[0], g[1], C.l, D.l, ket.comp) <= 64 * 1024;
@@ -1739,15 +1757,17 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                                                      b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
                                                      b->d_coefs.as<double>(), b->d_norms.as<double>(),
                                                      c->d_kappa.as<double>(), c->d_seeds.as<zc>(), c->d_col.as<zc>(),
-                                                     sc),
+                                                     gs),
                               "column launch");
             else
                 CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
                                                    b->d_exps.as<double>(), b->d_coefs.as<double>(),
                                                    b->d_norms.as<double>(), c->d_kappa.as<double>(),
-                                                   c->d_seeds.as<zc>(), c->d_col.as<zc>(), sc),
+                                                   c->d_seeds.as<zc>(), c->d_col.as<zc>(), gs),
                               "transform launch");
         }
+        CHOL_STEP_TRY(hipEventRecord(c->ev[9], c->s3), "hipEventRecord");
+        CHOL_STEP_TRY(hipStreamWaitEvent(sc, c->ev[9], 0), "hipStreamWaitEvent");
         CHOL_STEP_TRY(hipEventRecord(c->ev[1], sc), "hipEventRecord");
         if (new_block) {
             // New block: the largest residuals as candidates. Candidates still
@@ -1989,6 +2009,7 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     c->d_scratch = reinterpret_cast<int*>(c->d_seeds.as<zc>() + seed_zc);
     CHOL_TRY(hipMemset(c->d_flags.p, 0, sizeof(int)), "hipMemset");
     for (auto& e : c->ev) CHOL_TRY(hipEventCreate(&e), "hipEventCreate");
+    CHOL_TRY(hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking), "hipStreamCreate");
     CHOL_TRY(hipHostMalloc(&c->host, sizeof(CholHost), hipHostMallocDefault), "hipHostMalloc");
 
     // Diagonal over all ordered shell pairs a <= b (cholesky.py:143-169).
@@ -2136,6 +2157,7 @@ int giaohip_cholesky_destroy(giaohip_ctx* ctx, giaohip_cholesky* c) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamSynchronize(ctx->aux);
+        if (c->s3) (void)hipStreamSynchronize(c->s3);
     }
     c->release();
     delete c;
