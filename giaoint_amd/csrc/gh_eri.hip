@@ -849,13 +849,22 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
                                                          const double* __restrict__ coefs,
                                                          const double* __restrict__ norms,
                                                          const double* __restrict__ kappa,
-                                                         const zc* __restrict__ seeds, zc* __restrict__ out) {
+                                                         const zc* __restrict__ seeds, zc* __restrict__ out,
+                                                         int nitems, int per_pair_bytes) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    // ca.tile carries P, the bra pairs per wavefront: one per group of
+    // stride = 64 / P lanes, each with its own LDS region; the groups share
+    // only the wave fences
+    const int stride = 64 / ca.tile;
+    const int sub = (int)threadIdx.x / stride;
+    const int lane = (int)threadIdx.x % stride;
+    const int item = blockIdx.x * ca.tile + sub;
+    if (item >= nitems) return;  // no block barriers below: the rest of the wavefront continues
+    unsigned char* smem_pair = smem_raw + (size_t)sub * per_pair_bytes;
     const int la = ca.l[0], lb = ca.l[1], lc = ca.l[2], ld = ca.l[3];
     const int L = la + lb + lc + ld, Lab = la + lb, Lcd = lc + ld;
     const int NHL = nherm(L), NHab = nherm(Lab);
-    const int lane = threadIdx.x;
-    const WorkItem it = items[blockIdx.x];
+    const WorkItem it = items[item];
     const DevShell A = shells[it.sh[0]];
     const DevShell B = shells[it.sh[1]];
     const DevShell C = shells[it.sh[2]];
@@ -870,7 +879,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
     const int bx = cpow[0] + dpow[0] + 1, by = cpow[1] + dpow[1] + 1, bz = cpow[2] + dpow[2] + 1;
     const int nbox = bx * by * bz;
     const int ebra_i = (lb + 1) * (Lab + 1), ebra_a = (la + 1) * ebra_i;
-    zc* buf0 = reinterpret_cast<zc*>(smem_raw);
+    zc* buf0 = reinterpret_cast<zc*>(smem_pair);
     zc* buf1 = buf0 + NHL;
     zc* ebra = buf1 + NHL;
     zc* e1d = ebra + 3 * ebra_a;
@@ -879,7 +888,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
     zc* acc = vv + NHab;
     zc* sseed = acc + nab;
     const unsigned* rl = c_rlist_all + rlist_offset(L);
-    for (int o = lane; o < nab; o += 64) acc[o] = zc{0.0, 0.0};
+    for (int o = lane; o < nab; o += stride) acc[o] = zc{0.0, 0.0};
     const int Kb = B.nprim, Kd = D.nprim;
     const int Kab = A.nprim * Kb, Kcd = C.nprim * Kd;
     for (int pab = 0; pab < Kab; ++pab) {
@@ -887,7 +896,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
         const double ea = exps[A.prim_off + pa], eb = exps[B.prim_off + pb];
         const double p = ea + eb;
         wave_barrier();
-        for (int e = lane; e < 3 * (la + 1) * (lb + 1); e += 64) {
+        for (int e = lane; e < 3 * (la + 1) * (lb + 1); e += stride) {
             const int axis = e / ((la + 1) * (lb + 1));
             const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
             const double Pc = (ea * A.center[axis] + eb * B.center[axis]) / p;
@@ -895,7 +904,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
             build_phase_hermite(i, j, Pc - A.center[axis], Pc - B.center[axis], p, qv,
                                 ebra + axis * ebra_a + i * ebra_i + j * (Lab + 1));
         }
-        for (int k = lane; k < NHab; k += 64) vv[k] = zc{0.0, 0.0};
+        for (int k = lane; k < NHab; k += stride) vv[k] = zc{0.0, 0.0};
         for (int pcd = 0; pcd < Kcd; ++pcd) {
             const int pc = pcd / Kd, pd = pcd % Kd;
             const double ec = exps[C.prim_off + pc], ed = exps[D.prim_off + pd];
@@ -911,7 +920,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
                                     e1d + axis * (Lcd + 1));
             }
             wave_barrier();
-            for (int e = lane; e < nbox; e += 64) {
+            for (int e = lane; e < nbox; e += stride) {
                 const int jt = e / (by * bz), ju = (e / bz) % by, jv = e % bz;
                 zc v = zmul(zmul(e1d[jt], e1d[(Lcd + 1) + ju]), e1d[2 * (Lcd + 1) + jv]);
                 if ((jt + ju + jv) & 1) v = zneg(v);
@@ -925,7 +934,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
                 const zc* cur = ((L - n) & 1) ? buf0 : buf1;
                 zc* nxt = ((L - n) & 1) ? buf1 : buf0;
                 const int cnt = nherm(L - n);
-                for (int e = lane; e < cnt; e += 64) {
+                for (int e = lane; e < cnt; e += stride) {
                     const unsigned pk = rl[e];
                     const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = (int)(pk >> 18);
                     zc val;
@@ -960,7 +969,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
             const zc* R0 = (L & 1) ? buf1 : buf0;
             const double sc = coefs[C.coef_off + it.x[2] * C.nprim + pc] * norms[C.norm_off + cc * C.nprim + pc] *
                               coefs[D.coef_off + it.x[3] * D.nprim + pd] * norms[D.norm_off + dc * D.nprim + pd];
-            for (int e = lane; e < NHab; e += 64) {
+            for (int e = lane; e < NHab; e += stride) {
                 const unsigned pk = c_rlist_all[rlist_offset(Lab) + e];
                 const int kt = pk & 63, ku = (pk >> 6) & 63, kv = (pk >> 12) & 63, k = (int)(pk >> 18);
                 zc s{0.0, 0.0};
@@ -975,7 +984,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
             }
         }
         wave_barrier();
-        for (int o = lane; o < nab; o += 64) {
+        for (int o = lane; o < nab; o += stride) {
             const int cac = o / ncb, cbc = o % ncb;
             const unsigned pap = cart_of(la, cac), pbp = cart_of(lb, cbc);
             const int ax = pap & 255, ay = (pap >> 8) & 255, az = pap >> 16;
@@ -997,7 +1006,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
         }
     }
     const long long ic = (long long)it.x[2] * ncart(lc) + cc, id = (long long)it.x[3] * ncd + dc;
-    for (int o = lane; o < nab; o += 64) {
+    for (int o = lane; o < nab; o += stride) {
         const int cac = o / ncb, cbc = o % ncb;
         const long long ia = (long long)it.x[0] * nca + cac, ib = (long long)it.x[1] * ncb + cbc;
         out[it.out_base + ia * it.stride[0] + ib * it.stride[1] + ic * it.stride[2] + id * it.stride[3]] = acc[o];
@@ -1032,8 +1041,14 @@ hipError_t launch_chol_column(const ClassArgs& ca, int comp, const WorkItem* ite
     if (L > kColMaxL) return hipErrorInvalidValue;
     const int bytes = chol_column_lds_bytes(ca.l[0], ca.l[1], ca.l[2], ca.l[3], comp);
     if (bytes > 64 * 1024) return hipErrorInvalidValue;
-    chol_column_kernel<<<dim3((unsigned)nitems), dim3(64), bytes, s>>>(ca, items, shells, exps, coefs, norms, kappa,
-                                                                        seeds, out);
+    // several bra pairs per wavefront when the pairs are small (their
+    // loops are narrower than a wavefront and each pair is mostly latency)
+    const int lds_cap = 64 * 1024;
+    const int P = (L <= 6 && 4 * bytes <= lds_cap) ? 4 : ((L <= 10 && 2 * bytes <= lds_cap) ? 2 : 1);
+    ClassArgs cp = ca;
+    cp.tile = P;
+    chol_column_kernel<<<dim3((unsigned)((nitems + P - 1) / P)), dim3(64), P * bytes, s>>>(
+        cp, items, shells, exps, coefs, norms, kappa, seeds, out, nitems, bytes);
     return hipGetLastError();
 }
 
