@@ -1044,7 +1044,10 @@ hipError_t launch_chol_column(const ClassArgs& ca, int comp, const WorkItem* ite
     // several bra pairs per wavefront when the pairs are small (their
     // loops are narrower than a wavefront and each pair is mostly latency)
     const int lds_cap = 64 * 1024;
-    const int P = (L <= 6 && 4 * bytes <= lds_cap) ? 4 : ((L <= 10 && 2 * bytes <= lds_cap) ? 2 : 1);
+    const int P = (L <= 3 && 8 * bytes <= lds_cap)    ? 8
+                  : (L <= 6 && 4 * bytes <= lds_cap)  ? 4
+                  : (L <= 10 && 2 * bytes <= lds_cap) ? 2
+                                                      : 1;
     ClassArgs cp = ca;
     cp.tile = P;
     chol_column_kernel<<<dim3((unsigned)((nitems + P - 1) / P)), dim3(64), P * bytes, s>>>(
