@@ -530,20 +530,18 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
                         const zc* Ex = eket + 0 * eket_a + cx * eket_i + dx * (Lcd + 1);
                         const zc* Ey = eket + 1 * eket_a + cy * eket_i + dy * (Lcd + 1);
                         const zc* Ez = eket + 2 * eket_a + cz * eket_i + dz * (Lcd + 1);
+                        // sum_{tau,phi,chi} (-1)^(tau+phi+chi) Ex Ey Ez R0, one axis at a time
                         zc s{0.0, 0.0};
                         for (int tau = 0; tau <= cx + dx; ++tau) {
-                            zc ex = Ex[tau];
-                            if (tau & 1) ex = zneg(ex);
+                            zc st{0.0, 0.0};
                             for (int phi = 0; phi <= cy + dy; ++phi) {
-                                zc exy = zmul(ex, Ey[phi]);
-                                if (phi & 1) exy = zneg(exy);
                                 const zc* rr = R0 + offL[(t + tau) * (L + 1) + (u + phi)] + v;
-                                for (int chi = 0; chi <= cz + dz; ++chi) {
-                                    zc e3 = zmul(exy, Ez[chi]);
-                                    if (chi & 1) e3 = zneg(e3);
-                                    zmac(s, e3, rr[chi]);
-                                }
+                                zc stp{0.0, 0.0};
+                                for (int chi = 0; chi <= cz + dz; ++chi)
+                                    zmac(stp, (chi & 1) ? zneg(Ez[chi]) : Ez[chi], rr[chi]);
+                                zmac(st, (phi & 1) ? zneg(Ey[phi]) : Ey[phi], stp);
                             }
+                            zmac(s, (tau & 1) ? zneg(Ex[tau]) : Ex[tau], st);
                         }
                         const double sc = coefs[C.coef_off + it.x[2] * C.nprim + pc] *
                                           norms[C.norm_off + cc * C.nprim + pc] *
@@ -582,13 +580,17 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
                     const zc* Ex = ebra + 0 * ebra_a + ax * ebra_i + bx * (Lab + 1);
                     const zc* Ey = ebra + 1 * ebra_a + ay * ebra_i + by * (Lab + 1);
                     const zc* Ez = ebra + 2 * ebra_a + az * ebra_i + bz * (Lab + 1);
+                    // sum_{t,u,v} Ex Ey Ez W, one axis at a time
                     zc s{0.0, 0.0};
                     for (int t = 0; t <= ax + bx; ++t) {
+                        zc st{0.0, 0.0};
                         for (int u = 0; u <= ay + by; ++u) {
-                            const zc exy = zmul(Ex[t], Ey[u]);
                             const zc* ww = work + (offAB[t * (Lab + 1) + u]) * tl + cdl;
-                            for (int v = 0; v <= az + bz; ++v) zmac(s, zmul(exy, Ez[v]), ww[v * tl]);
+                            zc stu{0.0, 0.0};
+                            for (int v = 0; v <= az + bz; ++v) zmac(stu, Ez[v], ww[v * tl]);
+                            zmac(st, Ey[u], stu);
                         }
+                        zmac(s, Ex[t], st);
                     }
                     const double sc = coefs[A.coef_off + it.x[0] * A.nprim + pa] *
                                       norms[A.norm_off + cac * A.nprim + pa] *
