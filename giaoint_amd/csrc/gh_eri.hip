@@ -315,6 +315,13 @@ constexpr int kNT = 256;
 constexpr int kMaxW = 8;  // W entries per thread
 constexpr int kMaxO = 8;  // output entries per thread
 
+// threadIdx.x, opaque to the optimizer: what is derived from it is computed
+// where it is used rather than hoisted out of loops into live registers.
+__device__ inline int opaque_tid(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 struct LdsLayout {
     int offL, listL, offAB, tuvAB;  // int offsets (in ints)
     int seeds, eket, ebra, work;    // zc offsets (in zc) from zc base
@@ -358,7 +365,7 @@ __device__ __host__ inline int lex_off(int D, int t, int u) {
     return nherm(D) - nherm(D - t) + u * (D - t + 1) - u * (u - 1) / 2;
 }
 
-__global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const WorkItem* __restrict__ items,
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void transform_kernel(ClassArgs ca, const WorkItem* __restrict__ items,
                                                         const DevShell* __restrict__ shells,
                                                         const double* __restrict__ exps,
                                                         const double* __restrict__ coefs,
@@ -420,6 +427,10 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
     const DevShell B = shells[it.sh[1]];
     const DevShell C = shells[it.sh[2]];
     const DevShell D = shells[it.sh[3]];
+    // centres read through global memory: a runtime axis index into a copied
+    // struct would place the copies in scratch
+    const double *Ac = shells[it.sh[0]].center, *Bc = shells[it.sh[1]].center;
+    const double *Cc = shells[it.sh[2]].center, *Dc = shells[it.sh[3]].center;
     const int nca = ncart(la), ncb = ncart(lb), ncc = ncart(lc), ncd = ncart(ld);
     const int nab = nca * ncb;
     const int NHab = nherm(Lab);
@@ -429,6 +440,37 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
     const int stride_pq = L + 4;
     const int ebra_i = (lb + 1) * (Lab + 1), ebra_a = (la + 1) * ebra_i;
     const int eket_i = (ld + 1) * (Lcd + 1), eket_a = (lc + 1) * eket_i;
+
+    // The R0 recurrence step of entries tid and tid + kNT (fixed for the
+    // item): rs_a = lex | src1 << 16, rs_b = src2 | axis << 16 | (coef) << 18,
+    // axis 3 marking the seed entry.
+    int rs_a[2], rs_b[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int e = tid + r * kNT;
+        rs_a[r] = 0;
+        rs_b[r] = 3 << 16;
+        if (e < nherm(L)) {
+            const int pk = listL[e];
+            const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = pk >> 18;
+            int s1 = 0, s2 = 0, ax = 3, cf = 0;
+            if (t > 0) {
+                ax = 0;
+                s1 = offL[(t - 1) * (L + 1) + u] + v;
+                if (t > 1) s2 = offL[(t - 2) * (L + 1) + u] + v, cf = t - 1;
+            } else if (u > 0) {
+                ax = 1;
+                s1 = offL[u - 1] + v;
+                if (u > 1) s2 = offL[u - 2] + v, cf = u - 1;
+            } else if (v > 0) {
+                ax = 2;
+                s1 = v - 1;
+                if (v > 1) s2 = v - 2, cf = v - 1;
+            }
+            rs_a[r] = lex | (s1 << 16);
+            rs_b[r] = s2 | (ax << 16) | (cf << 18);
+        }
+    }
 
     for (int t0 = 0; t0 < nsel; t0 += tile) {
         const int tl = min(tile, nsel - t0);
@@ -444,9 +486,9 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
             for (int e = tid; e < 3 * (la + 1) * (lb + 1); e += kNT) {
                 const int axis = e / ((la + 1) * (lb + 1));
                 const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
-                const double Pc = (ea * A.center[axis] + eb * B.center[axis]) / p;
+                const double Pc = (ea * Ac[axis] + eb * Bc[axis]) / p;
                 const double qv = kappa[3 * it.sh[1] + axis] - kappa[3 * it.sh[0] + axis];
-                build_phase_hermite(i, j, Pc - A.center[axis], Pc - B.center[axis], p, qv,
+                build_phase_hermite(i, j, Pc - Ac[axis], Pc - Bc[axis], p, qv,
                                     ebra + axis * ebra_a + i * ebra_i + j * (Lab + 1));
             }
             zc w[kMaxW];
@@ -463,9 +505,9 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
                 for (int e = tid; e < 3 * (lc + 1) * (ld + 1); e += kNT) {
                     const int axis = e / ((lc + 1) * (ld + 1));
                     const int i = (e / (ld + 1)) % (lc + 1), j = e % (ld + 1);
-                    const double Qc = (ec * C.center[axis] + ed * D.center[axis]) / q;
+                    const double Qc = (ec * Cc[axis] + ed * Dc[axis]) / q;
                     const double qv = kappa[3 * it.sh[3] + axis] - kappa[3 * it.sh[2] + axis];
-                    build_phase_hermite(i, j, Qc - C.center[axis], Qc - D.center[axis], q, qv,
+                    build_phase_hermite(i, j, Qc - Cc[axis], Qc - Dc[axis], q, qv,
                                         eket + axis * eket_a + i * eket_i + j * (Lcd + 1));
                 }
                 __syncthreads();
@@ -479,7 +521,26 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
                     const zc* cur = ((L - n) & 1) ? buf0 : buf1;
                     zc* nxt = ((L - n) & 1) ? buf1 : buf0;
                     const int cnt = nherm(L - n);
-                    for (int e = tid; e < cnt; e += kNT) {
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        if (tid + r * kNT < cnt) {
+                            const int lex = rs_a[r] & 0xffff, s1 = rs_a[r] >> 16;
+                            const int s2 = rs_b[r] & 0xffff, ax = (rs_b[r] >> 16) & 3, cf = rs_b[r] >> 18;
+                            zc val;
+                            if (ax == 3) {
+                                val = sseed[3 + n];
+                            } else {
+                                val = zmul(ax == 0 ? X0 : (ax == 1 ? X1 : X2), cur[s1]);
+                                if (cf > 0) {
+                                    const zc r2 = cur[s2];
+                                    val.x += (double)cf * r2.x;
+                                    val.y += (double)cf * r2.y;
+                                }
+                            }
+                            nxt[lex] = val;
+                        }
+                    }
+                    for (int e = tid + 2 * kNT; e < cnt; e += kNT) {
                         const int pk = listL[e];
                         const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = pk >> 18;
                         zc val;
@@ -513,11 +574,14 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
                 }
                 __syncthreads();
                 const zc* R0 = (L & 1) ? buf1 : buf0;
-                // Ket transform into registers.
+                // Ket transform into registers. The entry decode is redone per
+                // primitive quartet (opaque_tid): hoisted out of the loop it
+                // would hold kMaxW copies of it in registers.
                 const int nW = NHab * tl;
+                const int tid_k = opaque_tid(tid);
 #pragma unroll
                 for (int r = 0; r < kMaxW; ++r) {
-                    const int e = tid + r * kNT;
+                    const int e = tid_k + r * kNT;
                     if (e < nW) {
                         const int cdl = e / NHab, k = e % NHab;
                         const int pk = tuvAB[k];
@@ -568,9 +632,10 @@ __global__ __launch_bounds__(kNT) void transform_kernel(ClassArgs ca, const Work
             __syncthreads();
             // Bra transform.
             const int nO = nab * tl;
+            const int tid_b = opaque_tid(tid);
 #pragma unroll
             for (int r = 0; r < kMaxO; ++r) {
-                const int o = tid + r * kNT;
+                const int o = tid_b + r * kNT;
                 if (o < nO) {
                     const int abcomp = o / tl, cdl = o % tl;
                     const int cac = abcomp / ncb, cbc = abcomp % ncb;
@@ -871,6 +936,10 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
     const DevShell B = shells[it.sh[1]];
     const DevShell C = shells[it.sh[2]];
     const DevShell D = shells[it.sh[3]];
+    // centres read through global memory: a runtime axis index into a copied
+    // struct would place the copies in scratch
+    const double *Ac = shells[it.sh[0]].center, *Bc = shells[it.sh[1]].center;
+    const double *Cc = shells[it.sh[2]].center, *Dc = shells[it.sh[3]].center;
     const int nca = ncart(la), ncb = ncart(lb), ncd = ncart(ld);
     const int nab = nca * ncb;
     const int cdcomp = it.cd0;
@@ -901,9 +970,9 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
         for (int e = lane; e < 3 * (la + 1) * (lb + 1); e += stride) {
             const int axis = e / ((la + 1) * (lb + 1));
             const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
-            const double Pc = (ea * A.center[axis] + eb * B.center[axis]) / p;
+            const double Pc = (ea * Ac[axis] + eb * Bc[axis]) / p;
             const double qv = kappa[3 * it.sh[1] + axis] - kappa[3 * it.sh[0] + axis];
-            build_phase_hermite(i, j, Pc - A.center[axis], Pc - B.center[axis], p, qv,
+            build_phase_hermite(i, j, Pc - Ac[axis], Pc - Bc[axis], p, qv,
                                 ebra + axis * ebra_a + i * ebra_i + j * (Lab + 1));
         }
         for (int k = lane; k < NHab; k += stride) vv[k] = zc{0.0, 0.0};
@@ -916,9 +985,9 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
             if (lane < L + 4) sseed[lane] = sp[lane];
             if (lane < 3) {
                 const int axis = lane;
-                const double Qc = (ec * C.center[axis] + ed * D.center[axis]) / q;
+                const double Qc = (ec * Cc[axis] + ed * Dc[axis]) / q;
                 const double qv = kappa[3 * it.sh[3] + axis] - kappa[3 * it.sh[2] + axis];
-                build_phase_hermite(cpow[axis], dpow[axis], Qc - C.center[axis], Qc - D.center[axis], q, qv,
+                build_phase_hermite((int)((pcp >> (8 * axis)) & 255), (int)((pdp >> (8 * axis)) & 255), Qc - Cc[axis], Qc - Dc[axis], q, qv,
                                     e1d + axis * (Lcd + 1));
             }
             wave_barrier();
@@ -994,13 +1063,18 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
             const zc* Ex = ebra + 0 * ebra_a + ax * ebra_i + bxp * (Lab + 1);
             const zc* Ey = ebra + 1 * ebra_a + ay * ebra_i + byp * (Lab + 1);
             const zc* Ez = ebra + 2 * ebra_a + az * ebra_i + bzp * (Lab + 1);
+            // sum_{t,u,v} Ex Ey Ez V, one axis at a time
             zc s{0.0, 0.0};
-            for (int t = 0; t <= ax + bxp; ++t)
+            for (int t = 0; t <= ax + bxp; ++t) {
+                zc st{0.0, 0.0};
                 for (int u = 0; u <= ay + byp; ++u) {
-                    const zc exy = zmul(Ex[t], Ey[u]);
                     const zc* ww = vv + lex_off(Lab, t, u);
-                    for (int v = 0; v <= az + bzp; ++v) zmac(s, zmul(exy, Ez[v]), ww[v]);
+                    zc stu{0.0, 0.0};
+                    for (int v = 0; v <= az + bzp; ++v) zmac(stu, Ez[v], ww[v]);
+                    zmac(st, Ey[u], stu);
                 }
+                zmac(s, Ex[t], st);
+            }
             const double sc = coefs[A.coef_off + it.x[0] * A.nprim + pa] * norms[A.norm_off + cac * A.nprim + pa] *
                               coefs[B.coef_off + it.x[1] * B.nprim + pb] * norms[B.norm_off + cbc * B.nprim + pb];
             acc[o].x = fma(sc, s.x, acc[o].x);
