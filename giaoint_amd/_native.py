@@ -95,7 +95,7 @@ _SIGNATURES = {
     "giaohip_cholesky_pivots": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "giaohip_cholesky_vectors": ([_vp, _vp, C.c_uint64, C.c_uint64, _dp], C.c_int),
     "giaohip_cholesky_resume": ([_vp, _vp, C.c_uint64], C.c_int),
-    "giaohip_cholesky_blocks": ([_vp, _u64p, _u64p, _dp, _u64p], C.c_int),
+    "giaohip_cholesky_blocks": ([_vp, _u64p, _u64p, _dp, _u64p, _u64p], C.c_int),
     "giaohip_cholesky_reconstruct": ([_vp, _vp, C.POINTER(C.c_int64), _dp], C.c_int),
     "giaohip_cholesky_coulomb": ([_vp, _vp, _dp, _dp], C.c_int),
     "giaohip_cholesky_destroy": ([_vp, _vp], C.c_int),
@@ -402,10 +402,11 @@ class DeviceCholesky:
         rows = C.c_uint64()
         bms = C.c_double()
         proj = C.c_uint64()
+        crows = C.c_uint64()
         self.ctx.check(self.ctx.lib.giaohip_cholesky_blocks(self.handle, C.byref(blocks), C.byref(rows), C.byref(bms),
-                                                            C.byref(proj)))
+                                                            C.byref(proj), C.byref(crows)))
         self.blocks = {"blocks": int(blocks.value), "factor_rows_read": int(rows.value), "block_ms": bms.value,
-                       "projected": int(proj.value)}
+                       "projected": int(proj.value), "candidate_rows": int(crows.value)}
 
     def resume(self, stop_after: int = 0):
         """Continue the run until converged or the rank reaches stop_after (0: no limit)."""

@@ -174,6 +174,7 @@ struct giaohip_cholesky {
     uint64_t blocks = 0;
     uint64_t rows_read = 0;  // factor rows read by the block projections
     uint64_t projected = 0;  // candidates projected (the others carried over)
+    uint64_t cand_rows = 0;  // sum of candidates projected x rank
     int real = 0;            // real arithmetic: zero field or one centre (symmetrized columns)
     double block_ms = 0.0;   // candidate selection + projection
     hipEvent_t ev[10]{};
@@ -1843,6 +1844,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
             c->blocks += 1;
             c->projected += nnew;
+            c->cand_rows += (uint64_t)nnew * c->rank;
             c->ncand = gh::kCholMaxBlock;  // slot table is searched by the open mask
             for (int q = 0; q < gh::kCholMaxBlock; ++q)
                 if (((c->open >> q) & 1u) && c->cand[q] == p) slot = q;
@@ -2069,10 +2071,11 @@ int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max
 }
 
 int giaohip_cholesky_blocks(const giaohip_cholesky* c, uint64_t* blocks, uint64_t* factor_rows_read,
-                            double* block_ms, uint64_t* projected) {
+                            double* block_ms, uint64_t* projected, uint64_t* candidate_rows) {
     if (!c) return GIAOHIP_INVALID_ARGUMENT;
     if (blocks) *blocks = c->blocks;
     if (projected) *projected = c->projected;
+    if (candidate_rows) *candidate_rows = c->cand_rows;
     if (factor_rows_read) *factor_rows_read = c->rows_read;
     if (block_ms) *block_ms = c->block_ms;
     return GIAOHIP_OK;

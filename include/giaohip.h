@@ -220,12 +220,13 @@ int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* 
 /* Pivot-block statistics of a run: blocks started, factor rows read by their
  * projections (sum of the rank at each block start that projected any
  * candidate; each row is n*n complex), device time of candidate selection +
- * projection (ms, included in update_ms), and the number of candidates
+ * projection (ms, included in update_ms), the number of candidates
  * projected (candidates still open from the previous block keep their
- * projections). No counterpart in the reference (its projection is one GEMV
- * per pivot). */
+ * projections), and candidate_rows = sum over blocks of candidates projected
+ * x rank (each is n*n complex multiply-adds). Any output pointer may be NULL.
+ * No counterpart in the reference (its projection is one GEMV per pivot). */
 int giaohip_cholesky_blocks(const giaohip_cholesky* chol, uint64_t* blocks, uint64_t* factor_rows_read,
-                            double* block_ms, uint64_t* projected);
+                            double* block_ms, uint64_t* projected, uint64_t* candidate_rows);
 /* Reconstructed ERIs from the device factors, (mu nu|lam sig) ~
  * sum_P L_P[mu,nu] conj(L_P[sig,lam]), on the AO ranges mu in
  * [ranges[0], ranges[1]), nu in [ranges[2], ranges[3]), lam in [ranges[4],

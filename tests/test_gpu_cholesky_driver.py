@@ -105,11 +105,15 @@ def test_device_cholesky_block_sizes_bitwise(gpu, port):
     # one candidate per block: one pass over all earlier factors per pivot
     assert ref.blocks["blocks"] == ref.rank
     assert ref.blocks["factor_rows_read"] == ref.rank * (ref.rank - 1) // 2
+    assert ref.blocks["candidate_rows"] == ref.blocks["factor_rows_read"]
     ref_piv, ref_vec = ref.pivots(), ref.vectors()
     for nb in (3, 16, 32):
         got = _run_with_block(gpu, db, f, nb)
         print(f"block {nb}: rank {got.rank}, {got.blocks['blocks']} blocks")
         assert got.blocks["blocks"] < ref.blocks["blocks"]
+        # every candidate of a block is projected against the rank at the block start
+        assert got.blocks["candidate_rows"] <= got.blocks["projected"] * got.blocks["factor_rows_read"]
+        assert got.blocks["candidate_rows"] >= got.blocks["factor_rows_read"]
         assert got.rank == ref.rank and got.converged
         np.testing.assert_array_equal(got.pivots(), ref_piv)
         np.testing.assert_array_equal(got.vectors(), ref_vec)
