@@ -170,6 +170,7 @@ struct giaohip_cholesky {
     int block = 1, slots = 1;
     int ncand = 0;
     long long cand[gh::kCholMaxBlock]{};
+    uint64_t complete[gh::kCholMaxBlock]{};  // rank up to which each slot's projection is summed
     unsigned open = 0;
     uint64_t blocks = 0;
     uint64_t rows_read = 0;  // factor rows read by the block projections
@@ -1652,21 +1653,19 @@ struct CholHost {
     int flags;
     int ntop;
     long long top[gh::kCholMaxBlock];  // a block's candidates, largest residual first
-    // uploads of a block start: slot table, new candidates and their slots
-    long long slot_piv[gh::kCholMaxBlock];
+    // uploads of a block start: new candidates and their slots
     long long new_piv[gh::kCholMaxBlock];
     int new_slot[gh::kCholMaxBlock];
 };
 // Device layout of giaohip_cholesky::d_cand (bytes).
-constexpr size_t kCandSlots = 0, kCandTop = 256, kCandNewPiv = 512, kCandNewSlot = 768, kCandCount = 896,
-                 kCandBytes = 1024;
+constexpr size_t kCandTop = 0, kCandNewPiv = 256, kCandNewSlot = 512, kCandCount = 640, kCandBytes = 768;
 
 // Pivot steps of a run until the residual is below the tolerance or the rank
 // reaches `target` (0: no target). Per step: when the pivot is not an open
 // candidate of the current block, a new block (top-B candidates and their
 // projections in one pass over the factors); then the pivot's column of ERIs,
-// the accept step (factor row, residual update, projections of the other
-// open candidates) and the next argmax. The host reads the pivot back to
+// the accept step (the pivot's projection brought up to the current rank,
+// factor row, residual update) and the next argmax. The host reads the pivot back to
 // choose the column's ket class and to test block membership.
 int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
     const giaohip_basis* b = c->basis;
@@ -1812,13 +1811,10 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                 while ((used >> q) & 1u) ++q;  // a free slot: at most ntop <= slots are in use
                 host->new_slot[m] = q;
                 c->cand[q] = host->new_piv[m];
+                c->complete[q] = c->rank;
                 used |= 1u << q;
             }
             c->open = used;
-            std::copy(c->cand, c->cand + gh::kCholMaxBlock, host->slot_piv);
-            CHOL_STEP_TRY(hipMemcpyAsync(dc + kCandSlots, host->slot_piv, sizeof host->slot_piv,
-                                         hipMemcpyHostToDevice, s),
-                          "copy");
             CHOL_STEP_TRY(hipMemcpyAsync(dc + kCandNewPiv, host->new_piv, sizeof host->new_piv, hipMemcpyHostToDevice,
                                          s),
                           "copy");
@@ -1851,12 +1847,17 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             CHOL_STEP_TRY(hipStreamWaitEvent(s, c->ev[1], 0), "hipStreamWaitEvent");
             block_pending = true;
         }
-        const unsigned rest = c->open & ~(1u << slot);
         CHOL_STEP_TRY(hipEventRecord(c->ev[5], s), "hipEventRecord");
-        CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2,
-                                             c->d_proj.as<zc>(), c->d_cand.as<long long>(), rest, c->real, n, n2,
-                                             c->d_state.as<gh::CholState>(), c->d_res.as<double>(), row, c->psd_tol,
-                                             c->d_flags.as<int>(), c->d_pmax.as<double>(), c->d_pidx.as<long long>(), s),
+        // the pivot's projection lacks the factors accepted since it was projected
+        CHOL_STEP_TRY(gh::launch_chol_catchup(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
+                                              (int)c->complete[slot], (int)c->rank, p,
+                                              c->d_proj.as<zc>() + (long long)slot * n2, s),
+                      "catch-up launch");
+        c->complete[slot] = c->rank;
+        CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2, c->real,
+                                             n, n2, c->d_state.as<gh::CholState>(), c->d_res.as<double>(), row,
+                                             c->psd_tol, c->d_flags.as<int>(), c->d_pmax.as<double>(),
+                                             c->d_pidx.as<long long>(), s),
                       "accept launch");
         CHOL_STEP_TRY(gh::launch_chol_argmax(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
                                              c->d_pidx.as<long long>(), c->d_state.as<gh::CholState>(), false, s),

@@ -17,12 +17,12 @@
 // its seed quartets and work items from a static bra table and the pivot's
 // ket (cholesky.cpp:203-282), the seed and column kernels of gh_eri.hip
 // evaluate it) and accept_kernel: L[rank] = (column - proj) / sqrt(res[p]),
-// res -= |L[rank]|^2 with the PSD check, clamp at 0, res[p] = 0, per-block
-// (max, first index) partials, and proj[m] += conj(L[rank][p_m]) L[rank] for
-// the candidates not yet taken. The argmax kernels finish the first maximum.
-// When that maximum is a candidate not yet taken, its projection is already
-// complete and the next pivot needs no pass over the factors; otherwise a new
-// block starts.
+// res -= |L[rank]|^2 with the PSD check, clamp at 0, res[p] = 0 and per-block
+// (max, first index) partials. The argmax kernels finish the first maximum.
+// When that maximum is a candidate not yet taken, the next pivot needs no
+// pass over all factors: catchup_kernel adds to its projection the terms of
+// the factors accepted since it was projected (the candidates never taken
+// never get them); otherwise a new block starts.
 //
 // Every projection accumulates conj(L[k][p]) L[k] in increasing k with the
 // same fused multiply-adds (cmac), starting from zero, and subtracts the sum
@@ -416,11 +416,44 @@ __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const
             if (m < nnew) proj[(long long)slots[m] * n2 + j] = acc[m];
 }
 
+// proj += sum_{from <= k < to} conj(L[k][pm]) L[k], k increasing: the terms
+// of the factors accepted since candidate pm's projection was complete.
+// Four elements per thread, each row's loads issued before its FMAs.
+constexpr int kCatchNT = 256, kCatchPer = 4;
+__global__ __launch_bounds__(kCatchNT) void chol_catchup_kernel(const zc* const* __restrict__ chunks,
+                                                                int rows_per_chunk, long long n2, int from, int to,
+                                                                long long pm, zc* __restrict__ proj) {
+    const long long base = (long long)blockIdx.x * (kCatchNT * kCatchPer) + threadIdx.x;
+    zc acc[kCatchPer];
+#pragma unroll
+    for (int e = 0; e < kCatchPer; ++e) {
+        const long long j = base + e * kCatchNT;
+        acc[e] = j < n2 ? proj[j] : zc{0.0, 0.0};
+    }
+    for (int k = from; k < to; ++k) {
+        const zc* row = chunks[k / rows_per_chunk] + (long long)(k % rows_per_chunk) * n2;
+        const zc lp = row[pm];
+        const zc c{lp.x, -lp.y};
+        zc v[kCatchPer];
+#pragma unroll
+        for (int e = 0; e < kCatchPer; ++e) {
+            const long long j = base + e * kCatchNT;
+            v[e] = row[j < n2 ? j : 0];
+        }
+#pragma unroll
+        for (int e = 0; e < kCatchPer; ++e) cmac(acc[e], c, v[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < kCatchPer; ++e) {
+        const long long j = base + e * kCatchNT;
+        if (j < n2) proj[j] = acc[e];
+    }
+}
+
 // Accepts the pivot st->pivot, whose raw column is `column` and whose
 // projection onto the earlier factors is `pproj`: writes the new factor row,
-// updates the residual diagonal (PSD check, clamp, res[p] = 0), the
-// per-block (max, first index) partials and the projections of the candidate
-// slots in `rest` (bit m: slot m at proj + m * n2, pivot cand[m]).
+// updates the residual diagonal (PSD check, clamp, res[p] = 0) and the
+// per-block (max, first index) partials.
 // With `real` set (zero-field or one-centre bases, real arithmetic in
 // cholesky.py:175-176,269-271) the projected column is first symmetrized:
 // (c[mu,nu] + c[nu,mu]) / 2, imaginary part dropped; n is the AO count.
@@ -437,9 +470,7 @@ __device__ inline zc chol_column_value(const zc* __restrict__ column, const zc* 
 }
 
 __global__ __launch_bounds__(kUpdNT) void chol_accept_kernel(const zc* __restrict__ column,
-                                                             const zc* __restrict__ pproj, zc* proj,
-                                                             const long long* __restrict__ cand, unsigned rest,
-                                                             int real, long long n,
+                                                             const zc* __restrict__ pproj, int real, long long n,
                                                              long long n2, const CholState* __restrict__ st,
                                                              double* __restrict__ res, zc* __restrict__ out_row,
                                                              double psd_tol, int* __restrict__ flags,
@@ -468,21 +499,6 @@ __global__ __launch_bounds__(kUpdNT) void chol_accept_kernel(const zc* __restric
             if (j[e] == p) nr = 0.0;
             res[j[e]] = nr;
             argmax_combine(best, bidx, nr, j[e]);
-        }
-    }
-    for (unsigned mask = rest; mask; mask &= mask - 1) {
-        const int m = __builtin_ctz(mask);
-        const long long pm = cand[m];
-        const zc cv = chol_column_value(column, pproj, pm, real, n);
-        const zc c{cv.x / root, -(cv.y / root)};
-        zc* pr = proj + (long long)m * n2;
-#pragma unroll
-        for (int e = 0; e < kUpdPerThread; ++e) {
-            if (j[e] < n2) {
-                zc acc = pr[j[e]];
-                cmac(acc, c, val[e]);
-                pr[j[e]] = acc;
-            }
         }
     }
     block_argmax<kUpdNT>(best, bidx, sv, si, pmax + blockIdx.x, pidx + blockIdx.x);
@@ -656,11 +672,20 @@ hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, 
     return hipGetLastError();
 }
 
-hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
-                              int real, long long n, long long n2, const CholState* st, double* res, zc* out_row,
-                              double psd_tol, int* flags, double* pmax, long long* pidx, hipStream_t s) {
+hipError_t launch_chol_catchup(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
+                               long long pm, zc* proj, hipStream_t s) {
+    if (from >= to) return hipSuccess;
+    const long long per = (long long)kCatchNT * kCatchPer;
+    chol_catchup_kernel<<<dim3((unsigned)((n2 + per - 1) / per)), dim3(kCatchNT), 0, s>>>(chunks, rows_per_chunk,
+                                                                                         n2, from, to, pm, proj);
+    return hipGetLastError();
+}
+
+hipError_t launch_chol_accept(const zc* column, const zc* pproj, int real, long long n, long long n2,
+                              const CholState* st, double* res, zc* out_row, double psd_tol, int* flags, double* pmax,
+                              long long* pidx, hipStream_t s) {
     chol_accept_kernel<<<dim3((unsigned)chol_partials(n2)), dim3(kUpdNT), 0, s>>>(
-        column, pproj, proj, cand, rest, real, n, n2, st, res, out_row, psd_tol, flags, pmax, pidx);
+        column, pproj, real, n, n2, st, res, out_row, psd_tol, flags, pmax, pidx);
     return hipGetLastError();
 }
 

@@ -51,9 +51,11 @@ hipError_t launch_chol_gamma(const zc* const* chunks, int rows_per_chunk, long l
                              zc* gamma, hipStream_t s);
 hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, long long n, int rank,
                                    const long long* ranges, long long total, zc* out, hipStream_t s);
-hipError_t launch_chol_accept(const zc* column, const zc* pproj, zc* proj, const long long* cand, unsigned rest,
-                              int real, long long n, long long n2, const CholState* st, double* res, zc* out_row,
-                              double psd_tol, int* flags, double* pmax, long long* pidx, hipStream_t s);
+hipError_t launch_chol_catchup(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
+                               long long pm, zc* proj, hipStream_t s);
+hipError_t launch_chol_accept(const zc* column, const zc* pproj, int real, long long n, long long n2,
+                              const CholState* st, double* res, zc* out_row, double psd_tol, int* flags, double* pmax,
+                              long long* pidx, hipStream_t s);
 hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
                               bool partial_first, hipStream_t s);
 
