@@ -171,6 +171,14 @@ struct giaohip_cholesky {
     int ncand = 0;
     long long cand[gh::kCholMaxBlock]{};
     uint64_t complete[gh::kCholMaxBlock]{};  // rank up to which each slot's projection is summed
+    // Boys seeds of recent ket shell pairs (a pivot's seeds depend only on
+    // its two shells): seed_slots buffers of seed_slot_zc complex each,
+    // least recently used replaced first
+    long long seed_slot_zc = 0;
+    int seed_slots = 1;
+    std::vector<long long> seed_key;     // ket shell pair per buffer (-1: empty)
+    std::vector<uint64_t> seed_used;     // last use (pivot count) per buffer
+    uint64_t seed_hits = 0, seed_misses = 0;
     unsigned open = 0;
     uint64_t blocks = 0;
     uint64_t rows_read = 0;  // factor rows read by the block projections
@@ -1729,10 +1737,29 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                                              c->d_wi.as<WorkItem>(), sc),
                       "expand launch");
         const int maxL = c->max_lab + ket.lcd;
-        CHOL_STEP_TRY(gh::launch_seed(nmax_bucket(maxL), c->d_sq.as<SeedQuartet>(), c->nunits, c->p1 * ket.kcd,
-                                      b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
-                                      c->d_seeds.as<zc>(), c->d_err.as<DevError>(), c->d_scratch, sc),
-                      "seed launch");
+        // the seeds of this ket shell pair, computed now unless still buffered
+        const long long skey = (long long)sb * (long long)b->shells.size() + sa;
+        int sslot = -1, lru = 0;
+        for (int q = 0; q < c->seed_slots; ++q) {
+            if (c->seed_key[q] == skey) sslot = q;
+            if (c->seed_used[q] < c->seed_used[lru]) lru = q;
+        }
+        const bool seeds_ready = sslot >= 0;
+        if (!seeds_ready) {
+            sslot = lru;
+            c->seed_key[sslot] = skey;
+        }
+        c->seed_used[sslot] = c->rank + 1;
+        zc* seeds = c->d_seeds.as<zc>() + (size_t)sslot * c->seed_slot_zc;
+        if (seeds_ready) {
+            c->seed_hits += 1;
+        } else {
+            c->seed_misses += 1;
+            CHOL_STEP_TRY(gh::launch_seed(nmax_bucket(maxL), c->d_sq.as<SeedQuartet>(), c->nunits, c->p1 * ket.kcd,
+                                          b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
+                                          seeds, c->d_err.as<DevError>(), c->d_scratch, sc),
+                          "seed launch");
+        }
         // the (la, lb) groups of bra pairs write disjoint rows of the column:
         // the larger ones first, spread over two streams
         CHOL_STEP_TRY(hipEventRecord(c->ev[8], sc), "hipEventRecord");
@@ -1756,14 +1783,13 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                 CHOL_STEP_TRY(gh::launch_chol_column(ca, ket.comp, c->d_wi.as<WorkItem>() + g[2], g[3],
                                                      b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
                                                      b->d_coefs.as<double>(), b->d_norms.as<double>(),
-                                                     c->d_kappa.as<double>(), c->d_seeds.as<zc>(), c->d_col.as<zc>(),
-                                                     gs),
+                                                     c->d_kappa.as<double>(), seeds, c->d_col.as<zc>(), gs),
                               "column launch");
             else
                 CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
                                                    b->d_exps.as<double>(), b->d_coefs.as<double>(),
-                                                   b->d_norms.as<double>(), c->d_kappa.as<double>(),
-                                                   c->d_seeds.as<zc>(), c->d_col.as<zc>(), gs),
+                                                   b->d_norms.as<double>(), c->d_kappa.as<double>(), seeds,
+                                                   c->d_col.as<zc>(), gs),
                               "transform launch");
         }
         CHOL_STEP_TRY(hipEventRecord(c->ev[9], c->s3), "hipEventRecord");
@@ -2008,8 +2034,19 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     c->slots = gh::chol_block_slots(c->block);
     CHOL_TRY(c->d_proj.ensure((size_t)c->slots * n2 * sizeof(zc)), "hipMalloc(projections)");
     CHOL_TRY(c->d_cand.ensure(kCandBytes), "hipMalloc");
-    CHOL_TRY(c->d_seeds.ensure((size_t)(seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
-    c->d_scratch = reinterpret_cast<int*>(c->d_seeds.as<zc>() + seed_zc);
+    {
+        // seed buffers: up to 1/16 of the free device memory, at most 8 GiB
+        size_t free_b = 0, total_b = 0;
+        CHOL_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const size_t budget = std::min<size_t>(size_t(8) << 30, free_b / 16);
+        const size_t per = (size_t)seed_zc * sizeof(zc);
+        c->seed_slot_zc = seed_zc;
+        c->seed_slots = (int)std::max<size_t>(1, std::min<size_t>(4096, per ? budget / per : 1));
+        c->seed_key.assign(c->seed_slots, -1);
+        c->seed_used.assign(c->seed_slots, 0);
+    }
+    CHOL_TRY(c->d_seeds.ensure(((size_t)c->seed_slots * seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
+    c->d_scratch = reinterpret_cast<int*>(c->d_seeds.as<zc>() + (size_t)c->seed_slots * seed_zc);
     CHOL_TRY(hipMemset(c->d_flags.p, 0, sizeof(int)), "hipMemset");
     for (auto& e : c->ev) CHOL_TRY(hipEventCreate(&e), "hipEventCreate");
     CHOL_TRY(hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking), "hipStreamCreate");
