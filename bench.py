@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cholesky-rank", type=int, default=0,
                     help="Cholesky leg: stop after this many pivots (0: run to convergence, -1: skip the leg)")
     ap.add_argument("--cholesky-block", type=int, default=0, help="candidates per pivot block (default: library's)")
+    ap.add_argument("--high-l-gggg", type=int, default=50_000, help="(gg|gg) quartets of the high-l leg (0 skips)")
+    ap.add_argument("--high-l-ffdd", type=int, default=200_000, help="(ff|dd) quartets of the high-l leg (0 skips)")
     ap.add_argument("--mixed", type=int, default=200_000,
                     help="mixed-class leg: quartets of the l<=3, degree 1..8 workload (0 skips it)")
     return ap.parse_args()
@@ -131,6 +133,41 @@ def mixed_leg(n: int):
         "tflops_convention": flops / dt / 1e12,
         "launches": st["launches"],
     }
+
+
+def high_l_leg(n_gggg: int, n_ffdd: int):
+    """The high-angular-momentum precision workloads at throughput: (gg|gg)
+    and (ff|dd), contraction degree 2, exponents 10^U(log10 0.05, log10 5e3).
+    (ff|dd) runs on the class-specialized kernels through its (dd|ff) image,
+    (gg|gg) on the generic kernel. A warmup call on the first quartets, then
+    one timed call per class, output resident on the device."""
+    from giaoint_amd import _native, workloads
+
+    g, fd = workloads.precision_batches(n_gggg=n_gggg, n_ffdd=n_ffdd)
+    out = {}
+    for name, batch in (("gggg", g), ("ffdd", fd)):
+        n = len(batch.quartets)
+        if n == 0:
+            continue
+        sh = batch.shells
+        coefs = _native.normalize_shells(sh.L, sh.nprim, sh.ncontr, sh.exps, sh.coefs)
+        ctx = _native.Context(0)
+        db = _native.DeviceBasis(ctx, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
+        total = int(db.quartet_sizes(batch.quartets).sum())
+        dev = ctx.alloc(total * 16)
+        f = _native.field6(batch.B, batch.origin)
+        flops = workloads.batch_flops(sh, batch.quartets)
+        db.eri_batch(batch.quartets[: min(n, 256)], f, device_out=dev)
+        ctx.synchronize()
+        t = time.perf_counter()
+        db.eri_batch(batch.quartets, f, device_out=dev)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        ctx.free(dev)
+        log(f"{name}: {n} quartets in {dt:.2f} s")
+        out[name] = {"workload": batch.description, "quartets": n, "quartets_per_s": n / dt, "seconds": dt,
+                     "tflops_convention": flops / dt / 1e12}
+    return out
 
 
 def projection_roofline(nbytes: float, flops: float, block_ms: float, unblocked_bytes: float) -> dict:
@@ -353,6 +390,8 @@ def main():
     del dbasis, ctx
     if args.mixed > 0:
         result["mixed_class"] = mixed_leg(args.mixed)
+    if args.high_l_gggg > 0 or args.high_l_ffdd > 0:
+        result["high_l"] = high_l_leg(args.high_l_gggg, args.high_l_ffdd)
     if args.cholesky_rank >= 0:
         result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block)
     print(json.dumps(result), flush=True)
