@@ -343,10 +343,14 @@ __device__ __forceinline__ void cmac4_bcast(zc (&acc)[NB], const zc& c, const zc
 }
 
 // The same projection as chol_project_kernel for NB = 16 or 32 slots, one
-// entry per thread: the coefficient row of factor k reaches the lanes as one
-// 16-byte load per lane (slot lane % 16, and 16 + lane % 16 for NB = 32) and
-// the FMAs take slot m's coefficient from lane m of the row by DPP broadcast,
-// instead of every wave fetching the row through the scalar cache.
+// entry per thread. The workgroup stages the coefficient rows of kStage
+// factors in LDS (one coalesced copy, instead of every wave fetching every
+// row from L2 and holding it in registers while it is in flight); per factor
+// row each lane reads one 16-byte coefficient (slot lane % 16, and 16 +
+// lane % 16 for NB = 32) and the FMAs take slot m's coefficient from lane m
+// of the 16-lane row by DPP broadcast. Factor rows stream U at a time with
+// the next U in flight.
+constexpr int kStage = 64;  // factor rows per coefficient stage
 template <int NB, int U>
 __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const* __restrict__ chunks,
                                                                  int rows_per_chunk, long long n2, int rank,
@@ -355,6 +359,7 @@ __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const
                                                                  zc* __restrict__ proj) {
     static_assert(NB == 16 || NB == 32, "one or two coefficients per lane");
     constexpr int NL = NB / 16;  // coefficients per lane and row
+    __shared__ zc sc[kStage * NB];
     const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long jc = j < n2 ? j : n2 - 1;
     const int l16 = threadIdx.x & 15;
@@ -373,41 +378,52 @@ __global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const
             cmac4_bcast<28>(acc, c[NL - 1], v);
         }
     };
+    auto coefs = [&](zc (&c)[NL], int r) {
+#pragma unroll
+        for (int h = 0; h < NL; ++h) c[h] = sc[r * NB + 16 * h + l16];
+    };
     for (int k0 = 0; k0 < rank; k0 += rows_per_chunk) {
         const zc* chunk = chunks[k0 / rows_per_chunk];
-        const int rows = min(rows_per_chunk, rank - k0);
-        const int full = rows - rows % (2 * U);
-        const zc* ck = coef + (long long)k0 * NB + l16;
-        // rows r..r+U-1, clamped to the chunk (loads past the last group repeat row rows-1, unused)
-        auto load = [&](zc (&v)[U], zc (&c)[U][NL], int r) {
+        const int crows = min(rows_per_chunk, rank - k0);
+        for (int s0 = 0; s0 < crows; s0 += kStage) {
+            const int rows = min(kStage, crows - s0);
+            __syncthreads();  // the previous stage's coefficients are no longer read
+            for (int i = threadIdx.x; i < rows * NB; i += 256) sc[i] = coef[(long long)(k0 + s0) * NB + i];
+            __syncthreads();
+            const zc* base = chunk + (long long)s0 * n2 + jc;
+            const int full = rows - rows % (2 * U);
+            // rows r..r+U-1 of the stage, clamped (loads past the last group repeat row rows-1, unused)
+            auto load = [&](zc (&v)[U], int r) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int rr = min(r + u, rows - 1);
-                v[u] = gload(chunk + (long long)rr * n2 + jc);
+                for (int u = 0; u < U; ++u) v[u] = gload(base + (long long)min(r + u, rows - 1) * n2);
+            };
+            zc va[U], vb[U];
+            load(va, 0);
+            for (int r = 0; r < full; r += 2 * U) {
+                load(vb, r + U);
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int h = 0; h < NL; ++h) c[u][h] = ck[(long long)rr * NB + 16 * h];
+                for (int u = 0; u < U; ++u) {
+                    zc c[NL];
+                    coefs(c, r + u);
+                    step(c, va[u]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                load(va, r + 2 * U);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    zc c[NL];
+                    coefs(c, r + U + u);
+                    step(c, vb[u]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
-        };
-        zc va[U], vb[U], ca[U][NL], cb[U][NL];
-        load(va, ca, 0);
-        for (int r = 0; r < full; r += 2 * U) {
-            load(vb, cb, r + U);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int u = 0; u < U; ++u) step(ca[u], va[u]);
-            __builtin_amdgcn_sched_barrier(0);
-            load(va, ca, r + 2 * U);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int u = 0; u < U; ++u) step(cb[u], vb[u]);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        for (int r = full; r < rows; ++r) {
-            zc c[NL];
-            const zc v = gload(chunk + (long long)r * n2 + jc);
-#pragma unroll
-            for (int h = 0; h < NL; ++h) c[h] = ck[(long long)r * NB + 16 * h];
-            step(c, v);
+            for (int r = full; r < rows; ++r) {
+                zc c[NL];
+                coefs(c, r);
+                step(c, gload(base + (long long)r * n2));
+            }
         }
     }
     if (j < n2)

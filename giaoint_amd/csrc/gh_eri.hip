@@ -51,6 +51,14 @@ constexpr int kColMaxL = kColumnMaxL;
 constexpr int kColListSize = (kColMaxL + 1) * (kColMaxL + 2) * (kColMaxL + 3) * (kColMaxL + 4) / 24;
 __constant__ unsigned c_rlist_all[kColListSize];
 __device__ __host__ inline int rlist_offset(int L) { return L * (L + 1) * (L + 2) * (L + 3) / 24; }
+// The R0 recurrence step of each entry of those lists (same offsets):
+// x = lex | src1 << 16, y = src2 | axis << 16 | coef << 18, where the entry is
+// X[axis] * R[src1] + coef * R[src2] of the level above (axis 3: the seed).
+__constant__ int2 c_rstep_all[kColListSize];
+// Lexicographic offsets lex_off(L, t, u) for L, t, u <= kColumnMaxL.
+constexpr int kLexDim = kColMaxL + 1;
+__constant__ int c_lexoff_all[kLexDim * kLexDim * kLexDim];
+__device__ inline const int* lexoff_table(int L) { return c_lexoff_all + L * kLexDim * kLexDim; }
 
 // ---------------------------------------------------------------------------
 // Pair data (overlap.cpp:60-91, same operation order)
@@ -824,6 +832,35 @@ hipError_t upload_constants() {
     }
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_rlist_all), rlist.data(), rlist.size() * sizeof(unsigned))) != hipSuccess)
         return e;
+    std::vector<int2> rstep(kColListSize);
+    for (int L = 0; L <= kColMaxL; ++L)
+        for (int i = 0; i < nherm(L); ++i) {
+            const unsigned pk = rlist[rlist_offset(L) + i];
+            const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = (int)(pk >> 18);
+            int s1 = 0, s2 = 0, ax = 3, cf = 0;
+            if (t > 0) {
+                ax = 0;
+                s1 = lex_off(L, t - 1, u) + v;
+                if (t > 1) s2 = lex_off(L, t - 2, u) + v, cf = t - 1;
+            } else if (u > 0) {
+                ax = 1;
+                s1 = lex_off(L, 0, u - 1) + v;
+                if (u > 1) s2 = lex_off(L, 0, u - 2) + v, cf = u - 1;
+            } else if (v > 0) {
+                ax = 2;
+                s1 = v - 1;
+                if (v > 1) s2 = v - 2, cf = v - 1;
+            }
+            rstep[rlist_offset(L) + i] = make_int2(lex | (s1 << 16), s2 | (ax << 16) | (cf << 18));
+        }
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_rstep_all), rstep.data(), rstep.size() * sizeof(int2))) != hipSuccess)
+        return e;
+    std::vector<int> lexoff(kLexDim * kLexDim * kLexDim, -1);
+    for (int L = 0; L <= kColMaxL; ++L)
+        for (int t = 0; t <= L; ++t)
+            for (int u = 0; t + u <= L; ++u) lexoff[(L * kLexDim + t) * kLexDim + u] = lex_off(L, t, u);
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_lexoff_all), lexoff.data(), lexoff.size() * sizeof(int))) != hipSuccess)
+        return e;
     return hipSuccess;
 }
 
@@ -958,7 +995,9 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
     zc* vv = eket + nbox;
     zc* acc = vv + NHab;
     zc* sseed = acc + nab;
-    const unsigned* rl = c_rlist_all + rlist_offset(L);
+    const int2* rs = c_rstep_all + rlist_offset(L);
+    const int* lexL = lexoff_table(L);
+    const int* lexAB = lexoff_table(Lab);
     for (int o = lane; o < nab; o += stride) acc[o] = zc{0.0, 0.0};
     const int Kb = B.nprim, Kd = D.nprim;
     const int Kab = A.nprim * Kb, Kcd = C.nprim * Kd;
@@ -1006,32 +1045,19 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
                 zc* nxt = ((L - n) & 1) ? buf1 : buf0;
                 const int cnt = nherm(L - n);
                 for (int e = lane; e < cnt; e += stride) {
-                    const unsigned pk = rl[e];
-                    const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = (int)(pk >> 18);
+                    const int2 st = rs[e];
+                    const int lex = st.x & 0xffff, s1 = st.x >> 16;
+                    const int s2 = st.y & 0xffff, ax = (st.y >> 16) & 3, cf = st.y >> 18;
                     zc val;
-                    if (t > 0) {
-                        val = zmul(X0, cur[lex_off(L, t - 1, u) + v]);
-                        if (t > 1) {
-                            const zc r2 = cur[lex_off(L, t - 2, u) + v];
-                            val.x += (double)(t - 1) * r2.x;
-                            val.y += (double)(t - 1) * r2.y;
-                        }
-                    } else if (u > 0) {
-                        val = zmul(X1, cur[lex_off(L, 0, u - 1) + v]);
-                        if (u > 1) {
-                            const zc r2 = cur[lex_off(L, 0, u - 2) + v];
-                            val.x += (double)(u - 1) * r2.x;
-                            val.y += (double)(u - 1) * r2.y;
-                        }
-                    } else if (v > 0) {
-                        val = zmul(X2, cur[v - 1]);
-                        if (v > 1) {
-                            const zc r2 = cur[v - 2];
-                            val.x += (double)(v - 1) * r2.x;
-                            val.y += (double)(v - 1) * r2.y;
-                        }
-                    } else {
+                    if (ax == 3) {
                         val = sseed[3 + n];
+                    } else {
+                        val = zmul(ax == 0 ? X0 : (ax == 1 ? X1 : X2), cur[s1]);
+                        if (cf > 0) {
+                            const zc r2 = cur[s2];
+                            val.x += (double)cf * r2.x;
+                            val.y += (double)cf * r2.y;
+                        }
                     }
                     nxt[lex] = val;
                 }
@@ -1046,7 +1072,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
                 zc s{0.0, 0.0};
                 for (int jt = 0; jt < bx; ++jt)
                     for (int ju = 0; ju < by; ++ju) {
-                        const zc* rr = R0 + lex_off(L, kt + jt, ku + ju) + kv;
+                        const zc* rr = R0 + lexL[(kt + jt) * kLexDim + ku + ju] + kv;
                         const zc* ee = eket + (jt * by + ju) * bz;
                         for (int jv = 0; jv < bz; ++jv) zmac(s, ee[jv], rr[jv]);
                     }
@@ -1068,7 +1094,7 @@ __global__ __launch_bounds__(64) void chol_column_kernel(ClassArgs ca, const Wor
             for (int t = 0; t <= ax + bxp; ++t) {
                 zc st{0.0, 0.0};
                 for (int u = 0; u <= ay + byp; ++u) {
-                    const zc* ww = vv + lex_off(Lab, t, u);
+                    const zc* ww = vv + lexAB[t * kLexDim + u];
                     zc stu{0.0, 0.0};
                     for (int v = 0; v <= az + bzp; ++v) zmac(stu, Ez[v], ww[v]);
                     zmac(st, Ey[u], stu);
