@@ -1819,6 +1819,9 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                                       "Cholesky pivot block does not start at the maximum residual");
                 return c->failed;
             }
+            // the block projects the factor rows below rank rounded down to a
+            // multiple of 4 (the matrix-core row groups); the catch-up adds the rest
+            const uint64_t rows4 = c->rank & ~uint64_t(3);
             unsigned keep = 0;
             int nnew = 0;
             for (int m = 0; m < ntop; ++m) {
@@ -1837,7 +1840,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                 while ((used >> q) & 1u) ++q;  // a free slot: at most ntop <= slots are in use
                 host->new_slot[m] = q;
                 c->cand[q] = host->new_piv[m];
-                c->complete[q] = c->rank;
+                c->complete[q] = rows4;
                 used |= 1u << q;
             }
             c->open = used;
@@ -1849,24 +1852,23 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                           "copy");
             CHOL_STEP_TRY(hipEventRecord(c->ev[7], s), "hipEventRecord");
             if (nnew > 0) {
-                const int nbt = gh::chol_block_slots(nnew);
-                CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(c->rank, 1) * nbt * sizeof(zc)),
-                              "hipMalloc");
-                CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
-                                                   (int)c->rank, reinterpret_cast<const long long*>(dc + kCandNewPiv),
-                                                   nnew, nbt, c->d_coef.as<zc>(), s),
+                const int nbt = nnew <= 16 ? 16 : 32;
+                CHOL_STEP_TRY(c->d_coef.ensure((size_t)std::max<uint64_t>(rows4, 1) * nbt * sizeof(zc)), "hipMalloc");
+                CHOL_STEP_TRY(gh::launch_chol_coef(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)rows4,
+                                                   reinterpret_cast<const long long*>(dc + kCandNewPiv), nnew, nbt,
+                                                   c->d_coef.as<zc>(), s),
                               "coefficient launch");
-                CHOL_STEP_TRY(gh::launch_chol_project(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
-                                                      (int)c->rank, c->d_coef.as<zc>(), nnew, nbt,
-                                                      reinterpret_cast<const int*>(dc + kCandNewSlot),
-                                                      c->d_proj.as<zc>(), s),
+                CHOL_STEP_TRY(gh::launch_chol_project_mfma(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
+                                                           (int)rows4, c->d_coef.as<zc>(), nnew, nbt,
+                                                           reinterpret_cast<const int*>(dc + kCandNewSlot),
+                                                           c->d_proj.as<zc>(), s),
                               "projection launch");
-                c->rows_read += c->rank;
+                c->rows_read += rows4;
             }
             CHOL_STEP_TRY(hipEventRecord(c->ev[4], s), "hipEventRecord");
             c->blocks += 1;
             c->projected += nnew;
-            c->cand_rows += (uint64_t)nnew * c->rank;
+            c->cand_rows += (uint64_t)nnew * rows4;
             c->ncand = gh::kCholMaxBlock;  // slot table is searched by the open mask
             for (int q = 0; q < gh::kCholMaxBlock; ++q)
                 if (((c->open >> q) & 1u) && c->cand[q] == p) slot = q;
@@ -1875,9 +1877,9 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         }
         CHOL_STEP_TRY(hipEventRecord(c->ev[5], s), "hipEventRecord");
         // the pivot's projection lacks the factors accepted since it was projected
-        CHOL_STEP_TRY(gh::launch_chol_catchup(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
-                                              (int)c->complete[slot], (int)c->rank, p,
-                                              c->d_proj.as<zc>() + (long long)slot * n2, s),
+        CHOL_STEP_TRY(gh::launch_chol_catchup_mfma(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2,
+                                                   (int)c->complete[slot], (int)c->rank, p,
+                                                   c->d_proj.as<zc>() + (long long)slot * n2, s),
                       "catch-up launch");
         c->complete[slot] = c->rank;
         CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2, c->real,

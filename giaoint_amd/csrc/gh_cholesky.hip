@@ -9,10 +9,12 @@
 //   topk_kernel     the B candidates, from the per-block maxima of the last
 //                   residual update (only the blocks holding a top-B maximum
 //                   can hold a top-B entry);
-//   coef_kernel     c[k][m] = conj(L[k][p_m]) for the k < rank factors;
-//   project_kernel  proj[m] = sum_k c[k][m] L[k] for all candidates in one
-//                   pass over the factors (the GEMV of cholesky.py:268 for B
-//                   columns at once: B times fewer factor reads).
+//   coef_kernel     c[k][m] = conj(L[k][p_m]) for the factors k < rank4
+//                   (rank rounded down to a multiple of 4);
+//   project_mfma_kernel  proj[m] = sum_{k < rank4} c[k][m] L[k] for all
+//                   candidates in one pass over the factors on the FP64
+//                   matrix cores (the GEMV of cholesky.py:268 for B columns at
+//                   once: B times fewer factor reads).
 // Every accepted pivot then needs its raw ERI column (expand_kernel writes
 // its seed quartets and work items from a static bra table and the pivot's
 // ket (cholesky.cpp:203-282), the seed and column kernels of gh_eri.hip
@@ -20,14 +22,17 @@
 // res -= |L[rank]|^2 with the PSD check, clamp at 0, res[p] = 0 and per-block
 // (max, first index) partials. The argmax kernels finish the first maximum.
 // When that maximum is a candidate not yet taken, the next pivot needs no
-// pass over all factors: catchup_kernel adds to its projection the terms of
-// the factors accepted since it was projected (the candidates never taken
-// never get them); otherwise a new block starts.
+// pass over all factors: catchup_mfma_kernel adds to its projection the terms
+// of the factors from its projection's rank4 up to the current rank (the
+// candidates never taken never get them); otherwise a new block starts.
 //
-// Every projection accumulates conj(L[k][p]) L[k] in increasing k with the
-// same fused multiply-adds (cmac), starting from zero, and subtracts the sum
-// from the raw column once. Blocked and one-at-a-time runs therefore produce
-// bitwise identical factors and pivots for every block size.
+// Every projection accumulates conj(L[k][p]) L[k] from zero in groups of four
+// factor rows aligned at row 0, each group by the same matrix-core
+// instruction sequence (a candidate's value does not depend on the other
+// rows of the tile), and is subtracted from the raw column once. Blocked and
+// one-at-a-time runs therefore produce bitwise identical factors and pivots
+// for every block size. (chol_project_kernel, FMAs in increasing k, serves
+// the Coulomb contraction of the finished factors.)
 #include "gh_internal.h"
 
 namespace gh {
@@ -311,159 +316,143 @@ __global__ __launch_bounds__(256) void chol_project_kernel(const zc* const* __re
                 if (base + e * 256 < n2) proj[(long long)slots[m] * n2 + base + e * 256] = acc[m][e];
 }
 
-// acc[M0 + j] += c(M0 + j) * v for j < 4, where c(m) is the coefficient that
-// lane m of the lane's 16-lane row holds (DPP row broadcast of src0). Each
-// accumulator component sees cmac's order (x += c.x v.x, x += -c.y v.y;
-// y += c.x v.y, y += c.y v.x); the four slots are interleaved so that
-// dependent FMAs are eight instructions apart. The leading s_nop provides the
-// wait states a DPP read needs after a VALU write of its source.
-template <int M0, int NB>
-__device__ __forceinline__ void cmac4_bcast(zc (&acc)[NB], const zc& c, const zc& v) {
-    asm("s_nop 1\n\t"
-        "v_fmac_f64_dpp %0, %8, %10 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %2, %8, %10 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %4, %8, %10 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %6, %8, %10 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %1, %8, %11 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %3, %8, %11 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %5, %8, %11 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %7, %8, %11 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %0, -%9, %11 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %2, -%9, %11 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %4, -%9, %11 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %6, -%9, %11 row_newbcast:%c15 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %1, %9, %10 row_newbcast:%c12 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %3, %9, %10 row_newbcast:%c13 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %5, %9, %10 row_newbcast:%c14 row_mask:0xf bank_mask:0xf\n\t"
-        "v_fmac_f64_dpp %7, %9, %10 row_newbcast:%c15 row_mask:0xf bank_mask:0xf"
-        : "+v"(acc[M0].x), "+v"(acc[M0].y), "+v"(acc[M0 + 1].x), "+v"(acc[M0 + 1].y), "+v"(acc[M0 + 2].x),
-          "+v"(acc[M0 + 2].y), "+v"(acc[M0 + 3].x), "+v"(acc[M0 + 3].y)
-        : "v"(c.x), "v"(c.y), "v"(v.x), "v"(v.y), "n"(M0 % 16), "n"(M0 % 16 + 1), "n"(M0 % 16 + 2),
-          "n"(M0 % 16 + 3));
+constexpr int kStage = 64;  // factor rows per coefficient stage in LDS
+
+// ---------------------------------------------------------------------------
+// Projections on the FP64 matrix cores. v_mfma_f64_16x16x4f64 computes
+// D = A B + C for a 16x4 A, a 4x16 B and a 16x16 C; lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and holds D[l/16 + 4 r][l%16] in register r.
+// A = coefficients (candidate x factor row), B = factor rows (row x entry):
+// per group of four factor rows and per complex part, two MFMAs,
+//   re = A_re B_re + (-A_im) B_im + re,   im = A_re B_im + A_im B_re + im,
+// in that order. Every projection (block start, catch-up) accumulates its
+// factor rows in these groups of four, aligned at row 0, from zero, so a
+// value never depends on which kernel or which block summed which group.
+// ---------------------------------------------------------------------------
+typedef double mf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void mfma_group(mf4& re, mf4& im, double are, double aim, double anim, double bre,
+                                           double bim) {
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, re, 0, 0, 0);
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(anim, bim, re, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, im, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, im, 0, 0, 0);
 }
 
-// The same projection as chol_project_kernel for NB = 16 or 32 slots, one
-// entry per thread. The workgroup stages the coefficient rows of kStage
-// factors in LDS (one coalesced copy, instead of every wave fetching every
-// row from L2 and holding it in registers while it is in flight); per factor
-// row each lane reads one 16-byte coefficient (slot lane % 16, and 16 +
-// lane % 16 for NB = 32) and the FMAs take slot m's coefficient from lane m
-// of the 16-lane row by DPP broadcast. Factor rows stream U at a time with
-// the next U in flight.
-constexpr int kStage = 64;  // factor rows per coefficient stage
-template <int NB, int U>
-__global__ __launch_bounds__(256) void chol_project_bcast_kernel(const zc* const* __restrict__ chunks,
-                                                                 int rows_per_chunk, long long n2, int rank,
-                                                                 const zc* __restrict__ coef, int nnew,
-                                                                 const int* __restrict__ slots,
-                                                                 zc* __restrict__ proj) {
-    static_assert(NB == 16 || NB == 32, "one or two coefficients per lane");
-    constexpr int NL = NB / 16;  // coefficients per lane and row
+// proj[slots[m]][j] = sum_{k < rows} coef[k][m] L[k][j] for m < nnew; rows is
+// a multiple of 4. MT tiles of 16 candidates (coef row stride 16 MT); each
+// wave takes NJ tiles of 16 entries; the workgroup stages kStage coefficient
+// rows in LDS. The next group's factor loads are issued before the current
+// group's MFMAs.
+template <int MT, int NJ>
+__global__ __launch_bounds__(256) void chol_project_mfma_kernel(const zc* const* __restrict__ chunks,
+                                                                int rows_per_chunk, long long n2, int rows,
+                                                                const zc* __restrict__ coef, int nnew,
+                                                                const int* __restrict__ slots,
+                                                                zc* __restrict__ proj) {
+    constexpr int NB = 16 * MT;
     __shared__ zc sc[kStage * NB];
-    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long jc = j < n2 ? j : n2 - 1;
-    const int l16 = threadIdx.x & 15;
-    zc acc[NB];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int li = lane % 16, lk = lane / 16;
+    const long long j0 = ((long long)blockIdx.x * 4 + wave) * (16 * NJ);
+    mf4 acc[MT][2][NJ];
 #pragma unroll
-    for (int m = 0; m < NB; ++m) acc[m] = zc{0.0, 0.0};
-    auto step = [&](const zc (&c)[NL], const zc& v) {
-        cmac4_bcast<0>(acc, c[0], v);
-        cmac4_bcast<4>(acc, c[0], v);
-        cmac4_bcast<8>(acc, c[0], v);
-        cmac4_bcast<12>(acc, c[0], v);
-        if constexpr (NL == 2) {
-            cmac4_bcast<16>(acc, c[NL - 1], v);
-            cmac4_bcast<20>(acc, c[NL - 1], v);
-            cmac4_bcast<24>(acc, c[NL - 1], v);
-            cmac4_bcast<28>(acc, c[NL - 1], v);
-        }
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NJ; ++t) acc[mt][0][t] = acc[mt][1][t] = mf4{0.0, 0.0, 0.0, 0.0};
+    long long jj[NJ];
+#pragma unroll
+    for (int t = 0; t < NJ; ++t) {
+        const long long j = j0 + 16 * t + li;
+        jj[t] = j < n2 ? j : n2 - 1;  // lanes past n2 read a valid entry and store nothing
+    }
+    auto load = [&](zc (&b)[NJ], int k) {
+        const zc* row = chunks[k / rows_per_chunk] + (long long)(k % rows_per_chunk) * n2;
+#pragma unroll
+        for (int t = 0; t < NJ; ++t) b[t] = gload(row + jj[t]);
     };
-    auto coefs = [&](zc (&c)[NL], int r) {
+    for (int s0 = 0; s0 < rows; s0 += kStage) {
+        const int srows = min(kStage, rows - s0);
+        __syncthreads();  // the previous stage's coefficients are no longer read
+        for (int i = threadIdx.x; i < srows * NB; i += 256) sc[i] = coef[(long long)s0 * NB + i];
+        __syncthreads();
+        zc b[NJ], bn[NJ];
+        load(b, s0 + lk);
+        for (int r = 0; r < srows; r += 4) {
+            load(bn, s0 + min(r + 4, srows - 4) + lk);  // the next group (the last group reloads itself)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int h = 0; h < NL; ++h) c[h] = sc[r * NB + 16 * h + l16];
-    };
-    for (int k0 = 0; k0 < rank; k0 += rows_per_chunk) {
-        const zc* chunk = chunks[k0 / rows_per_chunk];
-        const int crows = min(rows_per_chunk, rank - k0);
-        for (int s0 = 0; s0 < crows; s0 += kStage) {
-            const int rows = min(kStage, crows - s0);
-            __syncthreads();  // the previous stage's coefficients are no longer read
-            for (int i = threadIdx.x; i < rows * NB; i += 256) sc[i] = coef[(long long)(k0 + s0) * NB + i];
-            __syncthreads();
-            const zc* base = chunk + (long long)s0 * n2 + jc;
-            const int full = rows - rows % (2 * U);
-            // rows r..r+U-1 of the stage, clamped (loads past the last group repeat row rows-1, unused)
-            auto load = [&](zc (&v)[U], int r) {
+            for (int mt = 0; mt < MT; ++mt) {
+                const zc a = sc[(r + lk) * NB + 16 * mt + li];
 #pragma unroll
-                for (int u = 0; u < U; ++u) v[u] = gload(base + (long long)min(r + u, rows - 1) * n2);
-            };
-            zc va[U], vb[U];
-            load(va, 0);
-            for (int r = 0; r < full; r += 2 * U) {
-                load(vb, r + U);
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    zc c[NL];
-                    coefs(c, r + u);
-                    step(c, va[u]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                load(va, r + 2 * U);
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    zc c[NL];
-                    coefs(c, r + U + u);
-                    step(c, vb[u]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
+                for (int t = 0; t < NJ; ++t) mfma_group(acc[mt][0][t], acc[mt][1][t], a.x, a.y, -a.y, b[t].x, b[t].y);
             }
-            for (int r = full; r < rows; ++r) {
-                zc c[NL];
-                coefs(c, r);
-                step(c, gload(base + (long long)r * n2));
-            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < NJ; ++t) b[t] = bn[t];
         }
     }
-    if (j < n2)
 #pragma unroll
-        for (int m = 0; m < NB; ++m)
-            if (m < nnew) proj[(long long)slots[m] * n2 + j] = acc[m];
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int t = 0; t < NJ; ++t) {
+            const long long j = j0 + 16 * t + li;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 16 * mt + lk + 4 * q;
+                if (m < nnew && j < n2) proj[(long long)slots[m] * n2 + j] = zc{acc[mt][0][t][q], acc[mt][1][t][q]};
+            }
+        }
 }
 
-// proj += sum_{from <= k < to} conj(L[k][pm]) L[k], k increasing: the terms
-// of the factors accepted since candidate pm's projection was complete.
-// Four elements per thread, each row's loads issued before its FMAs.
-constexpr int kCatchNT = 256, kCatchPer = 4;
-__global__ __launch_bounds__(kCatchNT) void chol_catchup_kernel(const zc* const* __restrict__ chunks,
+// proj += sum_{from <= k < to} conj(L[k][pm]) L[k] in the groups of four rows
+// of chol_project_mfma_kernel (from is a multiple of 4; rows past `to` of the
+// last group are zero): the candidate is row 0 of the A tile, the other rows
+// zero. Each wave takes 16 NJ entries.
+constexpr int kCatchNJ = 4;
+__global__ __launch_bounds__(256) void chol_catchup_mfma_kernel(const zc* const* __restrict__ chunks,
                                                                 int rows_per_chunk, long long n2, int from, int to,
                                                                 long long pm, zc* __restrict__ proj) {
-    const long long base = (long long)blockIdx.x * (kCatchNT * kCatchPer) + threadIdx.x;
-    zc acc[kCatchPer];
+    constexpr int NJ = kCatchNJ;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int li = lane % 16, lk = lane / 16;
+    const long long j0 = ((long long)blockIdx.x * 4 + wave) * (16 * NJ);
+    mf4 re[NJ], im[NJ];
+    long long jj[NJ];
 #pragma unroll
-    for (int e = 0; e < kCatchPer; ++e) {
-        const long long j = base + e * kCatchNT;
-        acc[e] = j < n2 ? proj[j] : zc{0.0, 0.0};
+    for (int t = 0; t < NJ; ++t) {
+        const long long j = j0 + 16 * t + li;
+        jj[t] = j < n2 ? j : n2 - 1;
+        const zc p0 = lk == 0 ? proj[jj[t]] : zc{0.0, 0.0};  // D row 0 = lanes 0..15, register 0
+        re[t] = mf4{p0.x, 0.0, 0.0, 0.0};
+        im[t] = mf4{p0.y, 0.0, 0.0, 0.0};
     }
-    for (int k = from; k < to; ++k) {
-        const zc* row = chunks[k / rows_per_chunk] + (long long)(k % rows_per_chunk) * n2;
-        const zc lp = row[pm];
-        const zc c{lp.x, -lp.y};
-        zc v[kCatchPer];
+    for (int k4 = from; k4 < to; k4 += 4) {
+        const int k = k4 + lk;
+        double are = 0.0, aim = 0.0;
+        zc b[NJ];
+        if (k < to) {
+            const zc* row = chunks[k / rows_per_chunk] + (long long)(k % rows_per_chunk) * n2;
+            if (li == 0) {
+                const zc x = row[pm];
+                are = x.x;
+                aim = -x.y;
+            }
 #pragma unroll
-        for (int e = 0; e < kCatchPer; ++e) {
-            const long long j = base + e * kCatchNT;
-            v[e] = row[j < n2 ? j : 0];
+            for (int t = 0; t < NJ; ++t) b[t] = gload(row + jj[t]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NJ; ++t) b[t] = zc{0.0, 0.0};
         }
 #pragma unroll
-        for (int e = 0; e < kCatchPer; ++e) cmac(acc[e], c, v[e]);
+        for (int t = 0; t < NJ; ++t) mfma_group(re[t], im[t], are, aim, -aim, b[t].x, b[t].y);
     }
+    if (lk == 0)
 #pragma unroll
-    for (int e = 0; e < kCatchPer; ++e) {
-        const long long j = base + e * kCatchNT;
-        if (j < n2) proj[j] = acc[e];
-    }
+        for (int t = 0; t < NJ; ++t) {
+            const long long j = j0 + 16 * t + li;
+            if (j < n2) proj[j] = zc{re[t][0], im[t][0]};
+        }
 }
 
 // Accepts the pivot st->pivot, whose raw column is `column` and whose
@@ -661,16 +650,36 @@ hipError_t launch_chol_project(const zc* const* chunks, int rows_per_chunk, long
         case 2: return project<2, 4, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
         case 4: return project<4, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
         case 8: return project<8, 2, 2>(chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj, s);
-        case 16:
-            chol_project_bcast_kernel<16, 4><<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(
-                chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj);
-            return hipGetLastError();
-        case 32:
-            chol_project_bcast_kernel<32, 2><<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(
-                chunks, rows_per_chunk, n2, rank, coef, nnew, slots, proj);
-            return hipGetLastError();
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_chol_project_mfma(const zc* const* chunks, int rows_per_chunk, long long n2, int rows,
+                                    const zc* coef, int nnew, int nbt, const int* slots, zc* proj, hipStream_t s) {
+    if (nnew == 0) return hipSuccess;
+    if (rows % 4 != 0) return hipErrorInvalidValue;
+    constexpr int NJ = 4;
+    const long long per = 4LL * 16 * NJ;  // entries per workgroup
+    const dim3 grid((unsigned)((n2 + per - 1) / per));
+    if (nbt == 16)
+        chol_project_mfma_kernel<1, NJ><<<grid, dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rows, coef, nnew, slots,
+                                                                    proj);
+    else if (nbt == 32)
+        chol_project_mfma_kernel<2, NJ><<<grid, dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rows, coef, nnew, slots,
+                                                                    proj);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_chol_catchup_mfma(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
+                                    long long pm, zc* proj, hipStream_t s) {
+    if (from >= to) return hipSuccess;
+    if (from % 4 != 0) return hipErrorInvalidValue;
+    const long long per = 4LL * 16 * kCatchNJ;
+    chol_catchup_mfma_kernel<<<dim3((unsigned)((n2 + per - 1) / per)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n2,
+                                                                                       from, to, pm, proj);
+    return hipGetLastError();
 }
 
 hipError_t launch_chol_gamma(const zc* const* chunks, int rows_per_chunk, long long n, int rank, const zc* dens,
@@ -685,15 +694,6 @@ hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, 
     if (total == 0) return hipSuccess;
     chol_reconstruct_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n,
                                                                                        rank, ranges, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_chol_catchup(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
-                               long long pm, zc* proj, hipStream_t s) {
-    if (from >= to) return hipSuccess;
-    const long long per = (long long)kCatchNT * kCatchPer;
-    chol_catchup_kernel<<<dim3((unsigned)((n2 + per - 1) / per)), dim3(kCatchNT), 0, s>>>(chunks, rows_per_chunk,
-                                                                                         n2, from, to, pm, proj);
     return hipGetLastError();
 }
 

@@ -51,8 +51,12 @@ hipError_t launch_chol_gamma(const zc* const* chunks, int rows_per_chunk, long l
                              zc* gamma, hipStream_t s);
 hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, long long n, int rank,
                                    const long long* ranges, long long total, zc* out, hipStream_t s);
-hipError_t launch_chol_catchup(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
-                               long long pm, zc* proj, hipStream_t s);
+// Matrix-core projections: rows (a multiple of 4) factor rows for nbt = 16 or
+// 32 coefficient columns; the catch-up starts at a multiple of 4.
+hipError_t launch_chol_project_mfma(const zc* const* chunks, int rows_per_chunk, long long n2, int rows,
+                                    const zc* coef, int nnew, int nbt, const int* slots, zc* proj, hipStream_t s);
+hipError_t launch_chol_catchup_mfma(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
+                                    long long pm, zc* proj, hipStream_t s);
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, int real, long long n, long long n2,
                               const CholState* st, double* res, zc* out_row, double psd_tol, int* flags, double* pmax,
                               long long* pidx, hipStream_t s);

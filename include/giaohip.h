@@ -218,8 +218,9 @@ int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* chol, uint64_t s
 int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* max_residual, double* tolerance,
                           int* converged, double* column_ms, double* update_ms, double* diagonal_ms);
 /* Pivot-block statistics of a run: blocks started, factor rows read by their
- * projections (sum of the rank at each block start that projected any
- * candidate; each row is n*n complex), device time of candidate selection +
+ * projections (sum over the block starts that projected any candidate of the
+ * rank rounded down to a multiple of 4, the projections' row groups; each
+ * row is n*n complex), device time of candidate selection +
  * projection (ms, included in update_ms), the number of candidates
  * projected (candidates still open from the previous block keep their
  * projections), and candidate_rows = sum over blocks of candidates projected
