@@ -1,14 +1,16 @@
 """Pivoted Cholesky of the ordered-pair Coulomb matrix, over the HIP library.
 
-Mirrors python/giao_integrals/cholesky.py:16-297 (CholeskyERI, cholesky_eri)
-for bases with one contraction per shell, where the reference's pivot policy
-is strict global pivoting (cholesky.py:175-247: complex multi-centre runs
-take each candidate set as the pivot itself; zero-field and one-centre runs
-use real arithmetic with symmetrized columns and never override the global
-maximum). Those runs stay on the device end to end (giaohip_cholesky_run).
-General contractions (contraction-variant candidate blocks, the local pivot
-override and the column LRU, cholesky.py:193-262) follow another policy and
-raise NotImplementedError here.
+Mirrors python/giao_integrals/cholesky.py:16-297 (CholeskyERI, cholesky_eri).
+Every run stays on the device end to end (giaohip_cholesky_run). Zero-field
+and one-centre runs use real arithmetic with symmetrized columns; they, and
+complex runs on bases with one contraction per shell, pivot on the global
+maximum (cholesky.py:175-247). Complex runs on bases with general
+contractions follow the reference's candidate-block policy (contraction
+variants of the pivot's component pair, the pivot taken from the latest
+block when its largest residual is at least 1% of the maximum, and the
+raw-column cache whose membership decides when a new block forms,
+cholesky.py:193-262); the library keeps that bookkeeping on the host and
+computes every column when its pivot is taken.
 """
 
 from __future__ import annotations
@@ -78,13 +80,11 @@ def cholesky_eri(basis: Basis, *, field: MagneticField | None = None, threshold:
     if max_bytes is not None and (isinstance(max_bytes, (bool, np.bool_))
                                   or not isinstance(max_bytes, (int, np.integer)) or max_bytes < 0):
         raise ValueError("max_bytes must be a non-negative integer or None")
-    shells = basis.shells
     n = basis.ao_count
-    if any(s.contraction_count != 1 for s in shells):
-        raise NotImplementedError("the device Cholesky covers bases with one contraction per shell")
     row_bytes = n * n * 16 + 16
     limit = n * n if max_bytes is None else min(n * n, int(max_bytes) // row_bytes)
     dev = basis._dev()
+    dev.ctx.set_cholesky_pivot_batch(int(pivot_batch_size))
     f = _field6(field)
 
     def over(rank):

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <list>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -25,6 +26,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 using gh::DevError;
@@ -140,6 +142,7 @@ struct giaohip_ctx {
     std::set<void*> user_allocs;
     size_t seed_budget_bytes = size_t(2048) << 20;
     int chol_block = 32;  // candidates per Cholesky pivot block
+    int chol_pivot_batch = 32;  // cholesky_eri's pivot_batch_size
     size_t staging_budget_bytes = size_t(1) << 30;
 };
 
@@ -179,6 +182,17 @@ struct giaohip_cholesky {
     std::vector<long long> seed_key;     // ket shell pair per buffer (-1: empty)
     std::vector<uint64_t> seed_used;     // last use (pivot count) per buffer
     uint64_t seed_hits = 0, seed_misses = 0;
+    // Pivot policy of complex runs on bases with general contractions
+    // (cholesky.py:193-262): the pivot may come from the latest candidate
+    // block; the candidate sets and the membership of the raw-column cache
+    // decide when. Only the membership matters here (every column is
+    // computed when its pivot is taken), so the cache holds indices.
+    bool policy = false;
+    int pivot_batch = 32;
+    std::vector<long long> pol_block, pol_cand;
+    std::list<long long> pol_lru;
+    std::unordered_map<long long, std::list<long long>::iterator> pol_pos;
+    DevBuffer d_gidx, d_gval;
     unsigned open = 0;
     uint64_t blocks = 0;
     uint64_t rows_read = 0;  // factor rows read by the block projections
@@ -193,7 +207,7 @@ struct giaohip_cholesky {
         for (zc* c : chunks) (void)hipFree(c);
         chunks.clear();
         for (DevBuffer* x : {&d_units, &d_sq, &d_wi, &d_res, &d_col, &d_coef, &d_pmax, &d_pidx, &d_state, &d_flags,
-                             &d_chunkptr, &d_seeds, &d_kappa, &d_err, &d_proj, &d_cand})
+                             &d_chunkptr, &d_seeds, &d_kappa, &d_err, &d_proj, &d_cand, &d_gidx, &d_gval})
             x->release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -1083,6 +1097,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
         ctx->chol_block = value;
         return GIAOHIP_OK;
     }
+    if (option == GIAOHIP_OPTION_CHOL_PIVOT_BATCH && value >= 1) {
+        ctx->chol_pivot_batch = value;
+        return GIAOHIP_OK;
+    }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");
 }
 
@@ -1658,6 +1676,7 @@ namespace {
 
 struct CholHost {
     gh::CholState state;
+    gh::CholState chosen;  // the policy's pivot, uploaded for the accept step
     int flags;
     int ntop;
     long long top[gh::kCholMaxBlock];  // a block's candidates, largest residual first
@@ -1667,6 +1686,94 @@ struct CholHost {
 };
 // Device layout of giaohip_cholesky::d_cand (bytes).
 constexpr size_t kCandTop = 0, kCandNewPiv = 256, kCandNewSlot = 512, kCandCount = 640, kCandBytes = 768;
+
+// Residual entries res[idx[i]] read back to the host.
+int chol_gather(giaohip_ctx* ctx, giaohip_cholesky* c, const std::vector<long long>& idx, std::vector<double>& vals) {
+    hipStream_t s = ctx->stream;
+    vals.assign(idx.size(), 0.0);
+    if (idx.empty()) return GIAOHIP_OK;
+    HIP_TRY(c->d_gidx.ensure(idx.size() * sizeof(long long)), "hipMalloc");
+    HIP_TRY(c->d_gval.ensure(idx.size() * sizeof(double)), "hipMalloc");
+    HIP_TRY(hipMemcpyAsync(c->d_gidx.p, idx.data(), idx.size() * sizeof(long long), hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+    HIP_TRY(gh::launch_chol_gather(c->d_res.as<double>(), c->d_gidx.as<long long>(), (int)idx.size(),
+                                   c->d_gval.as<double>(), s),
+            "gather launch");
+    HIP_TRY(hipMemcpyAsync(vals.data(), c->d_gval.p, idx.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "Cholesky policy");
+    return GIAOHIP_OK;
+}
+
+// The pivot of a complex run on a basis with general contractions
+// (cholesky.py:193-262), from the global maximum (p, value): the largest
+// residual of the latest candidate block replaces it when above the
+// tolerance and at least 1% of the maximum. A pivot outside the current
+// candidate set (and not in the raw-column cache) starts a new set: the
+// contraction variants of its Cartesian component pair above the tolerance,
+// largest residual first (stable), at most 32, the pivot forced in; the
+// block is its first min(32, pivot_batch_size); the others enter the cache
+// (least recently inserted leave first beyond min(1024, max(8, rank + 8))).
+int policy_pivot(giaohip_ctx* ctx, giaohip_cholesky* c, long long& p, double& value) {
+    const giaohip_basis* b = c->basis;
+    const long long n = c->n;
+    std::vector<double> vals;
+    if (!c->pol_block.empty()) {
+        if (int st = chol_gather(ctx, c, c->pol_block, vals)) return st;
+        size_t best = 0;
+        for (size_t i = 1; i < vals.size(); ++i)
+            if (vals[i] > vals[best]) best = i;
+        if (vals[best] > c->tolerance && vals[best] >= 0.01 * value) {
+            p = c->pol_block[best];
+            value = vals[best];
+        }
+    }
+    auto cache_erase = [&](long long k) {
+        auto it = c->pol_pos.find(k);
+        if (it == c->pol_pos.end()) return false;
+        c->pol_lru.erase(it->second);
+        c->pol_pos.erase(it);
+        return true;
+    };
+    if (cache_erase(p)) c->pol_cand.assign(1, p);
+    if (std::find(c->pol_cand.begin(), c->pol_cand.end(), p) == c->pol_cand.end()) {
+        const long long mu = p / n, nu = p % n;
+        const int sa = b->owner[mu], sb = b->owner[nu];
+        const DevShell &A = b->shells[sa], &B = b->shells[sb];
+        const long long oa = b->ao_off[sa], ob = b->ao_off[sb];
+        std::vector<long long> idx;
+        for (int i = 0; i < A.ncontr; ++i)
+            for (int j = 0; j < B.ncontr; ++j)
+                idx.push_back((oa + (mu - oa) % A.ncart + (long long)A.ncart * i) * n +
+                              (ob + (nu - ob) % B.ncart + (long long)B.ncart * j));
+        if (int st = chol_gather(ctx, c, idx, vals)) return st;
+        std::vector<size_t> keep;
+        for (size_t i = 0; i < idx.size(); ++i)
+            if (vals[i] > c->tolerance) keep.push_back(i);
+        std::stable_sort(keep.begin(), keep.end(), [&](size_t x, size_t y) { return vals[x] > vals[y]; });
+        if (keep.size() > 32) keep.resize(32);
+        c->pol_cand.clear();
+        for (size_t i : keep) c->pol_cand.push_back(idx[i]);
+        if (std::find(c->pol_cand.begin(), c->pol_cand.end(), p) == c->pol_cand.end()) {
+            if (c->pol_cand.empty()) c->pol_cand.push_back(p);
+            else c->pol_cand.back() = p;
+        }
+        c->pol_block.assign(c->pol_cand.begin(),
+                            c->pol_cand.begin() + std::min<size_t>(c->pol_cand.size(), std::min(32, c->pivot_batch)));
+        for (long long k : c->pol_cand)
+            if (k != p && !c->pol_pos.count(k)) {
+                c->pol_lru.push_back(k);
+                c->pol_pos[k] = std::prev(c->pol_lru.end());
+            }
+        const size_t cap = (size_t)std::min<uint64_t>(1024, std::max<uint64_t>(8, c->rank + 8));
+        while (c->pol_lru.size() > cap) {
+            c->pol_pos.erase(c->pol_lru.front());
+            c->pol_lru.pop_front();
+        }
+    }
+    cache_erase(p);
+    return GIAOHIP_OK;
+}
 
 // Pivot steps of a run until the residual is below the tolerance or the rank
 // reaches `target` (0: no target). Per step: when the pivot is not an open
@@ -1697,7 +1804,14 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             c->failed = set_error(ctx, GIAOHIP_OUT_OF_MEMORY, m.str());
             return c->failed;
         }
-        const long long p = c->state.pivot;
+        long long p = c->state.pivot;
+        double pvalue = c->state.value;
+        if (c->policy) {
+            if (int st = policy_pivot(ctx, c, p, pvalue)) {
+                c->failed = st;
+                return st;
+            }
+        }
         int slot = -1;
         for (int m = 0; m < c->ncand; ++m)
             if (((c->open >> m) & 1u) && c->cand[m] == p) slot = m;
@@ -1802,16 +1916,24 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             // one pass over the factors.
             char* dc = static_cast<char*>(c->d_cand.p);
             CHOL_STEP_TRY(hipEventRecord(c->ev[3], s), "hipEventRecord");
-            CHOL_STEP_TRY(gh::launch_chol_topk(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
-                                               c->d_pidx.as<long long>(), c->block,
-                                               reinterpret_cast<long long*>(dc + kCandTop),
-                                               reinterpret_cast<int*>(dc + kCandCount), s),
-                          "top-k launch");
-            CHOL_STEP_TRY(hipEventRecord(c->ev[6], s), "hipEventRecord");
-            CHOL_STEP_TRY(hipMemcpyAsync(host->top, dc + kCandTop, sizeof host->top, hipMemcpyDeviceToHost, s),
-                          "copy");
-            CHOL_STEP_TRY(hipMemcpyAsync(&host->ntop, dc + kCandCount, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-            CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky block");
+            if (c->policy) {
+                // the policy's pivot is the block's only candidate
+                CHOL_STEP_TRY(hipEventRecord(c->ev[6], s), "hipEventRecord");
+                host->top[0] = p;
+                host->ntop = 1;
+            } else {
+                CHOL_STEP_TRY(gh::launch_chol_topk(c->d_res.as<double>(), n2, c->d_pmax.as<double>(),
+                                                   c->d_pidx.as<long long>(), c->block,
+                                                   reinterpret_cast<long long*>(dc + kCandTop),
+                                                   reinterpret_cast<int*>(dc + kCandCount), s),
+                              "top-k launch");
+                CHOL_STEP_TRY(hipEventRecord(c->ev[6], s), "hipEventRecord");
+                CHOL_STEP_TRY(hipMemcpyAsync(host->top, dc + kCandTop, sizeof host->top, hipMemcpyDeviceToHost, s),
+                              "copy");
+                CHOL_STEP_TRY(hipMemcpyAsync(&host->ntop, dc + kCandCount, sizeof(int), hipMemcpyDeviceToHost, s),
+                              "copy");
+                CHOL_STEP_TRY(hipStreamSynchronize(s), "Cholesky block");
+            }
             const int ntop = host->ntop;
             if (ntop < 1 || host->top[0] != p) {
                 (void)hipStreamSynchronize(sc);
@@ -1882,6 +2004,11 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
                                                    c->d_proj.as<zc>() + (long long)slot * n2, s),
                       "catch-up launch");
         c->complete[slot] = c->rank;
+        if (c->policy) {
+            host->chosen = gh::CholState{pvalue, p};
+            CHOL_STEP_TRY(hipMemcpyAsync(c->d_state.p, &host->chosen, sizeof(gh::CholState), hipMemcpyHostToDevice, s),
+                          "copy");
+        }
         CHOL_STEP_TRY(gh::launch_chol_accept(c->d_col.as<zc>(), c->d_proj.as<zc>() + (long long)slot * n2, c->real,
                                              n, n2, c->d_state.as<gh::CholState>(), c->d_res.as<double>(), row,
                                              c->psd_tol, c->d_flags.as<int>(), c->d_pmax.as<double>(),
@@ -1943,10 +2070,6 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     *out = nullptr;
     if (!std::isfinite(threshold) || threshold < 0.0)
         return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "threshold must be finite and non-negative");
-    for (const auto& sh : b->shells)
-        if (sh.ncontr != 1)
-            return set_error(ctx, GIAOHIP_UNSUPPORTED,
-                             "device-resident Cholesky needs one contraction per shell; use the policy driver");
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     const long long n = b->nao, n2 = n * n;
@@ -1982,13 +2105,18 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
                 if (b->shells[a].l != la) continue;
                 for (int bb = 0; bb < ns; ++bb) {
                     if (b->shells[bb].l != lb) continue;
-                    gh::CholUnit u{};
-                    u.sh_a = a;
-                    u.sh_b = bb;
-                    u.lab = la + lb;
-                    u.kab = b->shells[a].nprim * b->shells[bb].nprim;
-                    u.out_base = b->ao_off[a] * n + b->ao_off[bb];
-                    units.push_back(u);
+                    for (int xa = 0; xa < b->shells[a].ncontr; ++xa)
+                        for (int xb = 0; xb < b->shells[bb].ncontr; ++xb) {
+                            gh::CholUnit u{};
+                            u.sh_a = a;
+                            u.sh_b = bb;
+                            u.xa = xa;
+                            u.xb = xb;
+                            u.lab = la + lb;
+                            u.kab = b->shells[a].nprim * b->shells[bb].nprim;
+                            u.out_base = b->ao_off[a] * n + b->ao_off[bb];
+                            units.push_back(u);
+                        }
                 }
             }
             if ((int)units.size() > first) {
@@ -2032,7 +2160,13 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     CHOL_TRY(c->d_pidx.ensure(parts * sizeof(long long)), "hipMalloc");
     CHOL_TRY(c->d_state.ensure(sizeof(gh::CholState)), "hipMalloc");
     CHOL_TRY(c->d_flags.ensure(sizeof(int)), "hipMalloc");
-    c->block = ctx->chol_block;
+    c->pivot_batch = ctx->chol_pivot_batch;
+    {
+        bool general = false;
+        for (const auto& sh : b->shells) general = general || sh.ncontr > 1;
+        c->policy = !real && general && c->pivot_batch > 1;
+    }
+    c->block = c->policy ? 1 : ctx->chol_block;
     c->slots = gh::chol_block_slots(c->block);
     CHOL_TRY(c->d_proj.ensure((size_t)c->slots * n2 * sizeof(zc)), "hipMalloc(projections)");
     CHOL_TRY(c->d_cand.ensure(kCandBytes), "hipMalloc");

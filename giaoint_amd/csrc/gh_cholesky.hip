@@ -61,8 +61,8 @@ __global__ void chol_expand_kernel(const CholUnit* __restrict__ units, int nunit
     w.sh[1] = U.sh_b;
     w.sh[2] = ket.sh_c;
     w.sh[3] = ket.sh_d;
-    w.x[0] = 0;
-    w.x[1] = 0;
+    w.x[0] = U.xa;
+    w.x[1] = U.xb;
     w.x[2] = ket.xc;
     w.x[3] = ket.xd;
     w.cd0 = ket.comp;
@@ -694,6 +694,18 @@ hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, 
     if (total == 0) return hipSuccess;
     chol_reconstruct_kernel<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n,
                                                                                        rank, ranges, out);
+    return hipGetLastError();
+}
+
+__global__ void chol_gather_kernel(const double* __restrict__ res, const long long* __restrict__ idx, int m,
+                                   double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = res[idx[i]];
+}
+
+hipError_t launch_chol_gather(const double* res, const long long* idx, int m, double* out, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    chol_gather_kernel<<<dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s>>>(res, idx, m, out);
     return hipGetLastError();
 }
 

@@ -11,8 +11,9 @@ struct ClassArgs {
 };
 
 // Device-resident Cholesky (gh_cholesky.hip).
-struct CholUnit {  // one bra shell pair of a pivot column
+struct CholUnit {  // one bra shell pair (and contraction rows) of a pivot column
     int sh_a, sh_b;
+    int xa, xb;
     int lab, kab;
     long long p1;        // sum of kab over earlier units
     long long p2;        // sum of kab * lab over earlier units
@@ -57,6 +58,8 @@ hipError_t launch_chol_project_mfma(const zc* const* chunks, int rows_per_chunk,
                                     const zc* coef, int nnew, int nbt, const int* slots, zc* proj, hipStream_t s);
 hipError_t launch_chol_catchup_mfma(const zc* const* chunks, int rows_per_chunk, long long n2, int from, int to,
                                     long long pm, zc* proj, hipStream_t s);
+// out[i] = res[idx[i]] for i < m (residual entries the host pivot policy reads).
+hipError_t launch_chol_gather(const double* res, const long long* idx, int m, double* out, hipStream_t s);
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, int real, long long n, long long n2,
                               const CholState* st, double* res, zc* out_row, double psd_tol, int* flags, double* pmax,
                               long long* pidx, hipStream_t s);

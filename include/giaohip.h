@@ -64,7 +64,13 @@ enum {
        need no further pass. Factors and pivots are bitwise identical for
        every value; it only trades passes over the factors against the
        projection work per pass. Read when a run starts. */
-    GIAOHIP_OPTION_CHOL_BLOCK = 2
+    GIAOHIP_OPTION_CHOL_BLOCK = 2,
+    /* cholesky_eri's pivot_batch_size (python/giao_integrals/cholesky.py:
+       193-206), >= 1; default 32. It only matters for complex runs on bases
+       with general contractions, where the pivot may be taken from the
+       latest candidate block instead of the global maximum. Read when a run
+       starts. */
+    GIAOHIP_OPTION_CHOL_PIVOT_BATCH = 3
 };
 
 typedef struct giaohip_ctx giaohip_ctx;

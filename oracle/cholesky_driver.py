@@ -26,11 +26,13 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
     tolerance). ``max_pivots`` truncates the loop (prefix timing only);
     ``history`` (a list) receives a copy of the residual before every step.
 
-    ``follow`` (strict pivoting only) replays another implementation's pivot
-    sequence: at every step the followed pivot must be a valid argmax up to
-    rounding, i.e. its residual within ``tie_rtol * max(initial diagonal)`` of
-    the largest residual (residuals D - sum |L|^2 carry absolute rounding of
-    order eps * max D per update); otherwise AssertionError. ``stats`` (a dict) receives the number
+    ``follow`` replays another implementation's pivot sequence: at every
+    step the followed pivot must be the policy's choice up to rounding, i.e.
+    its residual within ``tie_rtol * max(initial diagonal)`` of the largest
+    residual of the pool the policy takes the pivot from (everything, or the
+    latest candidate block when the local override applies; residuals
+    D - sum |L|^2 carry absolute rounding of order eps * max D per update);
+    otherwise AssertionError. ``stats`` (a dict) receives the number
     of steps where the followed pivot differs from this run's own argmax.
     """
     offsets = np.asarray(ao_offsets, dtype=np.int64)
@@ -69,24 +71,32 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
         if history is not None:
             history.append(residual.copy())
         pivot = int(np.argmax(residual))
+        overridden = False
+        if reuse and not real and pivot_batch_size > 1 and len(block):
+            local = int(block[np.argmax(residual[block])])
+            if residual[local] > tolerance and residual[local] >= 0.01 * residual[pivot]:
+                pivot = local
+                overridden = True
+                if stats is not None:
+                    stats["overrides"] = stats.get("overrides", 0) + 1
         if follow is not None:
+            # the followed pivot must be this policy's choice up to rounding:
+            # a maximum of the pool the policy chose from (the block when it
+            # overrode the global maximum, else everything) within tie_rtol
             k = len(pivots)
             if k >= len(follow):
                 raise AssertionError(f"followed run stopped after {k} pivots; residual {residual[pivot]:.3e} > {tolerance:.3e}")
             fp = int(follow[k][0]) * n + int(follow[k][1])
             top = residual[pivot]
-            if not residual[fp] >= top - tie_rtol * scale:
+            in_pool = (not overridden) or fp in set(int(x) for x in block)
+            if not (in_pool and residual[fp] >= top - tie_rtol * scale):
                 raise AssertionError(
-                    f"step {k}: followed pivot {divmod(fp, n)} residual {residual[fp]!r} is not a maximum "
-                    f"({divmod(pivot, n)} has {top!r})"
+                    f"step {k}: followed pivot {divmod(fp, n)} residual {residual[fp]!r} is not the policy's "
+                    f"choice ({divmod(pivot, n)} has {top!r})"
                 )
             if fp != pivot and stats is not None:
                 stats["tie_flips"] = stats.get("tie_flips", 0) + 1
             pivot = fp
-        elif reuse and not real and pivot_batch_size > 1 and len(block):
-            local = int(block[np.argmax(residual[block])])
-            if residual[local] > tolerance and residual[local] >= 0.01 * residual[pivot]:
-                pivot = local
         rank = len(pivots)
         if rank >= limit:
             raise MemoryError(f"Cholesky ERI output requires at least {(rank + 1) * row_bytes} bytes")

@@ -91,9 +91,6 @@ def test_cholesky_argument_checks():
         cholesky.cholesky_eri(b, field=f, pivot_batch_size=0)
     with pytest.raises(ValueError, match="max_bytes"):
         cholesky.cholesky_eri(b, field=f, max_bytes=-1)
-    c2 = core.Shell((0, 0, 1.5), 0, [1.0, 0.5], [[1.0, 0.2], [0.3, 1.0]])
-    with pytest.raises(NotImplementedError):
-        cholesky.cholesky_eri(core.Basis([s1, c2]), field=f)  # general contraction
 
 
 def test_reconstruct_convention():

@@ -204,3 +204,45 @@ def test_device_factor_consumers(gpu, port):
     assert np.max(np.abs(J - Jx)) <= (1e-8 + 1e-12) * np.abs(D).sum()
     with pytest.raises(IndexError):
         res.reconstruct((0, n + 1))
+
+
+@pytest.mark.parametrize("case", ["water", "water-bent"])
+def test_device_cholesky_general_contractions(gpu, port, case):
+    """Complex runs on bases with general contractions follow the reference's
+    candidate-block policy (cholesky.py:193-262): the pivot may be the
+    largest residual of the latest block of contraction variants instead of
+    the global maximum. Water cc-pVDZ in a field (and a copy with one H moved
+    off the mirror plane): the checker replays the GPU's pivots, each of
+    which must be its own policy's choice up to rounding-level ties, and the
+    factors must agree to 1e-10."""
+    sh, B, O = workloads.water_ccpvdz()
+    shells = sh.shell_list()
+    if case == "water-bent":
+        moved = tuple(np.add(shells[-1][0], (0.13, -0.07, 0.05)))
+        shells = [s if i < len(shells) - 2 else (moved, s[1], s[2], s[3]) for i, s in enumerate(shells)]
+        B = (0.03, -0.05, 0.11)
+    b = make_basis(shells, port)
+    assert any(k > 1 for k in b.ncontr)
+    f = _native.field6(B, O)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    gpu.set_cholesky_pivot_batch(32)
+    res = _native.cholesky_run(db, f, 1e-8)
+    assert res.converged and res.max_residual <= res.tolerance
+    gp = res.pivots()
+    stats = {}
+    vec, piv, maxres, tol = cholesky_driver.cholesky_eri(
+        b.ao_offsets, [((l + 1) * (l + 2) // 2, k, p) for l, k, p in zip(b.L, b.ncontr, b.nprim)],
+        lambda pairs: port.cholesky_diagonal(b, pairs, f, 16),
+        lambda pv: port.cholesky_columns(b, pv, f, 16),
+        real=False, single_center=False, threshold=1e-8, follow=gp, stats=stats,
+    )
+    print(f"{case}: rank {res.rank}; {stats.get('overrides', 0)} pivots from the block instead of the global "
+          f"maximum; pivots decided by rounding-level ties {stats.get('tie_flips', 0)}")
+    assert stats.get("overrides", 0) > 0  # the policy's local override is exercised
+    assert len(piv) == res.rank
+    assert abs(res.tolerance - tol) <= 1e-12 * tol
+    worst = float(np.max(np.abs(res.vectors(0, res.rank) - vec)))
+    print(f"max |L_gpu - L_checker| = {worst:.3e}")
+    assert worst <= 1e-10
+    assert abs(res.max_residual - maxres) <= 1e-12
+    gpu.set_cholesky_pivot_batch(32)
