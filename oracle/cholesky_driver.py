@@ -8,6 +8,9 @@ reproduces the reference's pivot policy: strict global argmax (first maximum
 in flat mu*n+nu order), the local-block override for complex generally
 contracted factors, the candidate sets and the bounded raw-column cache.
 Schwarz-screened diagonals (screening_threshold > 0) are not restated.
+Which pivot the general-contraction policy picks is parity unpinned (no
+reference output for it is available offline); its factorization is pinned
+by the dense factorization in the same pivot order.
 """
 
 from __future__ import annotations
