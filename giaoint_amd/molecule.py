@@ -1,0 +1,245 @@
+"""Molecules for the ERI and Cholesky entry points.
+
+Mirrors the part of python/giao_integrals/molecule.py that feeds the ERI
+path: geometry in angstrom or bohr (1 bohr = 0.529177210544 angstrom,
+molecule.py:26,65-83), XYZ input (molecule.py:86-112), the basis built from
+Basis Set Exchange JSON in atom order and then stable increasing L order
+within each atom, a general contraction kept as one shell with several
+coefficient rows and combined (e.g. SP) shells split per angular momentum
+(molecule.py:191-267), and the eri / cholesky_eri / eri_batches methods
+(molecule.py:390-450). Named bases and Gaussian94/NWChem text need the
+basis-set-exchange package, which is not a dependency here: pass BSE JSON
+(a dictionary, a .json path or JSON text). One-electron integrals are not
+part of this package.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+from os import PathLike
+from pathlib import Path
+
+from . import cholesky as _cholesky
+from . import eri as _eri
+from .core import Basis, MagneticField, Shell
+
+BOHR_IN_ANGSTROM = 0.529177210544
+
+_ELEMENTS = (
+    "H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "
+    "Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb "
+    "Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At Rn Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf Es Fm Md No Lr Rf "
+    "Db Sg Bh Hs Mt Ds Rg Cn Nh Fl Mc Lv Ts Og"
+).split()
+_Z = {symbol: z for z, symbol in enumerate(_ELEMENTS, start=1)}
+
+
+def _float(value) -> float:
+    if isinstance(value, bool):
+        raise TypeError("expected a number")
+    return float(value)
+
+
+def _element(value) -> str:
+    """Element symbol from a symbol (any case) or an atomic number."""
+    if isinstance(value, str):
+        text = value.strip()
+        if text.isdigit():
+            z = int(text)
+        else:
+            symbol = text[:1].upper() + text[1:].lower()
+            if symbol in _Z:
+                return symbol
+            z = -1
+    elif isinstance(value, int) and not isinstance(value, bool):
+        z = value
+    else:
+        z = -1
+    if 1 <= z <= len(_ELEMENTS):
+        return _ELEMENTS[z - 1]
+    raise ValueError(f"Unknown element {value!r}; use an element symbol or atomic number")
+
+
+def _atoms_in_bohr(atoms, unit: str):
+    scale = {"angstrom": 1.0 / BOHR_IN_ANGSTROM, "bohr": 1.0}
+    if unit not in scale:
+        raise ValueError("unit must be 'angstrom' or 'bohr'")
+    out = []
+    for number, atom in enumerate(atoms, start=1):
+        try:
+            symbol, coordinates = atom
+            position = tuple(_float(x) * scale[unit] for x in coordinates)
+            if len(position) != 3 or not all(math.isfinite(x) for x in position):
+                raise ValueError("coordinates must be three finite numbers")
+            out.append((_element(symbol), position))
+        except (TypeError, ValueError, OverflowError) as error:
+            raise ValueError(f"Invalid atom {number}: {error}") from error
+    if not out:
+        raise ValueError("A molecule must contain at least one atom")
+    return tuple(out)
+
+
+def _xyz_atoms(source):
+    """Atoms of one XYZ frame, from a path or from multiline text."""
+    if isinstance(source, PathLike) or ("\n" not in source and "\r" not in source):
+        text = Path(source).read_text(encoding="utf-8-sig")
+    else:
+        text = source.lstrip("\ufeff")
+    lines = text.lstrip("\r\n").splitlines()
+    try:
+        count = int(lines[0].strip())
+    except (IndexError, ValueError) as error:
+        raise ValueError("XYZ must start with an integer atom count") from error
+    if count < 1:
+        raise ValueError("XYZ atom count must be positive")
+    if len(lines) < count + 2:
+        raise ValueError(f"XYZ requires a comment line and {count} atom lines")
+    if any(line.strip() for line in lines[count + 2:]):
+        raise ValueError("Expected one XYZ frame; extra atoms or frames follow the declared count")
+    atoms = []
+    for row, line in enumerate(lines[2:count + 2], start=3):
+        fields = line.split()
+        if len(fields) != 4:
+            raise ValueError(f"XYZ line {row} must contain an element and three coordinates")
+        atoms.append((fields[0], fields[1:]))
+    return atoms
+
+
+def _basis_json(source, basis_format):
+    """BSE JSON data from a dictionary, a path or JSON text."""
+    if basis_format not in (None, "json", "gaussian94", "nwchem"):
+        raise ValueError("basis_format must be 'json', 'gaussian94', or 'nwchem'")
+    if isinstance(source, dict):
+        if basis_format not in (None, "json"):
+            raise ValueError("A basis dictionary uses basis_format='json'")
+        return source
+    if not isinstance(source, (str, PathLike)):
+        raise TypeError("basis must be a basis name, file path, formatted text, or BSE dictionary")
+    if basis_format in ("gaussian94", "nwchem"):
+        raise ImportError("Gaussian94 and NWChem bases require basis-set-exchange, which is not available; "
+                          "pass BSE JSON instead")
+    if isinstance(source, str) and source.lstrip().startswith("{"):
+        text = source
+    else:
+        path = Path(source)
+        if not (isinstance(source, PathLike) or path.is_file() or path.suffix or "/" in str(source)):
+            raise ImportError(f"Named bases ({source!r}) require basis-set-exchange, which is not available; "
+                              "pass BSE JSON instead")
+        if path.suffix.lower() not in ("", ".json") and basis_format is None:
+            raise ImportError("Gaussian94 and NWChem bases require basis-set-exchange, which is not available; "
+                              "pass BSE JSON instead")
+        text = path.read_text(encoding="utf-8-sig")
+    try:
+        data = json.loads(text)
+    except ValueError as error:
+        raise ValueError(f"Invalid BSE JSON basis: {error}") from error
+    if not isinstance(data, dict):
+        raise ValueError("BSE JSON basis must be an object containing 'elements'")
+    return data
+
+
+def _shells(atoms, data):
+    """Shells in atom order, then stable increasing L per atom; the owning atom of each."""
+    elements = data.get("elements")
+    if not isinstance(elements, dict):
+        raise ValueError("BSE basis data must contain an 'elements' object keyed by atomic number")
+    shells, owners = [], []
+    for index, (symbol, position) in enumerate(atoms):
+        element = elements.get(str(_Z[symbol]))
+        if not isinstance(element, dict):
+            raise ValueError(f"Basis has no data for element {symbol}")
+        if element.get("ecp_potentials") or element.get("ecp_electrons", 0):
+            raise ValueError(f"Basis for {symbol} contains an ECP; only all-electron bases are supported")
+        entries = element.get("electron_shells")
+        if not isinstance(entries, list) or not entries:
+            raise ValueError(f"Basis for {symbol} has no electron shells")
+        specs = []
+        for k, entry in enumerate(entries):
+            try:
+                if entry.get("function_type", "gto") not in ("gto", "gto_cartesian", "gto_spherical"):
+                    raise ValueError("only Gaussian orbital shells are supported")
+                momenta = entry["angular_momentum"]
+                if not momenta or any(type(l) is not int or l < 0 for l in momenta):
+                    raise ValueError("angular momenta must be non-negative integers")
+                if len(set(momenta)) != len(momenta):
+                    raise ValueError("combined shells must have distinct angular momenta")
+                exps = [_float(x) for x in entry["exponents"]]
+                rows = [[_float(x) for x in row] for row in entry["coefficients"]]
+                if len(momenta) == 1:
+                    specs.append((momenta[0], exps, rows))
+                elif len(rows) == len(momenta):
+                    specs.extend((l, exps, [row]) for l, row in zip(momenta, rows))
+                else:
+                    raise ValueError("combined shells require one coefficient row per angular momentum")
+            except (AttributeError, KeyError, TypeError, ValueError, OverflowError) as error:
+                raise ValueError(f"Invalid basis shell {k} for {symbol}: {error}") from error
+        for l, exps, rows in sorted(specs, key=lambda spec: spec[0]):
+            try:
+                shells.append(Shell(position, l, exps, rows))
+            except (TypeError, ValueError, OverflowError) as error:
+                raise ValueError(f"Invalid basis shell for {symbol}: {error}") from error
+            owners.append(index)
+    return Basis(shells), tuple(owners)
+
+
+class Molecule:
+    """Geometry, a Cartesian basis and the ERI methods (molecule.py:285-450).
+
+    ``atoms`` is ``[(symbol, (x, y, z)), ...]`` in angstrom unless
+    ``unit='bohr'``; stored coordinates are in bohr. ``basis`` is BSE JSON
+    (dictionary, path or text). Output is Cartesian.
+    """
+
+    def __init__(self, atoms, basis, *, unit: str = "angstrom", basis_format=None, field=None):
+        if field is not None and not isinstance(field, MagneticField):
+            raise TypeError("field must be a MagneticField or None")
+        self._atoms = _atoms_in_bohr(atoms, unit)
+        self._basis, self._shell_atoms = _shells(self._atoms, _basis_json(basis, basis_format))
+        self._field = field if field is not None else MagneticField()
+
+    @classmethod
+    def from_xyz(cls, source, basis, *, unit: str = "angstrom", basis_format=None, field=None) -> "Molecule":
+        """One XYZ frame from a path or multiline text."""
+        return cls(_xyz_atoms(source), basis, unit=unit, basis_format=basis_format, field=field)
+
+    @property
+    def atoms(self):
+        return self._atoms
+
+    @property
+    def basis(self) -> Basis:
+        return self._basis
+
+    @property
+    def shell_atoms(self):
+        return self._shell_atoms
+
+    @property
+    def field(self) -> MagneticField:
+        return self._field
+
+    @property
+    def ao_count(self) -> int:
+        return self._basis.ao_count
+
+    def __repr__(self) -> str:
+        return f"Molecule(atoms={len(self._atoms)}, shells={len(self._shell_atoms)}, ao_count={self.ao_count})"
+
+    def eri(self, *, max_bytes=None, screening_threshold: float = 0.0, threads=None):
+        """The full (nao, nao, nao, nao) tensor; max_bytes=None means no cap."""
+        cap = self.ao_count**4 * 16 if max_bytes is None else max_bytes
+        return _eri.eri(self._basis, field=self._field, storage="full", max_bytes=cap,
+                        screening_threshold=screening_threshold, threads=threads)
+
+    def cholesky_eri(self, *, threshold: float = 1e-8, screening_threshold: float = 0.0, threads=None,
+                     max_bytes=None, pivot_batch_size: int = 32):
+        """Coulomb Cholesky factors (see giaoint_amd.cholesky.cholesky_eri)."""
+        return _cholesky.cholesky_eri(self._basis, field=self._field, threshold=threshold,
+                                      screening_threshold=screening_threshold, threads=threads,
+                                      max_bytes=max_bytes, pivot_batch_size=pivot_batch_size)
+
+    def eri_batches(self, *, target_bytes: int = 64 * 1024**2, screening_threshold: float = 0.0, threads=None):
+        """Canonical shell-quartet batches with the molecule's field."""
+        return _eri.eri_batches(self._basis, field=self._field, target_bytes=target_bytes,
+                                screening_threshold=screening_threshold, threads=threads)

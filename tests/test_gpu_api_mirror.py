@@ -104,3 +104,27 @@ def test_selected_integrals_and_cholesky_eri(port):
         cholesky.cholesky_eri(mb, field=field, max_bytes=3 * row)
     with pytest.raises(MemoryError, match="exceeding max_bytes=0"):
         cholesky.cholesky_eri(mb, field=field, max_bytes=0)
+
+
+def test_molecule_entry_points(gpu, port):
+    """Molecule (molecule.py:285-450) on water cc-pVDZ from the committed BSE
+    JSON: the full tensor against the checker's, and the Cholesky factors
+    (general contractions, complex field) reconstructing it within the
+    threshold."""
+    from giaoint_amd.molecule import Molecule
+    from oracle.pyoracle import make_basis
+
+    atoms = [("O", (0.0, 0.0, 0.1173)), ("H", (0.0, 0.7572, -0.4692)), ("H", (0.0, -0.7572, -0.4692))]
+    mol = Molecule(atoms, workloads.DATA / "cc-pvdz-h-o.json", field=core.MagneticField((0.0, 0.0, 0.1)))
+    T = mol.eri()
+    sh, B, O = workloads.water_ccpvdz()
+    b = make_basis(sh.shell_list(), port)
+    want = port.eri_tensor(b, port.field6(B, O), threads=8)
+    err = float(np.max(np.abs(T - want)))
+    print(f"Molecule.eri max |err| {err:.2e}")
+    assert err <= 1e-11
+    ch = mol.cholesky_eri(threshold=1e-6)
+    recon = ch.reconstruct()
+    gap = float(np.max(np.abs(recon - want)))
+    print(f"Molecule.cholesky_eri rank {len(ch.pivots)}, max |ERI - LL*| {gap:.2e}")
+    assert gap <= 1e-6 + 1e-11
