@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per (dd|dd) quartet of fast_transform_kernel<2,2,2,2>, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a copy-kernel
 # calibration (profiles/fast_transform_dddd_200k_pmc.md).
-TRANSFORM_HBM_BYTES_PER_QUARTET = 369_357
+TRANSFORM_HBM_BYTES_PER_QUARTET = 370_995
 TRANSFORM_TRAFFIC_NOTE = ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), "
-                          "factors from a 256 MiB copy; 369,357 B per (dd|dd) quartet x quartets per launch")
+                          "factors from a 256 MiB copy; 370,995 B per (dd|dd) quartet x quartets per launch")
 METRIC = "shell quartets/s + FP64 GFLOP/s vs peak; Cholesky wall time, 1 MI355X"
 
 
