@@ -207,6 +207,10 @@ class Context:
         """Candidates per pivot block of later Cholesky runs (1..32; results do not depend on it)."""
         self.check(self.lib.giaohip_set_option(self.handle, 2, int(candidates)))
 
+    def set_staging_mib(self, mib: int):
+        """Device staging of host-memory ERI outputs in MiB (1..65536)."""
+        self.check(self.lib.giaohip_set_option(self.handle, 4, int(mib)))
+
     def set_cholesky_pivot_batch(self, value: int):
         """cholesky_eri's pivot_batch_size for later Cholesky runs (>= 1)."""
         self.check(self.lib.giaohip_set_option(self.handle, 3, int(value)))

@@ -138,6 +138,13 @@ struct giaohip_ctx {
     hipEvent_t ev[5]{};
     std::vector<hipEvent_t> chunk_ev;  // per-chunk timing events of the batched path
     PinnedBuffer h_sq, h_pq, h_items;  // host staging of the batched path's descriptors
+    // host outputs: a second device staging buffer (one group is computed
+    // while the previous one is copied out), page-locked bounce buffers and
+    // the copy stream with its events
+    DevBuffer staging2;
+    PinnedBuffer bounce[2];
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_ev[2]{};
     DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
     std::set<void*> user_allocs;
     size_t seed_budget_bytes = size_t(2048) << 20;
@@ -1038,7 +1045,9 @@ int giaohip_create(int device, giaohip_ctx** out) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ctx->copy_ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = gh::upload_constants();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -1054,6 +1063,11 @@ int giaohip_destroy(giaohip_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    ctx->staging2.release();
+    for (PinnedBuffer& b : ctx->bounce) b.release();
+    for (auto& ev : ctx->copy_ev)
+        if (ev) (void)hipEventDestroy(ev);
     for (DevBuffer* b : {&ctx->seeds, &ctx->squartets, &ctx->items, &ctx->kappa, &ctx->errrec, &ctx->staging,
                          &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c, &ctx->fpq, &ctx->fitems})
         b->release();
@@ -1066,6 +1080,7 @@ int giaohip_destroy(giaohip_ctx* ctx) {
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx;
     return GIAOHIP_OK;
 }
@@ -1099,6 +1114,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_CHOL_PIVOT_BATCH && value >= 1) {
         ctx->chol_pivot_batch = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_STAGING_MIB && value >= 1 && value <= 65536) {
+        ctx->staging_budget_bytes = (size_t)value << 20;
         return GIAOHIP_OK;
     }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");
@@ -1286,6 +1305,37 @@ int giaohip_boys(giaohip_ctx* ctx, const double* z, size_t n, int nmax, int scal
     return GIAOHIP_OK;
 }
 
+// Device -> pageable host copy through two page-locked bounce buffers: the
+// DMA of piece k + 1 runs while piece k is copied out by host threads.
+// Returns a HIP error (hipSuccess when done).
+hipError_t copy_to_host(giaohip_ctx* ctx, const char* src, char* dst, size_t bytes) {
+    constexpr size_t kPiece = size_t(64) << 20;
+    for (PinnedBuffer& b : ctx->bounce)
+        if (hipError_t e = b.ensure(kPiece); e != hipSuccess) return e;
+    hipStream_t cs = ctx->copy_stream;
+    const size_t pieces = (bytes + kPiece - 1) / kPiece;
+    auto drain = [&](size_t k) -> hipError_t {
+        if (hipError_t e = hipEventSynchronize(ctx->copy_ev[k % 2]); e != hipSuccess) return e;
+        const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
+        const char* from = ctx->bounce[k % 2].as<char>();
+        parallel_for(len / 4096 + 1, [&](size_t lo, size_t hi) {
+            const size_t a = std::min(len, lo * 4096), z = std::min(len, hi * 4096);
+            if (z > a) std::memcpy(dst + off + a, from + a, z - a);
+        });
+        return hipSuccess;
+    };
+    for (size_t k = 0; k < pieces; ++k) {
+        const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
+        if (hipError_t e = hipMemcpyAsync(ctx->bounce[k % 2].p, src + off, len, hipMemcpyDeviceToHost, cs);
+            e != hipSuccess)
+            return e;
+        if (hipError_t e = hipEventRecord(ctx->copy_ev[k % 2], cs); e != hipSuccess) return e;
+        if (k > 0)
+            if (hipError_t e = drain(k - 1); e != hipSuccess) return e;
+    }
+    return pieces ? drain(pieces - 1) : hipSuccess;
+}
+
 int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
                       const double field[6], const uint64_t* offsets, double* out, int out_on_device) {
     if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
@@ -1366,15 +1416,23 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
         });
         st = run_requests(ctx, b, field, reqs, reinterpret_cast<zc*>(out));
     } else {
-        // Stage groups of quartets through device memory, then copy back.
+        // Stage groups of quartets through two device buffers: a group is
+        // computed while a copier thread moves the previous one to the host.
         const size_t budget = ctx->staging_budget_bytes / sizeof(zc);
+        DevBuffer* sbuf[2] = {&ctx->staging, &ctx->staging2};
+        std::thread copier;
+        hipError_t copy_err = hipSuccess;
+        auto join_copier = [&] {
+            if (copier.joinable()) copier.join();
+            if (copy_err != hipSuccess && st == GIAOHIP_OK) st = hip_fail(ctx, copy_err, "copy to host");
+        };
         size_t i = 0;
-        std::vector<zc> host_tmp;
-        while (i < n && st == GIAOHIP_OK) {
+        for (int g = 0; i < n && st == GIAOHIP_OK; ++g) {
             size_t j = i;
             uint64_t tot = 0;
             while (j < n && (j == i || tot + sizes[j] <= budget)) tot += sizes[j++];
-            if (hipError_t e = ctx->staging.ensure(tot * sizeof(zc)); e != hipSuccess) {
+            DevBuffer& buf = *sbuf[g % 2];
+            if (hipError_t e = buf.ensure(tot * sizeof(zc)); e != hipSuccess) {
                 st = hip_fail(ctx, e, "hipMalloc(staging)");
                 break;
             }
@@ -1384,21 +1442,24 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
                 reqs.push_back(make_req(k, (long long)local));
                 local += sizes[k];
             }
-            st = run_requests(ctx, b, field, reqs, ctx->staging.as<zc>());
+            st = run_requests(ctx, b, field, reqs, buf.as<zc>());
+            join_copier();  // the previous group is on the host; its buffer is free again
             if (st) break;
             bool packed = true;
             for (size_t k = i + 1; k < j; ++k) packed = packed && offs[k] == offs[k - 1] + sizes[k - 1];
+            const char* src = static_cast<const char*>(buf.p);
             if (packed) {
-                if (hipError_t e = hipMemcpy(out + 2 * offs[i], ctx->staging.p, tot * sizeof(zc), hipMemcpyDeviceToHost);
-                    e != hipSuccess) {
-                    st = hip_fail(ctx, e, "hipMemcpy");
-                    break;
-                }
+                char* dst = reinterpret_cast<char*>(out + 2 * offs[i]);
+                const size_t bytes = tot * sizeof(zc);
+                copier = std::thread([&ctx, &copy_err, src, dst, bytes] {
+                    (void)hipSetDevice(ctx->device);
+                    copy_err = copy_to_host(ctx, src, dst, bytes);
+                });
             } else {
-                host_tmp.resize(tot);
-                if (hipError_t e = hipMemcpy(host_tmp.data(), ctx->staging.p, tot * sizeof(zc), hipMemcpyDeviceToHost);
+                std::vector<zc> host_tmp(tot);
+                if (hipError_t e = copy_to_host(ctx, src, reinterpret_cast<char*>(host_tmp.data()), tot * sizeof(zc));
                     e != hipSuccess) {
-                    st = hip_fail(ctx, e, "hipMemcpy");
+                    st = hip_fail(ctx, e, "copy to host");
                     break;
                 }
                 uint64_t lo = 0;
@@ -1409,6 +1470,7 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
             }
             i = j;
         }
+        join_copier();
     }
     hipEvent_t t1;
     (void)hipEventCreate(&t1);

@@ -70,7 +70,12 @@ enum {
        with general contractions, where the pivot may be taken from the
        latest candidate block instead of the global maximum. Read when a run
        starts. */
-    GIAOHIP_OPTION_CHOL_PIVOT_BATCH = 3
+    GIAOHIP_OPTION_CHOL_PIVOT_BATCH = 3,
+    /* Device staging of ERI outputs written to host memory, in MiB,
+       1..65536; default 1024. Outputs are computed in groups of about this
+       size; a group is copied to the host while the next one is computed,
+       so two such buffers are held. */
+    GIAOHIP_OPTION_STAGING_MIB = 4
 };
 
 typedef struct giaohip_ctx giaohip_ctx;

@@ -186,3 +186,50 @@ def test_schwarz_bounds_and_screening(gpu, port):
     got = db.eri_batch(kept[:40], f)
     want, _ = port.eri_batch(b, kept[:40], f, 4)
     assert np.max(np.abs(got - want)) <= ATOL
+
+
+def test_host_output_groups_and_scattered_offsets(gpu):
+    """Outputs written to host memory go through device staging groups that
+    are copied out while the next group is computed. With a 1 MiB staging
+    budget, 2000 (dd|dd) quartets (41 MB of output) take ~40 groups; the host
+    result must equal the device-resident one bitwise, also when the caller
+    asks for blocks at scattered offsets (giaohip_eri_batch's out_offsets)."""
+    import ctypes as C
+
+    batch = workloads.dddd_batch(2000, seed=7)
+    sh = batch.shells
+    coefs = _native.normalize_shells(sh.L, sh.nprim, sh.ncontr, sh.exps, sh.coefs)
+    db = _native.DeviceBasis(gpu, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
+    f = _native.field6(batch.B, batch.origin)
+    sizes = db.quartet_sizes(batch.quartets)
+    total = int(sizes.sum())
+    dev = gpu.alloc(total * 16)
+    try:
+        db.eri_batch(batch.quartets, f, device_out=dev)
+        want = gpu.to_host(dev, np.empty(total, dtype=np.complex128))
+    finally:
+        gpu.free(dev)
+    gpu.set_staging_mib(1)
+    try:
+        got = db.eri_batch(batch.quartets, f)
+        np.testing.assert_array_equal(got, want)
+        # every block 7 elements apart, in reverse order
+        q = np.ascontiguousarray(batch.quartets, dtype=np.uint32)
+        gap = 7
+        offs = np.zeros(len(q), dtype=np.uint64)
+        pos = 0
+        for i in reversed(range(len(q))):
+            offs[i] = pos
+            pos += int(sizes[i]) + gap
+        out = np.full(pos, np.nan + 0j, dtype=np.complex128)
+        gpu.check(gpu.lib.giaohip_eri_batch(
+            gpu.handle, db.handle, q.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(len(q)),
+            f.ctypes.data_as(C.POINTER(C.c_double)), offs.ctypes.data_as(C.POINTER(C.c_uint64)),
+            out.ctypes.data_as(C.c_void_p), 0))
+        starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        for i in range(len(q)):
+            o, n = int(offs[i]), int(sizes[i])
+            np.testing.assert_array_equal(out[o:o + n], want[starts[i]:starts[i] + n])
+        assert np.isnan(out[int(offs[0]) + int(sizes[0]):int(offs[0]) + int(sizes[0]) + gap].real).all()
+    finally:
+        gpu.set_staging_mib(1024)
