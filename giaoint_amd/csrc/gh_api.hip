@@ -1604,9 +1604,10 @@ int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* b, const double fi
     HIP_TRY(gh::launch_tensor_scatter(d_boff, d_q, (int)boff.size(), total, b->d_shells.as<DevShell>(),
                                       b->d_aooff.as<long long>(), nao, ctx->blocks.as<zc>(), d_tensor, s),
             "scatter launch");
-    if (!out_on_device)
-        HIP_TRY(hipMemcpyAsync(out, d_tensor, tensor_bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     HIP_TRY(hipStreamSynchronize(s), "tensor execution");
+    if (!out_on_device)
+        HIP_TRY(copy_to_host(ctx, reinterpret_cast<const char*>(d_tensor), reinterpret_cast<char*>(out), tensor_bytes),
+                "copy");
     return GIAOHIP_OK;
 }
 
@@ -2381,12 +2382,17 @@ int giaohip_cholesky_vectors(giaohip_ctx* ctx, const giaohip_cholesky* c, uint64
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (first + count > c->rank) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "Cholesky vector range is out of range");
     const long long n2 = c->n * c->n;
-    for (uint64_t k = 0; k < count; ++k) {
-        const uint64_t r = first + k;
-        const zc* row = c->chunks[r / c->rows_per_chunk] + (long long)(r % c->rows_per_chunk) * n2;
-        HIP_TRY(hipMemcpyAsync(out + 2 * k * n2, row, n2 * sizeof(zc), hipMemcpyDeviceToHost, ctx->stream), "copy");
-    }
     HIP_TRY(hipStreamSynchronize(ctx->stream), "copy");
+    // rows of one factor chunk are contiguous: one copy per run of rows
+    for (uint64_t k = 0; k < count;) {
+        const uint64_t r = first + k;
+        const uint64_t in_chunk = std::min<uint64_t>(count - k, c->rows_per_chunk - r % c->rows_per_chunk);
+        const zc* row = c->chunks[r / c->rows_per_chunk] + (long long)(r % c->rows_per_chunk) * n2;
+        HIP_TRY(copy_to_host(ctx, reinterpret_cast<const char*>(row), reinterpret_cast<char*>(out + 2 * k * n2),
+                             in_chunk * n2 * sizeof(zc)),
+                "copy");
+        k += in_chunk;
+    }
     return GIAOHIP_OK;
 }
 
