@@ -51,7 +51,12 @@ def parse():
     ap.add_argument("--chunk-mib", type=int, default=0, help="batched-path chunk size (default: library's)")
     ap.add_argument("--cholesky-rank", type=int, default=0,
                     help="Cholesky leg: stop after this many pivots (0: run to convergence, -1: skip the leg)")
-    ap.add_argument("--cholesky-block", type=int, default=0, help="candidates per pivot block (default: library's)")
+    ap.add_argument("--cholesky-block", type=int, default=0,
+                    help="candidates per pivot block (default: autotuned, else the library's)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch parameters instead of autotuning them")
+    ap.add_argument("--tune-cholesky-rank", type=int, default=1500,
+                    help="pivots of each Cholesky run the autotuner times")
     ap.add_argument("--high-l-gggg", type=int, default=50_000, help="(gg|gg) quartets of the high-l leg (0 skips)")
     ap.add_argument("--high-l-ffdd", type=int, default=200_000, help="(ff|dd) quartets of the high-l leg (0 skips)")
     ap.add_argument("--mixed", type=int, default=200_000,
@@ -101,11 +106,13 @@ def cpu_baseline(batch, B, origin, target_s: float):
     }
 
 
-def mixed_leg(n: int):
+def mixed_leg(n: int, tune: bool = True):
     """Mixed classes l <= 3 (weights (1 + sum l)^-2), contraction degree U{1..8}:
     the generator of the 2M-quartet workload at n quartets. One warmup call,
-    one timed call, inputs and output resident on the device."""
-    from giaoint_amd import _native, workloads
+    one timed call, inputs and output resident on the device. With `tune`,
+    the launch form of the generic kernel is autotuned on the same batch
+    first (per-bucket grids, per-bucket work queues, one persistent launch)."""
+    from giaoint_amd import _native, autotune, workloads
 
     batch = workloads.mixed_batch(n)
     sh = batch.shells
@@ -116,6 +123,14 @@ def mixed_leg(n: int):
     out = ctx.alloc(total * 16)
     f = _native.field6(batch.B, batch.origin)
     flops = workloads.batch_flops(sh, batch.quartets)
+    tuned = None
+    if tune:
+        ctx.free(out)  # the tuner holds its own output buffer
+        tuned = autotune.tune_eri(db, batch.quartets, f, knobs=("eri_queue",), reps=1,
+                                  workload=f"mixed l<=3 degree 1..8, {n} quartets")
+        log(f"mixed: autotuned {tuned.values} " +
+            " ".join(f"{t.value}={t.seconds:.3f}s" for t in tuned.trials))
+        out = ctx.alloc(total * 16)
     db.eri_batch(batch.quartets, f, device_out=out)
     ctx.synchronize()
     t = time.perf_counter()
@@ -125,7 +140,8 @@ def mixed_leg(n: int):
     st = ctx.stats()
     ctx.free(out)
     log(f"mixed: {n} quartets in {dt:.2f} s")
-    return {
+    extra = {"autotune": tuned.as_dict()} if tuned else {}
+    return {**extra,
         "workload": f"{n} quartets, classes l<=3 weighted (1+sum l)^-2, degree U{{1..8}}, seed 3, "
                     f"{st['primitive_quartets']} primitive quartets",
         "quartets_per_s": n / dt,
@@ -193,7 +209,7 @@ def projection_roofline(nbytes: float, flops: float, block_ms: float, unblocked_
             "frac": t_hbm / sec, **common}
 
 
-def cholesky_leg(rank_cap: int, block: int = 0):
+def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0):
     """Streamed pivoted Cholesky of the 30-atom 4s3p2d1f set (1050 AOs), tau = 1e-8.
 
     Runs the device driver in steps of 1000 pivots (a progress line after
@@ -208,6 +224,14 @@ def cholesky_leg(rank_cap: int, block: int = 0):
         ctx.set_cholesky_block(block)
     db = _native.DeviceBasis(ctx, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
     field = _native.field6(B, O)
+    tuned = None
+    if not block and tune_rank > 0:
+        from giaoint_amd import autotune
+
+        tuned = autotune.tune_cholesky(db, field, 1e-8, tune_rank,
+                                       workload=f"30-atom 4s3p2d1f Cholesky, first {tune_rank} pivots")
+        log(f"cholesky: autotuned {tuned.values} " +
+            " ".join(f"B={t.value}:{t.seconds:.2f}s" for t in tuned.trials))
     step = 1000
     t0 = time.perf_counter()
     first = step if rank_cap <= 0 else min(step, rank_cap)
@@ -231,6 +255,7 @@ def cholesky_leg(rank_cap: int, block: int = 0):
     proj_flops = 8.0 * n * n * blk["candidate_rows"]
     unblocked_bytes = 16.0 * n * n * m * (m - 1) / 2
     result = {
+        **({"autotune": tuned.as_dict()} if tuned else {}),
         "workload": "30 atoms, min spacing 1.8 bohr in a 14-bohr box (seed 5), uncontracted 4s3p2d1f "
                     f"(alpha_i = 0.05*3^i), {n} Cartesian AOs, B=(0.05,0,0.1) a.u., tau=1e-8, complex128 factors",
         "rank": m,
@@ -281,6 +306,16 @@ def main():
     def step():
         dbasis.eri_batch(batch.quartets, field, device_out=out)
         return ctx.stats()
+
+    tuned = None
+    if not args.no_autotune:
+        from giaoint_amd import autotune
+
+        ctx.free(out)  # the tuner holds its own output buffer
+        tuned = autotune.tune_eri(dbasis, batch.quartets, field, knobs=("fast_waves",),
+                                  workload=f"(dd|dd) K=3, {n} quartets")
+        log(f"autotuned {tuned.values}: " + " ".join(f"{t.value}={t.seconds:.3f}s" for t in tuned.trials))
+        out = ctx.alloc(total * 16)
 
     # plain FP64 kernel (generic, one launch per class) on the same batch, for comparison
     plain = None
@@ -380,6 +415,8 @@ def main():
         },
         "checksum": float(np.abs(probe).sum()),
     }
+    if tuned is not None:
+        result["autotune"] = tuned.as_dict()
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(batch, batch.B, batch.origin, args.cpu_seconds)
@@ -389,11 +426,12 @@ def main():
     ctx.free(out)
     del dbasis, ctx
     if args.mixed > 0:
-        result["mixed_class"] = mixed_leg(args.mixed)
+        result["mixed_class"] = mixed_leg(args.mixed, tune=not args.no_autotune)
     if args.high_l_gggg > 0 or args.high_l_ffdd > 0:
         result["high_l"] = high_l_leg(args.high_l_gggg, args.high_l_ffdd)
     if args.cholesky_rank >= 0:
-        result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block)
+        result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block,
+                                          0 if args.no_autotune else args.tune_cholesky_rank)
     print(json.dumps(result), flush=True)
 
 

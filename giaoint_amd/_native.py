@@ -215,6 +215,20 @@ class Context:
         """cholesky_eri's pivot_batch_size for later Cholesky runs (>= 1)."""
         self.check(self.lib.giaohip_set_option(self.handle, 3, int(value)))
 
+    QUEUE_FORMS = {"none": 0, "buckets": 1, "single": 2}
+
+    def set_eri_queue(self, form: str):
+        """Launch form of the generic transform kernel: 'none' (one workgroup
+        per item and LDS-size bucket), 'buckets' (a persistent device work
+        queue per bucket) or 'single' (one persistent launch). Results are
+        bitwise identical for every form."""
+        self.check(self.lib.giaohip_set_option(self.handle, 5, self.QUEUE_FORMS[form]))
+
+    def set_fast_waves(self, waves: int):
+        """Largest workgroup of the class-specialized transform in waves
+        (0 = library default, else 4..8); it sets the bra-pair tile per pass."""
+        self.check(self.lib.giaohip_set_option(self.handle, 6, int(waves)))
+
     def stats(self) -> dict:
         s = Stats()
         self.check(self.lib.giaohip_last_stats(self.handle, C.byref(s)))

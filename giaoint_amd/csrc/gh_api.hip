@@ -151,6 +151,9 @@ struct giaohip_ctx {
     int chol_block = 32;  // candidates per Cholesky pivot block
     int chol_pivot_batch = 32;  // cholesky_eri's pivot_batch_size
     size_t staging_budget_bytes = size_t(1) << 30;
+    int eri_queue = GIAOHIP_QUEUE_NONE;  // generic-kernel launch form (GIAOHIP_OPTION_ERI_QUEUE)
+    int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
+    DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
 };
 
 // A device-resident Cholesky: factors, residual and every buffer a pivot step
@@ -523,6 +526,27 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             launches.push_back({lds_max, all.size(), sel.size(), cost});
             for (const Planned* pl : sel) all.push_back(pl->w);
         }
+        if (ctx->eri_queue == GIAOHIP_QUEUE_SINGLE && launches.size() > 1) {
+            // one persistent launch over every item, longest first, with the
+            // largest LDS footprint of any of them
+            std::vector<const Planned*> sel;
+            sel.reserve(planned.size());
+            for (const Planned& pl : planned) sel.push_back(&pl);
+            std::stable_sort(sel.begin(), sel.end(), [](const Planned* x, const Planned* y) { return x->cost > y->cost; });
+            MixedLaunch one{0, 0, sel.size(), 0.0};
+            all.clear();
+            for (const Planned* pl : sel) {
+                one.lds = std::max(one.lds, pl->lds);
+                one.cost += pl->cost;
+                all.push_back(pl->w);
+            }
+            launches.assign(1, one);
+        }
+        int* d_queue = nullptr;
+        if (ctx->eri_queue != GIAOHIP_QUEUE_NONE) {
+            HIP_TRY(ctx->queue.ensure(2 * sizeof(int)), "hipMalloc");
+            d_queue = ctx->queue.as<int>();
+        }
         HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
         HIP_TRY(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
@@ -550,7 +574,8 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             HIP_TRY(gh::launch_transform_mixed(L.lds, ctx->items.as<WorkItem>() + L.first, (int)L.count,
                                                b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
                                                b->d_coefs.as<double>(), b->d_norms.as<double>(),
-                                               ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_out, ls),
+                                               ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_out,
+                                               d_queue ? d_queue + side : nullptr, ls),
                     "transform kernel launch");
             ctx->stats.launches += 1;
         }
@@ -738,7 +763,7 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
         for (Group& g : groups) g.idx.clear();
         for (size_t i = begin; i < end; ++i) {
             const FastReq& f = reqs[i];
-            const int waves = gh::fast_waves(f.info.nhab, (int)need[i].kab);
+            const int waves = gh::fast_waves(f.info.nhab, (int)need[i].kab, ctx->fast_waves_cap);
             const std::pair<gh::FastLaunch, gh::FastLaunch> fn{f.info.produce, f.info.transform};
             Group* g = nullptr;
             for (Group& x : groups)
@@ -1069,7 +1094,8 @@ int giaohip_destroy(giaohip_ctx* ctx) {
     for (auto& ev : ctx->copy_ev)
         if (ev) (void)hipEventDestroy(ev);
     for (DevBuffer* b : {&ctx->seeds, &ctx->squartets, &ctx->items, &ctx->kappa, &ctx->errrec, &ctx->staging,
-                         &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c, &ctx->fpq, &ctx->fitems})
+                         &ctx->blocks, &ctx->misc_a, &ctx->misc_b, &ctx->misc_c, &ctx->fpq, &ctx->fitems,
+                         &ctx->queue})
         b->release();
     for (auto& ev : ctx->chunk_ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1118,6 +1144,15 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_STAGING_MIB && value >= 1 && value <= 65536) {
         ctx->staging_budget_bytes = (size_t)value << 20;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_ERI_QUEUE && value >= GIAOHIP_QUEUE_NONE && value <= GIAOHIP_QUEUE_SINGLE) {
+        ctx->eri_queue = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_FAST_WAVES &&
+        (value == 0 || (value >= gh::kFastBraGroups && value <= gh::kFastMaxWaves))) {
+        ctx->fast_waves_cap = value;
         return GIAOHIP_OK;
     }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");

@@ -373,15 +373,13 @@ __device__ __host__ inline int lex_off(int D, int t, int u) {
     return nherm(D) - nherm(D - t) + u * (D - t + 1) - u * (u - 1) / 2;
 }
 
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void transform_kernel(ClassArgs ca, const WorkItem* __restrict__ items,
-                                                        const DevShell* __restrict__ shells,
-                                                        const double* __restrict__ exps,
-                                                        const double* __restrict__ coefs,
-                                                        const double* __restrict__ norms,
-                                                        const double* __restrict__ kappa,
-                                                        const zc* __restrict__ seeds, zc* __restrict__ out) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    const WorkItem it = items[blockIdx.x];
+// One work item of the generic transform. Every thread of the 256-thread
+// workgroup calls it with the same item; smem_raw is the dynamic LDS region.
+__device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkItem it,
+                                               const DevShell* __restrict__ shells, const double* __restrict__ exps,
+                                               const double* __restrict__ coefs, const double* __restrict__ norms,
+                                               const double* __restrict__ kappa, const zc* __restrict__ seeds,
+                                               zc* __restrict__ out, unsigned char* smem_raw) {
     // the item's own class (mixed-class launches) or the launch's
     const int cls = it.cls;
     const int la = cls ? (cls & 15) : ca.l[0], lb = cls ? ((cls >> 4) & 15) : ca.l[1];
@@ -694,6 +692,40 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void t
     }
 }
 
+// One workgroup per work item.
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void transform_kernel(
+    ClassArgs ca, const WorkItem* __restrict__ items, const DevShell* __restrict__ shells,
+    const double* __restrict__ exps, const double* __restrict__ coefs, const double* __restrict__ norms,
+    const double* __restrict__ kappa, const zc* __restrict__ seeds, zc* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    transform_item(ca, items[blockIdx.x], shells, exps, coefs, norms, kappa, seeds, out, smem_raw);
+}
+
+// Persistent form: the grid's workgroups take items from a device work queue
+// (`queue` counts the items handed out; it is zero at launch) in list order,
+// which the host sorts longest first, until the list is exhausted. Every
+// workgroup leaves the loop once the counter has passed nitems.
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void transform_queue_kernel(
+    ClassArgs ca, const WorkItem* __restrict__ items, int nitems, int* __restrict__ queue,
+    const DevShell* __restrict__ shells, const double* __restrict__ exps, const double* __restrict__ coefs,
+    const double* __restrict__ norms, const double* __restrict__ kappa, const zc* __restrict__ seeds,
+    zc* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+        __syncthreads();
+        const int i = next;
+        if (i >= nitems) break;
+        transform_item(ca, items[i], shells, exps, coefs, norms, kappa, seeds, out, smem_raw);
+        __syncthreads();  // LDS and `next` are free for the next item
+    }
+}
+
+__global__ void queue_reset_kernel(int* __restrict__ queue, int n) {
+    if ((int)threadIdx.x < n) queue[threadIdx.x] = 0;
+}
+
 // Four symmetry images of canonical blocks into an n^4 tensor (eri.cpp:777-805).
 __global__ void tensor_scatter_kernel(const long long* __restrict__ block_off, const int* __restrict__ block_q,
                                       int nblocks, long long total, const DevShell* __restrict__ shells,
@@ -913,7 +945,7 @@ hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nite
 
 hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
                                   const double* exps, const double* coefs, const double* norms, const double* kappa,
-                                  const zc* seeds, zc* out, hipStream_t s) {
+                                  const zc* seeds, zc* out, int* queue, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
     if (lds_bytes > 160 * 1024) return hipErrorInvalidValue;
     if (lds_bytes > 64 * 1024) {
@@ -926,6 +958,31 @@ hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nite
         }
     }
     const ClassArgs none{{0, 0, 0, 0}, 1};  // every item carries its class
+    // the queue form also holds one int of static LDS
+    if (queue && lds_bytes + 256 <= 160 * 1024) {
+        hipError_t e = hipSuccess;
+        if (lds_bytes > 64 * 1024) {
+            static bool raised_q = false;
+            if (!raised_q) {
+                e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_queue_kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+                if (e != hipSuccess) return e;
+                raised_q = true;
+            }
+        }
+        int per_cu = 0, dev = 0, cus = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&transform_queue_kernel),
+                                                         kNT, lds_bytes);
+        if (e != hipSuccess) return e;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        const long long resident = (long long)(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+        const unsigned grid = (unsigned)(resident < nitems ? resident : nitems);
+        queue_reset_kernel<<<dim3(1), dim3(64), 0, s>>>(queue, 1);
+        transform_queue_kernel<<<dim3(grid), dim3(kNT), lds_bytes, s>>>(none, items, nitems, queue, shells, exps,
+                                                                         coefs, norms, kappa, seeds, out);
+        return hipGetLastError();
+    }
     transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), lds_bytes, s>>>(none, items, shells, exps, coefs, norms,
                                                                           kappa, seeds, out);
     return hipGetLastError();

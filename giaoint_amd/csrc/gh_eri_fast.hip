@@ -10,8 +10,9 @@ FastClassInfo fast_info_1111();
 FastClassInfo fast_info_1010();
 FastClassInfo fast_info_0000();
 
-int fast_waves(int nhab, int kab) {
-    const int fit = 64 * kFastMaxWaves / nhab;
+int fast_waves(int nhab, int kab, int cap) {
+    const int maxw = cap >= kFastBraGroups && cap <= kFastMaxWaves ? cap : kFastMaxWaves;
+    const int fit = 64 * maxw / nhab;
     const int ppp = kab < fit ? kab : fit;
     const int w = (ppp * nhab + 63) / 64;
     return w < kFastBraGroups ? kFastBraGroups : w;

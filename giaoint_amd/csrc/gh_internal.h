@@ -115,7 +115,10 @@ struct FastClassInfo {
 FastClassInfo fast_class_info(int la, int lb, int lc, int ld);
 // Transform workgroup size (wavefronts) for a bra side of nhab Hermite rows
 // and kab primitive pairs; the kernel derives the same pass split from it.
-int fast_waves(int nhab, int kab);
+// Waves per workgroup of the class-specialized transform for an item with kab
+// bra primitive pairs: enough rows for every pair in one pass, at most `cap`
+// waves (kFastBraGroups..kFastMaxWaves; 0 means kFastMaxWaves).
+int fast_waves(int nhab, int kab, int cap = 0);
 
 hipError_t upload_constants();
 int transform_lds_bytes(const ClassArgs& ca);
@@ -133,10 +136,13 @@ hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nite
                             const double* exps, const double* coefs, const double* norms, const double* kappa,
                             const zc* seeds, zc* out, hipStream_t s);
 // Transform of items of several classes in one launch: every item carries
-// its class and tile in WorkItem::cls; lds_bytes covers the largest.
+// its class and tile in WorkItem::cls; lds_bytes covers the largest. With
+// `queue` (one device int, used only by launches on stream s) the launch is
+// persistent: as many workgroups as fit on the device take items from a
+// work queue in list order; without it, one workgroup per item.
 hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
                                   const double* exps, const double* coefs, const double* norms, const double* kappa,
-                                  const zc* seeds, zc* out, hipStream_t s);
+                                  const zc* seeds, zc* out, int* queue, hipStream_t s);
 // Cholesky column ERIs for one (la, lb) group of bra shell pairs and the
 // pivot's ket class (lc, ld) and component `comp`: one wavefront per pair.
 // Needs la + lb + lc + ld <= kColumnMaxL; `items` as for launch_transform

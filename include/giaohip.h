@@ -75,7 +75,21 @@ enum {
        1..65536; default 1024. Outputs are computed in groups of about this
        size; a group is copied to the host while the next one is computed,
        so two such buffers are held. */
-    GIAOHIP_OPTION_STAGING_MIB = 4
+    GIAOHIP_OPTION_STAGING_MIB = 4,
+    /* Launch form of the generic transform kernel (classes without a
+       specialized kernel, and every class under GIAOHIP_ERI_PLAIN), one of
+       GIAOHIP_QUEUE_*; default GIAOHIP_QUEUE_NONE. Results are bitwise
+       identical for every form. */
+    GIAOHIP_OPTION_ERI_QUEUE = 5,
+    /* Largest workgroup of the class-specialized transform, in waves:
+       0 (default: 8) or 4..8. A workgroup takes as many bra primitive pairs
+       per pass as its rows hold, so the value sets the tile of bra pairs. */
+    GIAOHIP_OPTION_FAST_WAVES = 6
+};
+enum {
+    GIAOHIP_QUEUE_NONE = 0,    /* one workgroup per item, one launch per LDS-size bucket */
+    GIAOHIP_QUEUE_BUCKETS = 1, /* persistent device work queue per LDS-size bucket */
+    GIAOHIP_QUEUE_SINGLE = 2   /* one persistent launch over every item (largest LDS of any) */
 };
 
 typedef struct giaohip_ctx giaohip_ctx;
