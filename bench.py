@@ -30,12 +30,16 @@ sys.path.insert(0, str(ROOT))
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector / matrix (dense)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# HBM bytes per (dd|dd) quartet of fast_transform_kernel<2,2,2,2>, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a copy-kernel
-# calibration (profiles/fast_transform_dddd_200k_pmc.md).
-TRANSFORM_HBM_BYTES_PER_QUARTET = 370_995
+# Per (dd|dd) K=3 quartet of fast_transform_kernel<2,2,2,2>, by workgroup
+# size (4 waves: 7 + 2 bra pairs in two passes; the default cap gives 5 waves
+# and one pass): HBM bytes from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# scaled by a copy-kernel calibration, and FP64 operations issued from the
+# SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64 counters (64 lanes, FMA = 2)
+# (profiles/fast_transform_kernel_dddd_200k_pmc.md, profiles/fp64_mfma_lds_counters.md).
+TRANSFORM_HBM_BYTES_PER_QUARTET = {4: 466_622, 5: 370_995}
+TRANSFORM_FP64_ISSUED_PER_QUARTET = {4: 1.4337e7}
 TRANSFORM_TRAFFIC_NOTE = ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), "
-                          "factors from a 256 MiB copy; 370,995 B per (dd|dd) quartet x quartets per launch")
+                          "factors from a 256 MiB copy; {b:,} B per (dd|dd) quartet at {w} waves x quartets per launch")
 METRIC = "shell quartets/s + FP64 GFLOP/s vs peak; Cholesky wall time, 1 MI355X"
 
 
@@ -53,6 +57,11 @@ def parse():
                     help="Cholesky leg: stop after this many pivots (0: run to convergence, -1: skip the leg)")
     ap.add_argument("--cholesky-block", type=int, default=0,
                     help="candidates per pivot block (default: autotuned, else the library's)")
+    ap.add_argument("--fast-waves", type=int, default=-1,
+                    help="workgroup waves of the class-specialized kernel for the headline batch "
+                         "(0 = library default, 4..8; default: autotuned)")
+    ap.add_argument("--no-host-output", action="store_true",
+                    help="skip the PCIe-inclusive run into host memory (profiles of the timed steps only)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch parameters instead of autotuning them")
     ap.add_argument("--tune-cholesky-rank", type=int, default=1500,
@@ -308,7 +317,9 @@ def main():
         return ctx.stats()
 
     tuned = None
-    if not args.no_autotune:
+    if args.fast_waves >= 0:
+        ctx.set_fast_waves(args.fast_waves)
+    elif not args.no_autotune:
         from giaoint_amd import autotune
 
         ctx.free(out)  # the tuner holds its own output buffer
@@ -346,21 +357,28 @@ def main():
     contract_launches = stats[-1]["contract_launches"]
     launches = stats[-1]["launches"]
     # the same batch delivered into host memory (PCIe-inclusive rate, not the headline value)
-    host_out = np.empty(total, dtype=np.complex128)
-    host_out.fill(0.0)  # fault the pages in before timing
-    t = time.perf_counter()
-    dbasis.eri_batch(batch.quartets, field, out=host_out)
-    host_rate = n / (time.perf_counter() - t)
-    del host_out
+    host_rate = None
+    if not args.no_host_output:
+        host_out = np.empty(total, dtype=np.complex128)
+        host_out.fill(0.0)  # fault the pages in before timing
+        t = time.perf_counter()
+        dbasis.eri_batch(batch.quartets, field, out=host_out)
+        host_rate = n / (time.perf_counter() - t)
+        del host_out
     # checksum of the resident output for run-to-run comparison
     probe = np.empty(min(total, 1 << 20), dtype=np.complex128)
     ctx.to_host(out, probe)
     # dominant kernel: the E.R.E contraction, timed with events on the stream it runs on
     avg_launch_ms = contract_ms / contract_launches
     achieved = contract_flops / contract_launches / (avg_launch_ms * 1e-3) / 1e12
+    fast_waves = args.fast_waves if args.fast_waves >= 0 else (tuned.values["fast_waves"] if tuned else 0)
+    waves_used = 4 if fast_waves == 4 else 5  # 9 bra pairs x 35 Hermite rows (workload dddd K=3)
     traffic = None
-    if TRANSFORM_HBM_BYTES_PER_QUARTET is not None:
-        traffic = TRANSFORM_HBM_BYTES_PER_QUARTET * n / contract_launches
+    if waves_used in TRANSFORM_HBM_BYTES_PER_QUARTET:
+        traffic = TRANSFORM_HBM_BYTES_PER_QUARTET[waves_used] * n / contract_launches
+    issued = None
+    if waves_used in TRANSFORM_FP64_ISSUED_PER_QUARTET:
+        issued = TRANSFORM_FP64_ISSUED_PER_QUARTET[waves_used] * n / contract_launches / (avg_launch_ms * 1e-3) / 1e12
     result = {
         "metric": METRIC,
         "value": n / (ms_step * 1e-3),
@@ -411,12 +429,20 @@ def main():
             "executed_tflops": (executed - (flops - contract_flops)) / (contract_ms * 1e-3) / 1e12,
             "executed_note": "complex MACs the contraction executes (structural zeros of the Hermite products "
                              "skipped), 8 FLOP each",
-            "traffic_note": None if traffic is None else TRANSFORM_TRAFFIC_NOTE,
+            "fp64_issued_tflops": issued,
+            "fp64_issued_frac": None if issued is None else issued / FP64_PEAK_TFLOPS,
+            "fp64_issued_note": "FP64 operations the kernel issues (rocprofv3 SQ_INSTS_VALU_*_F64 per quartet) per "
+                                "average launch: the FP64 pipe's utilisation. The convention counts the dense "
+                                "contraction, whose structural zeros the kernel skips, so `frac` can exceed 1",
+            "waves_per_workgroup": waves_used,
+            "traffic_note": None if traffic is None else TRANSFORM_TRAFFIC_NOTE.format(
+                b=TRANSFORM_HBM_BYTES_PER_QUARTET[waves_used], w=waves_used),
         },
         "checksum": float(np.abs(probe).sum()),
     }
     if tuned is not None:
         result["autotune"] = tuned.as_dict()
+    result["fast_waves"] = fast_waves
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(batch, batch.B, batch.origin, args.cpu_seconds)
