@@ -323,9 +323,10 @@ def main():
         from giaoint_amd import autotune
 
         ctx.free(out)  # the tuner holds its own output buffer
-        tuned = autotune.tune_eri(dbasis, batch.quartets, field, knobs=("fast_waves",),
+        knobs = ("fast_waves",) if args.chunk_mib else ("fast_waves", "chunk_mib")
+        tuned = autotune.tune_eri(dbasis, batch.quartets, field, knobs=knobs,
                                   workload=f"(dd|dd) K=3, {n} quartets")
-        log(f"autotuned {tuned.values}: " + " ".join(f"{t.value}={t.seconds:.3f}s" for t in tuned.trials))
+        log(f"autotuned {tuned.values}: " + " ".join(f"{t.knob}={t.value}:{t.seconds:.3f}s" for t in tuned.trials))
         out = ctx.alloc(total * 16)
 
     # plain FP64 kernel (generic, one launch per class) on the same batch, for comparison

@@ -1,7 +1,7 @@
 """Autotuner for the launch parameters of the ERI and Cholesky kernels.
 
 GIAOINT has no counterpart: its OpenMP path has no tile or launch parameters
-(`src/eri.cpp:730-756` schedules shell quartets `dynamic,1`). On the GPU three
+(`src/eri.cpp:730-756` schedules shell quartets `dynamic,1`). On the GPU four
 parameters are chosen at run time, by timing the caller's own workload on the
 device in use:
 
@@ -12,12 +12,15 @@ device in use:
 * ``eri_queue`` -- launch form of the generic transform: one workgroup per
   item and LDS-size bucket, a persistent device work queue per bucket, or one
   persistent launch over every class.
+* ``chunk_mib`` -- work memory of one chunk of the batched path (chunks
+  alternate between the producer and the contraction stream, so the size
+  trades pipeline fill and drain against launch count).
 * ``cholesky_block`` -- candidates per pivot block of the streamed Cholesky
   (projection tile width against passes over the factors).
 
 Every candidate is checked before it can win: ERI candidates must reproduce
 the default configuration's integrals on a sample of the workload (bitwise
-for the launch forms, which do not change any sum; within ``tolerance`` for
+for the launch forms and the chunk size, which do not change any sum; within ``tolerance`` for
 the workgroup size, which changes the order of the bra-pair sums), and
 Cholesky candidates must give the same pivots and factors bitwise (the block
 size is designed not to change them). Choices can be kept in a JSON cache
@@ -38,8 +41,10 @@ from . import _native
 
 FAST_WAVES = (0, 4, 5, 6, 7)
 ERI_QUEUE = ("none", "buckets", "single")
+CHUNK_MIB = (512, 1024, 2048, 4096)
 CHOLESKY_BLOCK = (8, 16, 24, 32)
-DEFAULTS = {"fast_waves": 0, "eri_queue": "none", "cholesky_block": 32}
+DEFAULTS = {"fast_waves": 0, "eri_queue": "none", "chunk_mib": 2048, "cholesky_block": 32}
+EXACT = ("eri_queue", "chunk_mib")  # knobs that do not change any sum: candidates must match bitwise
 
 
 @dataclass
@@ -67,6 +72,8 @@ def apply(ctx: "_native.Context", values: dict) -> None:
         ctx.set_fast_waves(int(values["fast_waves"]))
     if "eri_queue" in values:
         ctx.set_eri_queue(str(values["eri_queue"]))
+    if "chunk_mib" in values:
+        ctx.set_chunk_mib(int(values["chunk_mib"]))
     if "cholesky_block" in values:
         ctx.set_cholesky_block(int(values["cholesky_block"]))
 
@@ -99,8 +106,8 @@ def tune_eri(basis: "_native.DeviceBasis", quartets, field6, knobs=("fast_waves"
     out = ctx.alloc(max(total, 1) * 16)
     try:
         for knob in knobs:
-            space = {"fast_waves": FAST_WAVES, "eri_queue": ERI_QUEUE}[knob]
-            exact = knob == "eri_queue"
+            space = {"fast_waves": FAST_WAVES, "eri_queue": ERI_QUEUE, "chunk_mib": CHUNK_MIB}[knob]
+            exact = knob in EXACT
             # an exact knob is compared with the integrals of the choices made so far
             base = basis.eri_batch(sample, field6) if exact else reference
             best = None

@@ -6,7 +6,8 @@
   mixed-class workload (l <= 3, degree 1..8), which match the oracle on a
   sample at 1e-11.
 * Every workgroup size of the class-specialized (dd|dd) kernel (the bra-pair
-  tile) matches the oracle at 1e-11.
+  tile) matches the oracle at 1e-11; chunk sizes of the batched path give
+  bitwise identical integrals.
 * The autotuner, run on a small ERI batch and on a short Cholesky run,
   returns only accepted values; Cholesky pivots are bitwise equal across
   the block sizes it tries.
@@ -81,6 +82,21 @@ def test_fast_waves_tiles(port):
         err = np.abs(_rows_of(db, batch, db.eri_batch(batch.quartets, f), rows) - want)
         print(f"  max |err| {err.max():.2e}")
         assert err.max() <= ATOL
+
+
+def test_chunk_sizes_bitwise():
+    """The batched path's chunk size only regroups independent quartets."""
+    ctx = _native.Context(0)
+    batch = workloads.dddd_batch(500)
+    db = _device(ctx, batch)
+    f = _native.field6(batch.B, batch.origin)
+    got = {}
+    for mib in (16, 64, 2048):
+        print(f"(dd|dd) 500 quartets, chunk {mib} MiB", flush=True)
+        ctx.set_chunk_mib(mib)
+        got[mib] = db.eri_batch(batch.quartets, f)
+        print(f"  launches {ctx.stats()['contract_launches']}")
+    assert np.array_equal(got[16], got[2048]) and np.array_equal(got[64], got[2048])
 
 
 def test_fast_waves_rejects_out_of_range():
