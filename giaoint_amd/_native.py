@@ -88,6 +88,7 @@ _SIGNATURES = {
     "giaohip_eri_batch_size": ([_vp, _u32p, C.c_size_t, _u64p], C.c_int),
     "giaohip_eri_tensor": ([_vp, _vp, _dp, _vp, C.c_int], C.c_int),
     "giaohip_primitive_eri": ([_vp, _dp, _ip, _dp, _dp], C.c_int),
+    "giaohip_nuclear_attraction": ([_vp, _vp, C.c_int, _dp, _dp, _dp, _vp], C.c_int),
     "giaohip_cholesky_diagonal": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp], C.c_int),
     "giaohip_cholesky_columns": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp, C.c_int], C.c_int),
     "giaohip_cholesky_run": ([_vp, _vp, _dp, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(_vp)], C.c_int),
@@ -366,6 +367,19 @@ class DeviceBasis:
         out = np.empty((n, n, n, n), dtype=np.complex128)
         f = np.ascontiguousarray(field, dtype=np.float64)
         self.ctx.check(self.ctx.lib.giaohip_eri_tensor(self.ctx.handle, self.handle, _p(f), out.ctypes.data_as(_vp), 0))
+        return out
+
+    def nuclear_attraction(self, charges, centers, field):
+        """(nao, nao) complex128 nuclear-attraction matrix (compute_nuclear_attraction_matrix)."""
+        n = self.nao
+        out = np.empty((n, n), dtype=np.complex128)
+        z = np.ascontiguousarray(charges, dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        if len(c) != len(z):
+            raise ValueError("one center per nuclear charge")
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.giaohip_nuclear_attraction(self.ctx.handle, self.handle, len(z), _p(z), _p(c),
+                                                               _p(f), out.ctypes.data_as(_vp)))
         return out
 
     def cholesky_diagonal(self, pairs, field):

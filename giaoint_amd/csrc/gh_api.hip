@@ -1646,6 +1646,61 @@ int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* b, const double fi
     return GIAOHIP_OK;
 }
 
+int giaohip_nuclear_attraction(giaohip_ctx* ctx, const giaohip_basis* b, int nnuc, const double* charges,
+                               const double* centers, const double field[6], double* out) {
+    if (!ctx || !b || nnuc < 0 || (nnuc > 0 && (!charges || !centers)) || !field || !out)
+        return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    for (int c = 0; c < nnuc; ++c) {
+        bool ok = std::isfinite(charges[c]);
+        for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(centers[3 * c + k]);
+        if (!ok) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "nucleus charge and center must be finite");
+    }
+    int lmax = 0;
+    for (const DevShell& sh : b->shells) lmax = std::max(lmax, sh.l);
+    if (2 * lmax > 32)
+        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT,
+                         "nuclear attraction supports combined angular order through 32");
+    if (2 * lmax > gh::kNucMaxL)
+        return set_error(ctx, GIAOHIP_UNSUPPORTED, "nuclear attraction on the GPU supports shells through l = 4");
+    const long long nao = b->nao;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const std::vector<double> kap = wave_vectors(b, field);
+    std::vector<double> nuc(4 * (size_t)std::max(nnuc, 1), 0.0);
+    for (int c = 0; c < nnuc; ++c) {
+        nuc[4 * c] = charges[c];
+        for (int k = 0; k < 3; ++k) nuc[4 * c + 1 + k] = centers[3 * c + k];
+    }
+    const size_t bytes = (size_t)nao * (size_t)nao * sizeof(zc);
+    HIP_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
+    HIP_TRY(ctx->misc_a.ensure(nuc.size() * sizeof(double)), "hipMalloc");
+    HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
+    HIP_TRY(ctx->staging.ensure(std::max<size_t>(bytes, 16)), "hipMalloc");
+    HIP_TRY(hipMemcpyAsync(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(ctx->misc_a.p, nuc.data(), nuc.size() * sizeof(double), hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+    HIP_TRY(hipMemsetAsync(ctx->errrec.p, 0, sizeof(DevError), s), "hipMemsetAsync");
+    HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
+    HIP_TRY(gh::launch_nuclear(b->ns, b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                               b->d_norms.as<double>(), ctx->kappa.as<double>(), ctx->misc_a.as<double>(), nnuc,
+                               b->d_aooff.as<long long>(), nao, ctx->staging.as<zc>(), ctx->errrec.as<DevError>(), s),
+            "nuclear attraction launch");
+    HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
+    HIP_TRY(hipStreamSynchronize(s), "nuclear attraction execution");
+    ctx->stats.transform_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.launches = 1;
+    DevError de{};
+    HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (de.code != 0) return boys_error(ctx, de);
+    if (bytes)
+        HIP_TRY(copy_to_host(ctx, reinterpret_cast<const char*>(ctx->staging.p), reinterpret_cast<char*>(out), bytes),
+                "copy");
+    return GIAOHIP_OK;
+}
+
 int giaohip_primitive_eri(giaohip_ctx* ctx, const double prim[4][6], const int ang[4][3], const double field[6],
                           double out[2]) {
     if (!ctx) return GIAOHIP_INVALID_ARGUMENT;

@@ -1248,4 +1248,180 @@ hipError_t launch_real_columns(const zc* cols, const int* owner, long long nao, 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Nuclear attraction (nuclear.cpp:24-207, compute_nuclear_attraction_matrix
+// nuclear.cpp:282-312): V[mu,nu] = sum_C -Z_C <mu| 1/|r - C| |nu> with London
+// orbitals. The reference builds a four-index auxiliary R^n_{tuv} with the
+// pair's wave term -i q; as for the ERI bra pair, the phase is folded into
+// the Hermite coefficients (E~, built by build_phase_hermite), leaving the
+// ordinary three-index R0 at the complex displacement X = P' - C. One
+// wavefront per ordered shell pair; lanes take component pairs for the
+// contraction and (t,u,v) cells for the R0 levels.
+constexpr int kNucSide = kNucMaxL + 1;
+constexpr int kNucCube = kNucSide * kNucSide * kNucSide;
+
+__global__ __launch_bounds__(64) void nuclear_kernel(int nshell, const DevShell* __restrict__ shells,
+                                                     const double* __restrict__ exps,
+                                                     const double* __restrict__ coefs,
+                                                     const double* __restrict__ norms,
+                                                     const double* __restrict__ kappa,
+                                                     const double* __restrict__ nuclei, int nnuc,
+                                                     const long long* __restrict__ ao_off, long long nao,
+                                                     zc* __restrict__ out, DevError* err) {
+    __shared__ zc R[2][kNucCube];
+    __shared__ zc E[3][kNucSide * kNucSide * kNucSide];  // [axis][i][j][k], k <= i + j
+    __shared__ zc seed[kNucSide];
+    const int lane = threadIdx.x;
+    const int sa = blockIdx.x / nshell, sb = blockIdx.x % nshell;
+    const DevShell& A = shells[sa];
+    const DevShell& B = shells[sb];
+    const int la = A.l, lb = B.l, L = la + lb;
+    const int na = A.ncart, nb = B.ncart, npair = na * nb;
+    const int side = L + 1;
+    for (int xa = 0; xa < A.ncontr; ++xa) {
+        for (int xb = 0; xb < B.ncontr; ++xb) {
+            zc acc[kNucMaxComp];
+#pragma unroll
+            for (int r = 0; r < kNucMaxComp; ++r) acc[r] = zc{0.0, 0.0};
+            for (int pa = 0; pa < A.nprim; ++pa) {
+                for (int pb = 0; pb < B.nprim; ++pb) {
+                    const double ea = exps[A.prim_off + pa], eb = exps[B.prim_off + pb];
+                    PairD P;
+                    make_pair(ea, A.center, kappa + 3 * sa, eb, B.center, kappa + 3 * sb, P);
+                    for (int e = lane; e < 3 * (la + 1) * (lb + 1); e += 64) {
+                        const int axis = e / ((la + 1) * (lb + 1));
+                        const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
+                        build_phase_hermite(i, j, P.P[axis] - A.center[axis], P.P[axis] - B.center[axis], P.p,
+                                            P.q[axis], &E[axis][(i * kNucSide + j) * kNucSide]);
+                    }
+                    zc vsum[kNucMaxComp];
+#pragma unroll
+                    for (int r = 0; r < kNucMaxComp; ++r) vsum[r] = zc{0.0, 0.0};
+                    for (int c = 0; c < nnuc; ++c) {
+                        const double Z = nuclei[4 * c];
+                        const double C[3] = {nuclei[4 * c + 1], nuclei[4 * c + 2], nuclei[4 * c + 3]};
+                        const zc X[3] = {zsub(P.Pc[0], zc{C[0], 0.0}), zsub(P.Pc[1], zc{C[1], 0.0}),
+                                         zsub(P.Pc[2], zc{C[2], 0.0})};
+                        if (lane == 0) {
+                            const zc T = zscale(P.p, zadd(zadd(zmul(X[0], X[0]), zmul(X[1], X[1])), zmul(X[2], X[2])));
+                            const bool scaled = T.x < 0.0;
+                            zc F[kNucMaxL + 1];
+                            int region, segs;
+                            double berr;
+                            const int st = boys_eval<kNucMaxL>(T, L, scaled, F, region, berr, segs);
+                            if (st != kErrNone) {
+                                record_error(err, st, blockIdx.x, T, L);
+#pragma unroll
+                                for (int n = 0; n <= kNucMaxL; ++n) F[n] = zc{0.0, 0.0};
+                            }
+                            zc common = P.london;
+                            if (scaled) {
+                                double d2 = 0.0, qc = 0.0;
+#pragma unroll
+                                for (int i = 0; i < 3; ++i) {
+                                    const double d = P.P[i] - C[i];
+                                    d2 += d * d;
+                                    qc += P.q[i] * C[i];
+                                }
+                                common = zexp(zc{-P.p * d2, -qc});
+                            }
+                            double rf = 1.0;
+#pragma unroll
+                            for (int n = 0; n <= kNucMaxL; ++n) {
+                                if (n <= L) seed[n] = zscale(rf, zmul(common, F[n]));
+                                rf *= -2.0 * P.p;
+                            }
+                        }
+                        __syncthreads();
+                        // R0 levels: level n from level n + 1, cells (t,u,v) with t+u+v <= L-n
+                        int cur = 0;
+                        if (lane == 0) R[0][0] = seed[L];
+                        for (int n = L - 1; n >= 0; --n) {
+                            __syncthreads();
+                            const zc* rc = R[cur];
+                            zc* rn = R[cur ^ 1];
+                            const int deg = L - n;
+                            for (int e = lane; e < side * side * side; e += 64) {
+                                const int t = e / (side * side), u = (e / side) % side, v = e % side;
+                                if (t + u + v > deg) continue;
+                                zc val;
+                                if (t > 0) {
+                                    val = zmul(X[0], rc[((t - 1) * side + u) * side + v]);
+                                    if (t > 1) val = zadd(val, zscale((double)(t - 1), rc[((t - 2) * side + u) * side + v]));
+                                } else if (u > 0) {
+                                    val = zmul(X[1], rc[(u - 1) * side + v]);
+                                    if (u > 1) val = zadd(val, zscale((double)(u - 1), rc[(u - 2) * side + v]));
+                                } else if (v > 0) {
+                                    val = zmul(X[2], rc[v - 1]);
+                                    if (v > 1) val = zadd(val, zscale((double)(v - 1), rc[v - 2]));
+                                } else {
+                                    val = seed[n];
+                                }
+                                rn[(t * side + u) * side + v] = val;
+                            }
+                            cur ^= 1;
+                        }
+                        __syncthreads();
+                        const zc* R0 = R[cur];
+#pragma unroll
+                        for (int r = 0; r < kNucMaxComp; ++r) {
+                            const int e = lane + 64 * r;
+                            if (e < npair) {
+                                const unsigned ca = cart_of(la, e / nb), cb = cart_of(lb, e % nb);
+                                const int ax = ca & 255, ay = (ca >> 8) & 255, az = ca >> 16;
+                                const int bx = cb & 255, by = (cb >> 8) & 255, bz = cb >> 16;
+                                const zc* Ex = &E[0][(ax * kNucSide + bx) * kNucSide];
+                                const zc* Ey = &E[1][(ay * kNucSide + by) * kNucSide];
+                                const zc* Ez = &E[2][(az * kNucSide + bz) * kNucSide];
+                                zc sx{0.0, 0.0};
+                                for (int t = 0; t <= ax + bx; ++t) {
+                                    zc sy{0.0, 0.0};
+                                    for (int u = 0; u <= ay + by; ++u) {
+                                        zc sz{0.0, 0.0};
+                                        for (int v = 0; v <= az + bz; ++v)
+                                            sz = zadd(sz, zmul(Ez[v], R0[(t * side + u) * side + v]));
+                                        sy = zadd(sy, zmul(Ey[u], sz));
+                                    }
+                                    sx = zadd(sx, zmul(Ex[t], sy));
+                                }
+                                vsum[r] = zadd(vsum[r], zscale(-Z, sx));
+                            }
+                        }
+                        __syncthreads();  // R, seed and E reused by the next nucleus / primitive pair
+                    }
+                    const double pref = 2.0 * kPi / P.p * P.K * coefs[A.coef_off + xa * A.nprim + pa] *
+                                        coefs[B.coef_off + xb * B.nprim + pb];
+#pragma unroll
+                    for (int r = 0; r < kNucMaxComp; ++r) {
+                        const int e = lane + 64 * r;
+                        if (e < npair) {
+                            const double nrm = norms[A.norm_off + (e / nb) * A.nprim + pa] *
+                                               norms[B.norm_off + (e % nb) * B.nprim + pb];
+                            acc[r] = zadd(acc[r], zscale(pref * nrm, vsum[r]));
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kNucMaxComp; ++r) {
+                const int e = lane + 64 * r;
+                if (e < npair) {
+                    const long long mu = ao_off[sa] + (long long)xa * na + e / nb;
+                    const long long nu = ao_off[sb] + (long long)xb * nb + e % nb;
+                    out[mu * nao + nu] = acc[r];
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_nuclear(int nshell, const DevShell* shells, const double* exps, const double* coefs,
+                          const double* norms, const double* kappa, const double* nuclei, int nnuc,
+                          const long long* ao_off, long long nao, zc* out, DevError* err, hipStream_t s) {
+    if (nshell == 0) return hipSuccess;
+    nuclear_kernel<<<dim3((unsigned)nshell * (unsigned)nshell), dim3(64), 0, s>>>(
+        nshell, shells, exps, coefs, norms, kappa, nuclei, nnuc, ao_off, nao, out, err);
+    return hipGetLastError();
+}
+
 }  // namespace gh

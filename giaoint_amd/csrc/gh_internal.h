@@ -66,6 +66,15 @@ hipError_t launch_chol_accept(const zc* column, const zc* pproj, int real, long 
 hipError_t launch_chol_argmax(const double* res, long long n2, double* pmax, long long* pidx, CholState* st,
                               bool partial_first, hipStream_t s);
 
+// Nuclear-attraction matrix (nao x nao complex, every element written): one
+// wavefront per ordered shell pair; nuclei as (Z, x, y, z) rows. Needs
+// la + lb <= kNucMaxL for every pair.
+constexpr int kNucMaxL = 8;
+constexpr int kNucMaxComp = 4;  // component pairs per lane: (g, g) has 225
+hipError_t launch_nuclear(int nshell, const DevShell* shells, const double* exps, const double* coefs,
+                          const double* norms, const double* kappa, const double* nuclei, int nnuc,
+                          const long long* ao_off, long long nao, zc* out, DevError* err, hipStream_t s);
+
 // Class-specialized batched transform (gh_fast.h, gh_fast_*.hip, dispatch in gh_eri_fast.hip).
 constexpr int kFastBraGroups = 4;  // waves running the bra step, each with a quarter of the bra components
 constexpr int kFastMaxWaves = 8;   // largest transform workgroup

@@ -197,6 +197,16 @@ int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* basis, const doubl
 int giaohip_primitive_eri(giaohip_ctx* ctx, const double prim[4][6], const int ang[4][3], const double field[6],
                           double out[2]);
 
+/* Replaces compute_nuclear_attraction_matrix (nuclear.cpp:282-312, decl.
+ * nuclear.hpp:59-61; pybind11 `nuclear_attraction`, bindings.cpp:577-596):
+ * V[mu,nu] = sum_C -Z_C <mu| 1/|r - C| |nu> with London orbitals, an
+ * (nao, nao) complex128 matrix in host memory (row-major, every element
+ * written). nuclei: charges[nnuc], centers[nnuc][3] in bohr. Shells through
+ * l = 4 (GIAOHIP_UNSUPPORTED above; combined order above 32 is the
+ * reference's invalid_argument). */
+int giaohip_nuclear_attraction(giaohip_ctx* ctx, const giaohip_basis* basis, int nnuc, const double* charges,
+                               const double* centers, const double field[6], double* out);
+
 /* --- streamed Cholesky -------------------------------------------------
  * Replaces detail::cholesky_diagonal (cholesky.cpp:285-314, decl.
  * src/cholesky.hpp:9-11): pairs = npairs*(a,b) shell pairs with a <= b;

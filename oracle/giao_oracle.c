@@ -1158,3 +1158,149 @@ int or_cholesky_columns(const or_basis* b, const uint64_t* pivots, size_t npiv, 
     }
     return OR_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* nuclear.cpp: nuclear attraction with London orbitals                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int side;
+    double wave[3];
+    cplx disp[3];
+    cplx* table;          /* [n][t][u][v] */
+    unsigned char* ready;
+} nuc_aux;
+
+/* nuclear.cpp:106-141: the four-index auxiliary with the pair's wave term,
+ * memoized, in the reference's recursion order. */
+static cplx nuc_auxiliary(nuc_aux* a, int t, int u, int v, int n) {
+    const size_t side = (size_t)a->side;
+    const size_t idx = (((size_t)n * side + t) * side + u) * side + v;
+    if (a->ready[idx]) return a->table[idx];
+    cplx value;
+    if (t > 0) {
+        value = CMPLX(0.0, -a->wave[0]) * nuc_auxiliary(a, t - 1, u, v, n) +
+                a->disp[0] * nuc_auxiliary(a, t - 1, u, v, n + 1);
+        if (t > 1) value += (double)(t - 1) * nuc_auxiliary(a, t - 2, u, v, n + 1);
+    } else if (u > 0) {
+        value = CMPLX(0.0, -a->wave[1]) * nuc_auxiliary(a, t, u - 1, v, n) +
+                a->disp[1] * nuc_auxiliary(a, t, u - 1, v, n + 1);
+        if (u > 1) value += (double)(u - 1) * nuc_auxiliary(a, t, u - 2, v, n + 1);
+    } else {
+        value = CMPLX(0.0, -a->wave[2]) * nuc_auxiliary(a, t, u, v - 1, n) +
+                a->disp[2] * nuc_auxiliary(a, t, u, v - 1, n + 1);
+        if (v > 1) value += (double)(v - 1) * nuc_auxiliary(a, t, u, v - 2, n + 1);
+    }
+    a->table[idx] = value;
+    a->ready[idx] = 1;
+    return value;
+}
+
+/* nuclear.cpp:24-153 NuclearKernel::compute for one nucleus, unnormalized
+ * primitives with unit coefficients (as compute_nuclear_attraction calls it). */
+static int nuclear_kernel(const prim_t* bra, const prim_t* ket, double charge, const double C[3],
+                          const double field[6], cplx* value) {
+    pair_data pair;
+    gaussian_product(bra->exponent, bra->center, ket->exponent, ket->center, field, &pair);
+    const int L = bra->ang[0] + bra->ang[1] + bra->ang[2] + ket->ang[0] + ket->ang[1] + ket->ang[2];
+    if (L > 32) return fail(OR_INVALID_ARGUMENT, "nuclear attraction supports combined angular order through 32");
+    double E[3][2 * BOYS_MAX_ORDER + 4];
+    for (int axis = 0; axis < 3; ++axis)
+        hermite_coefficients(bra->ang[axis], ket->ang[axis], bra->center[axis], ket->center[axis], pair.P[axis],
+                             pair.exponent, E[axis]);
+    nuc_aux a;
+    a.side = L + 1;
+    for (int i = 0; i < 3; ++i) {
+        a.wave[i] = pair.q[i];
+        a.disp[i] = pair.Pc[i] - C[i];
+    }
+    const cplx arg = pair.exponent * (a.disp[0] * a.disp[0] + a.disp[1] * a.disp[1] + a.disp[2] * a.disp[2]);
+    const int scaled = creal(arg) < 0.0;
+    cplx F[BOYS_MAX_ORDER + 1];
+    int region;
+    double err;
+    size_t segs;
+    int st = compute_boys(arg, L, scaled, F, &region, &err, &segs);
+    if (st) return st;
+    cplx common = pair.london;
+    if (scaled) {
+        const double d[3] = {pair.P[0] - C[0], pair.P[1] - C[1], pair.P[2] - C[2]};
+        common = cexp(CMPLX(-pair.exponent * dot3(d, d), -dot3(pair.q, C)));
+    }
+    const size_t side = (size_t)a.side, cells = side * side * side * side;
+    a.table = malloc(sizeof(cplx) * cells);
+    a.ready = calloc(cells, 1);
+    if (!a.table || !a.ready) {
+        free(a.table);
+        free(a.ready);
+        return fail(OR_RUNTIME_ERROR, "out of memory");
+    }
+    double radial = 1.0;
+    for (int n = 0; n <= L; ++n) {
+        const size_t idx = (size_t)n * side * side * side;
+        a.table[idx] = common * radial * F[n];
+        a.ready[idx] = 1;
+        radial *= -2.0 * pair.exponent;
+    }
+    cplx contraction = 0.0;
+    const int mx = bra->ang[0] + ket->ang[0], my = bra->ang[1] + ket->ang[1], mz = bra->ang[2] + ket->ang[2];
+    for (int t = 0; t <= mx; ++t)
+        for (int u = 0; u <= my; ++u)
+            for (int v = 0; v <= mz; ++v)
+                contraction += E[0][t] * E[1][u] * E[2][v] * nuc_auxiliary(&a, t, u, v, 0);
+    free(a.table);
+    free(a.ready);
+    *value = -charge * (2.0 * PI / pair.exponent) * pair.ordinary * contraction;
+    return OR_OK;
+}
+
+/* nuclear.cpp:209-280 compute_nuclear_attraction and 282-312
+ * compute_nuclear_attraction_matrix: every ordered shell pair's block. */
+int or_nuclear_matrix(const or_basis* b, int nnuc, const double* charges, const double* centers,
+                      const double field[6], double* out) {
+    shell_view* sh = NULL;
+    int st = make_all_shells(b, &sh);
+    if (st) return st;
+    size_t nao = 0;
+    size_t* off = malloc(sizeof(size_t) * (size_t)(b->nshell + 1));
+    if (!off) {
+        free_all_shells(b, sh);
+        return fail(OR_RUNTIME_ERROR, "out of memory");
+    }
+    for (int s = 0; s < b->nshell; ++s) {
+        off[s] = nao;
+        nao += shell_ao_count(b, s);
+    }
+    cplx* V = (cplx*)out;
+    for (size_t i = 0; i < nao * nao; ++i) V[i] = 0.0;
+    for (int sa = 0; sa < b->nshell && !st; ++sa) {
+        for (int sb = 0; sb < b->nshell && !st; ++sb) {
+            const shell_view *A = &sh[sa], *B = &sh[sb];
+            for (int pa = 0; pa < A->nprim && !st; ++pa)
+                for (int pb = 0; pb < B->nprim && !st; ++pb)
+                    for (int ca = 0; ca < A->ncart && !st; ++ca)
+                        for (int cb = 0; cb < B->ncart && !st; ++cb) {
+                            prim_t bra, ket;
+                            make_prim(A, pa, ca, &bra);
+                            make_prim(B, pb, cb, &ket);
+                            cplx value = 0.0;
+                            for (int c = 0; c < nnuc && !st; ++c) {
+                                cplx v;
+                                st = nuclear_kernel(&bra, &ket, charges[c], centers + 3 * c, field, &v);
+                                if (!st) value += v;
+                            }
+                            value *= A->norms[(size_t)ca * A->nprim + pa] * B->norms[(size_t)cb * B->nprim + pb];
+                            for (int xa = 0; xa < A->ncontr; ++xa)
+                                for (int xb = 0; xb < B->ncontr; ++xb) {
+                                    const size_t mu = off[sa] + (size_t)xa * A->ncart + ca;
+                                    const size_t nu = off[sb] + (size_t)xb * B->ncart + cb;
+                                    V[mu * nao + nu] += A->coefs[(size_t)xa * A->nprim + pa] *
+                                                        B->coefs[(size_t)xb * B->nprim + pb] * value;
+                                }
+                        }
+        }
+    }
+    free(off);
+    free_all_shells(b, sh);
+    return st;
+}

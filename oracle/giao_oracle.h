@@ -101,6 +101,11 @@ int or_cholesky_diagonal(const or_basis* basis, const uint64_t* pairs,
                          size_t npairs, const double field[6], double* out,
                          int threads);
 
+/* nuclear.cpp:282-312 compute_nuclear_attraction_matrix: (nao, nao)
+ * complex128, nuclei as charges[nnuc] and centers[nnuc][3]. */
+int or_nuclear_matrix(const or_basis* basis, int nnuc, const double* charges, const double* centers,
+                      const double field[6], double* out);
+
 /* cholesky.cpp:203-282. pivots: npiv*(mu,nu). out: npiv*n*n values, complex
  * interleaved unless real_output (which requires exact zero field). */
 int or_cholesky_columns(const or_basis* basis, const uint64_t* pivots,

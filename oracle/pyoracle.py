@@ -187,6 +187,16 @@ class Checker:
         self._check(self._fn("eri_tensor")(C.byref(basis.struct), _ptr(f), _ptr(out), C.c_int(threads)))
         return out.view(np.complex128).reshape(n, n, n, n)
 
+    def nuclear_matrix(self, basis: BasisArrays, charges, centers, field):
+        n = basis.nao
+        z = np.ascontiguousarray(charges, dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(2 * n * n)
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self._check(self._fn("nuclear_matrix")(C.byref(basis.struct), C.c_int(len(z)), _ptr(z), _ptr(c), _ptr(f),
+                                               _ptr(out)))
+        return out.view(np.complex128).reshape(n, n)
+
     def cholesky_diagonal(self, basis: BasisArrays, pairs, field, threads=1):
         n = basis.nao
         p = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint64).reshape(-1, 2))

@@ -4,6 +4,7 @@
 // signatures equal those of giao_oracle.h with the prefix ref_ instead of or_.
 #include "giao_integrals/boys.hpp"
 #include "giao_integrals/eri.hpp"
+#include "giao_integrals/nuclear.hpp"
 #include "cholesky.hpp"
 
 #include <array>
@@ -192,6 +193,19 @@ int ref_cholesky_columns(const RefBasis* b, const std::uint64_t* pivots, std::si
                 std::span<giao::Complex>(reinterpret_cast<giao::Complex*>(out), npiv * n * n),
                 static_cast<std::size_t>(threads));
         }
+    });
+}
+
+int ref_nuclear_matrix(const RefBasis* b, int nnuc, const double* charges, const double* centers,
+                       const double field[6], double* out) {
+    return guarded([&] {
+        const auto basis = make_basis(b);
+        const auto n = basis.ao_count();
+        std::vector<giao::Nucleus> nuclei;
+        for (int c = 0; c < nnuc; ++c)
+            nuclei.emplace_back(charges[c], giao::Vec3{centers[3 * c], centers[3 * c + 1], centers[3 * c + 2]});
+        giao::compute_nuclear_attraction_matrix(basis, nuclei, make_field(field),
+                                                std::span<giao::Complex>(reinterpret_cast<giao::Complex*>(out), n * n));
     });
 }
 
