@@ -405,6 +405,33 @@ def _eri_full(basis, *, field=None, screening_threshold=0.0, threads=None):
     return out
 
 
+class Nucleus:
+    """Point charge (types.hpp:134-139): charge and centre in bohr."""
+
+    def __init__(self, charge, center):
+        charge = float(charge)
+        if not np.isfinite(charge):
+            raise ValueError("nucleus charge must be finite")
+        self.charge = charge
+        self.center = _vec3(center, "nucleus center")
+
+
+def nuclear_attraction(basis, nuclei, *, field=None, out=None):
+    """(n, n) complex128 nuclear-attraction matrix, sum over nuclei of
+    -Z <mu| 1/|r - C| |nu> (pybind11 `nuclear_attraction`, bindings.cpp:577-596;
+    compute_nuclear_attraction_matrix, nuclear.cpp:282-312)."""
+    n = basis.ao_count
+    nuclei = list(nuclei)
+    z = np.array([nu.charge for nu in nuclei], dtype=np.float64)
+    c = np.array([nu.center for nu in nuclei], dtype=np.float64).reshape(-1, 3)
+    values = basis._dev().nuclear_attraction(z, c, _field6(field))
+    if out is None:
+        return values
+    _check_out(out, (n, n))
+    out[...] = values
+    return out
+
+
 def _canonical_shell_quartet(a, b, c, d):
     """Lexicographic minimum of {abcd, cdab, badc*, dcba*} and its conjugate flag (eri.cpp:414-429)."""
     images = [((a, b, c, d), False), ((c, d, a, b), False), ((b, a, d, c), True), ((d, c, b, a), True)]

@@ -60,3 +60,25 @@ def test_nuclear_error_paths(gpu, port):
     small = _device(gpu, make_basis([(np.zeros(3), 1, np.array([1.0]), np.array([[1.0]]))], port))
     with pytest.raises(ValueError, match="finite"):
         small.nuclear_attraction([np.nan], [[0.0, 0.0, 1.0]], _native.field6())
+
+
+def test_nuclear_reference_named_entry(port):
+    """giaoint_amd.core.nuclear_attraction (the pybind11 name) with Nucleus
+    objects and a strict out= array gives the C-ABI matrix."""
+    from giaoint_amd import core
+
+    sh, B, O = workloads.water_ccpvdz()
+    shells = sh.shell_list()
+    basis = core.Basis([core.Shell(c, l, e, k) for c, l, e, k in shells])
+    nuclei = [core.Nucleus(8.0, shells[0][0])] + [core.Nucleus(1.0, c) for c in
+                                                   {tuple(s[0]) for s in shells if abs(s[0][1]) > 1e-9}]
+    field = core.MagneticField(B, O)
+    out = np.empty((basis.ao_count, basis.ao_count), dtype=np.complex128)
+    got = core.nuclear_attraction(basis, nuclei, field=field, out=out)
+    assert got is out
+    z = np.array([n.charge for n in nuclei])
+    c = np.array([n.center for n in nuclei])
+    want = port.nuclear_matrix(make_basis(shells, port), z, c, _native.field6(B, O))
+    assert np.max(np.abs(out - want)) <= ATOL
+    with pytest.raises(ValueError):
+        core.nuclear_attraction(basis, nuclei, field=field, out=np.empty((2, 2), dtype=np.complex128))
