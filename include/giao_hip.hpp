@@ -483,6 +483,35 @@ inline void compute_eri_tensor(const Basis& basis, const MagneticField& field, s
                                               0));
 }
 
+// Point charge (types.hpp:134-139).
+struct Nucleus {
+    double charge{};
+    Vec3 center{};
+    Nucleus(double charge_in, Vec3 center_in) : charge(charge_in), center(center_in) {
+        if (!std::isfinite(charge) || !std::isfinite(center.x) || !std::isfinite(center.y) ||
+            !std::isfinite(center.z))
+            throw std::invalid_argument("nucleus charge and center must be finite");
+    }
+};
+
+// compute_nuclear_attraction_matrix (nuclear.cpp:282-312, nuclear.hpp:59-61):
+// the (n, n) matrix of sum_C -Z_C <mu| 1/|r - C| |nu>, row-major.
+inline void compute_nuclear_attraction_matrix(const Basis& basis, std::span<const Nucleus> nuclei,
+                                              const MagneticField& field, std::span<Complex> output) {
+    const std::size_t n = basis.ao_count();
+    if (output.size() != n * n) throw std::length_error("basis nuclear-attraction output has an incorrect size");
+    std::vector<double> charges, centers;
+    for (const Nucleus& nu : nuclei) {
+        charges.push_back(nu.charge);
+        centers.insert(centers.end(), {nu.center.x, nu.center.y, nu.center.z});
+    }
+    giaohip_ctx* ctx = hip_detail::context();
+    const auto f = hip_detail::field6(field.B, field.gauge_origin);
+    hip_detail::check(ctx, giaohip_nuclear_attraction(ctx, basis.device(), static_cast<int>(charges.size()),
+                                                      charges.data(), centers.data(), f.data(),
+                                                      reinterpret_cast<double*>(output.data())));
+}
+
 // Parallel evaluation is available (on the GPU), so thread counts above one are accepted.
 [[nodiscard]] inline bool openmp_enabled() noexcept { return true; }
 [[nodiscard]] inline std::size_t openmp_max_threads() noexcept { return 16; }
