@@ -78,7 +78,10 @@ _SIGNATURES = {
     "giaohip_device_free": ([_vp, _vp], C.c_int),
     "giaohip_memcpy_to_host": ([_vp, _vp, _vp, C.c_size_t], C.c_int),
     "giaohip_synchronize": ([_vp], C.c_int),
+    "giaohip_host_threads": ([], C.c_int),
+    "giaohip_device_allocations": ([], C.c_ulonglong),
     "giaohip_basis_create": ([_vp, C.c_int, _dp, _ip, _ip, _ip, _dp, _dp, C.POINTER(_vp)], C.c_int),
+    "giaohip_basis_update": ([_vp, _vp, C.c_int, _dp, _ip, _ip, _ip, _dp, _dp], C.c_int),
     "giaohip_basis_destroy": ([_vp, _vp], C.c_int),
     "giaohip_basis_ao_count": ([_vp], C.c_size_t),
     "giaohip_normalize_shells": ([C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, C.c_char_p, C.c_size_t], C.c_int),
@@ -93,6 +96,7 @@ _SIGNATURES = {
     "giaohip_cholesky_columns": ([_vp, _vp, _u64p, C.c_size_t, _dp, _dp, C.c_int], C.c_int),
     "giaohip_cholesky_run": ([_vp, _vp, _dp, C.c_double, C.c_uint64, C.c_uint64, C.POINTER(_vp)], C.c_int),
     "giaohip_cholesky_info": ([_vp, _u64p, _dp, _dp, _ip, _dp, _dp, _dp], C.c_int),
+    "giaohip_cholesky_remeasure": ([_vp, _vp, _dp], C.c_int),
     "giaohip_cholesky_pivots": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "giaohip_cholesky_vectors": ([_vp, _vp, C.c_uint64, C.c_uint64, _dp], C.c_int),
     "giaohip_cholesky_resume": ([_vp, _vp, C.c_uint64], C.c_int),
@@ -443,6 +447,13 @@ class DeviceCholesky:
                                                             C.byref(proj), C.byref(crows)))
         self.blocks = {"blocks": int(blocks.value), "factor_rows_read": int(rows.value), "block_ms": bms.value,
                        "projected": int(proj.value), "candidate_rows": int(crows.value)}
+
+    def remeasure(self) -> float:
+        """Final residual max(D - sum |L|^2), all diagonal blocks measured again (cholesky.py:283-288)."""
+        out = C.c_double()
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_remeasure(self.ctx.handle, self.handle, C.byref(out)))
+        self._refresh()
+        return float(out.value)
 
     def resume(self, stop_after: int = 0):
         """Continue the run until converged or the rank reaches stop_after (0: no limit)."""

@@ -20,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .core import Basis, MagneticField, _field6, _threads
+from .core import Basis, MagneticField, _EriSchwarzBounds, _field6, _threads
 
 
 @dataclass(frozen=True)
@@ -65,14 +65,19 @@ def cholesky_eri(basis: Basis, *, field: MagneticField | None = None, threshold:
     """Factor the ERIs without forming the AO tensor (cholesky.py:60-297).
 
     Stops when the largest residual diagonal is <= max(threshold,
-    64 eps max(initial diagonal)). screening_threshold only lets the
-    reference skip initial diagonal blocks whose bound is already below the
-    threshold; such entries can never be pivots, so the device run (which
-    measures every diagonal block) takes the same pivots. max_bytes caps the
-    returned vectors and pivot indices (MemoryError beyond it).
+    64 eps max(initial diagonal)). With screening_threshold > 0 the
+    reference starts a shell pair whose squared Schwarz bound is at most
+    min(threshold, screening_threshold) from that bound, measures it after
+    the loop and reports max(D - sum |L|^2) as the final residual
+    (cholesky.py:138-154,283-288). Such entries are never pivots or
+    candidates (their residual is below the tolerance), so the device run,
+    which measures every diagonal block up front, takes the same pivots; when
+    any pair would have been skipped, the final residual is re-measured the
+    same way on the device. max_bytes caps the returned vectors and pivot
+    indices (MemoryError beyond it).
     """
     threshold = _nonnegative("threshold", threshold)
-    _nonnegative("screening_threshold", screening_threshold)
+    screening_threshold = _nonnegative("screening_threshold", screening_threshold)
     _threads(threads)
     if isinstance(pivot_batch_size, (bool, np.bool_)) or not isinstance(pivot_batch_size, (int, np.integer)) \
             or pivot_batch_size < 1:
@@ -86,6 +91,14 @@ def cholesky_eri(basis: Basis, *, field: MagneticField | None = None, threshold:
     dev = basis._dev()
     dev.ctx.set_cholesky_pivot_batch(int(pivot_batch_size))
     f = _field6(field)
+    screened = False
+    if screening_threshold and len(basis.shells):
+        # cholesky.py:128-154: bounds at the integral field (zero on one centre)
+        centers = [s.center for s in basis.shells]
+        integral_field = None if all(c == centers[0] for c in centers) else field
+        bounds = _EriSchwarzBounds(basis, field=integral_field).values ** 2
+        upper = np.triu(bounds <= min(threshold, screening_threshold))
+        screened = bool(np.any(upper))
 
     def over(rank):
         return MemoryError(f"Cholesky ERI output requires at least {(rank + 1) * row_bytes} bytes, "
@@ -101,5 +114,7 @@ def cholesky_eri(basis: Basis, *, field: MagneticField | None = None, threshold:
             run = _native.cholesky_run(dev, f, threshold, limit if max_bytes is not None else 0)
         except MemoryError:
             raise over(limit) from None
+    if screened:
+        run.remeasure()
     vectors = run.vectors() if run.rank else np.zeros((0, n, n), dtype=np.complex128)
     return CholeskyERI(vectors, run.pivots().astype(np.int64).reshape(-1, 2), threshold, float(run.max_residual))

@@ -406,14 +406,21 @@ def _eri_full(basis, *, field=None, screening_threshold=0.0, threads=None):
 
 
 class Nucleus:
-    """Point charge (types.hpp:134-139): charge and centre in bohr."""
+    """Point charge (types.hpp:134-139): charge and centre in bohr.
+
+    Rejects what the reference's constructor rejects, with its messages
+    (types.cpp:188-194): a charge that is not finite and positive, and a
+    centre with a non-finite coordinate."""
 
     def __init__(self, charge, center):
         charge = float(charge)
-        if not np.isfinite(charge):
-            raise ValueError("nucleus charge must be finite")
+        center = _vec3(center, "center")
+        if not np.isfinite(charge) or charge <= 0.0:
+            raise ValueError("nuclear charge must be finite and positive")
+        if not all(np.isfinite(x) for x in center):
+            raise ValueError("nuclear center values must be finite")
         self.charge = charge
-        self.center = _vec3(center, "nucleus center")
+        self.center = center
 
 
 def nuclear_attraction(basis, nuclei, *, field=None, out=None):
@@ -421,13 +428,14 @@ def nuclear_attraction(basis, nuclei, *, field=None, out=None):
     -Z <mu| 1/|r - C| |nu> (pybind11 `nuclear_attraction`, bindings.cpp:577-596;
     compute_nuclear_attraction_matrix, nuclear.cpp:282-312)."""
     n = basis.ao_count
+    if out is not None:
+        _check_out(out, (n, n))
     nuclei = list(nuclei)
     z = np.array([nu.charge for nu in nuclei], dtype=np.float64)
     c = np.array([nu.center for nu in nuclei], dtype=np.float64).reshape(-1, 3)
     values = basis._dev().nuclear_attraction(z, c, _field6(field))
     if out is None:
         return values
-    _check_out(out, (n, n))
     out[...] = values
     return out
 
@@ -477,4 +485,5 @@ def openmp_enabled() -> bool:
 
 
 def openmp_max_threads() -> int:
-    return 16
+    """Host threads the library plans with (eri.cpp:863-869 reports OpenMP's)."""
+    return int(_native.load_library().giaohip_host_threads())

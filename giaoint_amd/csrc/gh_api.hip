@@ -37,6 +37,15 @@ using gh::zc;
 
 namespace {
 
+// Device allocations the library has made in this process (giaohip_device_allocations).
+std::atomic<unsigned long long> g_device_allocations{0};
+
+hipError_t counted_malloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) g_device_allocations.fetch_add(1, std::memory_order_relaxed);
+    return e;
+}
+
 struct DevBuffer {
     void* p = nullptr;
     size_t cap = 0;
@@ -49,7 +58,7 @@ struct DevBuffer {
             if (e != hipSuccess) return e;
         }
         size_t want = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&p, want);
+        hipError_t e = counted_malloc(&p, want);
         if (e == hipSuccess) cap = want;
         return e;
     }
@@ -209,6 +218,7 @@ struct giaohip_cholesky {
     uint64_t projected = 0;  // candidates projected (the others carried over)
     uint64_t cand_rows = 0;  // sum of candidates projected x rank
     int real = 0;            // real arithmetic: zero field or one centre (symmetrized columns)
+    double field[6]{};       // the field the integrals are taken at (zero on one centre)
     double block_ms = 0.0;   // candidate selection + projection
     hipEvent_t ev[10]{};
     hipStream_t s3 = nullptr;  // second stream of the column launches
@@ -347,11 +357,15 @@ int check_quartet_supported(giaohip_ctx* ctx, const giaohip_basis* b, const int 
     return class_error(ctx, b->cls_status[class_index(b, sh)]);
 }
 
-// Runs f(begin, end) over [0, n) split across up to 16 host threads.
+// Host threads of the library's planning work: the machine's hardware
+// threads, at most 16.
+size_t host_threads() { return std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency())); }
+
+// Runs f(begin, end) over [0, n) split across host_threads() threads.
 template <class F>
 void parallel_for(size_t n, F&& f) {
     constexpr size_t kGrain = 8192;
-    size_t T = std::min<size_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+    size_t T = host_threads();
     T = std::min(T, (n + kGrain - 1) / kGrain);
     if (T <= 1) {
         f((size_t)0, n);
@@ -395,25 +409,82 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Runs requests with the plain per-class transform kernel.
+// Ket-component tile, LDS bytes and number of work items of one request on
+// the plain transform kernel.
+int plain_plan(giaohip_ctx* ctx, const giaohip_basis* b, const Request& r, gh::ClassArgs& ca, int& lds,
+               long long& nitems) {
+    const int maxw = gh::transform_max_w(), maxo = gh::transform_max_o();
+    const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
+    const int ncd_all = S[2]->ncart * S[3]->ncart;
+    const int cd0 = r.cd1 < 0 ? 0 : r.cd0, cd1 = r.cd1 < 0 ? ncd_all : r.cd1;
+    const int nab = S[0]->ncart * S[1]->ncart;
+    const int NHab = gh::nherm(S[0]->l + S[1]->l);
+    int tile = std::min({cd1 - cd0, maxw / NHab, maxo / nab});
+    if (tile < 1) return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class too large for the GPU transform kernel");
+    ca = gh::ClassArgs{{S[0]->l, S[1]->l, S[2]->l, S[3]->l}, tile};
+    while (tile > 1 && gh::transform_lds_bytes(ca) > 160 * 1024) ca.tile = --tile;
+    lds = gh::transform_lds_bytes(ca);
+    if (lds > 160 * 1024)
+        return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class exceeds the LDS capacity of the transform kernel");
+    long long rows = 1;
+    for (int k = 0; k < 4; ++k) rows *= r.xfix[k] < 0 ? S[k]->ncontr : 1;
+    nitems = rows * ((cd1 - cd0 + ca.tile - 1) / ca.tile);
+    return GIAOHIP_OK;
+}
+
+// Fixed device memory of the plain path, used instead of the context's
+// buffers (which grow on demand). Capacities are in elements.
+struct FixedPlainMem {
+    double* kappa;
+    size_t kappa_cap;
+    DevError* err;
+    int* queue;
+    SeedQuartet* sq;
+    size_t sq_cap;
+    WorkItem* items;
+    size_t items_cap;
+    zc* seeds;  // seeds, then the Boys quadrature lists
+    size_t seeds_cap;
+};
+
+size_t seed_scratch_zc(long long pq_total) {
+    return (gh::seed_scratch_ints(pq_total) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
+}
+
+// Runs requests with the plain per-class transform kernel. With `fixed`, all
+// device memory comes from its slices (no allocation); chunks are then also
+// bounded by the slices' capacities.
 int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
-              zc* d_out) {
+              zc* d_out, const FixedPlainMem* fixed = nullptr) {
     if (reqs.empty()) return GIAOHIP_OK;
     hipStream_t s = ctx->stream;
     const std::vector<double> kap = wave_vectors(b, field);
-    HIP_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
-    HIP_TRY(hipMemcpyAsync(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
+    double* d_kappa;
+    DevError* d_err;
+    if (fixed) {
+        if (kap.size() > fixed->kappa_cap)
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT,
+                             "basis has more shells than the allocation-free ERI scratch holds");
+        d_kappa = fixed->kappa;
+        d_err = fixed->err;
+        HIP_TRY(gh::launch_zero_ints(reinterpret_cast<int*>(d_err), (int)(sizeof(DevError) / sizeof(int)), s),
+                "fill kernel launch");
+    } else {
+        HIP_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
+        HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
+        d_kappa = ctx->kappa.as<double>();
+        d_err = ctx->errrec.as<DevError>();
+        HIP_TRY(hipMemsetAsync(d_err, 0, sizeof(DevError), s), "hipMemsetAsync");
+    }
+    HIP_TRY(hipMemcpyAsync(d_kappa, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
             "hipMemcpyAsync");
-    HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
-    HIP_TRY(hipMemsetAsync(ctx->errrec.p, 0, sizeof(DevError), s), "hipMemsetAsync");
 
-    const int maxw = gh::transform_max_w(), maxo = gh::transform_max_o();
     size_t begin = 0;
     while (begin < reqs.size()) {
         // Chunk so that the seed buffer stays within budget.
         size_t end = begin;
         size_t seed_zc = 0;
-        long long pq_total = 0;
+        long long pq_total = 0, item_total = 0;
         int maxL = 0;
         std::vector<SeedQuartet> sq;
         while (end < reqs.size()) {
@@ -425,6 +496,22 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const size_t need = (size_t)(kab * kcd) * (size_t)(L + 4);
             // seeds are this path's only large buffer: four chunk budgets
             if (end > begin && (seed_zc + need) * sizeof(zc) > 4 * ctx->seed_budget_bytes) break;
+            if (fixed) {
+                gh::ClassArgs ca;
+                int lds;
+                long long nit;
+                const int st = plain_plan(ctx, b, r, ca, lds, nit);
+                if (st) return st;
+                const bool fits = sq.size() + 1 <= fixed->sq_cap && (size_t)(item_total + nit) <= fixed->items_cap &&
+                                  seed_zc + need + seed_scratch_zc(pq_total + kab * kcd) <= fixed->seeds_cap;
+                if (!fits) {
+                    if (end == begin)
+                        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT,
+                                         "shell quartet exceeds the allocation-free ERI scratch");
+                    break;
+                }
+                item_total += nit;
+            }
             SeedQuartet q{};
             for (int i = 0; i < 4; ++i) q.sh[i] = r.sh[i];
             q.L = L;
@@ -438,11 +525,20 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             maxL = std::max(maxL, L);
             ++end;
         }
-        const size_t scr_zc = (gh::seed_scratch_ints(pq_total) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
-        HIP_TRY(ctx->seeds.ensure((seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
-        int* d_scratch = reinterpret_cast<int*>(ctx->seeds.as<zc>() + seed_zc);
-        HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
-        HIP_TRY(hipMemcpyAsync(ctx->squartets.p, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
+        const size_t scr_zc = seed_scratch_zc(pq_total);
+        zc* d_seeds;
+        SeedQuartet* d_sq;
+        if (fixed) {
+            d_seeds = fixed->seeds;
+            d_sq = fixed->sq;
+        } else {
+            HIP_TRY(ctx->seeds.ensure((seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
+            HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
+            d_seeds = ctx->seeds.as<zc>();
+            d_sq = ctx->squartets.as<SeedQuartet>();
+        }
+        int* d_scratch = reinterpret_cast<int*>(d_seeds + seed_zc);
+        HIP_TRY(hipMemcpyAsync(d_sq, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
 
         // Work items of every class, one per pass of `tile` ket components,
@@ -460,15 +556,12 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
             const int ncd_all = S[2]->ncart * S[3]->ncart;
             const int cd0 = r.cd1 < 0 ? 0 : r.cd0, cd1 = r.cd1 < 0 ? ncd_all : r.cd1;
-            const int nab = S[0]->ncart * S[1]->ncart;
             const int NHab = gh::nherm(S[0]->l + S[1]->l);
-            int tile = std::min({cd1 - cd0, maxw / NHab, maxo / nab});
-            if (tile < 1) return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class too large for the GPU transform kernel");
-            gh::ClassArgs ca{{S[0]->l, S[1]->l, S[2]->l, S[3]->l}, tile};
-            while (tile > 1 && gh::transform_lds_bytes(ca) > 160 * 1024) ca.tile = --tile;
-            const int lds = gh::transform_lds_bytes(ca);
-            if (lds > 160 * 1024)
-                return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class exceeds the LDS capacity of the transform kernel");
+            gh::ClassArgs ca;
+            int lds;
+            long long nit;
+            const int st = plain_plan(ctx, b, r, ca, lds, nit);
+            if (st) return st;
             const int L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
             const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
             const int cls = S[0]->l | (S[1]->l << 4) | (S[2]->l << 8) | (S[3]->l << 12) | (ca.tile << 16);
@@ -544,17 +637,26 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         }
         int* d_queue = nullptr;
         if (ctx->eri_queue != GIAOHIP_QUEUE_NONE) {
-            HIP_TRY(ctx->queue.ensure(2 * sizeof(int)), "hipMalloc");
-            d_queue = ctx->queue.as<int>();
+            if (fixed) {
+                d_queue = fixed->queue;
+            } else {
+                HIP_TRY(ctx->queue.ensure(2 * sizeof(int)), "hipMalloc");
+                d_queue = ctx->queue.as<int>();
+            }
         }
-        HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
-        HIP_TRY(hipMemcpyAsync(ctx->items.p, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
+        WorkItem* d_items;
+        if (fixed) {
+            d_items = fixed->items;
+        } else {
+            HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
+            d_items = ctx->items.as<WorkItem>();
+        }
+        HIP_TRY(hipMemcpyAsync(d_items, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
                 "hipMemcpyAsync");
 
         HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
-        HIP_TRY(gh::launch_seed(nmax_bucket(maxL), ctx->squartets.as<SeedQuartet>(), (int)sq.size(), pq_total,
-                                b->d_shells.as<DevShell>(), b->d_exps.as<double>(), ctx->kappa.as<double>(),
-                                ctx->seeds.as<zc>(), ctx->errrec.as<DevError>(), d_scratch, s),
+        HIP_TRY(gh::launch_seed(nmax_bucket(maxL), d_sq, (int)sq.size(), pq_total, b->d_shells.as<DevShell>(),
+                                b->d_exps.as<double>(), d_kappa, d_seeds, d_err, d_scratch, s),
                 "seed kernel launch");
         HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
         // class launches alternate between the two streams (disjoint outputs),
@@ -571,10 +673,9 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const int side = queued[1] < queued[0] ? 1 : 0;
             queued[side] += L.cost;
             hipStream_t ls = side ? ctx->aux : s;
-            HIP_TRY(gh::launch_transform_mixed(L.lds, ctx->items.as<WorkItem>() + L.first, (int)L.count,
-                                               b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
-                                               b->d_coefs.as<double>(), b->d_norms.as<double>(),
-                                               ctx->kappa.as<double>(), ctx->seeds.as<zc>(), d_out,
+            HIP_TRY(gh::launch_transform_mixed(L.lds, d_items + L.first, (int)L.count, b->d_shells.as<DevShell>(),
+                                               b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                                               b->d_norms.as<double>(), d_kappa, d_seeds, d_out,
                                                d_queue ? d_queue + side : nullptr, ls),
                     "transform kernel launch");
             ctx->stats.launches += 1;
@@ -590,7 +691,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         ctx->stats.launches += 1;
         ctx->stats.primitive_quartets += (uint64_t)pq_total;
         DevError de{};
-        HIP_TRY(hipMemcpy(&de, ctx->errrec.p, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+        HIP_TRY(hipMemcpy(&de, d_err, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
         if (de.code != 0) return boys_error(ctx, de);
         begin = end;
     }
@@ -1010,6 +1111,10 @@ int diagonal_to_device(giaohip_ctx* ctx, const giaohip_basis* b, const uint64_t*
 
 extern "C" {
 
+int giaohip_host_threads(void) { return (int)host_threads(); }
+
+unsigned long long giaohip_device_allocations(void) { return g_device_allocations.load(); }
+
 double giaohip_primitive_normalization(double exponent, int ax, int ay, int az) {
     return primitive_normalization(exponent, ax, ay, az);
 }
@@ -1169,7 +1274,7 @@ int giaohip_device_alloc(giaohip_ctx* ctx, size_t bytes, void** out) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, std::max<size_t>(bytes, 16)), "hipMalloc");
+    HIP_TRY(counted_malloc(&p, std::max<size_t>(bytes, 16)), "hipMalloc");
     ctx->user_allocs.insert(p);
     *out = p;
     return GIAOHIP_OK;
@@ -1197,19 +1302,35 @@ int giaohip_synchronize(giaohip_ctx* ctx) {
     return GIAOHIP_OK;
 }
 
-int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, const int* L, const int* nprim,
-                         const int* ncontr, const double* exps, const double* coefs, giaohip_basis** out) {
-    if (!ctx || !out) return GIAOHIP_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Fills b's host tables from the C-ABI shell layout and uploads them. Device
+// buffers only grow (DevBuffer::ensure), so refilling a basis with shells that
+// fit its earlier tables allocates nothing. On failure b is left empty.
+int load_basis(giaohip_ctx* ctx, giaohip_basis* b, int nshell, const double* centers, const int* L,
+               const int* nprim, const int* ncontr, const double* exps, const double* coefs) {
+    auto clear = [b] {
+        b->ns = 0;
+        b->nao = 0;
+        b->shells.clear();
+        b->ao_off.clear();
+        b->exps.clear();
+        b->coefs.clear();
+        b->norms.clear();
+        b->radial.clear();
+        b->owner.clear();
+    };
+    clear();
     if (nshell < 0) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell count must be non-negative");
-    auto* b = new giaohip_basis;
-    b->ns = nshell;
+    if (nshell > 0 && (!centers || !L || !nprim || !ncontr || !exps || !coefs))
+        return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell description arrays must not be null");
     int po = 0, co = 0, no = 0;
     long long ao = 0;
     for (int s = 0; s < nshell; ++s) {
         if (L[s] < 0 || nprim[s] < 1 || ncontr[s] < 1) {
-            delete b;
+            clear();
             return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "invalid shell description");
         }
         DevShell d{};
@@ -1225,7 +1346,7 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
         for (int p = 0; p < d.nprim; ++p) {
             const double e = exps[po + p];
             if (!std::isfinite(e) || e <= 0.0) {
-                delete b;
+                clear();
                 return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell exponents must be finite and positive");
             }
             b->exps.push_back(e);
@@ -1244,6 +1365,7 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
         co += d.nprim * d.ncontr;
         no += (int)comps.size() * d.nprim;
     }
+    b->ns = nshell;
     b->nao = ao;
     build_class_tables(b);
     (void)hipSetDevice(ctx->device);
@@ -1261,11 +1383,43 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
     if (e == hipSuccess) e = up(b->d_owner, b->owner.data(), b->owner.size() * sizeof(int));
     if (e == hipSuccess) e = up(b->d_radial, b->radial.data(), b->radial.size() * sizeof(double));
     if (e != hipSuccess) {
-        delete b;
+        clear();
         return hip_fail(ctx, e, "basis upload");
+    }
+    return GIAOHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, const int* L, const int* nprim,
+                         const int* ncontr, const double* exps, const double* coefs, giaohip_basis** out) {
+    if (!ctx || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = nullptr;
+    auto* b = new giaohip_basis;
+    const int st = load_basis(ctx, b, nshell, centers, L, nprim, ncontr, exps, coefs);
+    if (st != GIAOHIP_OK) {
+        for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner,
+                             &b->d_radial})
+            d->release();
+        delete b;
+        return st;
     }
     *out = b;
     return GIAOHIP_OK;
+}
+
+int giaohip_basis_update(giaohip_ctx* ctx, giaohip_basis* b, int nshell, const double* centers, const int* L,
+                         const int* nprim, const int* ncontr, const double* exps, const double* coefs) {
+    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    // the previous call's kernels may still read the tables being replaced
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    HIP_TRY(hipStreamSynchronize(ctx->aux), "hipStreamSynchronize");
+    return load_basis(ctx, b, nshell, centers, L, nprim, ncontr, exps, coefs);
 }
 
 int giaohip_basis_destroy(giaohip_ctx* ctx, giaohip_basis* b) {
@@ -1300,7 +1454,7 @@ int giaohip_eri_batch_size(const giaohip_basis* b, const uint32_t* quartets, siz
 
 int giaohip_boys(giaohip_ctx* ctx, const double* z, size_t n, int nmax, int scaled, double* out, int* region,
                  double* error_estimate, int* segments) {
-    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || (n > 0 && (!z || !out))) return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (nmax < 0) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "Boys output must contain at least F_0");
     if (nmax > gh::kMaxBoysOrder) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "complex Boys supports orders through 32");
@@ -1340,48 +1494,14 @@ int giaohip_boys(giaohip_ctx* ctx, const double* z, size_t n, int nmax, int scal
     return GIAOHIP_OK;
 }
 
-// Device -> pageable host copy through two page-locked bounce buffers: the
-// DMA of piece k + 1 runs while piece k is copied out by host threads.
-// Returns a HIP error (hipSuccess when done).
-hipError_t copy_to_host(giaohip_ctx* ctx, const char* src, char* dst, size_t bytes) {
-    constexpr size_t kPiece = size_t(64) << 20;
-    for (PinnedBuffer& b : ctx->bounce)
-        if (hipError_t e = b.ensure(kPiece); e != hipSuccess) return e;
-    hipStream_t cs = ctx->copy_stream;
-    const size_t pieces = (bytes + kPiece - 1) / kPiece;
-    auto drain = [&](size_t k) -> hipError_t {
-        if (hipError_t e = hipEventSynchronize(ctx->copy_ev[k % 2]); e != hipSuccess) return e;
-        const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
-        const char* from = ctx->bounce[k % 2].as<char>();
-        parallel_for(len / 4096 + 1, [&](size_t lo, size_t hi) {
-            const size_t a = std::min(len, lo * 4096), z = std::min(len, hi * 4096);
-            if (z > a) std::memcpy(dst + off + a, from + a, z - a);
-        });
-        return hipSuccess;
-    };
-    for (size_t k = 0; k < pieces; ++k) {
-        const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
-        if (hipError_t e = hipMemcpyAsync(ctx->bounce[k % 2].p, src + off, len, hipMemcpyDeviceToHost, cs);
-            e != hipSuccess)
-            return e;
-        if (hipError_t e = hipEventRecord(ctx->copy_ev[k % 2], cs); e != hipSuccess) return e;
-        if (k > 0)
-            if (hipError_t e = drain(k - 1); e != hipSuccess) return e;
-    }
-    return pieces ? drain(pieces - 1) : hipSuccess;
-}
-
-int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
-                      const double field[6], const uint64_t* offsets, double* out, int out_on_device) {
-    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    reset_stats(ctx);
-    ctx->stats.requested_quartets = n;
-    const auto host_t0 = std::chrono::steady_clock::now();
+// The checks of evaluate_eri_shell_quartets (eri.cpp:644-688) on a quartet
+// list, in parallel (the first failing quartet decides the error), and the
+// size of each block in complex elements.
+static int check_quartets(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
+                   const double field[6], std::vector<uint64_t>& sizes) {
     for (int k = 0; k < 6; ++k)
         if (!std::isfinite(field[k])) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "field values must be finite");
     // validation and block sizes in parallel; the first failing quartet decides the error
-    std::vector<uint64_t> offs(n), sizes(n);
     std::vector<size_t> first_bad(16, n);
     std::vector<int> bad_status(16, 0);
     std::atomic<size_t> slot{0};
@@ -1415,6 +1535,69 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
         if (bad_st < 0) return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "ERI shell-quartet index is out of range");
         return class_error(ctx, bad_st);
     }
+    return GIAOHIP_OK;
+}
+
+// Request of quartet i of a list, written at complex offset `base` with the
+// C-order strides of its (a, b, c, d) block.
+static Request quartet_request(const giaohip_basis* b, const uint32_t* quartets, size_t i, long long base) {
+    Request r{};
+    for (int k = 0; k < 4; ++k) {
+        r.sh[k] = (int)quartets[4 * i + k];
+        r.xfix[k] = -1;
+    }
+    r.cd0 = 0;
+    r.cd1 = -1;
+    r.out_base = base;
+    const long long nb = b->shells[r.sh[1]].ao_count, nc = b->shells[r.sh[2]].ao_count,
+                    nd = b->shells[r.sh[3]].ao_count;
+    r.stride[0] = nb * nc * nd;
+    r.stride[1] = nc * nd;
+    r.stride[2] = nd;
+    r.stride[3] = 1;
+    return r;
+}
+
+// Device -> pageable host copy through two page-locked bounce buffers: the
+// DMA of piece k + 1 runs while piece k is copied out by host threads.
+// Returns a HIP error (hipSuccess when done).
+hipError_t copy_to_host(giaohip_ctx* ctx, const char* src, char* dst, size_t bytes) {
+    constexpr size_t kPiece = size_t(64) << 20;
+    for (PinnedBuffer& b : ctx->bounce)
+        if (hipError_t e = b.ensure(kPiece); e != hipSuccess) return e;
+    hipStream_t cs = ctx->copy_stream;
+    const size_t pieces = (bytes + kPiece - 1) / kPiece;
+    auto drain = [&](size_t k) -> hipError_t {
+        if (hipError_t e = hipEventSynchronize(ctx->copy_ev[k % 2]); e != hipSuccess) return e;
+        const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
+        const char* from = ctx->bounce[k % 2].as<char>();
+        parallel_for(len / 4096 + 1, [&](size_t lo, size_t hi) {
+            const size_t a = std::min(len, lo * 4096), z = std::min(len, hi * 4096);
+            if (z > a) std::memcpy(dst + off + a, from + a, z - a);
+        });
+        return hipSuccess;
+    };
+    for (size_t k = 0; k < pieces; ++k) {
+        const size_t off = k * kPiece, len = std::min(kPiece, bytes - off);
+        if (hipError_t e = hipMemcpyAsync(ctx->bounce[k % 2].p, src + off, len, hipMemcpyDeviceToHost, cs);
+            e != hipSuccess)
+            return e;
+        if (hipError_t e = hipEventRecord(ctx->copy_ev[k % 2], cs); e != hipSuccess) return e;
+        if (k > 0)
+            if (hipError_t e = drain(k - 1); e != hipSuccess) return e;
+    }
+    return pieces ? drain(pieces - 1) : hipSuccess;
+}
+
+int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
+                      const double field[6], const uint64_t* offsets, double* out, int out_on_device) {
+    if (!ctx || !b || !field || (n > 0 && (!quartets || !out))) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    ctx->stats.requested_quartets = n;
+    const auto host_t0 = std::chrono::steady_clock::now();
+    std::vector<uint64_t> offs(n), sizes(n);
+    if (const int vst = check_quartets(ctx, b, quartets, n, field, sizes)) return vst;
     uint64_t run = 0;
     for (size_t i = 0; i < n; ++i) {
         offs[i] = offsets ? offsets[i] : run;
@@ -1427,22 +1610,7 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     hipEvent_t t0;
     HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
     HIP_TRY(hipEventRecord(t0, s), "hipEventRecord");
-    auto make_req = [&](size_t i, long long base) {
-        Request r{};
-        for (int k = 0; k < 4; ++k) {
-            r.sh[k] = (int)quartets[4 * i + k];
-            r.xfix[k] = -1;
-        }
-        r.cd0 = 0;
-        r.cd1 = -1;
-        r.out_base = base;
-        const long long nb = ao_count(b, r.sh[1]), nc = ao_count(b, r.sh[2]), nd = ao_count(b, r.sh[3]);
-        r.stride[0] = nb * nc * nd;
-        r.stride[1] = nc * nd;
-        r.stride[2] = nd;
-        r.stride[3] = 1;
-        return r;
-    };
+    auto make_req = [&](size_t i, long long base) { return quartet_request(b, quartets, i, base); };
     int st = GIAOHIP_OK;
     if (out_on_device) {
         std::vector<Request> reqs(n);
@@ -1520,7 +1688,7 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
 }
 
 int giaohip_schwarz_bounds(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out) {
-    if (!ctx || !b || !out) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || !b || !out || !field) return GIAOHIP_INVALID_ARGUMENT;
     const int ns = b->ns;
     std::vector<uint32_t> q;
     q.reserve((size_t)ns * (ns + 1) * 2);
@@ -1578,7 +1746,7 @@ int giaohip_screen_quartets(giaohip_ctx* ctx, const giaohip_basis* b, const uint
 }
 
 int giaohip_eri_tensor(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out, int out_on_device) {
-    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || !b || !field || !out) return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
     const int ns = b->ns;
@@ -1652,10 +1820,13 @@ int giaohip_nuclear_attraction(giaohip_ctx* ctx, const giaohip_basis* b, int nnu
         return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
+    // Nucleus::Nucleus (types.cpp:188-194): the charge first, then the centre
     for (int c = 0; c < nnuc; ++c) {
-        bool ok = std::isfinite(charges[c]);
-        for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(centers[3 * c + k]);
-        if (!ok) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "nucleus charge and center must be finite");
+        if (!std::isfinite(charges[c]) || charges[c] <= 0.0)
+            return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "nuclear charge must be finite and positive");
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(centers[3 * c + k]))
+                return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "nuclear center values must be finite");
     }
     int lmax = 0;
     for (const DevShell& sh : b->shells) lmax = std::max(lmax, sh.l);
@@ -1703,7 +1874,7 @@ int giaohip_nuclear_attraction(giaohip_ctx* ctx, const giaohip_basis* b, int nnu
 
 int giaohip_primitive_eri(giaohip_ctx* ctx, const double prim[4][6], const int ang[4][3], const double field[6],
                           double out[2]) {
-    if (!ctx) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || !prim || !ang || !field || !out) return GIAOHIP_INVALID_ARGUMENT;
     double centers[12], exps[4], coefs[4] = {1.0, 1.0, 1.0, 1.0};
     int L[4], np[4] = {1, 1, 1, 1}, nc[4] = {1, 1, 1, 1};
     for (int i = 0; i < 4; ++i) {
@@ -1744,7 +1915,7 @@ int giaohip_primitive_eri(giaohip_ctx* ctx, const double prim[4][6], const int a
 
 int giaohip_cholesky_diagonal(giaohip_ctx* ctx, const giaohip_basis* b, const uint64_t* pairs, size_t npairs,
                               const double field[6], double* out) {
-    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || !b || !field || !out || (npairs > 0 && !pairs)) return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
     (void)hipSetDevice(ctx->device);
@@ -1759,7 +1930,7 @@ int giaohip_cholesky_diagonal(giaohip_ctx* ctx, const giaohip_basis* b, const ui
 
 int giaohip_cholesky_columns(giaohip_ctx* ctx, const giaohip_basis* b, const uint64_t* pivots, size_t npiv,
                              const double field[6], double* out, int real_output) {
-    if (!ctx || !b) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || !b || !field || (npiv > 0 && (!pivots || !out))) return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
     const bool zero_field = field[0] == 0.0 && field[1] == 0.0 && field[2] == 0.0;
@@ -1988,7 +2159,7 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         const int chunk = (int)(c->rank / c->rows_per_chunk);
         if (chunk >= (int)c->chunks.size()) {
             void* mem = nullptr;
-            CHOL_STEP_TRY(hipMalloc(&mem, (size_t)c->rows_per_chunk * n2 * sizeof(zc)), "hipMalloc(factors)");
+            CHOL_STEP_TRY(counted_malloc(&mem, (size_t)c->rows_per_chunk * n2 * sizeof(zc)), "hipMalloc(factors)");
             c->chunks.push_back(static_cast<zc*>(mem));
             CHOL_STEP_TRY(c->d_chunkptr.ensure(c->chunks.size() * sizeof(zc*)), "hipMalloc");
             CHOL_STEP_TRY(hipMemcpy(c->d_chunkptr.p, c->chunks.data(), c->chunks.size() * sizeof(zc*),
@@ -2217,7 +2388,7 @@ extern "C" {
 
 int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double threshold,
                          uint64_t max_rank, uint64_t stop_after, giaohip_cholesky** out) {
-    if (!ctx || !b || !out) return GIAOHIP_INVALID_ARGUMENT;
+    if (!ctx || !b || !out || !field) return GIAOHIP_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ctx->mu);
     reset_stats(ctx);
     *out = nullptr;
@@ -2241,6 +2412,7 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* b, const double 
     c->n = n;
     c->basis = b;
     c->real = real ? 1 : 0;
+    for (int k = 0; k < 6; ++k) c->field[k] = ifield[k];
     // Factor rows live in chunks of about 2 GiB (at least one row).
     c->rows_per_chunk = (int)std::max<long long>(1, (2LL << 30) / (n2 * (long long)sizeof(zc)));
     auto cleanup = [&](int code) {
@@ -2382,6 +2554,43 @@ int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t stop
     if (c->converged) return GIAOHIP_OK;
     (void)hipSetDevice(ctx->device);
     return chol_steps(ctx, c, stop_after);
+}
+
+int giaohip_cholesky_remeasure(giaohip_ctx* ctx, giaohip_cholesky* c, double* max_residual) {
+    if (!ctx || !c) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    if (c->failed) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "this Cholesky run ended with an error");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const giaohip_basis* b = c->basis;
+    const long long n2 = c->n * c->n;
+    const int ns = b->ns;
+    std::vector<uint64_t> pairs;
+    for (int a = 0; a < ns; ++a)
+        for (int bb = a; bb < ns; ++bb) {
+            pairs.push_back((uint64_t)a);
+            pairs.push_back((uint64_t)bb);
+        }
+    HIP_TRY(ctx->staging.ensure(std::max<size_t>((size_t)n2 * sizeof(double), 16)), "hipMalloc");
+    const int st = diagonal_to_device(ctx, b, pairs.data(), pairs.size() / 2, c->field, ctx->staging.as<double>());
+    if (st) return st;
+    const long long nb = (n2 + 255) / 256;
+    HIP_TRY(ctx->misc_b.ensure(std::max<size_t>((size_t)nb * sizeof(double), 16)), "hipMalloc");
+    HIP_TRY(gh::launch_chol_remeasure(c->d_chunkptr.as<const zc*>(), c->rows_per_chunk, n2, (int)c->rank,
+                                      ctx->staging.as<double>(), ctx->misc_b.as<double>(), s),
+            "remeasure launch");
+    std::vector<double> part((size_t)nb);
+    if (nb)
+        HIP_TRY(hipMemcpyAsync(part.data(), ctx->misc_b.p, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(s), "remeasure execution");
+    // np.max(residual, initial=0.0) (cholesky.py:297)
+    double m = 0.0;
+    for (double v : part) m = std::max(m, v);
+    c->max_residual = m;
+    if (max_residual) *max_residual = m;
+    return GIAOHIP_OK;
 }
 
 int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max_residual, double* tolerance,

@@ -697,6 +697,43 @@ hipError_t launch_chol_reconstruct(const zc* const* chunks, int rows_per_chunk, 
     return hipGetLastError();
 }
 
+// Final residual of a run whose screened diagonal blocks were measured at
+// the end (cholesky.py:283-288): out[e] = D[e] - sum_k |L_k[e]|^2 over the
+// factor rows in order k = 0..rank-1, |L|^2 taken as hypot(re, im)^2, with
+// no clamp and no zeroed pivots; pmax[block] = the block's largest value.
+__global__ __launch_bounds__(256) void chol_remeasure_kernel(const zc* const* __restrict__ chunks,
+                                                             int rows_per_chunk, long long n2, int rank,
+                                                             const double* __restrict__ diag,
+                                                             double* __restrict__ pmax) {
+    __shared__ double red[256];
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    double v = -HUGE_VAL;
+    if (e < n2) {
+        double sum = 0.0;
+        for (int k = 0; k < rank; ++k) {
+            const zc x = factor_row(chunks, rows_per_chunk, n2, k)[e];
+            const double a = hypot(x.x, x.y);
+            sum += a * a;
+        }
+        v = diag[e] - sum;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pmax[blockIdx.x] = red[0];
+}
+
+hipError_t launch_chol_remeasure(const zc* const* chunks, int rows_per_chunk, long long n2, int rank,
+                                 const double* diag, double* pmax, hipStream_t s) {
+    if (n2 == 0) return hipSuccess;
+    chol_remeasure_kernel<<<dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s>>>(chunks, rows_per_chunk, n2, rank,
+                                                                                  diag, pmax);
+    return hipGetLastError();
+}
+
 __global__ void chol_gather_kernel(const double* __restrict__ res, const long long* __restrict__ idx, int m,
                                    double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

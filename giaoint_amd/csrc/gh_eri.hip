@@ -726,6 +726,17 @@ __global__ void queue_reset_kernel(int* __restrict__ queue, int n) {
     if ((int)threadIdx.x < n) queue[threadIdx.x] = 0;
 }
 
+__global__ void zero_ints_kernel(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    zero_ints_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(p, n);
+    return hipGetLastError();
+}
+
 // Four symmetry images of canonical blocks into an n^4 tensor (eri.cpp:777-805).
 __global__ void tensor_scatter_kernel(const long long* __restrict__ block_off, const int* __restrict__ block_q,
                                       int nblocks, long long total, const DevShell* __restrict__ shells,
@@ -902,7 +913,9 @@ hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long 
                        const double* exps, const double* kappa, zc* seeds, DevError* err, int* scratch,
                        hipStream_t s, const double* radial) {
     if (total == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(scratch, 0, 16 * sizeof(int), s);
+    // the quadrature list counters start at zero (a fill kernel, so the
+    // sequence holds no memset node when a caller captures it)
+    hipError_t e = launch_zero_ints(scratch, 16, s);
     if (e != hipSuccess) return e;
     const dim3 block(128);
     const dim3 grid((unsigned)((total + 127) / 128));

@@ -60,6 +60,9 @@ hipError_t launch_chol_catchup_mfma(const zc* const* chunks, int rows_per_chunk,
                                     long long pm, zc* proj, hipStream_t s);
 // out[i] = res[idx[i]] for i < m (residual entries the host pivot policy reads).
 hipError_t launch_chol_gather(const double* res, const long long* idx, int m, double* out, hipStream_t s);
+// pmax[(n2 + 255) / 256] = per-block max of D - sum_k |L_k|^2 (screened runs' final residual)
+hipError_t launch_chol_remeasure(const zc* const* chunks, int rows_per_chunk, long long n2, int rank,
+                                 const double* diag, double* pmax, hipStream_t s);
 hipError_t launch_chol_accept(const zc* column, const zc* pproj, int real, long long n, long long n2,
                               const CholState* st, double* res, zc* out_row, double psd_tol, int* flags, double* pmax,
                               long long* pidx, hipStream_t s);
@@ -138,6 +141,8 @@ int transform_max_o();
 // ints (quadrature work lists); it is reused by the next launch in stream order.
 constexpr int kBoysBins = 11;
 size_t seed_scratch_ints(long long total);
+// p[0..n) = 0 by a kernel on stream s.
+hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
                        const double* exps, const double* kappa, zc* seeds, DevError* err, int* scratch,
                        hipStream_t s, const double* radial = nullptr);

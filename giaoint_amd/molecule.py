@@ -35,9 +35,16 @@ _ELEMENTS = (
 _Z = {symbol: z for z, symbol in enumerate(_ELEMENTS, start=1)}
 
 
-def _float(value) -> float:
+_UNIT_SCALE = {"angstrom": 1.0 / BOHR_IN_ANGSTROM, "bohr": 1.0}
+_FORTRAN_EXPONENT = str.maketrans("Dd", "Ee")
+
+
+def _real(value) -> float:
+    """A float from a number or from numeric text (Fortran D exponents allowed)."""
     if isinstance(value, bool):
         raise TypeError("expected a number")
+    if isinstance(value, str):
+        value = value.translate(_FORTRAN_EXPONENT)
     return float(value)
 
 
@@ -61,47 +68,62 @@ def _element(value) -> str:
     raise ValueError(f"Unknown element {value!r}; use an element symbol or atomic number")
 
 
+def _atom_record(entry, scale: float):
+    """(symbol, position in bohr) of one (symbol, (x, y, z)) entry; the
+    coordinates are checked before the symbol (molecule.py:65-83)."""
+    symbol, coordinates = entry
+    position = tuple(scale * _real(c) for c in coordinates)
+    if len(position) != 3 or not all(map(math.isfinite, position)):
+        raise ValueError("coordinates must be three finite numbers")
+    return _element(symbol), position
+
+
 def _atoms_in_bohr(atoms, unit: str):
-    scale = {"angstrom": 1.0 / BOHR_IN_ANGSTROM, "bohr": 1.0}
-    if unit not in scale:
+    scale = _UNIT_SCALE.get(unit) if isinstance(unit, str) else None
+    if scale is None:
         raise ValueError("unit must be 'angstrom' or 'bohr'")
-    out = []
-    for number, atom in enumerate(atoms, start=1):
+    records = []
+    for number, entry in enumerate(atoms, start=1):
         try:
-            symbol, coordinates = atom
-            position = tuple(_float(x) * scale[unit] for x in coordinates)
-            if len(position) != 3 or not all(math.isfinite(x) for x in position):
-                raise ValueError("coordinates must be three finite numbers")
-            out.append((_element(symbol), position))
+            records.append(_atom_record(entry, scale))
         except (TypeError, ValueError, OverflowError) as error:
             raise ValueError(f"Invalid atom {number}: {error}") from error
-    if not out:
+    if not records:
         raise ValueError("A molecule must contain at least one atom")
-    return tuple(out)
+    return tuple(records)
+
+
+def _xyz_text(source) -> str:
+    """Text of an XYZ source: multiline text as given (a leading BOM removed),
+    anything else a path read as UTF-8 (with or without BOM)."""
+    if isinstance(source, str) and ("\n" in source or "\r" in source):
+        return source.removeprefix("\ufeff")
+    return Path(source).read_text(encoding="utf-8-sig")
 
 
 def _xyz_atoms(source):
-    """Atoms of one XYZ frame, from a path or from multiline text."""
-    if isinstance(source, PathLike) or ("\n" not in source and "\r" not in source):
-        text = Path(source).read_text(encoding="utf-8-sig")
-    else:
-        text = source.lstrip("\ufeff")
-    lines = text.lstrip("\r\n").splitlines()
+    """Atoms of one XYZ frame (molecule.py:86-112): a count line, a comment
+    line, the atom lines, and nothing but blank lines after them. Blank lines
+    before the count are ignored; line numbers in messages count from it."""
+    rows = _xyz_text(source).splitlines()
+    first = next((k for k, row in enumerate(rows) if row.strip()), len(rows))
+    header, body = rows[first:first + 1], rows[first + 1:]
     try:
-        count = int(lines[0].strip())
+        count = int(header[0])
     except (IndexError, ValueError) as error:
         raise ValueError("XYZ must start with an integer atom count") from error
     if count < 1:
         raise ValueError("XYZ atom count must be positive")
-    if len(lines) < count + 2:
+    frame, rest = body[1:count + 1], body[count + 1:]
+    if len(frame) < count:
         raise ValueError(f"XYZ requires a comment line and {count} atom lines")
-    if any(line.strip() for line in lines[count + 2:]):
+    if any(row.strip() for row in rest):
         raise ValueError("Expected one XYZ frame; extra atoms or frames follow the declared count")
     atoms = []
-    for row, line in enumerate(lines[2:count + 2], start=3):
-        fields = line.split()
+    for line_number, row in zip(range(3, count + 3), frame):
+        fields = row.split()
         if len(fields) != 4:
-            raise ValueError(f"XYZ line {row} must contain an element and three coordinates")
+            raise ValueError(f"XYZ line {line_number} must contain an element and three coordinates")
         atoms.append((fields[0], fields[1:]))
     return atoms
 
@@ -139,47 +161,67 @@ def _basis_json(source, basis_format):
     return data
 
 
+_SHELL_TYPES = frozenset({"gto", "gto_cartesian", "gto_spherical"})
+
+
+def _entry_shells(entry):
+    """(L, exponents, coefficient rows) of one BSE shell entry; a combined
+    shell (e.g. SP) gives one shell per angular momentum, each with its own
+    coefficient row, and a general contraction stays one shell."""
+    if entry.get("function_type", "gto") not in _SHELL_TYPES:
+        raise ValueError("only Gaussian orbital shells are supported")
+    momenta = entry["angular_momentum"]
+    if not momenta or any(type(l) is not int or l < 0 for l in momenta):
+        raise ValueError("angular momenta must be non-negative integers")
+    if len(set(momenta)) != len(momenta):
+        raise ValueError("combined shells must have distinct angular momenta")
+    exponents = [_real(x) for x in entry["exponents"]]
+    rows = [[_real(x) for x in row] for row in entry["coefficients"]]
+    if len(momenta) == 1:
+        return [(momenta[0], exponents, rows)]
+    if len(rows) != len(momenta):
+        raise ValueError("combined shells require one coefficient row per angular momentum")
+    return [(l, exponents, [row]) for l, row in zip(momenta, rows)]
+
+
+def _element_shells(elements, symbol):
+    """Shell specifications of one element in increasing L, stable within
+    equal L (the order of the basis file), molecule.py:191-267."""
+    element = elements.get(str(_Z[symbol]))
+    if not isinstance(element, dict):
+        raise ValueError(f"Basis has no data for element {symbol}")
+    if element.get("ecp_potentials") or element.get("ecp_electrons", 0):
+        raise ValueError(f"Basis for {symbol} contains an ECP; only all-electron bases are supported")
+    entries = element.get("electron_shells")
+    if not isinstance(entries, list) or not entries:
+        raise ValueError(f"Basis for {symbol} has no electron shells")
+    specs = []
+    for index, entry in enumerate(entries):
+        try:
+            specs.extend(_entry_shells(entry))
+        except (AttributeError, KeyError, TypeError, ValueError, OverflowError) as error:
+            raise ValueError(f"Invalid basis shell {index} for {symbol}: {error}") from error
+    specs.sort(key=lambda spec: spec[0])
+    return specs
+
+
 def _shells(atoms, data):
-    """Shells in atom order, then stable increasing L per atom; the owning atom of each."""
+    """The basis in atom order (each element's shells parsed once) and the
+    owning atom of every shell."""
     elements = data.get("elements")
     if not isinstance(elements, dict):
         raise ValueError("BSE basis data must contain an 'elements' object keyed by atomic number")
+    parsed = {}
     shells, owners = [], []
-    for index, (symbol, position) in enumerate(atoms):
-        element = elements.get(str(_Z[symbol]))
-        if not isinstance(element, dict):
-            raise ValueError(f"Basis has no data for element {symbol}")
-        if element.get("ecp_potentials") or element.get("ecp_electrons", 0):
-            raise ValueError(f"Basis for {symbol} contains an ECP; only all-electron bases are supported")
-        entries = element.get("electron_shells")
-        if not isinstance(entries, list) or not entries:
-            raise ValueError(f"Basis for {symbol} has no electron shells")
-        specs = []
-        for k, entry in enumerate(entries):
+    for owner, (symbol, position) in enumerate(atoms):
+        if symbol not in parsed:
+            parsed[symbol] = _element_shells(elements, symbol)
+        for l, exponents, rows in parsed[symbol]:
             try:
-                if entry.get("function_type", "gto") not in ("gto", "gto_cartesian", "gto_spherical"):
-                    raise ValueError("only Gaussian orbital shells are supported")
-                momenta = entry["angular_momentum"]
-                if not momenta or any(type(l) is not int or l < 0 for l in momenta):
-                    raise ValueError("angular momenta must be non-negative integers")
-                if len(set(momenta)) != len(momenta):
-                    raise ValueError("combined shells must have distinct angular momenta")
-                exps = [_float(x) for x in entry["exponents"]]
-                rows = [[_float(x) for x in row] for row in entry["coefficients"]]
-                if len(momenta) == 1:
-                    specs.append((momenta[0], exps, rows))
-                elif len(rows) == len(momenta):
-                    specs.extend((l, exps, [row]) for l, row in zip(momenta, rows))
-                else:
-                    raise ValueError("combined shells require one coefficient row per angular momentum")
-            except (AttributeError, KeyError, TypeError, ValueError, OverflowError) as error:
-                raise ValueError(f"Invalid basis shell {k} for {symbol}: {error}") from error
-        for l, exps, rows in sorted(specs, key=lambda spec: spec[0]):
-            try:
-                shells.append(Shell(position, l, exps, rows))
+                shells.append(Shell(position, l, exponents, rows))
             except (TypeError, ValueError, OverflowError) as error:
                 raise ValueError(f"Invalid basis shell for {symbol}: {error}") from error
-            owners.append(index)
+            owners.append(owner)
     return Basis(shells), tuple(owners)
 
 

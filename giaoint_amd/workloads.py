@@ -89,25 +89,17 @@ def _from_shells(shells) -> ShellArrays:
 # Water cc-pVDZ
 # ---------------------------------------------------------------------------
 WATER_ANGSTROM = [("O", (0.0, 0.0, 0.1173)), ("H", (0.0, 0.7572, -0.4692)), ("H", (0.0, -0.7572, -0.4692))]
-Z = {"H": 1, "O": 8}
 
 
 def bse_shells(atoms_bohr, data) -> list:
-    """Shells from BSE JSON data: atom order, then stable increasing L; a
-    general contraction stays one shell (molecule.py:191-267)."""
+    """Shells from BSE JSON data in the Molecule front end's order (atom
+    order, then stable increasing L; a general contraction stays one shell),
+    as (centre, L, exponents, raw coefficient rows) tuples."""
+    from .molecule import _element_shells
+
     shells = []
     for sym, pos in atoms_bohr:
-        element = data["elements"][str(Z[sym])]
-        specs = []
-        for entry in element["electron_shells"]:
-            ang = entry["angular_momentum"]
-            exps = [float(x) for x in entry["exponents"]]
-            rows = [[float(x) for x in r] for r in entry["coefficients"]]
-            if len(ang) == 1:
-                specs.append((ang[0], exps, rows))
-            else:
-                specs.extend((m, exps, [r]) for m, r in zip(ang, rows))
-        for l, exps, rows in sorted(specs, key=lambda x: x[0]):
+        for l, exps, rows in _element_shells(data["elements"], sym):
             shells.append((np.array(pos, float), int(l), np.array(exps), np.array(rows)))
     return shells
 

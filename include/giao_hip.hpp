@@ -95,6 +95,20 @@ inline std::vector<CartesianExponent> cartesian(std::uint16_t l) {
 
 }  // namespace hip_detail
 
+// primitive_normalization (types.hpp:141-142, types.cpp:60-72): the norm of
+// a Cartesian primitive x^ax y^ay z^az exp(-alpha r^2).
+[[nodiscard]] inline double primitive_normalization(double exponent, CartesianExponent angular) {
+    if (!std::isfinite(exponent) || exponent <= 0.0)
+        throw std::invalid_argument("Gaussian exponent must be finite and positive");
+    return giaohip_primitive_normalization(exponent, angular.x, angular.y, angular.z);
+}
+
+// cartesian_components (types.hpp:144-145, types.cpp:74-90): x-major order,
+// x descending, then y descending.
+[[nodiscard]] inline std::vector<CartesianExponent> cartesian_components(std::uint16_t total_angular_momentum) {
+    return hip_detail::cartesian(total_angular_momentum);
+}
+
 struct PrimitiveGaussian {
     double exponent{};
     double coefficient{1.0};
@@ -254,9 +268,28 @@ struct BoysDiagnostics {
 }
 
 // --- ERIs (eri.hpp:19-146) ---------------------------------------------------
+// Per-caller scratch of compute_eri (eri.hpp:19-46): a four-shell device
+// basis that each call refills in place (giaohip_basis_update). Its device
+// tables only grow, so once warm a call allocates no device memory, as the
+// reference's workspace allocates nothing after warm-up
+// (tests/cpp/test_overlap.cpp:387-401). Not shared between threads.
 class EriWorkspace {
   public:
+    EriWorkspace() = default;
+    EriWorkspace(const EriWorkspace&) = delete;
+    EriWorkspace& operator=(const EriWorkspace&) = delete;
+    ~EriWorkspace() {
+        if (basis_) giaohip_basis_destroy(hip_detail::context(), basis_);
+    }
     void prepare(std::array<std::size_t, 7>, std::size_t, std::array<std::size_t, 4> = {}) {}
+
+    // The scratch basis holding the four given shells.
+    giaohip_basis* load(giaohip_ctx* ctx, const std::array<const Shell*, 4>& shells);
+
+  private:
+    giaohip_basis* basis_ = nullptr;
+    std::vector<double> centers_, exps_, coefs_;
+    std::vector<int> L_, np_, nc_;
 };
 
 struct ShellQuartetIndex {
@@ -367,14 +400,41 @@ class EriSchwarzBounds {
     return primitive_eri(a, b, c, d, field);
 }
 
+inline giaohip_basis* EriWorkspace::load(giaohip_ctx* ctx, const std::array<const Shell*, 4>& shells) {
+    centers_.clear();
+    exps_.clear();
+    coefs_.clear();
+    L_.clear();
+    np_.clear();
+    nc_.clear();
+    for (const Shell* s : shells) {
+        centers_.insert(centers_.end(), {s->center().x, s->center().y, s->center().z});
+        L_.push_back(s->angular_momentum());
+        np_.push_back(static_cast<int>(s->primitive_count()));
+        nc_.push_back(static_cast<int>(s->contraction_count()));
+        exps_.insert(exps_.end(), s->exponents().begin(), s->exponents().end());
+        coefs_.insert(coefs_.end(), s->coefficients().begin(), s->coefficients().end());
+    }
+    if (!basis_) {
+        hip_detail::check(ctx, giaohip_basis_create(ctx, 4, centers_.data(), L_.data(), np_.data(), nc_.data(),
+                                                    exps_.data(), coefs_.data(), &basis_));
+    } else {
+        hip_detail::check(ctx, giaohip_basis_update(ctx, basis_, 4, centers_.data(), L_.data(), np_.data(),
+                                                    nc_.data(), exps_.data(), coefs_.data()));
+    }
+    return basis_;
+}
+
+// compute_eri (eri.cpp:454-585): the contracted block (ab|cd) into output,
+// through the workspace's scratch basis.
 inline void compute_eri(const Shell& a, const Shell& b, const Shell& c, const Shell& d, const MagneticField& field,
-                        std::span<Complex> output, EriWorkspace&) {
+                        std::span<Complex> output, EriWorkspace& workspace) {
     if (output.size() != shell_quartet_size(a, b, c, d)) throw std::length_error("ERI output has an incorrect size");
-    const Basis four({a, b, c, d});
     giaohip_ctx* ctx = hip_detail::context();
+    giaohip_basis* four = workspace.load(ctx, {&a, &b, &c, &d});
     const std::uint32_t q[4] = {0, 1, 2, 3};
     const auto f = hip_detail::field6(field.B, field.gauge_origin);
-    hip_detail::check(ctx, giaohip_eri_batch(ctx, four.device(), q, 1, f.data(), nullptr,
+    hip_detail::check(ctx, giaohip_eri_batch(ctx, four, q, 1, f.data(), nullptr,
                                              reinterpret_cast<double*>(output.data()), 0));
 }
 
@@ -483,14 +543,15 @@ inline void compute_eri_tensor(const Basis& basis, const MagneticField& field, s
                                               0));
 }
 
-// Point charge (types.hpp:134-139).
+// Point charge (types.hpp:134-139), validated as types.cpp:188-194 does.
 struct Nucleus {
     double charge{};
     Vec3 center{};
     Nucleus(double charge_in, Vec3 center_in) : charge(charge_in), center(center_in) {
-        if (!std::isfinite(charge) || !std::isfinite(center.x) || !std::isfinite(center.y) ||
-            !std::isfinite(center.z))
-            throw std::invalid_argument("nucleus charge and center must be finite");
+        if (!std::isfinite(charge) || charge <= 0.0)
+            throw std::invalid_argument("nuclear charge must be finite and positive");
+        if (!std::isfinite(center.x) || !std::isfinite(center.y) || !std::isfinite(center.z))
+            throw std::invalid_argument("nuclear center values must be finite");
     }
 };
 
@@ -512,9 +573,12 @@ inline void compute_nuclear_attraction_matrix(const Basis& basis, std::span<cons
                                                       reinterpret_cast<double*>(output.data())));
 }
 
-// Parallel evaluation is available (on the GPU), so thread counts above one are accepted.
+// Parallel evaluation is available (on the GPU), so thread counts above one
+// are accepted; the host threads the library plans with (eri.cpp:855-869).
 [[nodiscard]] inline bool openmp_enabled() noexcept { return true; }
-[[nodiscard]] inline std::size_t openmp_max_threads() noexcept { return 16; }
+[[nodiscard]] inline std::size_t openmp_max_threads() noexcept {
+    return static_cast<std::size_t>(giaohip_host_threads());
+}
 
 // --- selected Cholesky integrals (src/cholesky.hpp:9-17) --------------------
 namespace detail {

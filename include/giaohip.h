@@ -135,9 +135,26 @@ int giaohip_device_free(giaohip_ctx* ctx, void* ptr);
 int giaohip_memcpy_to_host(giaohip_ctx* ctx, void* dst, const void* src, size_t bytes);
 int giaohip_synchronize(giaohip_ctx* ctx);
 
+/* Host threads the library uses for its planning work (validation, work
+ * lists, host copies): the hardware threads, at most 16. Replaces
+ * openmp_max_threads (eri.cpp:863-869, decl. eri.hpp:145-146). */
+int giaohip_host_threads(void);
+/* Device allocations (hipMalloc calls) the library has made in this process
+ * so far, on every device. A diagnostic with no counterpart in the
+ * reference: it shows which calls run without allocating. */
+unsigned long long giaohip_device_allocations(void);
+
 /* --- basis (giao::Basis, types.hpp:108-124) ----------------------------- */
 int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, const int* L, const int* nprim,
                          const int* ncontr, const double* exps, const double* coefs, giaohip_basis** out);
+/* Replaces the shells of an existing basis (same layout as
+ * giaohip_basis_create). Its device tables only grow: shells that fit the
+ * tables of an earlier fill are uploaded without allocating device memory.
+ * This is what keeps compute_eri's warm path free of device allocations,
+ * as the reference's EriWorkspace keeps its own (eri.hpp:19-46,
+ * eri.cpp:388-412). On failure the basis is left with no shells. */
+int giaohip_basis_update(giaohip_ctx* ctx, giaohip_basis* basis, int nshell, const double* centers, const int* L,
+                         const int* nprim, const int* ncontr, const double* exps, const double* coefs);
 int giaohip_basis_destroy(giaohip_ctx* ctx, giaohip_basis* basis);
 /* Number of AOs (Basis::ao_count, types.hpp:118). */
 size_t giaohip_basis_ao_count(const giaohip_basis* basis);
@@ -248,6 +265,13 @@ int giaohip_cholesky_run(giaohip_ctx* ctx, const giaohip_basis* basis, const dou
  * (python/giao_integrals/cholesky.py:183-297) continued in steps; it has no
  * counterpart call in the reference. */
 int giaohip_cholesky_resume(giaohip_ctx* ctx, giaohip_cholesky* chol, uint64_t stop_after);
+/* The final residual of a run on a basis with Schwarz-screened diagonal
+ * blocks (cholesky_eri's screening_threshold, python/giao_integrals/
+ * cholesky.py:138-154,283-288): every diagonal block is measured again and
+ * max_residual = max(0, max over (mu,nu) of D[mu,nu] - sum_k |L_k[mu,nu]|^2),
+ * unclamped entries, pivots not zeroed; also what giaohip_cholesky_info
+ * reports afterwards. One pass over the factors. */
+int giaohip_cholesky_remeasure(giaohip_ctx* ctx, giaohip_cholesky* chol, double* max_residual);
 /* Rank, the measured largest residual diagonal, the stopping tolerance,
  * whether the stopping test was met, and device time split (ms). */
 int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* max_residual, double* tolerance,

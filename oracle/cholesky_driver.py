@@ -7,10 +7,15 @@ the C oracle's (or the reference-built library's) diagonal and columns this
 reproduces the reference's pivot policy: strict global argmax (first maximum
 in flat mu*n+nu order), the local-block override for complex generally
 contracted factors, the candidate sets and the bounded raw-column cache.
-Schwarz-screened diagonals (screening_threshold > 0) are not restated.
-Which pivot the general-contraction policy picks is parity unpinned (no
-reference output for it is available offline); its factorization is pinned
-by the dense factorization in the same pivot order.
+Schwarz-screened diagonals (screening_threshold > 0) follow
+cholesky.py:138-154,283-288: a shell pair whose squared Schwarz bound is at
+most min(threshold, screening_threshold) starts from that bound instead of
+its measured diagonal and is measured once the loop ends, when the final
+residual becomes D - sum |L|^2 (``bounds`` gives the squared bounds).
+The general-contraction policy is pinned by the reference's recorded runs on
+water cc-pVTZ (ranks 656 / 668 / 687 and their residuals,
+tests/test_oracle_ccpvtz.py); its factorization also by the dense
+factorization in the same pivot order.
 """
 
 from __future__ import annotations
@@ -22,7 +27,8 @@ import numpy as np
 
 def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_center: bool,
                  threshold: float = 1e-8, max_bytes=None, pivot_batch_size: int = 32, max_pivots=None,
-                 history=None, follow=None, tie_rtol: float = 1e-12, stats=None):
+                 history=None, follow=None, tie_rtol: float = 1e-12, stats=None, bounds=None,
+                 screening_threshold: float = 0.0):
     """shells: list of (cartesian_count, contraction_count, primitive_count).
 
     Returns (vectors (rank, n, n) complex128, pivots (rank, 2), max residual,
@@ -47,12 +53,28 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
     ns = len(shells)
     spans = [slice(int(o), int(o + a)) for o, a in zip(offsets, ao)]
     owners = np.repeat(np.arange(ns), ao)
-    pairs = [(a, b) for a in range(ns) for b in range(a, ns)]
-    measured = diagonal(pairs)
     diag = np.zeros((n, n))
-    for a, b in pairs:
-        diag[spans[a], spans[b]] = measured[spans[a], spans[b]]
-        diag[spans[b], spans[a]] = measured[spans[b], spans[a]]
+    skipped, measured_pairs = [], []
+    for a in range(ns):
+        for b in range(a, ns):
+            # cholesky.py:138-154: squared Schwarz bounds stand in for pairs
+            # already below min(threshold, screening_threshold)
+            if screening_threshold and bounds is not None and bounds[a, b] <= min(threshold, screening_threshold):
+                diag[spans[a], spans[b]] = bounds[a, b]
+                diag[spans[b], spans[a]] = bounds[a, b]
+                skipped.append((a, b))
+            else:
+                measured_pairs.append((a, b))
+
+    def read_diagonal(pairs):
+        if not pairs:
+            return
+        measured = diagonal(pairs)
+        for a, b in pairs:
+            diag[spans[a], spans[b]] = measured[spans[a], spans[b]]
+            diag[spans[b], spans[a]] = measured[spans[b], spans[a]]
+
+    read_diagonal(measured_pairs)
     residual = diag.ravel().copy()
     scale = float(np.max(residual, initial=0.0))
     floor = 64 * np.finfo(float).eps * scale
@@ -154,5 +176,12 @@ def cholesky_eri(ao_offsets, shells, diagonal, columns, *, real: bool, single_ce
         residual[pivot] = 0
     if follow is not None and len(follow) != len(pivots) and max_pivots is None:
         raise AssertionError(f"followed run has {len(follow)} pivots, this run stopped after {len(pivots)}")
+    if skipped:
+        # cholesky.py:283-288: measure the skipped pairs, then the final
+        # residual is D - sum |L|^2 (not clamped, pivots not zeroed)
+        read_diagonal(skipped)
+        residual = diag.ravel() - np.sum(np.abs(factors[: len(pivots)]) ** 2, axis=0)
+        if stats is not None:
+            stats["skipped_pairs"] = len(skipped)
     vectors = np.array(factors[: len(pivots)], dtype=np.complex128).reshape(len(pivots), n, n)
     return vectors, np.asarray(pivots, dtype=np.int64).reshape(-1, 2), float(np.max(residual, initial=0.0)), tolerance

@@ -20,8 +20,9 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIBS = {"port": HERE / "liboracle.so", "reference": HERE / "_ref" / "libgiao_ref.so"}
-PREFIX = {"port": "or_", "reference": "ref_"}
+LIBS = {"port": HERE / "liboracle.so", "reference": HERE / "_ref" / "libgiao_ref.so",
+        "reference_fma": HERE / "_ref" / "libgiao_ref_fma.so"}
+PREFIX = {"port": "or_", "reference": "ref_", "reference_fma": "ref_"}
 
 ERRORS = {1: ValueError, 2: ValueError, 3: IndexError, 4: ArithmeticError, 5: RuntimeError}
 ERROR_NAMES = {1: "invalid_argument", 2: "length_error", 3: "out_of_range", 4: "BoysNumericalError", 5: "runtime"}

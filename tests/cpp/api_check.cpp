@@ -6,6 +6,8 @@
 // Prints one "name value" line per result; exceptions print their type.
 #include "giao_hip.hpp"
 
+#include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -74,6 +76,19 @@ int main(int argc, char** argv) {
         const Vec3 kappa = MagneticField({0.1, -0.2, 0.3}, {1.0, 0.0, -1.0}).london_wave_vector({0.5, 2.0, -3.0});
         std::printf("kappa %.17g %.17g %.17g\n", kappa.x, kappa.y, kappa.z);
         expect("boys_without_gpu", [] { (void)boys_values({1.0, 0.5}, 4); });
+        expect("nucleus_zero_charge", [] { Nucleus(0.0, Vec3{}); });
+        expect("nucleus_negative_charge", [] { Nucleus(-1.0, Vec3{}); });
+        expect("nucleus_nan_center", [] { Nucleus(8.0, Vec3{0.0, std::nan(""), 0.0}); });
+        expect("nucleus_ok", [] { Nucleus(1.0, Vec3{0.0, 0.5, 0.0}); });
+        std::printf("primitive_normalization %.17g\n", primitive_normalization(0.7, CartesianExponent{2, 1, 0}));
+        expect("normalization_exponent", [] { (void)primitive_normalization(-1.0, CartesianExponent{}); });
+        {
+            const auto comps = cartesian_components(3);
+            std::printf("components %zu", comps.size());
+            for (const auto& c : comps) std::printf(" %u%u%u", c.x, c.y, c.z);
+            std::printf("\n");
+        }
+        std::printf("max_threads %zu\n", openmp_max_threads());
         return 0;
     }
     const Basis basis(read_shells(dir + "/shells.txt"));
@@ -137,6 +152,35 @@ int main(int argc, char** argv) {
     std::vector<Complex> col(n * n);
     detail::cholesky_columns(basis, std::span(&piv, 1), field, col, 4);
     dump(dir + "/column.bin", col);
+    // warm compute_eri: 1000 calls over changing shell quartets reuse the
+    // workspace's device basis, so no call allocates device memory
+    {
+        std::vector<Complex> warm(64 * 64 * 64);
+        std::size_t max_size = 0;
+        for (std::size_t a = 0; a < sh.size(); ++a) max_size = std::max(max_size, sh[a].ao_count());
+        warm.resize(max_size * max_size * max_size * max_size);
+        // first pass over every quartet of the basis grows the tables once
+        for (std::size_t i = 0; i < 256; ++i) {
+            const Shell &x = sh[i % 4], &y = sh[(i / 4) % 4], &z = sh[(i / 16) % 4], &w = sh[(i / 64) % 4];
+            compute_eri(x, y, z, w, field, std::span(warm.data(), shell_quartet_size(x, y, z, w)), ws);
+        }
+        const unsigned long long before = giaohip_device_allocations();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (std::size_t i = 0; i < 1000; ++i) {
+            const Shell &x = sh[(i * 7) % 4], &y = sh[(i * 3 + 1) % 4], &z = sh[(i / 3) % 4], &w = sh[(i / 5) % 4];
+            compute_eri(x, y, z, w, field, std::span(warm.data(), shell_quartet_size(x, y, z, w)), ws);
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        const unsigned long long after = giaohip_device_allocations();
+        // the last call's block, for the oracle comparison
+        const std::size_t i = 999;
+        std::printf("warm_last %zu %zu %zu %zu\n", (i * 7) % 4, (i * 3 + 1) % 4, (i / 3) % 4, (i / 5) % 4);
+        const Shell &x = sh[(i * 7) % 4], &y = sh[(i * 3 + 1) % 4], &z = sh[(i / 3) % 4], &w = sh[(i / 5) % 4];
+        dump(dir + "/warm_last.bin",
+             std::vector<Complex>(warm.begin(), warm.begin() + (long)shell_quartet_size(x, y, z, w)));
+        std::printf("warm_allocations %llu\n", after - before);
+        std::printf("warm_us_per_call %.3f\n", std::chrono::duration<double, std::micro>(t1 - t0).count() / 1000.0);
+    }
     // error mapping
     expect("bad_output_size", [&] {
         std::vector<Complex> small(3);

@@ -103,3 +103,30 @@ def test_reconstruct_convention():
     np.testing.assert_allclose(r.reconstruct(sl), want[0:2, 1:3, 0:3, 2:3], atol=1e-13)
     with pytest.raises(ValueError, match="four slices"):
         r.reconstruct((slice(None),) * 3)
+
+
+def test_nucleus_checks():
+    """Nucleus rejects what types.cpp:188-194 rejects, with its messages."""
+    for charge in (0.0, -1.0, float("nan"), float("inf")):
+        with pytest.raises(ValueError, match="nuclear charge must be finite and positive"):
+            core.Nucleus(charge, (0.0, 0.0, 0.0))
+    with pytest.raises(ValueError, match="nuclear center values must be finite"):
+        core.Nucleus(8.0, (0.0, float("nan"), 0.0))
+    with pytest.raises(ValueError, match="center must be a one-dimensional array with length 3"):
+        core.Nucleus(8.0, (0.0, 0.0))
+    nu = core.Nucleus(1, (0.0, 0.5, -1.0))
+    assert nu.charge == 1.0 and nu.center == (0.0, 0.5, -1.0)
+
+
+def test_nuclear_attraction_checks_out_before_any_work():
+    """A bad out= is rejected before the device is touched (bindings.cpp:554-573
+    validates the output first), so this needs no GPU."""
+    b = core.Basis([core.Shell((0, 0, 0), 1, [1.0], [1.0])])
+    with pytest.raises(ValueError, match="out has an incorrect shape"):
+        core.nuclear_attraction(b, [core.Nucleus(1.0, (0, 0, 0))], out=np.zeros((2, 2), np.complex128))
+    with pytest.raises(ValueError, match="out must have dtype complex128"):
+        core.nuclear_attraction(b, [core.Nucleus(1.0, (0, 0, 0))], out=np.zeros((3, 3)))
+
+
+def test_openmp_max_threads_reports_library_threads():
+    assert 1 <= core.openmp_max_threads() <= 16

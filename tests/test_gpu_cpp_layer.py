@@ -63,6 +63,12 @@ def test_cpp_layer_against_oracle(tmp_path, port):
     assert np.max(np.abs(d - port.cholesky_diagonal(b, pairs, f, 4).reshape(-1))) <= ATOL
     col = c128("column.bin")
     assert np.max(np.abs(col - port.cholesky_columns(b, [(1, 0)], f, 4).reshape(-1))) <= ATOL
+    # 1000 warm compute_eri calls on changing quartets: no device allocation
+    assert int(r["warm_allocations"]) == 0
+    q = tuple(int(x) for x in r["warm_last"].split())
+    want, _ = port.eri_batch(b, [q], f, 4)
+    assert np.max(np.abs(c128("warm_last.bin") - want)) <= ATOL
+    print(f"compute_eri warm: {float(r['warm_us_per_call']):.1f} us per call", flush=True)
     assert r["bad_output_size"] == "length_error ERI output has an incorrect size"
     assert r["bad_index"] == "out_of_range ERI shell-quartet index is out of range"
     assert r["null_consumer"] == "invalid_argument ERI quartet consumer must not be null"
