@@ -89,6 +89,7 @@ _SIGNATURES = {
     "giaohip_boys": ([_vp, _dp, C.c_size_t, C.c_int, C.c_int, _dp, _ip, _dp, _ip], C.c_int),
     "giaohip_eri_batch": ([_vp, _vp, _u32p, C.c_size_t, _dp, _u64p, _vp, C.c_int], C.c_int),
     "giaohip_eri_batch_size": ([_vp, _u32p, C.c_size_t, _u64p], C.c_int),
+    "giaohip_eri_batch_noalloc": ([_vp, _vp, _u32p, C.c_size_t, _dp, _u64p, _vp], C.c_int),
     "giaohip_eri_tensor": ([_vp, _vp, _dp, _vp, C.c_int], C.c_int),
     "giaohip_primitive_eri": ([_vp, _dp, _ip, _dp, _dp], C.c_int),
     "giaohip_nuclear_attraction": ([_vp, _vp, C.c_int, _dp, _dp, _dp, _vp], C.c_int),

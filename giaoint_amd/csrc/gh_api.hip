@@ -35,6 +35,14 @@ using gh::SeedQuartet;
 using gh::WorkItem;
 using gh::zc;
 
+// Device memory of giaohip_eri_batch_noalloc: one array on every device the
+// library runs on, held from the load of the code object until the process
+// ends. On the host the name is only a stand-in for the symbol: host code
+// uses the device address that hipGetSymbolAddress gives per device
+// (noalloc_scratch_init), never the name itself.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 // Device allocations the library has made in this process (giaohip_device_allocations).
@@ -45,6 +53,41 @@ hipError_t counted_malloc(void** p, size_t bytes) {
     if (e == hipSuccess) g_device_allocations.fetch_add(1, std::memory_order_relaxed);
     return e;
 }
+
+// Per-device address of `scratch`, looked up once per device.
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;
+
+// Looks up `scratch` on `device` (made current here) the first time; later
+// calls return the recorded address.
+hipError_t noalloc_scratch_init(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (ptr == nullptr) return hipErrorInvalidSymbol;
+    g_scratch_addr[device] = ptr;
+    *out = ptr;
+    return hipSuccess;
+}
+
+// Layout of the allocation-free scratch (bytes): wave vectors, the device
+// error record, the work-queue counters, seed-quartet descriptors, work
+// items, then the seeds and the Boys quadrature lists.
+constexpr size_t kNaKappa = 0, kNaKappaBytes = size_t(64) << 10;
+constexpr size_t kNaErr = kNaKappa + kNaKappaBytes;
+constexpr size_t kNaQueue = kNaErr + 128;
+constexpr size_t kNaSq = kNaErr + 4096, kNaSqBytes = size_t(1) << 20;
+constexpr size_t kNaItems = kNaSq + kNaSqBytes, kNaItemsBytes = size_t(4) << 20;
+constexpr size_t kNaSeeds = kNaItems + kNaItemsBytes;
+static_assert(kNaSeeds % 256 == 0 && kNaSeeds < kNoAllocScratchBytes, "scratch layout");
 
 struct DevBuffer {
     void* p = nullptr;
@@ -163,6 +206,7 @@ struct giaohip_ctx {
     int eri_queue = GIAOHIP_QUEUE_NONE;  // generic-kernel launch form (GIAOHIP_OPTION_ERI_QUEUE)
     int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
+    void* noalloc = nullptr;                 // device address of `scratch` on this context's device
 };
 
 // A device-resident Cholesky: factors, residual and every buffer a pivot step
@@ -432,7 +476,8 @@ int plain_plan(giaohip_ctx* ctx, const giaohip_basis* b, const Request& r, gh::C
     return GIAOHIP_OK;
 }
 
-// Fixed device memory of the plain path, used instead of the context's
+// Fixed device memory of the plain path: slices of the library's static
+// scratch array, used by giaohip_eri_batch_noalloc instead of the context's
 // buffers (which grow on demand). Capacities are in elements.
 struct FixedPlainMem {
     double* kappa;
@@ -1179,6 +1224,7 @@ int giaohip_create(int device, giaohip_ctx** out) {
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ctx->copy_ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = gh::upload_constants();
+    if (e == hipSuccess) e = noalloc_scratch_init(device, &ctx->noalloc);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         giaohip_destroy(ctx);
@@ -1685,6 +1731,39 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return st;
+}
+
+int giaohip_eri_batch_noalloc(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
+                              const double field[6], const uint64_t* offsets, double* out_device) {
+    if (!ctx || !b || !field || (n > 0 && (!quartets || !out_device))) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    ctx->stats.requested_quartets = n;
+    if (!ctx->noalloc) return set_error(ctx, GIAOHIP_DEVICE_ERROR, "allocation-free ERI scratch is not available");
+    std::vector<uint64_t> sizes(n);
+    if (const int vst = check_quartets(ctx, b, quartets, n, field, sizes)) return vst;
+    ctx->stats.computed_quartets = n;
+    if (n == 0) return GIAOHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    char* base = static_cast<char*>(ctx->noalloc);
+    FixedPlainMem mem{};
+    mem.kappa = reinterpret_cast<double*>(base + kNaKappa);
+    mem.kappa_cap = kNaKappaBytes / sizeof(double);
+    mem.err = reinterpret_cast<DevError*>(base + kNaErr);
+    mem.queue = reinterpret_cast<int*>(base + kNaQueue);
+    mem.sq = reinterpret_cast<SeedQuartet*>(base + kNaSq);
+    mem.sq_cap = kNaSqBytes / sizeof(SeedQuartet);
+    mem.items = reinterpret_cast<WorkItem*>(base + kNaItems);
+    mem.items_cap = kNaItemsBytes / sizeof(WorkItem);
+    mem.seeds = reinterpret_cast<zc*>(base + kNaSeeds);
+    mem.seeds_cap = (kNoAllocScratchBytes - kNaSeeds) / sizeof(zc);
+    std::vector<Request> reqs(n);
+    uint64_t run = 0;
+    for (size_t i = 0; i < n; ++i) {
+        reqs[i] = quartet_request(b, quartets, i, (long long)(offsets ? offsets[i] : run));
+        run += sizes[i];
+    }
+    return run_plain(ctx, b, field, reqs, reinterpret_cast<zc*>(out_device), &mem);
 }
 
 int giaohip_schwarz_bounds(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out) {

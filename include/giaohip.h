@@ -188,6 +188,22 @@ int giaohip_boys(giaohip_ctx* ctx, const double* z, size_t n, int nmax, int scal
 int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32_t* quartets, size_t n,
                       const double field[6], const uint64_t* offsets, double* out, int out_on_device);
 
+/* giaohip_eri_batch into caller-provided device memory without allocating:
+ * no counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same inputs, checks, errors and output
+ * layout as giaohip_eri_batch with out_on_device = 1; every integral runs on
+ * the plain FP64 transform kernel. Its work memory (seeds, descriptors,
+ * error record) is a 64 MiB array compiled into the library, one per device,
+ * resident until the process ends, and the last call's contents stay in it.
+ * Calls that overlap on one device (several contexts on one GPU) race on it;
+ * calls on one context are serialized. Limits: at most 8192 / 3 shells in the
+ * basis, and each shell quartet's seeds (16 * (L + 4) bytes per primitive
+ * quartet) within about 59 MiB; beyond them GIAOHIP_INVALID_ARGUMENT. Device
+ * memory of the basis and of the output is the caller's (allocated earlier). */
+int giaohip_eri_batch_noalloc(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32_t* quartets, size_t n,
+                              const double field[6], const uint64_t* offsets, double* out_device);
+
 /* Size in complex elements of the packed output of a quartet list. */
 int giaohip_eri_batch_size(const giaohip_basis* basis, const uint32_t* quartets, size_t n, uint64_t* total);
 
