@@ -68,8 +68,10 @@ def parse():
                     help="pivots of each Cholesky run the autotuner times")
     ap.add_argument("--high-l-gggg", type=int, default=50_000, help="(gg|gg) quartets of the high-l leg (0 skips)")
     ap.add_argument("--high-l-ffdd", type=int, default=200_000, help="(ff|dd) quartets of the high-l leg (0 skips)")
-    ap.add_argument("--mixed", type=int, default=200_000,
+    ap.add_argument("--mixed", type=int, default=2_000_000,
                     help="mixed-class leg: quartets of the l<=3, degree 1..8 workload (0 skips it)")
+    ap.add_argument("--no-cpu-legs", action="store_true",
+                    help="skip the CPU baselines of the other workloads (water tensor, mixed, high l, Cholesky)")
     return ap.parse_args()
 
 
@@ -77,78 +79,189 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_threads() -> int:
+def physical_cores() -> int:
+    """Physical cores available to this process, counted as the reference
+    counts them for its default thread budget (python/giao_integrals/
+    _threads.py:33-64): distinct (package, core) pairs over the CPUs of the
+    affinity mask; 1 when the topology cannot be read."""
     try:
-        return len(os.sched_getaffinity(0))
-    except AttributeError:
-        return os.cpu_count() or 1
+        cores = set()
+        for cpu in os.sched_getaffinity(0):
+            topo = Path(f"/sys/devices/system/cpu/cpu{cpu}/topology")
+            cores.add(((topo / "physical_package_id").read_text().strip(), (topo / "core_id").read_text().strip()))
+        return max(1, len(cores))
+    except (AttributeError, OSError):
+        return 1
 
 
-def cpu_baseline(batch, B, origin, target_s: float):
+def _checker():
     from oracle import pyoracle
 
     kind = "reference" if pyoracle.available("reference") else "port"
-    chk = pyoracle.load(kind)
-    basis = pyoracle.make_basis(batch.shells.shell_list()[: 4 * 64], chk)
-    threads = cpu_threads()
-    f = chk.field6(B, origin)
-    # calibrate on 2*threads quartets, then size the sample to ~target_s
-    n0 = min(2 * threads, 64)
-    t = time.perf_counter()
-    chk.eri_batch(basis, batch.quartets[:n0], f, threads=threads)
-    dt = time.perf_counter() - t
-    per = dt / n0
-    n = int(min(max(n0, target_s / max(per, 1e-9)), len(batch.quartets)))
-    n = max(threads, (n // threads) * threads)
-    sub = batch.shells.subset(np.arange(4 * n))
+    return kind, pyoracle.load(kind)
+
+
+def _time_quartets(chk, batch, rows, threads: int) -> float:
+    """Seconds the checker takes for the quartets `rows` of a batch (each
+    quartet has four private shells), one OpenMP call."""
+    from oracle import pyoracle
+
+    rows = np.asarray(rows, dtype=np.int64)
+    sub = batch.shells.subset(np.concatenate([batch.quartets[i] for i in rows]).astype(np.int64))
     basis = pyoracle.make_basis(sub.shell_list(), chk)
-    q = np.arange(4 * n, dtype=np.uint32).reshape(n, 4)
+    q = np.arange(4 * len(rows), dtype=np.uint32).reshape(-1, 4)
+    f = chk.field6(batch.B, batch.origin)
     t = time.perf_counter()
     chk.eri_batch(basis, q, f, threads=threads)
-    dt = time.perf_counter() - t
+    return time.perf_counter() - t
+
+
+def _sized_sample(chk, batch, rows, threads: int, target_s: float):
+    """Times a calibration prefix of `rows`, then the longest prefix expected
+    to take about target_s; returns (rows timed, seconds)."""
+    n0 = min(len(rows), max(threads, 16))
+    dt0 = _time_quartets(chk, batch, rows[:n0], threads)
+    n = int(min(len(rows), max(n0, target_s / max(dt0 / n0, 1e-9))))
+    n = max(min(n0, n), (n // threads) * threads)
+    return n, _time_quartets(chk, batch, rows[:n], threads)
+
+
+def cpu_baseline(batch, target_s: float):
+    """The headline (dd|dd) batch on the host cores: the reference's own
+    OpenMP path (oracle/_ref, kind "reference"), or the C restatement when
+    that library is absent (kind "port"), on a prefix of the same batch."""
+    kind, chk = _checker()
+    threads = physical_cores()
+    n, dt = _sized_sample(chk, batch, np.arange(len(batch.quartets)), threads, target_s)
     return {
         "value": n / dt,
         "unit": "shell quartets/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"first {n} of the same (dd|dd) batch, OpenMP schedule(dynamic,1), {dt:.1f} s",
+        "sample": f"first {n} of the same (dd|dd) batch, OpenMP schedule(dynamic,1) on {threads} physical cores, "
+                  f"{dt:.1f} s",
     }
 
 
-def mixed_leg(n: int, tune: bool = True):
+def cpu_baselines(mixed_batch, target_s: float) -> dict:
+    """The other workloads on the host cores, each on a bounded sample
+    (rates are per sample, labelled as such):
+
+    * water cc-pVDZ, full 25^4 tensor at B = (0,0,0.1): ms per tensor;
+    * mixed classes: a stratified sample (every k-th quartet of the 2M batch);
+    * (gg|gg): one quartet on one core (about 40 s of the reference's CPU
+      time each); (ff|dd): a prefix of the 200k batch on every core;
+    * 30-atom Cholesky: the diagonal and the first 32 pivots of the checker
+      driver (oracle/cholesky_driver.py, the reference's driver restated, over
+      the checker's integrals), seconds per pivot, and the full run's time
+      extrapolated from it (labelled as an extrapolation)."""
+    from giaoint_amd import workloads
+    from oracle import cholesky_driver, pyoracle
+
+    kind, chk = _checker()
+    threads = physical_cores()
+    out = {}
+    # water cc-pVDZ full tensor
+    sh, B, O = workloads.water_ccpvdz()
+    basis = pyoracle.make_basis(sh.shell_list(), chk)
+    f = chk.field6(B, O)
+    reps, t = 0, time.perf_counter()
+    while reps < 3 or (time.perf_counter() - t < 2.0 and reps < 50):
+        chk.eri_tensor(basis, f, threads)
+        reps += 1
+    dt = (time.perf_counter() - t) / reps
+    out["water_ccpvdz_tensor"] = {"value": dt * 1e3, "unit": "ms per 25^4 tensor", "cores": threads, "kind": kind,
+                                  "sample": f"{reps} full tensors (637 canonical quartets each)"}
+    log(f"cpu: water tensor {dt * 1e3:.1f} ms")
+    # mixed classes: stratified sample
+    if mixed_batch is not None:
+        stride = max(1, len(mixed_batch.quartets) // 4000)
+        rows = np.arange(0, len(mixed_batch.quartets), stride)
+        n, dt = _sized_sample(chk, mixed_batch, rows, threads, target_s)
+        out["mixed_class"] = {"value": n / dt, "unit": "shell quartets/s", "cores": threads, "kind": kind,
+                              "sample": f"{n} quartets, every {stride}th of the {len(mixed_batch.quartets)}-quartet "
+                                        f"mixed batch, {dt:.1f} s"}
+        log(f"cpu: mixed {n / dt:.1f} quartets/s")
+    # high l
+    g, fd = workloads.precision_batches(n_gggg=1, n_ffdd=2000)
+    dt = _time_quartets(chk, g, [0], 1)
+    out["gggg"] = {"value": 1.0 / dt, "unit": "shell quartets/s", "cores": 1, "kind": kind,
+                   "sample": f"1 (gg|gg) K=2^4 quartet on one core, {dt:.1f} s"}
+    log(f"cpu: gggg {dt:.1f} s per quartet")
+    n, dt = _sized_sample(chk, fd, np.arange(len(fd.quartets)), threads, target_s / 2)
+    out["ffdd"] = {"value": n / dt, "unit": "shell quartets/s", "cores": threads, "kind": kind,
+                   "sample": f"first {n} (ff|dd) K=2^4 quartets, {dt:.1f} s"}
+    # 30-atom Cholesky prefix
+    sh, B, O = workloads.cholesky_atoms()
+    basis = pyoracle.make_basis(sh.shell_list(), chk)
+    f = chk.field6(B, O)
+    desc = [((l + 1) * (l + 2) // 2, 1, 1) for l in basis.L]
+    t = time.perf_counter()
+    chk.cholesky_diagonal(basis, [(a, c) for a in range(len(desc)) for c in range(a, len(desc))], f, threads)
+    diag_s = time.perf_counter() - t
+    t = time.perf_counter()
+    _, piv, _, _ = cholesky_driver.cholesky_eri(
+        basis.ao_offsets, desc, lambda pairs: chk.cholesky_diagonal(basis, pairs, f, threads),
+        lambda pv: chk.cholesky_columns(basis, pv, f, threads), real=False, single_center=False, threshold=1e-8,
+        max_pivots=32)
+    run_s = time.perf_counter() - t
+    per_pivot = (run_s - diag_s) / len(piv)
+    out["cholesky_30_atoms"] = {
+        "value": per_pivot, "unit": "s per pivot", "cores": threads, "kind": kind,
+        "sample": f"diagonal ({diag_s:.2f} s) and the first {len(piv)} pivots of the checker driver over the "
+                  f"checker's integrals, {run_s:.1f} s",
+        "extrapolated_full_run_s": diag_s + per_pivot * 6348,
+        "extrapolation_note": "per-pivot time of the first 32 pivots x rank 6348; the projection term grows with "
+                              "the rank, so the full run would take longer",
+    }
+    log(f"cpu: cholesky {per_pivot:.3f} s per pivot")
+    return out
+
+
+def mixed_leg(n: int, tune: bool = True, tune_n: int = 200_000):
     """Mixed classes l <= 3 (weights (1 + sum l)^-2), contraction degree U{1..8}:
-    the generator of the 2M-quartet workload at n quartets. One warmup call,
-    one timed call, inputs and output resident on the device. With `tune`,
-    the launch form of the generic kernel is autotuned on the same batch
-    first (per-bucket grids, per-bucket work queues, one persistent launch)."""
+    the 2M-quartet workload. With `tune`, the launch form of the generic
+    kernel is autotuned on its first tune_n quartets (per-bucket grids,
+    per-bucket work queues, one persistent launch); then one warmup call and
+    one timed call of the batched variant (class-specialized kernels where
+    they exist) and of the plain variant (the generic kernel for every
+    class), inputs and output resident on the device. Returns the leg's
+    record and the batch."""
     from giaoint_amd import _native, autotune, workloads
 
+    t0 = time.perf_counter()
     batch = workloads.mixed_batch(n)
+    log(f"mixed: {n} quartets generated in {time.perf_counter() - t0:.1f} s")
     sh = batch.shells
     coefs = _native.normalize_shells(sh.L, sh.nprim, sh.ncontr, sh.exps, sh.coefs)
     ctx = _native.Context(0)
     db = _native.DeviceBasis(ctx, sh.centers, sh.L, sh.nprim, sh.ncontr, sh.exps, coefs)
     total = int(db.quartet_sizes(batch.quartets).sum())
-    out = ctx.alloc(total * 16)
     f = _native.field6(batch.B, batch.origin)
     flops = workloads.batch_flops(sh, batch.quartets)
     tuned = None
     if tune:
-        ctx.free(out)  # the tuner holds its own output buffer
-        tuned = autotune.tune_eri(db, batch.quartets, f, knobs=("eri_queue",), reps=1,
-                                  workload=f"mixed l<=3 degree 1..8, {n} quartets")
+        m = min(n, tune_n)
+        tuned = autotune.tune_eri(db, batch.quartets[:m], f, knobs=("eri_queue",), reps=1,
+                                  workload=f"mixed l<=3 degree 1..8, first {m} of {n} quartets")
         log(f"mixed: autotuned {tuned.values} " +
             " ".join(f"{t.value}={t.seconds:.3f}s" for t in tuned.trials))
-        out = ctx.alloc(total * 16)
-    db.eri_batch(batch.quartets, f, device_out=out)
-    ctx.synchronize()
-    t = time.perf_counter()
-    db.eri_batch(batch.quartets, f, device_out=out)
-    ctx.synchronize()
-    dt = time.perf_counter() - t
-    st = ctx.stats()
-    ctx.free(out)
-    log(f"mixed: {n} quartets in {dt:.2f} s")
+    out = ctx.alloc(total * 16)
+    timings = {}
+    try:
+        for variant in ("batched", "plain"):
+            ctx.set_variant(variant)
+            db.eri_batch(batch.quartets[: min(n, 20_000)], f, device_out=out)
+            ctx.synchronize()
+            t = time.perf_counter()
+            db.eri_batch(batch.quartets, f, device_out=out)
+            ctx.synchronize()
+            timings[variant] = (time.perf_counter() - t, ctx.stats())
+            log(f"mixed ({variant}): {n} quartets in {timings[variant][0]:.2f} s")
+    finally:
+        ctx.set_variant("batched")
+        ctx.free(out)
+    dt, st = timings["batched"]
     extra = {"autotune": tuned.as_dict()} if tuned else {}
     return {**extra,
         "workload": f"{n} quartets, classes l<=3 weighted (1+sum l)^-2, degree U{{1..8}}, seed 3, "
@@ -157,7 +270,10 @@ def mixed_leg(n: int, tune: bool = True):
         "seconds": dt,
         "tflops_convention": flops / dt / 1e12,
         "launches": st["launches"],
-    }
+        "plain_kernel": {"quartets_per_s": n / timings["plain"][0], "seconds": timings["plain"][0],
+                         "tflops_convention": flops / timings["plain"][0] / 1e12},
+        "speedup_batched_vs_plain": timings["plain"][0] / dt,
+    }, batch
 
 
 def high_l_leg(n_gggg: int, n_ffdd: int):
@@ -371,7 +487,12 @@ def main():
     ctx.to_host(out, probe)
     # dominant kernel: the E.R.E contraction, timed with events on the stream it runs on
     avg_launch_ms = contract_ms / contract_launches
-    achieved = contract_flops / contract_launches / (avg_launch_ms * 1e-3) / 1e12
+    # the contraction's own work: 8 FLOP per complex MAC over the nonzero
+    # Hermite products (the executed count without the R0 part, which runs in
+    # r0_kernel), a count the kernel cannot exceed at the FP64 peak
+    contract_useful = executed - (flops - contract_flops)
+    achieved = contract_useful / contract_launches / (avg_launch_ms * 1e-3) / 1e12
+    convention_rate = contract_flops / contract_launches / (avg_launch_ms * 1e-3) / 1e12
     fast_waves = args.fast_waves if args.fast_waves >= 0 else (tuned.values["fast_waves"] if tuned else 0)
     waves_used = 4 if fast_waves == 4 else 5  # 9 bra pairs x 35 Hermite rows (workload dddd K=3)
     traffic = None
@@ -414,29 +535,36 @@ def main():
         "kernel_ms": {"seed": seed_ms, "pair_and_r0": produce_ms, "contract": contract_ms,
                       "launches_per_step": launches, "contract_launches_per_step": contract_launches},
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
-            "kernel": "gh::fast::fast_transform_kernel<2,2,2,2> (E.R.E contraction)",
+            "kernel": "gh::fast::fast_transform_kernel<2,2,2,2> (E.R.E contraction, FP64 VALU v_fmac_f64)",
             "avg_launch_ms": avg_launch_ms,
-            "flops_per_launch": contract_flops / contract_launches,
-            "flop_convention": "8 per complex MAC of the cheaper dense Hermite-to-Cartesian contraction (per quartet "
-                               f"{contract_flops / n:.4g}); the R0 part (12 per entry, {(flops - contract_flops) / n:.4g}"
-                               " per quartet) runs in r0_kernel",
-            "whole_path_tflops": flops / (ms_step * 1e-3) / 1e12,
-            "executed_tflops": (executed - (flops - contract_flops)) / (contract_ms * 1e-3) / 1e12,
-            "executed_note": "complex MACs the contraction executes (structural zeros of the Hermite products "
-                             "skipped), 8 FLOP each",
+            "flops_per_launch": contract_useful / contract_launches,
+            "flop_count": "8 per complex MAC over the nonzero complex Hermite products: every bra row (primitive "
+                          "pair, Hermite index) meets every nonzero ket product of every ket pair, then every "
+                          f"nonzero bra product meets every ket component ({contract_useful / n:.4g} per quartet); "
+                          "the kernel must execute these, so achieved <= peak",
+            "convention_tflops": convention_rate,
+            "convention_note": "the dense-contraction convention (8 per complex MAC of the cheaper dense "
+                               f"Hermite-to-Cartesian contraction, {contract_flops / n:.4g} per quartet) counts "
+                               "structural zeros of the Hermite products that the kernel skips",
+            "whole_path": {
+                "executed_tflops": executed / (ms_step * 1e-3) / 1e12,
+                "convention_tflops": flops / (ms_step * 1e-3) / 1e12,
+                "note": "per step: the contraction's MACs plus the R0 recurrences (12 FLOP per entry, "
+                        f"{(flops - contract_flops) / n:.4g} per quartet) over the step time",
+            },
             "fp64_issued_tflops": issued,
             "fp64_issued_frac": None if issued is None else issued / FP64_PEAK_TFLOPS,
-            "fp64_issued_note": "FP64 operations the kernel issues (rocprofv3 SQ_INSTS_VALU_*_F64 per quartet) per "
-                                "average launch: the FP64 pipe's utilisation. The convention counts the dense "
-                                "contraction, whose structural zeros the kernel skips, so `frac` can exceed 1",
+            "fp64_issued_note": "profile-derived constant: FP64 operations the kernel issues per quartet "
+                                "(rocprofv3 SQ_INSTS_VALU_*_F64 counters, profiles/fp64_mfma_lds_counters.md) "
+                                "over the live average launch time; includes padding of plan groups to 16 entries",
             "waves_per_workgroup": waves_used,
-            "traffic_note": None if traffic is None else TRANSFORM_TRAFFIC_NOTE.format(
+            "traffic_note": None if traffic is None else "profile-derived constant: " + TRANSFORM_TRAFFIC_NOTE.format(
                 b=TRANSFORM_HBM_BYTES_PER_QUARTET[waves_used], w=waves_used),
         },
         "checksum": float(np.abs(probe).sum()),
@@ -446,19 +574,32 @@ def main():
     result["fast_waves"] = fast_waves
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = cpu_baseline(batch, batch.B, batch.origin, args.cpu_seconds)
+            result["cpu_baseline"] = cpu_baseline(batch, args.cpu_seconds)
             result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
         except Exception as exc:  # the baseline is reported, never required
             result["cpu_baseline"] = {"error": str(exc)}
     ctx.free(out)
     del dbasis, ctx
+    mixed = None
     if args.mixed > 0:
-        result["mixed_class"] = mixed_leg(args.mixed, tune=not args.no_autotune)
+        result["mixed_class"], mixed = mixed_leg(args.mixed, tune=not args.no_autotune)
     if args.high_l_gggg > 0 or args.high_l_ffdd > 0:
         result["high_l"] = high_l_leg(args.high_l_gggg, args.high_l_ffdd)
     if args.cholesky_rank >= 0:
         result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block,
                                           0 if args.no_autotune else args.tune_cholesky_rank)
+    if not args.no_cpu and not args.no_cpu_legs:
+        try:
+            result["cpu_baselines"] = cpu_baselines(mixed, args.cpu_seconds)
+            legs = result["cpu_baselines"]
+            gpu_rates = {"mixed_class": result.get("mixed_class", {}).get("quartets_per_s"),
+                         "gggg": result.get("high_l", {}).get("gggg", {}).get("quartets_per_s"),
+                         "ffdd": result.get("high_l", {}).get("ffdd", {}).get("quartets_per_s")}
+            for name, rate in gpu_rates.items():
+                if rate and name in legs:
+                    legs[name]["gpu_over_cpu"] = rate / legs[name]["value"]
+        except Exception as exc:  # the baselines are reported, never required
+            result["cpu_baselines"] = {"error": str(exc)}
     print(json.dumps(result), flush=True)
 
 

@@ -195,12 +195,8 @@ def random_quartets(classes, degrees, *, seed, box=6.0, log_exp=(-1.0, 4.0), B=N
                 )
                 bad.extend(sel[s][~ok].tolist())
         todo = np.array(sorted(bad), dtype=np.int64)
-    coefs_list = []
-    exps_list = []
-    for i in range(n):
-        for j in range(4):
-            d = degrees[i, j]
-            exps_list.append(exps[i, j, :d])
+    # the first degrees[i, j] exponents of every shell, shell by shell
+    used = np.arange(kmax)[None, None, :] < degrees[:, :, None]
     coef_rng = np.random.Generator(np.random.PCG64(seed + 1000))
     flat_deg = degrees.reshape(-1)
     allc = coef_rng.normal(size=int(flat_deg.sum()))
@@ -209,12 +205,11 @@ def random_quartets(classes, degrees, *, seed, box=6.0, log_exp=(-1.0, 4.0), B=N
         classes.reshape(-1).copy(),
         flat_deg.copy(),
         np.ones(4 * n, dtype=np.int32),
-        np.concatenate(exps_list) if n else np.zeros(0),
+        exps[used] if n else np.zeros(0),
         allc,
     )
     # guard against all-zero contractions (N(0,1) draws are never exactly 0)
     quartets = np.arange(4 * n, dtype=np.uint32).reshape(n, 4)
-    del coefs_list
     return QuartetBatch(shells, quartets, tuple(B), (0.0, 0.0, 0.0), seed, name)
 
 
