@@ -235,6 +235,14 @@ class Context:
         (0 = library default, else 4..8); it sets the bra-pair tile per pass."""
         self.check(self.lib.giaohip_set_option(self.handle, 6, int(waves)))
 
+    MFMA_MODES = {"off": 0, "default": 1, "all": 2}
+
+    def set_mfma(self, mode: str):
+        """Classes on the FP64 matrix cores under the batched variant: "off",
+        "default" (classes without a class-specialized VALU kernel) or "all"
+        (every class with an MFMA instance)."""
+        self.check(self.lib.giaohip_set_option(self.handle, 7, self.MFMA_MODES[mode]))
+
     def stats(self) -> dict:
         s = Stats()
         self.check(self.lib.giaohip_last_stats(self.handle, C.byref(s)))

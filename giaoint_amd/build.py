@@ -31,6 +31,9 @@ SOURCES = [
     "gh_fast_pppp.hip",
     "gh_fast_psps.hip",
     "gh_fast_ssss.hip",
+    "gh_mfma_a.hip",
+    "gh_mfma_b.hip",
+    "gh_mfma_c.hip",
 ]
 
 HIPCC_FLAGS = [

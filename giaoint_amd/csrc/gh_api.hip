@@ -205,6 +205,7 @@ struct giaohip_ctx {
     size_t staging_budget_bytes = size_t(1) << 30;
     int eri_queue = GIAOHIP_QUEUE_NONE;  // generic-kernel launch form (GIAOHIP_OPTION_ERI_QUEUE)
     int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
+    int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     void* noalloc = nullptr;                 // device address of `scratch` on this context's device
 };
@@ -476,6 +477,47 @@ int plain_plan(giaohip_ctx* ctx, const giaohip_basis* b, const Request& r, gh::C
     return GIAOHIP_OK;
 }
 
+// FP64-MFMA instance of a request, with the request turned into its (cd|ab)
+// image when that image's bra side is the larger one (the kernels are
+// instantiated for lab >= lcd): returns lab * 16 + lcd of the instance, or -1.
+// A request restricted to part of its ket components is not turned.
+// Under GIAOHIP_MFMA_DEFAULT only instances with both momenta >= kMfmaDefaultMinL
+// are used: below that the measured FP64 VALU kernels are faster (the dense
+// products carry the structural zeros of the Hermite tables).
+constexpr int kMfmaDefaultMinL = 7;
+
+int mfma_orient(const giaohip_basis* b, Request& r, int mode) {
+    const DevShell *S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
+    const int lab = S[0]->l + S[1]->l, lcd = S[2]->l + S[3]->l;
+    if (mode == GIAOHIP_MFMA_OFF) return -1;
+    if (mode == GIAOHIP_MFMA_DEFAULT && std::min(lab, lcd) < kMfmaDefaultMinL) return -1;
+    const bool keep = lab >= lcd && gh::mfma_instance(lab, lcd);
+    const bool turn = r.cd1 < 0 && lcd >= lab && gh::mfma_instance(lcd, lab);
+    if (!keep && !turn) return -1;
+    bool do_turn = turn && !keep;
+    if (keep && turn) {
+        // equal momenta: the ket with fewer components and primitive pairs costs less
+        const double kab = (double)S[0]->nprim * S[1]->nprim, kcd = (double)S[2]->nprim * S[3]->nprim;
+        const double nab = (double)S[0]->ncart * S[1]->ncart, ncd = (double)S[2]->ncart * S[3]->ncart;
+        const double nh = (double)gh::nherm(lab);
+        const double as_is = kab * kcd * nh * nh * ncd + kab * nab * nh * ncd;
+        const double turned = kab * kcd * nh * nh * nab + kcd * ncd * nh * nab;
+        do_turn = turned < as_is;
+    }
+    if (do_turn) {
+        Request t = r;
+        const int map[4] = {2, 3, 0, 1};
+        for (int k = 0; k < 4; ++k) {
+            t.sh[k] = r.sh[map[k]];
+            t.xfix[k] = r.xfix[map[k]];
+            t.stride[k] = r.stride[map[k]];
+        }
+        r = t;
+        return lcd * 16 + lab;
+    }
+    return lab * 16 + lcd;
+}
+
 // Fixed device memory of the plain path: slices of the library's static
 // scratch array, used by giaohip_eri_batch_noalloc instead of the context's
 // buffers (which grow on demand). Capacities are in elements.
@@ -499,9 +541,20 @@ size_t seed_scratch_zc(long long pq_total) {
 // Runs requests with the plain per-class transform kernel. With `fixed`, all
 // device memory comes from its slices (no allocation); chunks are then also
 // bounded by the slices' capacities.
-int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
-              zc* d_out, const FixedPlainMem* fixed = nullptr) {
-    if (reqs.empty()) return GIAOHIP_OK;
+int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs_in,
+              zc* d_out, const FixedPlainMem* fixed = nullptr, bool mfma = false) {
+    if (reqs_in.empty()) return GIAOHIP_OK;
+    // requests with an FP64-MFMA instance, oriented for it (mfk: instance key or -1)
+    std::vector<Request> oriented;
+    std::vector<int> mfk;
+    if (mfma && !fixed) {
+        oriented = reqs_in;
+        mfk.assign(oriented.size(), -1);
+        parallel_for(oriented.size(), [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) mfk[i] = mfma_orient(b, oriented[i], ctx->mfma_mode);
+        });
+    }
+    const std::vector<Request>& reqs = mfk.empty() ? reqs_in : oriented;
     hipStream_t s = ctx->stream;
     const std::vector<double> kap = wave_vectors(b, field);
     double* d_kappa;
@@ -596,12 +649,45 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             double cost;
         };
         std::vector<Planned> planned;
+        std::map<int, std::vector<Planned>> mf_planned;  // FP64-MFMA items per instance
         for (size_t i = begin; i < end; ++i) {
             const Request& r = reqs[i];
             const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
             const int ncd_all = S[2]->ncart * S[3]->ncart;
             const int cd0 = r.cd1 < 0 ? 0 : r.cd0, cd1 = r.cd1 < 0 ? ncd_all : r.cd1;
             const int NHab = gh::nherm(S[0]->l + S[1]->l);
+            if (!mfk.empty() && mfk[i] >= 0) {
+                const int key = mfk[i], tile_w = gh::mfma_cd_tile(key >> 4);
+                const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
+                int lo[4], hi[4];
+                for (int k = 0; k < 4; ++k) {
+                    lo[k] = r.xfix[k] < 0 ? 0 : r.xfix[k];
+                    hi[k] = r.xfix[k] < 0 ? S[k]->ncontr : r.xfix[k] + 1;
+                }
+                for (int xa = lo[0]; xa < hi[0]; ++xa)
+                    for (int xb = lo[1]; xb < hi[1]; ++xb)
+                        for (int xc = lo[2]; xc < hi[2]; ++xc)
+                            for (int xd = lo[3]; xd < hi[3]; ++xd)
+                                for (int t0 = cd0; t0 < cd1; t0 += tile_w) {
+                                    Planned pl{};
+                                    WorkItem& w = pl.w;
+                                    for (int k = 0; k < 4; ++k) w.sh[k] = r.sh[k];
+                                    w.x[0] = xa;
+                                    w.x[1] = xb;
+                                    w.x[2] = xc;
+                                    w.x[3] = xd;
+                                    w.cd0 = t0;
+                                    w.cd1 = std::min(t0 + tile_w, cd1);
+                                    w.seed_off = sq[i - begin].seed_off;
+                                    w.out_base = r.out_base;
+                                    for (int k = 0; k < 4; ++k) w.stride[k] = r.stride[k];
+                                    w.cls = 0;
+                                    // matrix-core work is about a tenth of the generic kernel's per MAC
+                                    pl.cost = 0.1 * kq * NHab * gh::nherm(S[2]->l + S[3]->l) * tile_w;
+                                    mf_planned[key].push_back(pl);
+                                }
+                continue;
+            }
             gh::ClassArgs ca;
             int lds;
             long long nit;
@@ -646,6 +732,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             int lds;
             size_t first, count;
             double cost;
+            int mf = -1;  // FP64-MFMA instance key, or -1 for the generic kernel
         };
         std::vector<MixedLaunch> launches;
         for (size_t bk = 0; bk < std::size(kBuckets); ++bk) {
@@ -679,6 +766,14 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                 all.push_back(pl->w);
             }
             launches.assign(1, one);
+        }
+        for (auto& [key, list] : mf_planned) {
+            std::stable_sort(list.begin(), list.end(),
+                             [](const Planned& x, const Planned& y) { return x.cost > y.cost; });
+            double cost = 0.0;
+            for (const Planned& pl : list) cost += pl.cost;
+            launches.push_back({0, all.size(), list.size(), cost, key});
+            for (const Planned& pl : list) all.push_back(pl.w);
         }
         int* d_queue = nullptr;
         if (ctx->eri_queue != GIAOHIP_QUEUE_NONE) {
@@ -718,6 +813,14 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const int side = queued[1] < queued[0] ? 1 : 0;
             queued[side] += L.cost;
             hipStream_t ls = side ? ctx->aux : s;
+            if (L.mf >= 0) {
+                HIP_TRY(gh::launch_mfma(L.mf >> 4, L.mf & 15, d_items + L.first, (int)L.count,
+                                        b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                                        b->d_norms.as<double>(), d_kappa, d_seeds, d_out, ls),
+                        "MFMA transform kernel launch");
+                ctx->stats.launches += 1;
+                continue;
+            }
             HIP_TRY(gh::launch_transform_mixed(L.lds, d_items + L.first, (int)L.count, b->d_shells.as<DevShell>(),
                                                b->d_exps.as<double>(), b->d_coefs.as<double>(),
                                                b->d_norms.as<double>(), d_kappa, d_seeds, d_out,
@@ -1060,8 +1163,15 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
     const bool batched = ctx->eri_variant == GIAOHIP_ERI_BATCHED;
     std::vector<FastReq> fast_all(n);
     std::vector<char> is_fast(n);
+    const bool mfma = batched && ctx->mfma_mode != GIAOHIP_MFMA_OFF;
     parallel_for(n, [&](size_t lo, size_t hi) {
-        for (size_t i = lo; i < hi; ++i) is_fast[i] = batched && reqs[i].cd1 < 0 && orient(b, reqs[i], fast_all[i]);
+        for (size_t i = lo; i < hi; ++i) {
+            is_fast[i] = batched && reqs[i].cd1 < 0 && orient(b, reqs[i], fast_all[i]);
+            if (is_fast[i] && ctx->mfma_mode == GIAOHIP_MFMA_ALL) {
+                Request r = reqs[i];
+                is_fast[i] = mfma_orient(b, r, ctx->mfma_mode) < 0;
+            }
+        }
     });
     std::vector<Request> plain;
     std::vector<FastReq> fast;
@@ -1073,7 +1183,7 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
         if (is_fast[i]) fast.push_back(fast_all[i]);
         else plain.push_back(reqs[i]);
     }
-    int st = run_plain(ctx, b, field, plain, d_out);
+    int st = run_plain(ctx, b, field, plain, d_out, nullptr, mfma);
     if (st || fast.empty()) return st;
     // kappa and the error record were prepared by run_plain only if it ran
     const std::vector<double> kap = wave_vectors(b, field);
@@ -1304,6 +1414,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     if (option == GIAOHIP_OPTION_FAST_WAVES &&
         (value == 0 || (value >= gh::kFastBraGroups && value <= gh::kFastMaxWaves))) {
         ctx->fast_waves_cap = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_ALL) {
+        ctx->mfma_mode = value;
         return GIAOHIP_OK;
     }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");

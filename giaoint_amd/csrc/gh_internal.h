@@ -141,6 +141,15 @@ int transform_max_o();
 // ints (quadrature work lists); it is reused by the next launch in stream order.
 constexpr int kBoysBins = 11;
 size_t seed_scratch_ints(long long total);
+// E.R.E contraction on the FP64 matrix cores (gh_mfma.h) for bra / ket
+// pair momenta (lab, lcd), lab >= lcd: whether an instance exists, the ket
+// components one work item covers (items carry cd1 - cd0 <= that), and the
+// launch (seeds as for launch_transform, one 512-thread workgroup per item).
+bool mfma_instance(int lab, int lcd);
+int mfma_cd_tile(int lab);
+hipError_t launch_mfma(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+                       const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
+                       hipStream_t s);
 // p[0..n) = 0 by a kernel on stream s.
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,

@@ -84,7 +84,19 @@ enum {
     /* Largest workgroup of the class-specialized transform, in waves:
        0 (default: 8) or 4..8. A workgroup takes as many bra primitive pairs
        per pass as its rows hold, so the value sets the tile of bra pairs. */
-    GIAOHIP_OPTION_FAST_WAVES = 6
+    GIAOHIP_OPTION_FAST_WAVES = 6,
+    /* Classes whose E.R.E contraction runs on the FP64 matrix cores
+       (v_mfma_f64_16x16x4f64, dense complex Hermite products) under the
+       batched variant, one of GIAOHIP_MFMA_*; default GIAOHIP_MFMA_DEFAULT.
+       Instances exist for bra/ket pair momenta (LAB, LCD) with
+       2 <= LCD <= LAB <= 6, and (7,7), (8,8), in either orientation. The
+       plain variant never uses them. */
+    GIAOHIP_OPTION_MFMA = 7
+};
+enum {
+    GIAOHIP_MFMA_OFF = 0,     /* FP64 VALU kernels only */
+    GIAOHIP_MFMA_DEFAULT = 1, /* matrix cores where they are measured faster: bra and ket momenta >= 7 */
+    GIAOHIP_MFMA_ALL = 2      /* matrix cores for every class with an instance (precision studies) */
 };
 enum {
     GIAOHIP_QUEUE_NONE = 0,    /* one workgroup per item, one launch per LDS-size bucket */
