@@ -276,12 +276,16 @@ def mixed_leg(n: int, tune: bool = True, tune_n: int = 200_000):
     }, batch
 
 
-def high_l_leg(n_gggg: int, n_ffdd: int):
-    """The high-angular-momentum precision workloads at throughput: (gg|gg)
-    and (ff|dd), contraction degree 2, exponents 10^U(log10 0.05, log10 5e3).
-    (ff|dd) runs on the class-specialized kernels through its (dd|ff) image,
-    (gg|gg) on the generic kernel. A warmup call on the first quartets, then
-    one timed call per class, output resident on the device."""
+def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
+    """The high-angular-momentum precision workloads: (gg|gg) and (ff|dd),
+    contraction degree 2, exponents 10^U(log10 0.05, log10 5e3), each with
+    its E.R.E contraction on FP64 VALU (the generic kernel for (gg|gg), the
+    class-specialized (dd|ff) kernel for (ff|dd)) and on the FP64 matrix
+    cores (v_mfma_f64_16x16x4f64, dense complex Hermite products). A warmup
+    call, then one timed call per variant, output resident on the device.
+    Errors: max and RMS of |variant - FP64 VALU| over the first
+    `probe_values` integrals of the batch (the VALU path is held to the C
+    oracle at 1e-11 by the GPU tests)."""
     from giaoint_amd import _native, workloads
 
     g, fd = workloads.precision_batches(n_gggg=n_gggg, n_ffdd=n_ffdd)
@@ -298,16 +302,39 @@ def high_l_leg(n_gggg: int, n_ffdd: int):
         dev = ctx.alloc(total * 16)
         f = _native.field6(batch.B, batch.origin)
         flops = workloads.batch_flops(sh, batch.quartets)
-        db.eri_batch(batch.quartets[: min(n, 256)], f, device_out=dev)
-        ctx.synchronize()
-        t = time.perf_counter()
-        db.eri_batch(batch.quartets, f, device_out=dev)
-        ctx.synchronize()
-        dt = time.perf_counter() - t
-        ctx.free(dev)
-        log(f"{name}: {n} quartets in {dt:.2f} s")
-        out[name] = {"workload": batch.description, "quartets": n, "quartets_per_s": n / dt, "seconds": dt,
-                     "tflops_convention": flops / dt / 1e12}
+        probe = min(total, probe_values)
+        rec = {"workload": batch.description, "quartets": n, "variants": {}}
+        ref = None
+        try:
+            for variant, mode in (("fp64_valu", "off"), ("fp64_mfma", "all")):
+                ctx.set_mfma(mode)
+                db.eri_batch(batch.quartets[: min(n, 256)], f, device_out=dev)
+                ctx.synchronize()
+                t = time.perf_counter()
+                db.eri_batch(batch.quartets, f, device_out=dev)
+                ctx.synchronize()
+                dt = time.perf_counter() - t
+                vals = ctx.to_host(dev, np.empty(probe, dtype=np.complex128))
+                v = {"quartets_per_s": n / dt, "seconds": dt, "tflops_convention": flops / dt / 1e12}
+                if ref is None:
+                    ref = vals
+                else:
+                    err = np.abs(vals - ref)
+                    v["max_abs_err_vs_fp64_valu"] = float(err.max())
+                    v["rms_err_vs_fp64_valu"] = float(np.sqrt(np.mean(err**2)))
+                    v["max_abs_value"] = float(np.abs(ref).max())
+                    v["error_sample"] = f"first {probe} integrals ({probe * n // max(total, 1)} quartets)"
+                rec["variants"][variant] = v
+                log(f"{name} {variant}: {n} quartets in {dt:.3f} s")
+        finally:
+            ctx.set_mfma("default")
+            ctx.free(dev)
+        best = max(rec["variants"].items(), key=lambda kv: kv[1]["quartets_per_s"])
+        rec["quartets_per_s"] = best[1]["quartets_per_s"]
+        rec["fastest"] = best[0]
+        out[name] = rec
+    out["split_fp32"] = ("not built: FP32 accumulation rounds each partial sum at 2^-24 relative, far above "
+                         "the 1e-10 bar unless compensated in FP64 (which moves the work back to FP64 units)")
     return out
 
 

@@ -73,3 +73,28 @@ def test_high_l_precision_samples(gpu, port):
         rms = float(np.sqrt(np.mean(err**2)))
         print(f"{batch.description}: max |err| {err.max():.2e}, rms {rms:.2e}, max |value| {np.abs(want).max():.2e}")
         assert err.max() <= ATOL
+
+
+def test_high_l_mfma_variant_errors(gpu, port):
+    """The FP64 matrix-core contraction on the precision workloads: within
+    1e-10 of the FP64 VALU path on 200 (gg|gg) and 2000 (ff|dd) quartets,
+    and of the oracle on 2 and 50 of them; max and RMS errors printed."""
+    g, fd = workloads.precision_batches(n_gggg=200, n_ffdd=2000)
+    for batch, n_oracle in ((g, 2), (fd, 50)):
+        db = _device(gpu, batch)
+        f = _native.field6(batch.B, batch.origin)
+        try:
+            gpu.set_mfma("off")
+            valu = db.eri_batch(batch.quartets, f)
+            gpu.set_mfma("all")
+            mfma = db.eri_batch(batch.quartets, f)
+        finally:
+            gpu.set_mfma("default")
+        err = np.abs(mfma - valu)
+        offs, sizes = _blocks(db, batch)
+        want = _oracle(port, batch, np.arange(n_oracle))
+        oerr = np.abs(mfma[: offs[n_oracle]] - want)
+        print(f"{batch.description}: MFMA vs FP64 VALU max {err.max():.2e} rms {np.sqrt(np.mean(err**2)):.2e}; "
+              f"vs oracle ({n_oracle} quartets) max {oerr.max():.2e}; max |value| {np.abs(valu).max():.2e}")
+        assert err.max() <= 1e-10
+        assert oerr.max() <= 1e-10
