@@ -401,6 +401,10 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0):
     tj = time.perf_counter()
     ch.coulomb(dens)
     coulomb_s = time.perf_counter() - tj
+    tk = time.perf_counter()
+    ch.exchange(dens)
+    exchange_s = time.perf_counter() - tk
+    exchange_ms = ctx.stats()["transform_ms"]
     # projection: each pivot block reads the factors computed before it once
     blk = ch.blocks
     proj_bytes = 16.0 * n * n * blk["factor_rows_read"]
@@ -420,6 +424,13 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0):
             "seconds": coulomb_s,
             "factor_passes_gbs": 2 * 16.0 * n * n * m / coulomb_s / 1e9,
             "note": "J = sum_P L_P (sum conj(L_P^T) D): two passes over the device factors, density and J over PCIe",
+        },
+        "exchange_contraction": {
+            "seconds": exchange_s,
+            "device_ms": exchange_ms,
+            "tflops": 2 * 8.0 * n**3 * m / (exchange_ms * 1e-3) / 1e12,
+            "note": "K = sum_P L_P D L_P^H: two rocBLAS zgemm of order n per factor (8 FLOP per complex MAC), "
+                    "density and K over PCIe",
         },
         "pivot_blocks": blk["blocks"],
         "pivots_per_block": m / max(blk["blocks"], 1),

@@ -104,6 +104,7 @@ _SIGNATURES = {
     "giaohip_cholesky_blocks": ([_vp, _u64p, _u64p, _dp, _u64p, _u64p], C.c_int),
     "giaohip_cholesky_reconstruct": ([_vp, _vp, C.POINTER(C.c_int64), _dp], C.c_int),
     "giaohip_cholesky_coulomb": ([_vp, _vp, _dp, _dp], C.c_int),
+    "giaohip_cholesky_exchange": ([_vp, _vp, _dp, _dp], C.c_int),
     "giaohip_cholesky_destroy": ([_vp, _vp], C.c_int),
 }
 
@@ -506,6 +507,15 @@ class DeviceCholesky:
         out = np.empty((n, n), dtype=np.complex128)
         self.ctx.check(self.ctx.lib.giaohip_cholesky_coulomb(self.ctx.handle, self.handle, d.ctypes.data_as(_dp),
                                                              out.ctypes.data_as(_dp)))
+        return out
+
+    def exchange(self, density) -> np.ndarray:
+        """K[mu,nu] = sum_{lam,sig} (mu lam|sig nu) D[lam,sig] from the device factors."""
+        n = self.basis.nao
+        d = np.ascontiguousarray(density, dtype=np.complex128).reshape(n, n)
+        out = np.empty((n, n), dtype=np.complex128)
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_exchange(self.ctx.handle, self.handle, d.ctypes.data_as(_dp),
+                                                              out.ctypes.data_as(_dp)))
         return out
 
     def close(self):

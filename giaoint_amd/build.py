@@ -52,6 +52,11 @@ HIPCC_FLAGS = [
 ]
 
 
+# rocBLAS: the plain complex GEMMs of the exchange contraction
+LINK_LIBS = [f"-L{os.environ.get('ROCM_PATH', '/opt/rocm')}/lib", "-lrocblas",
+             f"-Wl,-rpath,{os.environ.get('ROCM_PATH', '/opt/rocm')}/lib"]
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():
@@ -97,7 +102,7 @@ def build(verbose: bool = False, jobs: int | None = None, incremental: bool = Fa
     with ThreadPoolExecutor(max_workers=workers) as pool:
         objects = list(pool.map(compile_one, SOURCES))
     tmp = LIBRARY.with_suffix(".so.tmp")
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objects], "-o", str(tmp)]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objects], *LINK_LIBS, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -8,6 +8,8 @@
 #include "giaohip.h"
 #include "gh_internal.h"
 
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -208,6 +210,7 @@ struct giaohip_ctx {
     int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     void* noalloc = nullptr;                 // device address of `scratch` on this context's device
+    rocblas_handle blas = nullptr;           // library GEMMs (exchange contraction), created on first use
 };
 
 // A device-resident Cholesky: factors, residual and every buffer a pivot step
@@ -1365,6 +1368,7 @@ int giaohip_destroy(giaohip_ctx* ctx) {
     for (void* p : ctx->user_allocs) (void)hipFree(p);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
@@ -2860,6 +2864,53 @@ int giaohip_cholesky_coulomb(giaohip_ctx* ctx, const giaohip_cholesky* c, const 
             "coulomb launch");
     HIP_TRY(hipMemcpyAsync(out, d_j.p, (size_t)n2 * sizeof(zc), hipMemcpyDeviceToHost, s), "copy");
     HIP_TRY(hipStreamSynchronize(s), "coulomb");
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_exchange(giaohip_ctx* ctx, const giaohip_cholesky* c, const double* density, double* out) {
+    if (!ctx || !c || !density || !out) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    const long long n = c->n, n2 = n * n;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+            ctx->blas = nullptr;
+            return set_error(ctx, GIAOHIP_DEVICE_ERROR, "rocBLAS handle creation failed");
+        }
+    }
+    if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success)
+        return set_error(ctx, GIAOHIP_DEVICE_ERROR, "rocBLAS stream setup failed");
+    ScopedBuffer d_dens, d_t, d_k;
+    HIP_TRY(d_dens.ensure((size_t)n2 * sizeof(zc)), "hipMalloc");
+    HIP_TRY(d_t.ensure((size_t)n2 * sizeof(zc)), "hipMalloc");
+    HIP_TRY(d_k.ensure((size_t)n2 * sizeof(zc)), "hipMalloc");
+    HIP_TRY(hipMemcpyAsync(d_dens.p, density, (size_t)n2 * sizeof(zc), hipMemcpyHostToDevice, s), "copy");
+    HIP_TRY(gh::launch_zero_ints(d_k.as<int>(), (int)(n2 * (long long)(sizeof(zc) / sizeof(int))), s), "fill");
+    HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
+    // K = sum_P L_P D L_P^H (row-major n x n). Column-major, the memory of a
+    // row-major X is X^T, so per factor T^T = D^T L_P^T and
+    // K^T += conj(L_P) T^T = (L_P^T)^H T^T.
+    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+    const rocblas_int m = (rocblas_int)n;
+    auto* D = reinterpret_cast<const rocblas_double_complex*>(d_dens.p);
+    auto* T = reinterpret_cast<rocblas_double_complex*>(d_t.p);
+    auto* K = reinterpret_cast<rocblas_double_complex*>(d_k.p);
+    for (uint64_t p = 0; p < c->rank; ++p) {
+        auto* Lp = reinterpret_cast<const rocblas_double_complex*>(c->chunks[p / c->rows_per_chunk] +
+                                                                   (long long)(p % c->rows_per_chunk) * n2);
+        if (rocblas_zgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, m, m, m, &one, D, m, Lp, m,
+                          &zero, T, m) != rocblas_status_success ||
+            rocblas_zgemm(ctx->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, m, m, &one,
+                          Lp, m, T, m, &one, K, m) != rocblas_status_success)
+            return set_error(ctx, GIAOHIP_DEVICE_ERROR, "rocBLAS zgemm failed");
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
+    HIP_TRY(hipMemcpyAsync(out, d_k.p, (size_t)n2 * sizeof(zc), hipMemcpyDeviceToHost, s), "copy");
+    HIP_TRY(hipStreamSynchronize(s), "exchange");
+    ctx->stats.transform_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.launches = 2 * c->rank;
     return GIAOHIP_OK;
 }
 

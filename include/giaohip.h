@@ -329,6 +329,13 @@ int giaohip_cholesky_reconstruct(giaohip_ctx* ctx, const giaohip_cholesky* chol,
  * n*n complex128 (host). Two passes over the factors. No counterpart in the
  * reference (a device-resident consumer of its CholeskyERI factors). */
 int giaohip_cholesky_coulomb(giaohip_ctx* ctx, const giaohip_cholesky* chol, const double* density, double* out);
+/* Exchange-type contraction with the device factors:
+ * K[mu,nu] = sum_{lam,sig} (mu lam|sig nu) D[lam,sig] ~ sum_P (L_P D L_P^H)[mu,nu]
+ * (the factor convention of giaohip_cholesky_reconstruct); two complex
+ * GEMMs of order n per factor (rocBLAS zgemm on the context's stream).
+ * density and out are n*n complex128 (host). No counterpart in the reference
+ * (a device-resident consumer of its CholeskyERI factors). */
+int giaohip_cholesky_exchange(giaohip_ctx* ctx, const giaohip_cholesky* chol, const double* density, double* out);
 /* Pivot AO pairs (mu, nu), rank*2 int64 values. */
 int giaohip_cholesky_pivots(const giaohip_cholesky* chol, int64_t* out);
 /* Factor rows first..first+count-1 as complex128 (count*n*n values, host). */

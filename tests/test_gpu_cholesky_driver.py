@@ -202,6 +202,12 @@ def test_device_factor_consumers(gpu, port):
     assert np.max(np.abs(J - Jw)) <= 1e-11 * np.abs(Jw).max()
     Jx = np.einsum("mnls,ls->mn", exact, D)
     assert np.max(np.abs(J - Jx)) <= (1e-8 + 1e-12) * np.abs(D).sum()
+    # exchange: K[mu,nu] = sum (mu lam|sig nu) D[lam,sig] ~ sum_P L_P D L_P^H
+    K = res.exchange(D)
+    Kw = np.einsum("Pml,ls,Pns->mn", L, D, L.conj())
+    assert np.max(np.abs(K - Kw)) <= 1e-11 * np.abs(Kw).max()
+    Kx = np.einsum("mlsn,ls->mn", exact, D)
+    assert np.max(np.abs(K - Kx)) <= (1e-8 + 1e-12) * np.abs(D).sum()
     with pytest.raises(IndexError):
         res.reconstruct((0, n + 1))
 
