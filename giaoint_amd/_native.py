@@ -88,6 +88,8 @@ _SIGNATURES = {
     "giaohip_primitive_normalization": ([C.c_double, C.c_int, C.c_int, C.c_int], C.c_double),
     "giaohip_boys": ([_vp, _dp, C.c_size_t, C.c_int, C.c_int, _dp, _ip, _dp, _ip], C.c_int),
     "giaohip_eri_batch": ([_vp, _vp, _u32p, C.c_size_t, _dp, _u64p, _vp, C.c_int], C.c_int),
+    "giaohip_eri_derivative_batch": ([_vp, _vp, _u32p, C.c_size_t, _dp, _u64p, _vp, C.c_uint64, C.c_int], C.c_int),
+    "giaohip_eri_derivative_pair_bounds": ([_vp, _vp, _dp, _dp], C.c_int),
     "giaohip_eri_batch_size": ([_vp, _u32p, C.c_size_t, _u64p], C.c_int),
     "giaohip_eri_batch_noalloc": ([_vp, _vp, _u32p, C.c_size_t, _dp, _u64p, _vp], C.c_int),
     "giaohip_eri_tensor": ([_vp, _vp, _dp, _vp, C.c_int], C.c_int),
@@ -353,6 +355,33 @@ class DeviceBasis:
                 out.ctypes.data_as(_vp), 0,
             )
         )
+        return out
+
+    def eri_derivative_batch(self, quartets, field, out=None, device_out: int | None = None):
+        """Packed (4, 3, na, nb, nc, nd) centre-derivative blocks of the
+        quartets (host complex128 array), or into device_out."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.uint32).reshape(-1, 4))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        ctx = self.ctx
+        ok = len(q) and int(q.max()) < self.nshell  # else the library reports the index
+        total = 12 * int(np.prod(self.ao_counts[q.astype(np.int64)], axis=1).sum()) if ok else 0
+        if device_out is not None:
+            ctx.check(ctx.lib.giaohip_eri_derivative_batch(
+                ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)), _p(f), None, _vp(device_out),
+                C.c_uint64(total), 1))
+            return None
+        if out is None:
+            out = np.empty(total, dtype=np.complex128)
+        ctx.check(ctx.lib.giaohip_eri_derivative_batch(
+            ctx.handle, self.handle, _p(q, C.c_uint32), C.c_size_t(len(q)), _p(f), None, out.ctypes.data_as(_vp),
+            C.c_uint64(out.size), 0))
+        return out
+
+    def eri_derivative_pair_bounds(self, field) -> np.ndarray:
+        """EriDerivativeSchwarzBounds values, nshell x nshell."""
+        out = np.zeros((self.nshell, self.nshell))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.giaohip_eri_derivative_pair_bounds(self.ctx.handle, self.handle, _p(f), _p(out)))
         return out
 
     def schwarz_bounds(self, field) -> np.ndarray:

@@ -25,6 +25,7 @@ SOURCES = [
     "gh_api.hip",
     "gh_eri.hip",
     "gh_cholesky.hip",
+    "gh_deriv.hip",
     "gh_eri_fast.hip",
     "gh_fast_dddd.hip",
     "gh_fast_ddff.hip",

@@ -36,6 +36,9 @@ __all__ = [
     "_eri_full",
     "_eri_batch",
     "_EriSchwarzBounds",
+    "eri_center_derivatives_shell",
+    "_eri_derivative_batch",
+    "_EriDerivativeSchwarzBounds",
     "_canonical_shell_quartet",
     "_cholesky_diagonal",
     "_cholesky_columns",
@@ -322,6 +325,85 @@ def eri_shell(a, b, c, d, *, field=None, out=None):
     basis = Basis(shells)
     basis._dev().eri_batch([(0, 1, 2, 3)], _field6(field), out=out.reshape(-1))
     return out
+
+
+def eri_center_derivatives_shell(a, b, c, d, *, field=None, out=None):
+    """Centre derivatives of (ab|cd) with the London phase response, shape
+    (4, 3, na, nb, nc, nd): quartet slot, axis, AOs (bindings.cpp:1308-1323,
+    derivatives.cpp:952-959)."""
+    shells = (a, b, c, d)
+    out = _check_out(out, (4, 3) + tuple(s.ao_count for s in shells))
+    Basis(shells)._dev().eri_derivative_batch([(0, 1, 2, 3)], _field6(field), out=out.reshape(-1))
+    return out
+
+
+class _EriDerivativeSchwarzBounds:
+    """Derivative-density Schwarz bounds (derivatives.cpp:1307-1387): per
+    shell pair the triangle/Schwarz bound of every centre derivative of the
+    pair density, London terms and |contraction coefficients| included; a
+    quartet's bound pairs it with the ordinary Schwarz factor of the other
+    pair. Both tables come from the device."""
+
+    def __init__(self, basis, *, field=None, threads=None):
+        _threads(threads)
+        self._basis_ref = weakref.ref(basis)
+        self._field = (0.0,) * 6 if field is None else tuple(field._field6())
+        dev = basis._dev()
+        f = _field6(field)
+        self._ordinary = dev.schwarz_bounds(f)
+        self._values = dev.eri_derivative_pair_bounds(f)
+        for v in (self._ordinary, self._values):
+            v.setflags(write=False)
+
+    def quartet_bound(self, quartet) -> float:
+        q = tuple(int(i) for i in quartet)
+        n = self._values.shape[0]
+        if len(q) != 4 or any(i < 0 or i >= n for i in q):
+            raise IndexError("derivative shell quartet index is out of range")
+        return float(self._bounds(np.asarray([q], dtype=np.int64))[0])
+
+    def _bounds(self, q: np.ndarray) -> np.ndarray:
+        """derivatives.cpp:1374-1383 for rows of q (validated indices)."""
+        D, Q = self._values, self._ordinary
+        a, b, c, d = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+        bound = np.maximum(D[a, b] * Q[c, d], Q[a, b] * D[c, d])
+        return np.nextafter(bound * (1.0 + 128.0 * np.finfo(np.float64).eps), np.inf)
+
+    def _matches(self, basis, field) -> bool:
+        f = (0.0,) * 6 if field is None else tuple(field._field6())
+        return self._basis_ref() is basis and f == self._field
+
+
+def _eri_derivative_batch(basis, quartets, *, field=None, screening_threshold=0.0, threads=None, bounds=None):
+    """Packed centre-derivative blocks (bindings.cpp:2255-2304): returns
+    (quartets uint32[m,4], shapes int64[m,6] = (4, 3, na, nb, nc, nd),
+    offsets int64[m+1], values complex128[offsets[m]], requested, screened);
+    quartets whose derivative bound is below a positive threshold are left out."""
+    _threads(threads)
+    t = float(screening_threshold)
+    if not np.isfinite(t) or t < 0.0:
+        raise ValueError("screening_threshold must be finite and non-negative")
+    if t > 0.0 and bounds is None:
+        raise ValueError("derivative screening requires derivative bounds")
+    if bounds is not None and not bounds._matches(basis, field):
+        raise ValueError("derivative bounds do not match basis and field")
+    q = np.asarray(quartets, dtype=np.int64).reshape(-1, 4)
+    if q.size and (q.min() < 0 or q.max() >= len(basis.shells)):
+        raise IndexError("derivative shell quartet index is out of range")
+    requested = len(q)
+    if t > 0.0 and len(q):
+        q = q[~(bounds._bounds(q) < t)]
+    ao = np.array([s.ao_count for s in basis.shells], dtype=np.int64)
+    m = len(q)
+    shapes = np.zeros((m, 6), dtype=np.int64)
+    if m:
+        shapes[:, 0], shapes[:, 1] = 4, 3
+        shapes[:, 2:] = ao[q]
+    sizes = np.prod(shapes, axis=1)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    kept = np.ascontiguousarray(q, dtype=np.uint32)
+    values = basis._dev().eri_derivative_batch(kept, _field6(field)) if m else np.zeros(0, dtype=np.complex128)
+    return kept, shapes, offsets, values, requested, requested - m
 
 
 class _EriSchwarzBounds:

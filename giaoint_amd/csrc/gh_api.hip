@@ -31,6 +31,7 @@
 #include <unordered_map>
 #include <vector>
 
+using gh::DerivItem;
 using gh::DevError;
 using gh::DevShell;
 using gh::SeedQuartet;
@@ -178,7 +179,19 @@ struct giaohip_basis {
     std::vector<double> exps, coefs, norms, radial;
     std::vector<int> owner;
     DevBuffer d_shells, d_exps, d_coefs, d_norms, d_aooff, d_owner, d_radial;
+    // shifted-momentum companion basis of the centre derivatives, built on
+    // first use and dropped when the shells change
+    giaohip_basis* deriv = nullptr;
 };
+
+void release_basis(giaohip_basis* b) {
+    if (!b) return;
+    release_basis(b->deriv);
+    b->deriv = nullptr;
+    for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner, &b->d_radial})
+        d->release();
+    delete b;
+}
 
 struct giaohip_ctx {
     int device = 0;
@@ -1485,6 +1498,8 @@ int load_basis(giaohip_ctx* ctx, giaohip_basis* b, int nshell, const double* cen
         b->norms.clear();
         b->radial.clear();
         b->owner.clear();
+        release_basis(b->deriv);
+        b->deriv = nullptr;
     };
     clear();
     if (nshell < 0) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "shell count must be non-negative");
@@ -1553,6 +1568,58 @@ int load_basis(giaohip_ctx* ctx, giaohip_basis* b, int nshell, const double* cen
     return GIAOHIP_OK;
 }
 
+// The shifted-momentum companion of b for centre derivatives (built on first
+// use, kept with b): shells [0, ns) are b's; s + ns has l + 1 and 2K rows,
+// c sqrt(alpha) (the 2 alpha (a + 1| term with the radial normalization
+// ratio R_l / R_{l+1} = 1 / (2 sqrt(alpha)) folded in) then
+// c / (2 sqrt(alpha)) (the London phase's (a + 1| term); s + 2 ns has the
+// first K of those rows only (zero field); s + 3 ns has l - 1 and
+// c 2 sqrt(alpha) (R_l / R_{l-1}), or is a copy of s when l = 0 (never used).
+int derivative_basis(giaohip_ctx* ctx, const giaohip_basis* b, giaohip_basis** out) {
+    auto* mb = const_cast<giaohip_basis*>(b);
+    if (mb->deriv) {
+        *out = mb->deriv;
+        return GIAOHIP_OK;
+    }
+    const int ns = b->ns;
+    std::vector<double> centers, exps, coefs;
+    std::vector<int> L, nprim, ncontr;
+    for (int part = 0; part < 4; ++part)
+        for (int s = 0; s < ns; ++s) {
+            const DevShell& d = b->shells[s];
+            const int K = d.ncontr, P = d.nprim;
+            const double* e = b->exps.data() + d.prim_off;
+            const double* c = b->coefs.data() + d.coef_off;
+            int l = d.l, rows = K;
+            if (part == 1) l = d.l + 1, rows = 2 * K;
+            if (part == 2) l = d.l + 1;
+            if (part == 3 && d.l > 0) l = d.l - 1;
+            for (int i = 0; i < 3; ++i) centers.push_back(d.center[i]);
+            L.push_back(l);
+            nprim.push_back(P);
+            ncontr.push_back(rows);
+            exps.insert(exps.end(), e, e + P);
+            for (int r = 0; r < rows; ++r)
+                for (int p = 0; p < P; ++p) {
+                    const double ra = std::sqrt(e[p]);
+                    double f = 1.0;
+                    if (part == 1 || part == 2) f = r < K ? ra : 1.0 / (2.0 * ra);
+                    if (part == 3 && d.l > 0) f = 2.0 * ra;
+                    coefs.push_back(c[(size_t)(r % K) * P + p] * f);
+                }
+        }
+    auto* db = new giaohip_basis;
+    const int st = load_basis(ctx, db, 4 * ns, centers.data(), L.data(), nprim.data(), ncontr.data(), exps.data(),
+                              coefs.data());
+    if (st != GIAOHIP_OK) {
+        release_basis(db);
+        return st;
+    }
+    mb->deriv = db;
+    *out = db;
+    return GIAOHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1565,10 +1632,7 @@ int giaohip_basis_create(giaohip_ctx* ctx, int nshell, const double* centers, co
     auto* b = new giaohip_basis;
     const int st = load_basis(ctx, b, nshell, centers, L, nprim, ncontr, exps, coefs);
     if (st != GIAOHIP_OK) {
-        for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner,
-                             &b->d_radial})
-            d->release();
-        delete b;
+        release_basis(b);
         return st;
     }
     *out = b;
@@ -1593,9 +1657,7 @@ int giaohip_basis_destroy(giaohip_ctx* ctx, giaohip_basis* b) {
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
     }
-    for (DevBuffer* d : {&b->d_shells, &b->d_exps, &b->d_coefs, &b->d_norms, &b->d_aooff, &b->d_owner, &b->d_radial})
-        d->release();
-    delete b;
+    release_basis(b);
     return GIAOHIP_OK;
 }
 
@@ -1882,6 +1944,370 @@ int giaohip_eri_batch_noalloc(giaohip_ctx* ctx, const giaohip_basis* b, const ui
         run += sizes[i];
     }
     return run_plain(ctx, b, field, reqs, reinterpret_cast<zc*>(out_device), &mem);
+}
+
+// compute_eri_derivative_blocks (derivatives.cpp:1389-1415) on the device:
+// the shifted-momentum quartets of each derivative quartet through the ERI
+// path on the companion basis, then the combination kernel (gh_deriv.hip).
+int giaohip_eri_derivative_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* quartets, size_t n,
+                                 const double field[6], const uint64_t* offsets, double* out, uint64_t out_len,
+                                 int out_on_device) {
+    if (!ctx || !b || !field || (n > 0 && (!quartets || !out))) return GIAOHIP_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    reset_stats(ctx);
+    ctx->stats.requested_quartets = n;
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(field[k])) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "field values must be finite");
+    if (offsets && offsets[0] != 0)
+        return set_error(ctx, GIAOHIP_LENGTH_ERROR, "derivative batch offsets have an incorrect size or start");
+    std::vector<uint64_t> sizes(n), offs(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t size = 1;
+        for (int k = 0; k < 4; ++k) {
+            if (quartets[4 * i + k] >= (uint32_t)b->ns)
+                return set_error(ctx, GIAOHIP_OUT_OF_RANGE, "derivative shell quartet index is out of range");
+            size *= (uint64_t)b->shells[quartets[4 * i + k]].ao_count;
+        }
+        sizes[i] = size;
+        if (offsets) {
+            if (offsets[i] > out_len || offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] != 12 * size)
+                return set_error(ctx, GIAOHIP_LENGTH_ERROR, "derivative batch offsets do not match block sizes");
+            offs[i + 1] = offsets[i + 1];
+        } else {
+            offs[i + 1] = offs[i] + 12 * size;
+        }
+    }
+    if (offs[n] != out_len) return set_error(ctx, GIAOHIP_LENGTH_ERROR, "derivative batch output has an incorrect size");
+    if (n == 0) return GIAOHIP_OK;
+    giaohip_basis* db = nullptr;
+    if (const int st = derivative_basis(ctx, b, &db)) return st;
+    const int ns = b->ns;
+    const bool phase = field[0] != 0.0 || field[1] != 0.0 || field[2] != 0.0;
+    const int plus_part = phase ? 1 : 2;
+    // Every shifted class must be admitted (the reference evaluates them
+    // all). The reference's own limits (combined order, auxiliary size) are
+    // reported before the device kernels' limits.
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t* q = quartets + 4 * i;
+            const int sh[4] = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+            auto check = [&](const int* t) {
+                const int cs = db->cls_status[class_index(db, t)];
+                const bool reference_limit = cs == kClsOrder || cs == kClsAux;
+                return (pass == 0) == reference_limit ? class_error(ctx, cs) : GIAOHIP_OK;
+            };
+            if (const int st = check(sh)) return st;
+            for (int k = 0; k < 4; ++k) {
+                int t[4] = {sh[0], sh[1], sh[2], sh[3]};
+                t[k] = sh[k] + plus_part * ns;
+                if (const int st = check(t)) return st;
+                if (b->shells[sh[k]].l > 0) {
+                    t[k] = sh[k] + 3 * ns;
+                    if (const int st = check(t)) return st;
+                }
+            }
+        }
+    ctx->stats.computed_quartets = n;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0), "hipEventCreate");
+    if (hipError_t e = hipEventCreate(&t1); e != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(ctx, e, "hipEventCreate");
+    }
+    (void)hipEventRecord(t0, s);
+    // work sizes of one derivative quartet: the shifted integrals and the output
+    auto temp_size = [&](size_t i) {
+        uint64_t t = sizes[i];
+        for (int k = 0; k < 4; ++k) {
+            const DevShell& d = b->shells[quartets[4 * i + k]];
+            const uint64_t rest = sizes[i] / (uint64_t)d.ao_count;
+            t += rest * (uint64_t)db->shells[quartets[4 * i + k] + plus_part * ns].ao_count;
+            if (d.l > 0) t += rest * (uint64_t)db->shells[quartets[4 * i + k] + 3 * ns].ao_count;
+        }
+        return t;
+    };
+    const uint64_t budget = std::max<uint64_t>(ctx->staging_budget_bytes / sizeof(zc), 1);
+    int st = GIAOHIP_OK;
+    std::vector<Request> reqs;
+    std::vector<DerivItem> items;
+    std::vector<int> tiles;
+    size_t i = 0;
+#define DERIV_TRY(expr, what)                  \
+    if (hipError_t _e = (expr); _e != hipSuccess) { \
+        st = hip_fail(ctx, _e, what);          \
+        break;                                 \
+    }
+    while (i < n && st == GIAOHIP_OK) {
+        size_t j = i;
+        uint64_t tt = 0, to = 0;
+        while (j < n && (j == i || (tt + temp_size(j) <= budget && to + 12 * sizes[j] <= budget))) {
+            tt += temp_size(j);
+            to += 12 * sizes[j];
+            ++j;
+        }
+        DERIV_TRY(ctx->blocks.ensure(tt * sizeof(zc)), "hipMalloc(blocks)");
+        zc* temp = ctx->blocks.as<zc>();
+        zc* dst;
+        if (out_on_device) {
+            dst = reinterpret_cast<zc*>(out) + offs[i];
+        } else {
+            DERIV_TRY(ctx->staging.ensure(to * sizeof(zc)), "hipMalloc(staging)");
+            dst = ctx->staging.as<zc>();
+        }
+        reqs.clear();
+        items.clear();
+        tiles.clear();
+        long long tpos = 0;
+        auto add_request = [&](const int sh[4], long long base) {
+            Request r{};
+            for (int k = 0; k < 4; ++k) {
+                r.sh[k] = sh[k];
+                r.xfix[k] = -1;
+            }
+            r.cd0 = 0;
+            r.cd1 = -1;
+            r.out_base = base;
+            const long long nb = db->shells[sh[1]].ao_count, nc = db->shells[sh[2]].ao_count,
+                            nd = db->shells[sh[3]].ao_count;
+            r.stride[0] = nb * nc * nd;
+            r.stride[1] = nc * nd;
+            r.stride[2] = nd;
+            r.stride[3] = 1;
+            reqs.push_back(r);
+            return (long long)db->shells[sh[0]].ao_count * nb * nc * nd;
+        };
+        for (size_t k = i; k < j; ++k) {
+            const uint32_t* q = quartets + 4 * k;
+            const int sh[4] = {(int)q[0], (int)q[1], (int)q[2], (int)q[3]};
+            DerivItem it{};
+            it.out = (long long)(offs[k] - offs[i]);
+            it.size = (long long)sizes[k];
+            it.base = tpos;
+            tpos += add_request(sh, tpos);
+            for (int m = 0; m < 4; ++m) {
+                const DevShell& d = b->shells[sh[m]];
+                it.n[m] = d.ao_count;
+                it.l[m] = d.l;
+                it.krow[m] = d.ncontr;
+                it.prow[m] = phase ? 2 * d.ncontr : d.ncontr;
+                for (int x = 0; x < 3; ++x) it.center[m][x] = d.center[x];
+                int t[4] = {sh[0], sh[1], sh[2], sh[3]};
+                t[m] = sh[m] + plus_part * ns;
+                it.plus[m] = tpos;
+                tpos += add_request(t, tpos);
+                it.minus[m] = tpos;
+                if (d.l > 0) {
+                    t[m] = sh[m] + 3 * ns;
+                    tpos += add_request(t, tpos);
+                }
+            }
+            it.first_tile = (int)tiles.size();
+            const long long nt = (it.size + 255) / 256;
+            for (long long x = 0; x < nt; ++x) tiles.push_back((int)items.size());
+            items.push_back(it);
+        }
+        st = run_requests(ctx, db, field, reqs, temp);
+        if (st) break;
+        const size_t ib = items.size() * sizeof(DerivItem);
+        DERIV_TRY(ctx->misc_b.ensure(ib + tiles.size() * sizeof(int) + 64), "hipMalloc");
+        auto* d_items = ctx->misc_b.as<DerivItem>();
+        int* d_tiles = reinterpret_cast<int*>(ctx->misc_b.as<char>() + ib);
+        DERIV_TRY(hipMemcpyAsync(d_items, items.data(), ib, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        DERIV_TRY(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(int), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+        DERIV_TRY(launch_deriv_assemble(d_items, d_tiles, (int)tiles.size(), temp, dst, field, phase, s),
+                "deriv_assemble_kernel");
+        // the descriptors and the work buffers are reused by the next group
+        DERIV_TRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (!out_on_device)
+            DERIV_TRY(copy_to_host(ctx, reinterpret_cast<const char*>(dst), reinterpret_cast<char*>(out + 2 * offs[i]),
+                                 to * sizeof(zc)),
+                    "copy to host");
+        i = j;
+    }
+#undef DERIV_TRY
+    (void)hipEventRecord(t1, s);
+    (void)hipEventSynchronize(t1);
+    ctx->stats.total_ms = elapsed(t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return st;
+}
+
+// EriDerivativeSchwarzBounds values (derivatives.cpp:1307-1372). The
+// primitive Coulomb norms sqrt|(right left|left right)| of the shifted raw
+// primitives are diagonal elements of ERI blocks of single-primitive shells
+// (l, l + 1, l - 1 at each primitive), evaluated on the device; the device
+// normalizes the shifted component, so each norm is rescaled by
+// N(alpha, c) / N(alpha, c'), then combined as the reference does.
+int giaohip_eri_derivative_pair_bounds(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out) {
+    if (!ctx || !b || !field || !out) return GIAOHIP_INVALID_ARGUMENT;
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(field[k])) return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "field values must be finite");
+    const int ns = b->ns;
+    if (ns == 0) return GIAOHIP_OK;
+    std::vector<int> pstart(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) pstart[s + 1] = pstart[s] + b->shells[s].nprim;
+    const int np = pstart[ns];
+    std::vector<double> centers, exps, coefs(3 * (size_t)np, 1.0);
+    std::vector<int> L, nprim(3 * (size_t)np, 1), ncontr(3 * (size_t)np, 1);
+    for (int s = 0; s < ns; ++s) {
+        const DevShell& d = b->shells[s];
+        for (int p = 0; p < d.nprim; ++p)
+            for (int v = 0; v < 3; ++v) {
+                for (int i = 0; i < 3; ++i) centers.push_back(d.center[i]);
+                exps.push_back(b->exps[d.prim_off + p]);
+                L.push_back(v == 0 ? d.l : (v == 1 ? d.l + 1 : std::max(d.l - 1, 0)));
+            }
+    }
+    giaohip_basis* pb = nullptr;
+    if (const int st = giaohip_basis_create(ctx, 3 * np, centers.data(), L.data(), nprim.data(), ncontr.data(),
+                                            exps.data(), coefs.data(), &pb))
+        return st;
+    struct Guard {
+        giaohip_ctx* c;
+        giaohip_basis* p;
+        ~Guard() { giaohip_basis_destroy(c, p); }
+    } guard{ctx, pb};
+    // quartets of one shell pair a <= b and primitive pair: (B A|A B) with
+    // A in {l, l+1, l-1} at fixed B, then B in {l+1, l-1} at fixed A
+    auto quartets_of = [&](int a, int c, std::vector<uint32_t>& q) {
+        const int la = b->shells[a].l, lc = b->shells[c].l;
+        for (int pa = 0; pa < b->shells[a].nprim; ++pa)
+            for (int pc = 0; pc < b->shells[c].nprim; ++pc) {
+                const uint32_t A = 3 * (uint32_t)(pstart[a] + pa), C = 3 * (uint32_t)(pstart[c] + pc);
+                q.insert(q.end(), {C, A, A, C, C, A + 1, A + 1, C});
+                if (la > 0) q.insert(q.end(), {C, A + 2, A + 2, C});
+                q.insert(q.end(), {C + 1, A, A, C + 1});
+                if (lc > 0) q.insert(q.end(), {C + 2, A, A, C + 2});
+            }
+    };
+    std::vector<std::pair<int, int>> pairs;
+    for (int a = 0; a < ns; ++a)
+        for (int c = a; c < ns; ++c) pairs.emplace_back(a, c);
+    const uint64_t budget = std::max<uint64_t>(ctx->staging_budget_bytes / sizeof(zc), 1);
+    size_t i = 0;
+    std::vector<double> blocks;
+    while (i < pairs.size()) {
+        // a group of shell pairs within the staging budget
+        size_t j = i;
+        std::vector<uint32_t> q;
+        std::vector<size_t> qfirst;
+        uint64_t tot = 0;
+        while (j < pairs.size()) {
+            std::vector<uint32_t> mine;
+            quartets_of(pairs[j].first, pairs[j].second, mine);
+            uint64_t sz = 0;
+            for (size_t k = 0; k < mine.size() / 4; ++k) {
+                uint64_t e = 1;
+                for (int m = 0; m < 4; ++m) e *= (uint64_t)pb->shells[mine[4 * k + m]].ao_count;
+                sz += e;
+            }
+            if (j > i && tot + sz > budget) break;
+            qfirst.push_back(q.size() / 4);
+            q.insert(q.end(), mine.begin(), mine.end());
+            tot += sz;
+            ++j;
+        }
+        const size_t nq = q.size() / 4;
+        std::vector<uint64_t> offs(nq + 1, 0);
+        for (size_t k = 0; k < nq; ++k) {
+            uint64_t e = 1;
+            for (int m = 0; m < 4; ++m) e *= (uint64_t)pb->shells[q[4 * k + m]].ao_count;
+            offs[k + 1] = offs[k] + e;
+        }
+        blocks.resize(2 * std::max<uint64_t>(offs[nq], 1));
+        if (const int st = giaohip_eri_batch(ctx, pb, q.data(), nq, field, offs.data(), blocks.data(), 0)) return st;
+        parallel_for(j - i, [&](size_t lo, size_t hi) {
+            for (size_t t = lo; t < hi; ++t) {
+                const int a = pairs[i + t].first, c = pairs[i + t].second;
+                const DevShell &sa = b->shells[a], &sc = b->shells[c];
+                const auto ca_list = cartesian_components(sa.l), cc_list = cartesian_components(sc.l);
+                const int Ka = sa.ncontr, Kc = sc.ncontr, Pa = sa.nprim, Pc = sc.nprim;
+                std::vector<double> contracted(6 * (size_t)Ka * Kc);
+                double maximum = 0.0;
+                for (size_t ia = 0; ia < ca_list.size(); ++ia)
+                    for (size_t ic = 0; ic < cc_list.size(); ++ic) {
+                        std::fill(contracted.begin(), contracted.end(), 0.0);
+                        size_t k = qfirst[t];
+                        for (int pa = 0; pa < Pa; ++pa)
+                            for (int pc = 0; pc < Pc; ++pc) {
+                                const double alpha = b->exps[sa.prim_off + pa], gamma = b->exps[sc.prim_off + pc];
+                                // the primitive pair's blocks: k0 (B A|A B), k1 A+1, k2 A-1, k3 B+1, k4 B-1
+                                const size_t k0 = k++, k1 = k++, k2 = sa.l > 0 ? k++ : k0, k3 = k++,
+                                             k4 = sc.l > 0 ? k++ : k0;
+                                // sqrt|(C x|x C)| of block kb at AO pair (cc_i, ca_i) with
+                                // the unshifted normalization restored on the shifted side
+                                auto norm = [&](size_t kb, int na, int nc, int ai, int ci, double ratio) {
+                                    const uint64_t e = (((uint64_t)ci * na + ai) * na + ai) * nc + ci;
+                                    const double* v = blocks.data() + 2 * (offs[kb] + e);
+                                    return std::sqrt(std::hypot(v[0], v[1])) * ratio;
+                                };
+                                const auto& pa3 = ca_list[ia];
+                                const auto& pc3 = cc_list[ic];
+                                const int na0 = (int)ca_list.size(), nc0 = (int)cc_list.size();
+                                const double same = norm(k0, na0, nc0, (int)ia, (int)ic, 1.0);
+                                double pbnd[6];
+                                for (int slot = 0; slot < 2; ++slot) {
+                                    const auto& raw = slot == 0 ? pa3 : pc3;
+                                    const double ex = slot == 0 ? alpha : gamma;
+                                    const int l = slot == 0 ? sa.l : sc.l;
+                                    const double* cen = slot == 0 ? sa.center : sc.center;
+                                    const double n0 = primitive_normalization(ex, raw[0], raw[1], raw[2]);
+                                    // norm with the slot's function at powers w (|w| = l +- 1)
+                                    auto changed = [&](std::array<int, 3> w, bool up) {
+                                        const int lw = l + (up ? 1 : -1);
+                                        const int idx = (lw - w[0]) * (lw - w[0] + 1) / 2 + w[2];
+                                        const int nw = (lw + 1) * (lw + 2) / 2;
+                                        const double ratio = n0 / primitive_normalization(ex, w[0], w[1], w[2]);
+                                        if (slot == 0) return norm(up ? k1 : k2, nw, nc0, idx, (int)ic, ratio);
+                                        return norm(up ? k3 : k4, na0, nw, (int)ia, idx, ratio);
+                                    };
+                                    for (int axis = 0; axis < 3; ++axis) {
+                                        std::array<int, 3> w = raw;
+                                        w[axis] += 1;
+                                        double bound = 2.0 * ex * changed(w, true);
+                                        if (raw[axis] != 0) {
+                                            w = raw;
+                                            w[axis] -= 1;
+                                            bound += (double)raw[axis] * changed(w, false);
+                                        }
+                                        const double e3[3] = {axis == 0 ? 1.0 : 0.0, axis == 1 ? 1.0 : 0.0,
+                                                              axis == 2 ? 1.0 : 0.0};
+                                        const double phase[3] = {field[1] * e3[2] - field[2] * e3[1],
+                                                                 field[2] * e3[0] - field[0] * e3[2],
+                                                                 field[0] * e3[1] - field[1] * e3[0]};
+                                        for (int y = 0; y < 3; ++y) {
+                                            const double weight = 0.5 * std::abs(phase[y]);
+                                            if (weight == 0.0) continue;
+                                            w = raw;
+                                            w[y] += 1;
+                                            bound += weight * (changed(w, true) + std::abs(cen[y]) * same);
+                                        }
+                                        pbnd[slot * 3 + axis] = bound;
+                                    }
+                                }
+                                for (int xa = 0; xa < Ka; ++xa)
+                                    for (int xc = 0; xc < Kc; ++xc) {
+                                        const double weight = std::abs(b->coefs[sa.coef_off + (size_t)xa * Pa + pa] *
+                                                                       b->coefs[sc.coef_off + (size_t)xc * Pc + pc]);
+                                        for (int m = 0; m < 6; ++m)
+                                            contracted[((size_t)xa * Kc + xc) * 6 + m] += weight * pbnd[m];
+                                    }
+                            }
+                        for (double v : contracted) maximum = std::max(maximum, v);
+                    }
+                const double bound = std::nextafter(maximum * (1.0 + 128.0 * 2.220446049250313e-16),
+                                                    std::numeric_limits<double>::infinity());
+                out[(size_t)a * ns + c] = bound;
+                out[(size_t)c * ns + a] = bound;
+            }
+        });
+        i = j;
+    }
+    return GIAOHIP_OK;
 }
 
 int giaohip_schwarz_bounds(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], double* out) {

@@ -151,6 +151,23 @@ hipError_t launch_mfma(int lab, int lcd, const WorkItem* items, int n, const Dev
                        const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                        hipStream_t s);
 // p[0..n) = 0 by a kernel on stream s.
+// ERI centre derivatives (gh_deriv.hip): one derivative quartet's block of
+// 12 x size values formed from the shifted-momentum integrals in `t`.
+struct DerivItem {
+    long long out;       // output offset of the (4, 3, na, nb, nc, nd) block
+    long long base;      // offset of (ab|cd) in t
+    long long plus[4];   // offset of the slot-k (l + 1) block in t
+    long long minus[4];  // offset of the slot-k (l - 1) block in t (unused for l = 0)
+    long long size;      // na nb nc nd
+    int n[4];            // AO counts of the four shells
+    int l[4];
+    int krow[4];         // contraction rows of the shells
+    int prow[4];         // rows of the (l + 1) shells: krow, or 2 krow with the phase rows
+    int first_tile, pad;
+    double center[4][3];
+};
+hipError_t launch_deriv_assemble(const DerivItem* items, const int* tile_item, int ntiles, const zc* t, zc* out,
+                                 const double B[3], bool phase, hipStream_t s);
 hipError_t launch_zero_ints(int* p, int n, hipStream_t s);
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
                        const double* exps, const double* kappa, zc* seeds, DevError* err, int* scratch,

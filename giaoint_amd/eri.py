@@ -12,7 +12,8 @@ from math import prod
 
 import numpy as np
 
-from .core import Basis, MagneticField, _eri_batch, _eri_full, _EriSchwarzBounds, _threads
+from .core import (Basis, MagneticField, _eri_batch, _eri_derivative_batch, _eri_full, _EriDerivativeSchwarzBounds,
+                   _EriSchwarzBounds, _threads)
 
 
 @dataclass(frozen=True)
@@ -118,3 +119,48 @@ def eri(basis: Basis, *, field: MagneticField | None = None, storage: str = "blo
 def eri_schwarz_bounds(basis: Basis, *, field: MagneticField | None = None) -> np.ndarray:
     """Shell-pair Schwarz factors (eri.py:204-208)."""
     return np.asarray(_EriSchwarzBounds(basis, field=field).values, dtype=np.float64)
+
+
+@dataclass(frozen=True)
+class EriDerivativeBatch(EriBatch):
+    """Packed local centre derivatives, blocks (4, 3, na, nb, nc, nd)
+    (eri.py:212-217): centre slots in quartet order, independent even for
+    repeated shells; block() is a view of values; counts include screened
+    requests."""
+
+
+def eri_derivative_batches(basis: Basis, *, field: MagneticField | None = None, quartets=None,
+                           target_bytes: int = 64 << 20, screening_threshold: float = 0.0, threads: int | None = None):
+    """Stream ERI centre derivatives with the London phase response, at
+    fixed B and gauge origin, without a full derivative AO tensor
+    (eri.py:220-288). Canonical quartets when quartets is None, else the given
+    ones in order; about target_bytes of packed values per batch (one
+    oversized quartet alone). A positive screening_threshold drops quartets
+    whose derivative-density Schwarz bound is below it."""
+    if isinstance(target_bytes, (bool, np.bool_)) or not isinstance(target_bytes, (int, np.integer)) \
+            or target_bytes <= 0:
+        raise ValueError("target_bytes must be a positive integer")
+    threshold = float(screening_threshold)
+    if not np.isfinite(threshold) or threshold < 0:
+        raise ValueError("screening_threshold must be finite and non-negative")
+    _threads(threads)
+    bounds = _EriDerivativeSchwarzBounds(basis, field=field, threads=threads) if threshold > 0 else None
+    shells = basis.shells
+    pending: list = []
+    size = 0
+
+    def emit():
+        q, shapes, offsets, values, requested, screened = _eri_derivative_batch(
+            basis, pending, field=field, screening_threshold=threshold, threads=threads, bounds=bounds)
+        return EriDerivativeBatch(q, shapes, offsets, values, requested, screened)
+
+    for q in _quartets(len(shells), quartets):
+        block = 12 * prod(shells[i].ao_count for i in q) * 16
+        if pending and size + block > target_bytes:
+            yield emit()
+            pending.clear()
+            size = 0
+        pending.append(q)
+        size += block
+    if pending:
+        yield emit()

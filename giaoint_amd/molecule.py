@@ -285,3 +285,12 @@ class Molecule:
         """Canonical shell-quartet batches with the molecule's field."""
         return _eri.eri_batches(self._basis, field=self._field, target_bytes=target_bytes,
                                 screening_threshold=screening_threshold, threads=threads)
+
+    def eri_derivative_batches(self, *, quartets=None, target_bytes: int = 64 * 1024**2,
+                               screening_threshold: float = 0.0, threads=None):
+        """ERI centre-derivative batches with the molecule's field; blocks
+        (4, 3, na, nb, nc, nd), slots in quartet order (shell_atoms maps
+        shells to atoms; repeated slots add)."""
+        return _eri.eri_derivative_batches(self._basis, field=self._field, quartets=quartets,
+                                           target_bytes=target_bytes, screening_threshold=screening_threshold,
+                                           threads=threads)

@@ -543,6 +543,79 @@ inline void compute_eri_tensor(const Basis& basis, const MagneticField& field, s
                                               0));
 }
 
+// --- ERI centre derivatives (derivatives.hpp:226-261) ------------------------
+// compute_eri_center_derivatives (derivatives.cpp:952-959): the (4, 3, na,
+// nb, nc, nd) block of (ab|cd), London phase response included, through the
+// workspace's scratch basis.
+inline void compute_eri_center_derivatives(const Shell& a, const Shell& b, const Shell& c, const Shell& d,
+                                           const MagneticField& field, std::span<Complex> output,
+                                           EriWorkspace& workspace) {
+    if (output.size() != 12U * shell_quartet_size(a, b, c, d))
+        throw std::length_error("center-derivative shell-quartet output has an incorrect size");
+    giaohip_ctx* ctx = hip_detail::context();
+    giaohip_basis* four = workspace.load(ctx, {&a, &b, &c, &d});
+    const std::uint32_t q[4] = {0, 1, 2, 3};
+    const auto f = hip_detail::field6(field.B, field.gauge_origin);
+    hip_detail::check(ctx, giaohip_eri_derivative_batch(ctx, four, q, 1, f.data(), nullptr,
+                                                        reinterpret_cast<double*>(output.data()), output.size(), 0));
+}
+inline void compute_eri_center_derivatives(const Shell& a, const Shell& b, const Shell& c, const Shell& d,
+                                           const MagneticField& field, std::span<Complex> output) {
+    EriWorkspace workspace;
+    compute_eri_center_derivatives(a, b, c, d, field, output, workspace);
+}
+
+// EriDerivativeSchwarzBounds (derivatives.hpp:244-255, derivatives.cpp:
+// 1307-1387): both tables evaluated on the device.
+class EriDerivativeSchwarzBounds {
+  public:
+    explicit EriDerivativeSchwarzBounds(const Basis& basis, const MagneticField& field = {}, std::size_t = 1U)
+        : ordinary_(basis, field), values_(basis.shells().size() * basis.shells().size()) {
+        giaohip_ctx* ctx = hip_detail::context();
+        const auto f = hip_detail::field6(field.B, field.gauge_origin);
+        hip_detail::check(ctx, giaohip_eri_derivative_pair_bounds(ctx, basis.device(), f.data(), values_.data()));
+    }
+    [[nodiscard]] double quartet_bound(ShellQuartetIndex quartet) const {
+        const auto n = shell_count();
+        for (const auto index : quartet.as_array())
+            if (index >= n) throw std::out_of_range("derivative shell quartet index is out of range");
+        const double bound = std::max(values_[quartet.a * n + quartet.b] * ordinary_(quartet.c, quartet.d),
+                                      ordinary_(quartet.a, quartet.b) * values_[quartet.c * n + quartet.d]);
+        return std::nextafter(bound * (1.0 + 128.0 * std::numeric_limits<double>::epsilon()),
+                              std::numeric_limits<double>::infinity());
+    }
+    [[nodiscard]] bool matches(const Basis& basis, const MagneticField& field) const noexcept {
+        return ordinary_.matches(basis, field);
+    }
+    [[nodiscard]] std::span<const double> values() const noexcept { return values_; }
+    [[nodiscard]] std::size_t shell_count() const noexcept { return ordinary_.shell_count(); }
+
+  private:
+    EriSchwarzBounds ordinary_;
+    std::vector<double> values_;
+};
+
+// compute_eri_derivative_blocks (derivatives.cpp:1389-1415): concatenated
+// (4, 3, na, nb, nc, nd) blocks at explicit cumulative offsets (n + 1 of
+// them); the checks and messages are the reference's, made by the library.
+inline void compute_eri_derivative_blocks(const Basis& basis, std::span<const ShellQuartetIndex> quartets,
+                                          std::span<const std::size_t> offsets, const MagneticField& field,
+                                          std::span<Complex> output, std::size_t = 1U) {
+    if (offsets.size() != quartets.size() + 1U || offsets[0] != 0U)
+        throw std::length_error("derivative batch offsets have an incorrect size or start");
+    giaohip_ctx* ctx = hip_detail::context();
+    std::vector<std::uint32_t> q(4 * quartets.size());
+    for (std::size_t i = 0; i < quartets.size(); ++i) {
+        const auto x = quartets[i].as_array();
+        std::copy(x.begin(), x.end(), q.begin() + 4 * i);
+    }
+    const std::vector<std::uint64_t> offs(offsets.begin(), offsets.end());
+    const auto f = hip_detail::field6(field.B, field.gauge_origin);
+    hip_detail::check(ctx, giaohip_eri_derivative_batch(ctx, basis.device(), q.data(), quartets.size(), f.data(),
+                                                        offs.data(), reinterpret_cast<double*>(output.data()),
+                                                        output.size(), 0));
+}
+
 // Point charge (types.hpp:134-139), validated as types.cpp:188-194 does.
 struct Nucleus {
     double charge{};

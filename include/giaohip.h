@@ -216,6 +216,35 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32
 int giaohip_eri_batch_noalloc(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32_t* quartets, size_t n,
                               const double field[6], const uint64_t* offsets, double* out_device);
 
+/* Replaces compute_eri_derivative_blocks (derivatives.cpp:1389-1415, decl.
+ * derivatives.hpp:257-261; Python binding _eri_derivative_batch,
+ * bindings.cpp:2255-2304): centre derivatives of the ERIs of n shell
+ * quartets, London phase response included at fixed B and gauge origin.
+ * Block i is (4, 3, na, nb, nc, nd) complex values (quartet slot, axis,
+ * AOs), slots in quartet order even for repeated shells, at out +
+ * 2*offsets[i]. offsets holds n + 1 entries (complex elements) with
+ * offsets[0] = 0 and offsets[i+1] - offsets[i] = 12 * block size, or is NULL
+ * (packed); out_len must equal offsets[n]. Errors, in the reference's order
+ * and words: GIAOHIP_LENGTH_ERROR "derivative batch offsets have an incorrect
+ * size or start" / "... do not match block sizes" / "derivative batch output
+ * has an incorrect size", GIAOHIP_OUT_OF_RANGE "derivative shell quartet
+ * index is out of range", then the ERI class errors of the shifted classes
+ * (one more unit of angular momentum on each slot). out is device memory when
+ * out_on_device, else host memory. */
+int giaohip_eri_derivative_batch(giaohip_ctx* ctx, const giaohip_basis* basis, const uint32_t* quartets, size_t n,
+                                 const double field[6], const uint64_t* offsets, double* out, uint64_t out_len,
+                                 int out_on_device);
+
+/* Replaces the values of EriDerivativeSchwarzBounds (derivatives.cpp:
+ * 1307-1372, decl. derivatives.hpp:244-255): out = nshell*nshell doubles
+ * (host); out[a*nshell+b] bounds every centre derivative of the pair density
+ * (a b| by the triangle/Schwarz sum of the reference (shifted raw primitives,
+ * London terms, |contraction coefficients|), padded by 128 eps and one ulp.
+ * A quartet's bound is max(out[ab] Q[cd], Q[ab] out[cd]) with the ordinary
+ * Schwarz factors Q of giaohip_schwarz_bounds (derivatives.cpp:1374-1383). */
+int giaohip_eri_derivative_pair_bounds(giaohip_ctx* ctx, const giaohip_basis* basis, const double field[6],
+                                       double* out);
+
 /* Size in complex elements of the packed output of a quartet list. */
 int giaohip_eri_batch_size(const giaohip_basis* basis, const uint32_t* quartets, size_t n, uint64_t* total);
 

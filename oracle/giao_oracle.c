@@ -847,6 +847,259 @@ int or_schwarz_bounds(const or_basis* b, const double field[6], double* out) {
     return st;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Centre derivatives (derivatives.cpp)                                     */
+/* ------------------------------------------------------------------------ */
+
+/* derivatives.cpp:43-51: (B x e_axis) */
+static void field_cross_axis(const double field[6], int axis, double out[3]) {
+    const double e[3] = {axis == 0, axis == 1, axis == 2};
+    out[0] = field[1] * e[2] - field[2] * e[1];
+    out[1] = field[2] * e[0] - field[0] * e[2];
+    out[2] = field[0] * e[1] - field[1] * e[0];
+}
+
+/* derivatives.cpp:54-84 center_response: d/dA_axis of the raw primitive
+ * quartet P with respect to the centre of slot `slot` (slots 0 and 2 are the
+ * conjugated functions, London phase sign +1/2; slots 1 and 3 take -1/2). */
+static int center_response(const prim_t P[4], int slot, int axis, const double field[6], const pair_data* ab,
+                           const pair_data* cd, int real_zero, workspace* w, cplx* out) {
+    prim_t Q[4];
+    memcpy(Q, P, sizeof Q);
+    cplx result = 0.0, v;
+    int st;
+    const int a = P[slot].ang[axis];
+    if (a != 0) {
+        Q[slot].ang[axis] = a - 1;
+        if ((st = eri_kernel(&Q[0], &Q[1], &Q[2], &Q[3], ab, cd, real_zero, w, &v))) return st;
+        result -= (double)a * v;
+    }
+    Q[slot].ang[axis] = a + 1;
+    if ((st = eri_kernel(&Q[0], &Q[1], &Q[2], &Q[3], ab, cd, real_zero, w, &v))) return st;
+    result += 2.0 * P[slot].exponent * v;
+    Q[slot].ang[axis] = a;
+    double phase[3];
+    field_cross_axis(field, axis, phase);
+    const double sign = slot % 2 == 0 ? 0.5 : -0.5;
+    cplx base;
+    if ((st = eri_kernel(&P[0], &P[1], &P[2], &P[3], ab, cd, real_zero, w, &base))) return st;
+    for (int k = 0; k < 3; ++k) {
+        const double coef = sign * phase[k];
+        if (coef == 0.0) continue;
+        Q[slot].ang[k] += 1;
+        st = eri_kernel(&Q[0], &Q[1], &Q[2], &Q[3], ab, cd, real_zero, w, &v);
+        Q[slot].ang[k] -= 1;
+        if (st) return st;
+        const cplx cv = v + P[slot].center[k] * base;
+        result += CMPLX(-coef * cimag(cv), coef * creal(cv));
+    }
+    *out = result;
+    return OR_OK;
+}
+
+/* derivatives.cpp:556-660 compute_quartet_block with the evaluator of
+ * derivatives.cpp:232-241,836-842: block (4, 3, na, nb, nc, nd). Each raw
+ * primitive carries its normalization in the coefficient (raw_primitive,
+ * derivatives.cpp:30-37); contraction weights multiply the 12 responses. */
+static int compute_derivative_shell(const shell_view* sh[4], const double field[6], cplx* out, workspace* w) {
+    const shell_view *a = sh[0], *b = sh[1], *c = sh[2], *d = sh[3];
+    const size_t na = (size_t)a->ncontr * a->ncart, nb = (size_t)b->ncontr * b->ncart;
+    const size_t nc = (size_t)c->ncontr * c->ncart, nd = (size_t)d->ncontr * d->ncart;
+    const size_t size = na * nb * nc * nd;
+    for (size_t i = 0; i < 12 * size; ++i) out[i] = 0.0;
+    const int real_zero = use_real_path(field);
+    for (int pa = 0; pa < a->nprim; ++pa)
+        for (int pb = 0; pb < b->nprim; ++pb) {
+            pair_data ab;
+            gaussian_product(a->exps[pa], a->center, b->exps[pb], b->center, field, &ab);
+            for (int pc = 0; pc < c->nprim; ++pc)
+                for (int pd = 0; pd < d->nprim; ++pd) {
+                    pair_data cdp;
+                    gaussian_product(c->exps[pc], c->center, d->exps[pd], d->center, field, &cdp);
+                    for (int ca = 0; ca < a->ncart; ++ca)
+                        for (int cb = 0; cb < b->ncart; ++cb)
+                            for (int cc = 0; cc < c->ncart; ++cc)
+                                for (int cq = 0; cq < d->ncart; ++cq) {
+                                    prim_t P[4];
+                                    const shell_view* s4[4] = {a, b, c, d};
+                                    const int pp[4] = {pa, pb, pc, pd}, cp[4] = {ca, cb, cc, cq};
+                                    for (int k = 0; k < 4; ++k) {
+                                        make_prim(s4[k], pp[k], cp[k], &P[k]);
+                                        P[k].coefficient = s4[k]->norms[(size_t)cp[k] * s4[k]->nprim + pp[k]];
+                                    }
+                                    cplx values[12];
+                                    for (int slot = 0; slot < 4; ++slot)
+                                        for (int axis = 0; axis < 3; ++axis) {
+                                            const int st = center_response(P, slot, axis, field, &ab, &cdp, real_zero, w,
+                                                                           &values[slot * 3 + axis]);
+                                            if (st) return st;
+                                        }
+                                    for (int xa = 0; xa < a->ncontr; ++xa) {
+                                        const size_t ia = (size_t)xa * a->ncart + ca;
+                                        for (int xb = 0; xb < b->ncontr; ++xb) {
+                                            const size_t ib = (size_t)xb * b->ncart + cb;
+                                            for (int xc = 0; xc < c->ncontr; ++xc) {
+                                                const size_t ic = (size_t)xc * c->ncart + cc;
+                                                for (int xd = 0; xd < d->ncontr; ++xd) {
+                                                    const size_t id = (size_t)xd * d->ncart + cq;
+                                                    const double weight = a->coefs[(size_t)xa * a->nprim + pa] *
+                                                                          b->coefs[(size_t)xb * b->nprim + pb] *
+                                                                          c->coefs[(size_t)xc * c->nprim + pc] *
+                                                                          d->coefs[(size_t)xd * d->nprim + pd];
+                                                    const size_t e = ((ia * nb + ib) * nc + ic) * nd + id;
+                                                    for (int k = 0; k < 12; ++k) out[k * size + e] += weight * values[k];
+                                                }
+                                            }
+                                        }
+                                    }
+                                }
+                }
+        }
+    return OR_OK;
+}
+
+/* compute_eri_derivative_blocks (derivatives.cpp:1389-1415): offsets holds
+ * n + 1 entries starting at 0, block i spans offsets[i]..offsets[i+1] and
+ * has 12 x the quartet size; offsets[n] must equal out_len. */
+int or_eri_derivative_blocks(const or_basis* b, const uint32_t* quartets, size_t n, const double field[6],
+                             const uint64_t* offsets, double* out, size_t out_len, int threads) {
+    if (threads < 1) return fail(OR_INVALID_ARGUMENT, "ERI thread count must be positive");
+    if (offsets[0] != 0) return fail(OR_LENGTH_ERROR, "derivative batch offsets have an incorrect size or start");
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 4; ++k)
+            if (quartets[4 * i + k] >= (uint32_t)b->nshell)
+                return fail(OR_OUT_OF_RANGE, "derivative shell quartet index is out of range");
+        const size_t size = or_quartet_size(b, quartets + 4 * i);
+        if (offsets[i] > out_len || offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] != 12 * size)
+            return fail(OR_LENGTH_ERROR, "derivative batch offsets do not match block sizes");
+    }
+    if (offsets[n] != out_len) return fail(OR_LENGTH_ERROR, "derivative batch output has an incorrect size");
+    shell_view* sv;
+    int st = make_all_shells(b, &sv);
+    if (st) return st;
+    int first_error = OR_OK;
+    char first_msg[512] = {0};
+#pragma omp parallel num_threads(threads)
+    {
+        workspace w;
+        memset(&w, 0, sizeof w);
+#pragma omp for schedule(dynamic, 1)
+        for (long long i = 0; i < (long long)n; ++i) {
+            int failed;
+#pragma omp atomic read
+            failed = first_error;
+            if (failed) continue;
+            const uint32_t* q = quartets + 4 * i;
+            const shell_view* sh[4] = {&sv[q[0]], &sv[q[1]], &sv[q[2]], &sv[q[3]]};
+            const int s = compute_derivative_shell(sh, field, (cplx*)(out + 2 * offsets[i]), &w);
+            if (s) {
+#pragma omp critical
+                {
+                    if (!first_error) {
+                        first_error = s;
+                        memcpy(first_msg, g_msg, sizeof first_msg);
+                    }
+                }
+            }
+        }
+        ws_free(&w);
+    }
+    free_all_shells(b, sv);
+    if (first_error) {
+        memcpy(g_msg, first_msg, sizeof g_msg);
+        return first_error;
+    }
+    return OR_OK;
+}
+
+/* EriDerivativeSchwarzBounds (derivatives.cpp:1307-1372): out[a*ns+b] bounds
+ * every centre derivative of the pair density (a b| over slots 0 and 1 by a
+ * triangle/Schwarz sum of sqrt|(right left|left right)| of the shifted raw
+ * primitives, contracted with |coefficients|, padded by 128 eps and one ulp. */
+int or_eri_derivative_pair_bounds(const or_basis* bs, const double field[6], double* out) {
+    const int ns = bs->nshell;
+    shell_view* sv;
+    int st = make_all_shells(bs, &sv);
+    if (st) return st;
+    workspace w;
+    memset(&w, 0, sizeof w);
+    const int real_zero = use_real_path(field);
+    /* norm(left, right) of derivatives.cpp:1325-1333 */
+#define DERIV_NORM(L, R, OUT)                                                               \
+    do {                                                                                    \
+        pair_data p1, p2;                                                                   \
+        gaussian_product((R).exponent, (R).center, (L).exponent, (L).center, field, &p1);   \
+        gaussian_product((L).exponent, (L).center, (R).exponent, (R).center, field, &p2);   \
+        cplx v_;                                                                            \
+        st = eri_kernel(&(R), &(L), &(L), &(R), &p1, &p2, real_zero, &w, &v_);              \
+        (OUT) = sqrt(cabs(v_));                                                             \
+    } while (0)
+    for (int ia = 0; ia < ns && !st; ++ia)
+        for (int ib = 0; ib < ns && !st; ++ib) {
+            const shell_view *a = &sv[ia], *b = &sv[ib];
+            double maximum = 0.0;
+            double* contracted = calloc((size_t)6 * a->ncontr * b->ncontr, sizeof(double));
+            for (int ca = 0; ca < a->ncart && !st; ++ca)
+                for (int cb = 0; cb < b->ncart && !st; ++cb) {
+                    memset(contracted, 0, sizeof(double) * 6 * (size_t)a->ncontr * b->ncontr);
+                    for (int pa = 0; pa < a->nprim && !st; ++pa)
+                        for (int pb = 0; pb < b->nprim && !st; ++pb) {
+                            prim_t ga, gb;
+                            make_prim(a, pa, ca, &ga);
+                            make_prim(b, pb, cb, &gb);
+                            ga.coefficient = a->norms[(size_t)ca * a->nprim + pa];
+                            gb.coefficient = b->norms[(size_t)cb * b->nprim + pb];
+                            double pbnd[6];
+                            for (int slot = 0; slot < 2 && !st; ++slot) {
+                                const prim_t raw = slot == 0 ? ga : gb;
+                                for (int axis = 0; axis < 3 && !st; ++axis) {
+                                    prim_t ch = raw;
+                                    double nv;
+                                    ch.ang[axis] += 1;
+                                    if (slot == 0) DERIV_NORM(ch, gb, nv); else DERIV_NORM(ga, ch, nv);
+                                    double bound = 2.0 * raw.exponent * nv;
+                                    if (raw.ang[axis] != 0) {
+                                        ch = raw;
+                                        ch.ang[axis] -= 1;
+                                        if (slot == 0) DERIV_NORM(ch, gb, nv); else DERIV_NORM(ga, ch, nv);
+                                        bound += (double)raw.ang[axis] * nv;
+                                    }
+                                    double phase[3];
+                                    field_cross_axis(field, axis, phase);
+                                    for (int k = 0; k < 3 && !st; ++k) {
+                                        const double weight = 0.5 * fabs(phase[k]);
+                                        if (weight == 0.0) continue;
+                                        double up, same;
+                                        ch = raw;
+                                        ch.ang[k] += 1;
+                                        if (slot == 0) DERIV_NORM(ch, gb, up); else DERIV_NORM(ga, ch, up);
+                                        DERIV_NORM(ga, gb, same);
+                                        bound += weight * (up + fabs(raw.center[k]) * same);
+                                    }
+                                    pbnd[slot * 3 + axis] = bound;
+                                }
+                            }
+                            for (int xa = 0; xa < a->ncontr; ++xa)
+                                for (int xb = 0; xb < b->ncontr; ++xb) {
+                                    const double weight = fabs(a->coefs[(size_t)xa * a->nprim + pa] *
+                                                               b->coefs[(size_t)xb * b->nprim + pb]);
+                                    for (int k = 0; k < 6; ++k)
+                                        contracted[((size_t)xa * b->ncontr + xb) * 6 + k] += weight * pbnd[k];
+                                }
+                        }
+                    for (size_t k = 0; k < (size_t)6 * a->ncontr * b->ncontr; ++k)
+                        if (contracted[k] > maximum) maximum = contracted[k];
+                }
+            free(contracted);
+            out[(size_t)ia * ns + ib] =
+                nextafter(maximum * (1.0 + 128.0 * 2.220446049250313e-16), INFINITY);
+        }
+#undef DERIV_NORM
+    ws_free(&w);
+    free_all_shells(bs, sv);
+    return st;
+}
+
 /* eri.cpp:414-429 */
 int or_canonicalize(const uint32_t in[4], uint32_t out[4]) {
     const uint32_t cand[4][4] = {{in[0], in[1], in[2], in[3]},

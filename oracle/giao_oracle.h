@@ -96,6 +96,16 @@ int or_canonicalize(const uint32_t in[4], uint32_t out[4]);
 /* EriSchwarzBounds (eri.cpp:587-620): out = nshell*nshell doubles. */
 int or_schwarz_bounds(const or_basis* basis, const double field[6], double* out);
 
+/* compute_eri_derivative_blocks (derivatives.cpp:1389-1415): offsets has
+ * n + 1 entries (complex elements, offsets[0] = 0), block i is (4, 3, na,
+ * nb, nc, nd) at out + 2*offsets[i]; out_len = offsets[n]. */
+int or_eri_derivative_blocks(const or_basis* basis, const uint32_t* quartets, size_t n,
+                             const double field[6], const uint64_t* offsets, double* out,
+                             size_t out_len, int threads);
+
+/* EriDerivativeSchwarzBounds values (derivatives.cpp:1307-1372): nshell*nshell. */
+int or_eri_derivative_pair_bounds(const or_basis* basis, const double field[6], double* out);
+
 /* cholesky.cpp:285-314. pairs: npairs*(a,b) with a<=b. out: n*n doubles. */
 int or_cholesky_diagonal(const or_basis* basis, const uint64_t* pairs,
                          size_t npairs, const double field[6], double* out,

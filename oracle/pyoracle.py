@@ -174,6 +174,30 @@ class Checker:
         )
         return out[: 2 * int(offsets[-1])].view(np.complex128), offsets
 
+    def eri_derivative_blocks(self, basis: BasisArrays, quartets, field, threads=1, offsets=None, out_len=None):
+        """compute_eri_derivative_blocks: packed (4, 3, na, nb, nc, nd) blocks
+        and the n + 1 offsets (complex elements)."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.uint32).reshape(-1, 4))
+        if offsets is None:
+            sizes = np.array([12 * basis.quartet_size(x) for x in q], dtype=np.uint64)
+            offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n_out = int(offsets[-1]) if out_len is None else int(out_len)
+        out = np.zeros(2 * max(n_out, 1))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self._check(
+            self._fn("eri_derivative_blocks")(
+                C.byref(basis.struct), _ptr(q, C.c_uint32), C.c_size_t(len(q)), _ptr(f), _ptr(offsets, C.c_uint64),
+                _ptr(out), C.c_size_t(n_out), C.c_int(threads)))
+        return out[: 2 * n_out].view(np.complex128), offsets
+
+    def eri_derivative_pair_bounds(self, basis: BasisArrays, field):
+        ns = len(basis.L)
+        out = np.zeros((ns, ns))
+        f = np.ascontiguousarray(field, dtype=np.float64)
+        self._check(self._fn("eri_derivative_pair_bounds")(C.byref(basis.struct), _ptr(f), _ptr(out)))
+        return out
+
     def schwarz_bounds(self, basis: BasisArrays, field):
         ns = len(basis.L)
         out = np.zeros((ns, ns))

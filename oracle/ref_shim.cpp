@@ -3,6 +3,7 @@
 // ONLY: used to pin the C oracle and as the "reference" CPU baseline. The
 // signatures equal those of giao_oracle.h with the prefix ref_ instead of or_.
 #include "giao_integrals/boys.hpp"
+#include "giao_integrals/derivatives.hpp"
 #include "giao_integrals/eri.hpp"
 #include "giao_integrals/nuclear.hpp"
 #include "cholesky.hpp"
@@ -206,6 +207,39 @@ int ref_nuclear_matrix(const RefBasis* b, int nnuc, const double* charges, const
             nuclei.emplace_back(charges[c], giao::Vec3{centers[3 * c], centers[3 * c + 1], centers[3 * c + 2]});
         giao::compute_nuclear_attraction_matrix(basis, nuclei, make_field(field),
                                                 std::span<giao::Complex>(reinterpret_cast<giao::Complex*>(out), n * n));
+    });
+}
+
+int ref_eri_derivative_blocks(const RefBasis* b, const std::uint32_t* quartets, std::size_t n, const double field[6],
+                              const std::uint64_t* offsets, double* out, std::size_t out_len, int threads) {
+    return guarded([&] {
+        const auto basis = make_basis(b);
+        std::vector<giao::ShellQuartetIndex> q(n);
+        for (std::size_t i = 0; i < n; ++i)
+            q[i] = {quartets[4 * i], quartets[4 * i + 1], quartets[4 * i + 2], quartets[4 * i + 3]};
+        const std::vector<std::size_t> offs(offsets, offsets + n + 1);
+        giao::compute_eri_derivative_blocks(basis, q, offs, make_field(field),
+                                            {reinterpret_cast<giao::Complex*>(out), out_len},
+                                            static_cast<std::size_t>(threads));
+    });
+}
+
+int ref_eri_derivative_pair_bounds(const RefBasis* b, const double field[6], double* out) {
+    return guarded([&] {
+        const auto basis = make_basis(b);
+        const giao::EriDerivativeSchwarzBounds bounds(basis, make_field(field));
+        const auto v = bounds.values();
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+int ref_eri_derivative_quartet_bounds(const RefBasis* b, const double field[6], const std::uint32_t* quartets,
+                                      std::size_t n, double* out) {
+    return guarded([&] {
+        const auto basis = make_basis(b);
+        const giao::EriDerivativeSchwarzBounds bounds(basis, make_field(field));
+        for (std::size_t i = 0; i < n; ++i)
+            out[i] = bounds.quartet_bound({quartets[4 * i], quartets[4 * i + 1], quartets[4 * i + 2], quartets[4 * i + 3]});
     });
 }
 

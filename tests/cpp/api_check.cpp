@@ -181,6 +181,33 @@ int main(int argc, char** argv) {
         std::printf("warm_allocations %llu\n", after - before);
         std::printf("warm_us_per_call %.3f\n", std::chrono::duration<double, std::micro>(t1 - t0).count() / 1000.0);
     }
+    // centre derivatives: one block, a packed batch, and the derivative bounds
+    {
+        std::vector<Complex> dblock(12 * shell_quartet_size(sh[0], sh[1], sh[2], sh[3]));
+        compute_eri_center_derivatives(sh[0], sh[1], sh[2], sh[3], field, dblock, ws);
+        dump(dir + "/deriv_block.bin", dblock);
+        const std::vector<ShellQuartetIndex> dq{{0, 1, 2, 3}, {3, 3, 1, 0}, {2, 2, 2, 2}};
+        std::vector<std::size_t> offs{0};
+        for (const auto& q : dq)
+            offs.push_back(offs.back() + 12 * shell_quartet_size(sh[q.a], sh[q.b], sh[q.c], sh[q.d]));
+        std::vector<Complex> dpacked(offs.back());
+        compute_eri_derivative_blocks(basis, dq, offs, field, dpacked);
+        dump(dir + "/deriv_packed.bin", dpacked);
+        const EriDerivativeSchwarzBounds dbounds(basis, field);
+        dump(dir + "/deriv_bounds.bin", std::vector<double>(dbounds.values().begin(), dbounds.values().end()));
+        std::printf("deriv_quartet_bound %.17g\n", dbounds.quartet_bound({0, 1, 2, 3}));
+        expect("deriv_bad_offsets", [&] {
+            std::vector<std::size_t> bad(offs);
+            bad[1] += 1;
+            compute_eri_derivative_blocks(basis, dq, bad, field, dpacked);
+        });
+        expect("deriv_bad_start", [&] {
+            std::vector<std::size_t> bad(offs);
+            bad[0] = 1;
+            compute_eri_derivative_blocks(basis, dq, bad, field, dpacked);
+        });
+        expect("deriv_bad_index", [&] { (void)dbounds.quartet_bound({0, 0, 0, ns}); });
+    }
     // error mapping
     expect("bad_output_size", [&] {
         std::vector<Complex> small(3);

@@ -69,6 +69,19 @@ def test_cpp_layer_against_oracle(tmp_path, port):
     want, _ = port.eri_batch(b, [q], f, 4)
     assert np.max(np.abs(c128("warm_last.bin") - want)) <= ATOL
     print(f"compute_eri warm: {float(r['warm_us_per_call']):.1f} us per call", flush=True)
+    # centre derivatives (derivatives.cpp:952-959,1307-1415)
+    want, _ = port.eri_derivative_blocks(b, [(0, 1, 2, 3)], f, 4)
+    scale = float(np.max(np.abs(want)))
+    assert np.max(np.abs(c128("deriv_block.bin") - want)) <= 1e-11 * scale
+    want, _ = port.eri_derivative_blocks(b, [(0, 1, 2, 3), (3, 3, 1, 0), (2, 2, 2, 2)], f, 4)
+    assert np.max(np.abs(c128("deriv_packed.bin") - want)) <= 1e-11 * float(np.max(np.abs(want)))
+    D = np.fromfile(tmp_path / "deriv_bounds.bin").reshape(4, 4)
+    np.testing.assert_allclose(D, port.eri_derivative_pair_bounds(b, f), rtol=1e-12)
+    qb = max(D[0, 1] * Q[2, 3], Q[0, 1] * D[2, 3])
+    assert float(r["deriv_quartet_bound"]) == np.nextafter(qb * (1 + 128 * np.finfo(float).eps), np.inf)
+    assert r["deriv_bad_offsets"] == "length_error derivative batch offsets do not match block sizes"
+    assert r["deriv_bad_start"] == "length_error derivative batch offsets have an incorrect size or start"
+    assert r["deriv_bad_index"] == "out_of_range derivative shell quartet index is out of range"
     assert r["bad_output_size"] == "length_error ERI output has an incorrect size"
     assert r["bad_index"] == "out_of_range ERI shell-quartet index is out of range"
     assert r["null_consumer"] == "invalid_argument ERI quartet consumer must not be null"
