@@ -70,6 +70,7 @@ def parse():
     ap.add_argument("--high-l-ffdd", type=int, default=200_000, help="(ff|dd) quartets of the high-l leg (0 skips)")
     ap.add_argument("--mixed", type=int, default=2_000_000,
                     help="mixed-class leg: quartets of the l<=3, degree 1..8 workload (0 skips it)")
+    ap.add_argument("--no-derivatives", action="store_true", help="skip the ERI centre-derivative leg")
     ap.add_argument("--no-cpu-legs", action="store_true",
                     help="skip the CPU baselines of the other workloads (water tensor, mixed, high l, Cholesky)")
     return ap.parse_args()
@@ -336,6 +337,69 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
     out["split_fp32"] = ("not built: FP32 accumulation rounds each partial sum at 2^-24 relative, far above "
                          "the 1e-10 bar unless compensated in FP64 (which moves the work back to FP64 units)")
     return out
+
+
+def derivative_leg(target_s: float, cpu: bool):
+    """ERI centre derivatives (compute_eri_derivative_blocks) of every
+    canonical shell quartet of water in cc-pVTZ (65 AOs, general
+    contractions, up to f) at B = (0.05, -0.03, 0.2), gauge (0.37, -0.29,
+    0.43), blocks resident on the device: a warmup call, then one timed call.
+    CPU: the reference's own derivatives.cpp (oracle/_ref) over every k-th
+    quartet, sized to about target_s on the physical cores."""
+    from giaoint_amd import core, workloads
+    from giaoint_amd.eri import _quartets
+    from giaoint_amd.molecule import Molecule
+
+    atoms = [("O", (0.0, 0.1, 0.2)), ("H", (0.1, 1.4, -0.9)), ("H", (-0.2, -1.3, -1.0))]
+    field = core.MagneticField((0.05, -0.03, 0.2), (0.37, -0.29, 0.43))
+    mol = Molecule(atoms, workloads.DATA / "cc-pvtz-h-o.json", unit="bohr", field=field)
+    dev = mol.basis._dev()
+    ctx = dev.ctx
+    qs = np.array(list(_quartets(len(mol.basis.shells), None)), dtype=np.uint32)
+    total = 12 * int(dev.quartet_sizes(qs).sum())
+    f = np.asarray(field._field6(), dtype=np.float64)
+    out = ctx.alloc(total * 16)
+    try:
+        dev.eri_derivative_batch(qs, f, device_out=out)
+        ctx.synchronize()
+        t = time.perf_counter()
+        dev.eri_derivative_batch(qs, f, device_out=out)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        device_ms = ctx.stats()["total_ms"]
+        checksum = float(np.abs(ctx.to_host(out, np.empty(min(total, 1 << 22), dtype=np.complex128))).sum())
+    finally:
+        ctx.free(out)
+    rec = {"workload": f"water cc-pVTZ ({mol.ao_count} AOs, {len(mol.basis.shells)} shells), all {len(qs)} "
+                       "canonical shell quartets, 12 centre derivatives each, London phase at B = (0.05, -0.03, 0.2)",
+           "quartets_per_s": len(qs) / dt, "seconds": dt, "device_ms": device_ms,
+           "output_gb_per_s": total * 16 / dt / 1e9, "output_values": total, "checksum": checksum}
+    log(f"derivatives: {len(qs)} quartets in {dt:.3f} s")
+    if cpu:
+        try:
+            from oracle import pyoracle
+
+            kind, chk = _checker()
+            threads = physical_cores()
+            # the molecule's shells with their contraction-normalized coefficients
+            shells = [(s.center, s.angular_momentum, s._exps, s._coefs) for s in mol.basis.shells]
+            arrays = pyoracle.BasisArrays(shells)
+            t0 = time.perf_counter()
+            chk.eri_derivative_blocks(arrays, qs[::97][:threads], f, threads=threads)
+            dt0 = time.perf_counter() - t0
+            per = dt0 / min(threads, len(qs[::97]))
+            step = max(1, int(np.ceil(len(qs) * per / target_s)))
+            sample = qs[::step]
+            t0 = time.perf_counter()
+            chk.eri_derivative_blocks(arrays, sample, f, threads=threads)
+            cdt = time.perf_counter() - t0
+            rec["cpu_baseline"] = {"value": len(sample) / cdt, "unit": "derivative shell quartets/s", "cores": threads,
+                                   "kind": kind, "sample": f"every {step}th canonical quartet ({len(sample)}), "
+                                                           f"{cdt:.1f} s"}
+            rec["gpu_over_cpu"] = rec["quartets_per_s"] / rec["cpu_baseline"]["value"]
+        except Exception as exc:  # the baseline is reported, never required
+            rec["cpu_baseline"] = {"error": str(exc)}
+    return rec
 
 
 def projection_roofline(nbytes: float, flops: float, block_ms: float, unblocked_bytes: float) -> dict:
@@ -623,6 +687,8 @@ def main():
         result["mixed_class"], mixed = mixed_leg(args.mixed, tune=not args.no_autotune)
     if args.high_l_gggg > 0 or args.high_l_ffdd > 0:
         result["high_l"] = high_l_leg(args.high_l_gggg, args.high_l_ffdd)
+    if not args.no_derivatives:
+        result["eri_derivatives"] = derivative_leg(args.cpu_seconds, not args.no_cpu and not args.no_cpu_legs)
     if args.cholesky_rank >= 0:
         result["cholesky"] = cholesky_leg(args.cholesky_rank, args.cholesky_block,
                                           0 if args.no_autotune else args.tune_cholesky_rank)

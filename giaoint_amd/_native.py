@@ -246,6 +246,13 @@ class Context:
         (every class with an MFMA instance)."""
         self.check(self.lib.giaohip_set_option(self.handle, 7, self.MFMA_MODES[mode]))
 
+    PIPELINES = {"overlap": 0, "serial": 1}
+
+    def set_pipeline(self, mode: str):
+        """Producer scheduling of the batched path: "overlap" (second stream)
+        or "serial" (context stream)."""
+        self.check(self.lib.giaohip_set_option(self.handle, 8, self.PIPELINES[mode]))
+
     def stats(self) -> dict:
         s = Stats()
         self.check(self.lib.giaohip_last_stats(self.handle, C.byref(s)))

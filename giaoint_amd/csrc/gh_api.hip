@@ -221,6 +221,7 @@ struct giaohip_ctx {
     int eri_queue = GIAOHIP_QUEUE_NONE;  // generic-kernel launch form (GIAOHIP_OPTION_ERI_QUEUE)
     int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
     int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
+    int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     void* noalloc = nullptr;                 // device address of `scratch` on this context's device
     rocblas_handle blas = nullptr;           // library GEMMs (exchange contraction), created on first use
@@ -935,7 +936,9 @@ bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
 // page-locked staging while the device works, and the host waits once.
 int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq>& reqs, zc* d_out) {
     if (reqs.empty()) return GIAOHIP_OK;
-    hipStream_t sc = ctx->stream, sp = ctx->aux;
+    // the producers run on the second stream (overlapping the previous
+    // chunk's contraction) or, serially, on the context stream
+    hipStream_t sc = ctx->stream, sp = ctx->pipeline == GIAOHIP_PIPELINE_SERIAL ? ctx->stream : ctx->aux;
     constexpr size_t kScratchHead = 4;  // zc holding the 16 counter ints of the quadrature lists
     struct Need {
         size_t seed, r0, tab, scr;
@@ -1435,6 +1438,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_ALL) {
         ctx->mfma_mode = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_PIPELINE && (value == GIAOHIP_PIPELINE_OVERLAP || value == GIAOHIP_PIPELINE_SERIAL)) {
+        ctx->pipeline = value;
         return GIAOHIP_OK;
     }
     return set_error(ctx, GIAOHIP_INVALID_ARGUMENT, "unknown option or value");

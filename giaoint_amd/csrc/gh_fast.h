@@ -111,56 +111,90 @@ __constant__ RList<L> c_rlist = make_rlist<L>();
 // ---------------------------------------------------------------------------
 // Pair kernel: products of one primitive pair in plan order.
 // ---------------------------------------------------------------------------
-template <int L1, int L2, int NG, int MODE>
-__constant__ typename plan::Side<L1, L2, NG, MODE>::Tables c_side = plan::Side<L1, L2, NG, MODE>::make();
-
-// One 64-lane workgroup per shell quartet; it loops over the quartet's
-// primitive pairs on one side (side 0: bra pair (A, B), side 1: ket pair (C, D)).
+// Per padded slot of a side's plan, the three 1-D Hermite table indices of
+// its product (axis x, y, z: ((axis (L1+1) + a) (L2+1) + b) (D+1) + h) and
+// the sign, packed as ix | iy << 10 | iz << 20 | neg << 30; -1 for padding.
 template <int L1, int L2, int NG, int MODE, bool SIGN>
-__global__ __launch_bounds__(64) void pair_kernel(const FastPairQuartet* __restrict__ pq, int side,
-                                                  const DevShell* __restrict__ shells,
-                                                  const double* __restrict__ exps, const double* __restrict__ kappa,
-                                                  zc* __restrict__ table) {
+struct PairDesc {
+    int d[plan::Side<L1, L2, NG, MODE>::NEP];
+};
+template <int L1, int L2, int NG, int MODE, bool SIGN>
+constexpr PairDesc<L1, L2, NG, MODE, SIGN> make_pair_desc() {
+    using S = plan::Side<L1, L2, NG, MODE>;
+    constexpr int D = L1 + L2;
+    static_assert(3 * (L1 + 1) * (L2 + 1) * (D + 1) <= 1024, "1-D table index exceeds 10 bits");
+    constexpr auto T = S::make();
+    PairDesc<L1, L2, NG, MODE, SIGN> r{};
+    for (int slot = 0; slot < S::NEP; ++slot) {
+        const int e = T.ent[slot];
+        if (e == 0xffff) {
+            r.d[slot] = -1;
+            continue;
+        }
+        const int c = T.comp[e];
+        const plan::P3 h = plan::herm(D, T.hidx[e]);
+        const plan::P3 a = plan::cart(L1, c / plan::nc(L2)), b = plan::cart(L2, c % plan::nc(L2));
+        const int ix = ((0 * (L1 + 1) + a.x) * (L2 + 1) + b.x) * (D + 1) + h.x;
+        const int iy = ((1 * (L1 + 1) + a.y) * (L2 + 1) + b.y) * (D + 1) + h.y;
+        const int iz = ((2 * (L1 + 1) + a.z) * (L2 + 1) + b.z) * (D + 1) + h.z;
+        const int neg = SIGN && ((h.x + h.y + h.z) & 1) ? 1 : 0;
+        r.d[slot] = ix | (iy << 10) | (iz << 20) | (neg << 30);
+    }
+    return r;
+}
+template <int L1, int L2, int NG, int MODE, bool SIGN>
+__constant__ PairDesc<L1, L2, NG, MODE, SIGN> c_pair_desc = make_pair_desc<L1, L2, NG, MODE, SIGN>();
+
+// One workgroup of four wavefronts per shell quartet; each wavefront takes
+// the quartet's primitive pairs on one side in turn (side 0: bra pair (A, B),
+// side 1: ket pair (C, D)), builds the pair's 1-D tables in its own LDS
+// region and writes the products in plan order. A lane's slot descriptors
+// are read once and serve every pair.
+template <int L1, int L2, int NG, int MODE, bool SIGN>
+__global__ __launch_bounds__(256) void pair_kernel(const FastPairQuartet* __restrict__ pq, int side,
+                                                   const DevShell* __restrict__ shells,
+                                                   const double* __restrict__ exps, const double* __restrict__ kappa,
+                                                   zc* __restrict__ table) {
     using S = plan::Side<L1, L2, NG, MODE>;
     constexpr int D = L1 + L2;
     constexpr int NT1D = 3 * (L1 + 1) * (L2 + 1);
-    __shared__ zc et[NT1D * (D + 1)];
+    constexpr int NS = (S::NEP + 63) / 64;
+    __shared__ zc et_all[4][NT1D * (D + 1)];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+    zc* et = et_all[wave];
+    int desc[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+        desc[i] = 64 * i + lane < S::NEP ? c_pair_desc<L1, L2, NG, MODE, SIGN>.d[64 * i + lane] : -2;
     const FastPairQuartet& Q = pq[blockIdx.x];
     const int s1 = Q.sh[2 * side], s2 = Q.sh[2 * side + 1];
     const DevShell& A = shells[s1];
     const DevShell& B = shells[s2];
     const int npairs = A.nprim * B.nprim;
-    for (int local = 0; local < npairs; ++local) {
+    for (int local = wave; local < npairs; local += 4) {
         const int p1 = local / B.nprim, p2 = local % B.nprim;
         const double ea = exps[A.prim_off + p1], eb = exps[B.prim_off + p2];
         const double p = ea + eb;
-        __syncthreads();  // the previous pair's 1-D tables are no longer read
-        for (int e = threadIdx.x; e < NT1D; e += 64) {
+        wave_sync();  // the previous pair's 1-D tables are no longer read
+        for (int e = lane; e < NT1D; e += 64) {
             const int axis = e / ((L1 + 1) * (L2 + 1));
             const int i = (e / (L2 + 1)) % (L1 + 1), j = e % (L2 + 1);
             const double P = (ea * A.center[axis] + eb * B.center[axis]) / p;
             const double q = kappa[3 * s2 + axis] - kappa[3 * s1 + axis];
             build_phase_hermite(i, j, P - A.center[axis], P - B.center[axis], p, q, et + e * (D + 1));
         }
-        __syncthreads();
+        wave_sync();
         zc* out = table + Q.table_off[side] + (long long)local * S::NEP;
-        for (int slot = threadIdx.x; slot < S::NEP; slot += 64) {
-            const int e = c_side<L1, L2, NG, MODE>.ent[slot];
-            if (e == 0xffff) {
-                out[slot] = zc{0.0, 0.0};
-                continue;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int d = desc[i];
+            if (d == -2) continue;
+            zc v = zc{0.0, 0.0};
+            if (d >= 0) {
+                v = zmul(zmul(et[d & 1023], et[(d >> 10) & 1023]), et[(d >> 20) & 1023]);
+                if (d >> 30) v = zneg(v);
             }
-            const int c = c_side<L1, L2, NG, MODE>.comp[e];
-            const int hx = c_side<L1, L2, NG, MODE>.hidx[e];
-            const plan::P3 h = plan::herm(D, hx);
-            const int ca = c / plan::nc(L2), cb = c % plan::nc(L2);
-            const plan::P3 a = plan::cart(L1, ca), b = plan::cart(L2, cb);
-            const zc ex = et[((0 * (L1 + 1) + a.x) * (L2 + 1) + b.x) * (D + 1) + h.x];
-            const zc ey = et[((1 * (L1 + 1) + a.y) * (L2 + 1) + b.y) * (D + 1) + h.y];
-            const zc ez = et[((2 * (L1 + 1) + a.z) * (L2 + 1) + b.z) * (D + 1) + h.z];
-            zc v = zmul(zmul(ex, ey), ez);
-            if (SIGN && ((h.x + h.y + h.z) & 1)) v = zneg(v);
-            out[slot] = v;
+            out[64 * i + lane] = v;
         }
     }
 }
@@ -172,13 +206,16 @@ __global__ __launch_bounds__(64) void pair_kernel(const FastPairQuartet* __restr
 // (level n in buffer n & 1, both indexed lexicographically over degree <= L)
 // and level 0 is stored contiguously to HBM.
 // ---------------------------------------------------------------------------
-// One R0 table: seeds sp (X then s_0..s_L) -> dst (nh(L) entries).
+// One R0 table: the lane's seed value (lane < L + 4: X then s_0..s_L) ->
+// dst (nh(L) entries). pk holds the lane's entries e = 64 i + lane of the
+// degree-sorted list (the same at every level: level n uses its first
+// nh(L - n) entries), read from constant memory once per kernel.
 template <int L>
-__device__ inline void r0_table(const zc* __restrict__ sp, zc* __restrict__ dst, zc* const (&bufs)[2], zc* sseed,
-                                int lane) {
+__device__ inline void r0_table(const zc seed, zc* __restrict__ dst, zc* const (&bufs)[2], zc* sseed, int lane,
+                                const int (&pk)[(plan::nh(L) + 63) / 64]) {
     constexpr int NHL = plan::nh(L);
     wave_sync();  // the previous table's LDS values are no longer read
-    if (lane < L + 4) sseed[lane] = sp[lane];
+    if (lane < L + 4) sseed[lane] = seed;
     wave_sync();
     const zc X0 = sseed[0], X1 = sseed[1], X2 = sseed[2];
     if (lane == 0) bufs[L & 1][0] = sseed[3 + L];
@@ -192,8 +229,8 @@ __device__ inline void r0_table(const zc* __restrict__ sp, zc* __restrict__ dst,
         for (int base = 0; base < NHL; base += 64) {
             const int e = base + lane;
             if (base < cnt && e < cnt) {
-                const int pk = c_rlist<L>.e[e];
-                const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = pk >> 18;
+                const int p = pk[base / 64];
+                const int t = p & 63, u = (p >> 6) & 63, v = (p >> 12) & 63, lex = p >> 18;
                 zc val;
                 if (t > 0) {
                     val = zmul(X0, cur[plan::lexoff(L, t - 1, u) + v]);
@@ -231,6 +268,7 @@ template <int L>
 __global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restrict__ pq,
                                                  const zc* __restrict__ seeds, zc* __restrict__ r0) {
     constexpr int NHL = plan::nh(L);
+    constexpr int NIT = (NHL + 63) / 64;
     constexpr int WZ = 2 * NHL + L + 4;
     __shared__ zc lds[4 * WZ];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
@@ -239,10 +277,17 @@ __global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restri
     zc* wl = lds + wave * WZ;
     zc* bufs[2] = {wl, wl + NHL};
     zc* sseed = wl + 2 * NHL;
-    // the four wavefronts of the workgroup take the quartet's primitive quartets in turn
+    int pk[NIT];
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) pk[i] = 64 * i + lane < NHL ? c_rlist<L>.e[64 * i + lane] : 0;
+    const zc* sbase = seeds + Q.seed_off + (lane < L + 4 ? lane : 0);
+    // the four wavefronts of the workgroup take the quartet's primitive
+    // quartets in turn; each prefetches its next table's seeds
+    zc next = wave < units ? sbase[(long long)wave * (L + 4)] : zc{0.0, 0.0};
     for (int local = wave; local < units; local += 4) {
-        r0_table<L>(seeds + Q.seed_off + (long long)local * (L + 4), r0 + Q.r0_off + (long long)local * NHL, bufs,
-                    sseed, lane);
+        const zc seed = next;
+        if (local + 4 < units) next = sbase[(long long)(local + 4) * (L + 4)];
+        r0_table<L>(seed, r0 + Q.r0_off + (long long)local * NHL, bufs, sseed, lane, pk);
     }
 }
 
@@ -594,10 +639,10 @@ template <int LA, int LB, int LC, int LD>
 hipError_t launch_produce(const FastLaunchArgs& a, hipStream_t s) {
     using C = Cls<LA, LB, LC, LD>;
     if (a.npq) {
-        pair_kernel<LA, LB, kG, 1, false><<<dim3((unsigned)a.npq), dim3(64), 0, s>>>(a.pq, 0, a.shells, a.exps,
-                                                                                    a.kappa, a.etab);
-        pair_kernel<LC, LD, C::NCHUNK, 0, true><<<dim3((unsigned)a.npq), dim3(64), 0, s>>>(a.pq, 1, a.shells, a.exps,
-                                                                                          a.kappa, a.etab);
+        pair_kernel<LA, LB, kG, 1, false><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, 0, a.shells, a.exps,
+                                                                                     a.kappa, a.etab);
+        pair_kernel<LC, LD, C::NCHUNK, 0, true><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, 1, a.shells,
+                                                                                           a.exps, a.kappa, a.etab);
         r0_kernel<C::L><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, a.seeds, a.r0);
     }
     return hipGetLastError();

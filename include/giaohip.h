@@ -91,7 +91,15 @@ enum {
        Instances exist for bra/ket pair momenta (LAB, LCD) with
        2 <= LCD <= LAB <= 6, and (7,7), (8,8), in either orientation. The
        plain variant never uses them. */
-    GIAOHIP_OPTION_MFMA = 7
+    GIAOHIP_OPTION_MFMA = 7,
+    /* Scheduling of the batched path's producer kernels (seeds, pair
+       tables, R0 tables), one of GIAOHIP_PIPELINE_*; default
+       GIAOHIP_PIPELINE_OVERLAP. Results are bitwise identical for both. */
+    GIAOHIP_OPTION_PIPELINE = 8
+};
+enum {
+    GIAOHIP_PIPELINE_OVERLAP = 0, /* on a second stream, beside the previous chunk's contraction */
+    GIAOHIP_PIPELINE_SERIAL = 1   /* on the context stream, between the contractions */
 };
 enum {
     GIAOHIP_MFMA_OFF = 0,     /* FP64 VALU kernels only */
