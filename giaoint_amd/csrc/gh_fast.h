@@ -570,13 +570,18 @@ __device__ inline void run_chunk(const FastItem& it, const ItemView& v, const zc
         zc W[C::TILE];
 #pragma unroll
         for (int t = 0; t < C::TILE; ++t) W[t] = zc{0.0, 0.0};
-        int kb[C::NTP];  // recomputed per pass: not live through the bra rounds
-        r0_offsets<C>(opaque_v(k), kb);
-        int roff = pab * Kcd * C::NHL;
-        for (int pcd = 0; pcd < Kcd; ++pcd, roff += C::NHL) {
-            const int pc = pcd / v.Kd, pd = pcd - pc * v.Kd;
-            const double ccd = v.cc[pc] * v.cd[pd];
-            ket_step<C, CI>(W, rq, opaque_v(roff), kb, ccd, ek + opaque_s(pcd) * C::Ket::NEP);
+        // A wavefront with no row of this pass (the last pass of a quartet
+        // whose bra pairs do not fill the workgroup) skips the ket
+        // contraction: its W is never stored. The test is wave-uniform.
+        if (wave * 64 < npab * C::NHAB) {
+            int kb[C::NTP];  // recomputed per pass: not live through the bra rounds
+            r0_offsets<C>(opaque_v(k), kb);
+            int roff = pab * Kcd * C::NHL;
+            for (int pcd = 0; pcd < Kcd; ++pcd, roff += C::NHL) {
+                const int pc = pcd / v.Kd, pd = pcd - pc * v.Kd;
+                const double ccd = v.cc[pc] * v.cd[pd];
+                ket_step<C, CI>(W, rq, opaque_v(roff), kb, ccd, ek + opaque_s(pcd) * C::Ket::NEP);
+            }
         }
         const int pa = pab / v.Kb, pb = pab - pa * v.Kb;
         const double cab = v.ca[pa] * v.cb[pb];
