@@ -11,7 +11,8 @@
 //    primitive pair are written once, in the compile-time order of a plan
 //    (gh_plan.h), with the (-1)^|j| sign folded in for the ket side;
 //  * R0 kernel: one wavefront per primitive quartet runs the recurrence level
-//    by level in LDS and stores the final table (nh(L) entries) to HBM;
+//    by level, a lane per (u, v) column of the table in registers, and stores
+//    the final table (nh(L) entries) to HBM;
 //  * transform: a workgroup of 4..8 wavefronts takes one work item and its
 //    lanes are the rows (bra primitive pair, bra Hermite index k), so a dd bra
 //    side with three-term contractions (9 x 35 rows) fills 5 wavefronts.
@@ -88,25 +89,6 @@ struct Cls {
     static_assert(TILE <= 64, "ket tile exceeds one wavefront");
 };
 
-// R0 entries of degree <= L sorted by degree: t | u << 6 | v << 12 | lexidx << 18.
-template <int L>
-struct RList {
-    int e[plan::nh(L)];
-};
-template <int L>
-constexpr RList<L> make_rlist() {
-    RList<L> r{};
-    int k = 0;
-    for (int d = 0; d <= L; ++d)
-        for (int t = d; t >= 0; --t)
-            for (int u = d - t; u >= 0; --u) {
-                const int v = d - t - u;
-                r.e[k++] = t | (u << 6) | (v << 12) | (plan::lexidx(L, t, u, v) << 18);
-            }
-    return r;
-}
-template <int L>
-__constant__ RList<L> c_rlist = make_rlist<L>();
 
 // ---------------------------------------------------------------------------
 // Pair kernel: products of one primitive pair in plan order.
@@ -202,10 +184,111 @@ __global__ __launch_bounds__(256) void pair_kernel(const FastPairQuartet* __rest
 // ---------------------------------------------------------------------------
 // R0 kernel: one wavefront per primitive quartet. Seeds X (3) and
 // s_0..s_L come from the seed kernel; levels n = L-1 .. 0 of
-// R0^n_{t+1} = t R0^{n+1}_{t-1} + X R0^{n+1}_t are built in LDS
-// (level n in buffer n & 1, both indexed lexicographically over degree <= L)
-// and level 0 is stored contiguously to HBM.
+//   R^n_{t,u,v} = X0 R^{n+1}_{t-1,u,v} + (t-1) R^{n+1}_{t-2,u,v}    (t > 0)
+//   R^n_{0,u,v} = X1 R^{n+1}_{0,u-1,v} + (u-1) R^{n+1}_{0,u-2,v}    (u > 0)
+//   R^n_{0,0,v} = X2 R^{n+1}_{0,0,v-1} + (v-1) R^{n+1}_{0,0,v-2}    (v > 0)
+//   R^n_{0,0,0} = s_n
+// Lane c < (L+1)(L+2)/2 owns the column (u, v) of the table and keeps its
+// entries t = 0 .. L-n-u-v of the current and the previous level in
+// registers: the t recurrence stays in the lane, and only the t = 0 entries
+// of the two neighbouring columns cross lanes (shuffles), twice per level.
+// Level 0 is stored in lexicographic order.
 // ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restrict__ pq,
+                                                 const zc* __restrict__ seeds, zc* __restrict__ r0) {
+    constexpr int NHL = plan::nh(L);
+    constexpr int NCOL = (L + 1) * (L + 2) / 2;
+    static_assert(NCOL <= 64, "one column per lane (r0_lds_kernel above that)");
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+    const FastPairQuartet& Q = pq[blockIdx.x];
+    const int units = Q.kab * Q.kcd;
+    // the lane's column (u, v) and the lanes holding the columns its t = 0
+    // entry is built from: (u-1, v), (u-2, v) or (0, v-1), (0, v-2)
+    int u = 0, v = 0;
+    {
+        int c = lane < NCOL ? lane : 0, uu = 0;
+        while (c > L - uu) {
+            c -= L - uu + 1;
+            ++uu;
+        }
+        u = uu;
+        v = c;
+    }
+    const bool own = lane < NCOL;
+    auto column = [](int cu, int cv) { return cu * (L + 1) - cu * (cu - 1) / 2 + cv; };
+    const int src1 = u > 0 ? column(u - 1, v) : (v > 0 ? column(0, v - 1) : 0);
+    const int src2 = u > 1 ? column(u - 2, v) : (u == 0 && v > 1 ? column(0, v - 2) : 0);
+    const double k2 = u > 0 ? (double)(u - 1) : (double)(v - 1);
+    const bool origin = u == 0 && v == 0;
+    // the four wavefronts of the workgroup take the quartet's primitive quartets in turn
+    for (int local = wave; local < units; local += 4) {
+        // the table's seeds: a wave-uniform address, read into scalar registers
+        const zc* sp = seeds + Q.seed_off + (long long)local * (L + 4);
+        zc sd[L + 4];
+#pragma unroll
+        for (int i = 0; i < L + 4; ++i) sd[i] = sp[i];
+        const zc X0 = sd[0];
+        const zc Xb = u > 0 ? sd[1] : sd[2];
+        // the column at the current level, updated in place from high t down:
+        // entries above the column's last (L - n - u - v) are never read by
+        // entries that exist
+        zc cur[L + 1];
+#pragma unroll
+        for (int t = 0; t <= L; ++t) cur[t] = zc{0.0, 0.0};
+        if (origin) cur[0] = sd[3 + L];
+#pragma unroll
+        for (int n = L - 1; n >= 0; --n) {
+            const zc r1 = zc{__shfl(cur[0].x, src1), __shfl(cur[0].y, src1)};
+            const zc r2 = zc{__shfl(cur[0].x, src2), __shfl(cur[0].y, src2)};
+#pragma unroll
+            for (int t = L - n; t >= 1; --t) {
+                zc val = zmul(X0, cur[t - 1]);
+                if (t > 1) {
+                    val.x += (double)(t - 1) * cur[t - 2].x;
+                    val.y += (double)(t - 1) * cur[t - 2].y;
+                }
+                cur[t] = val;
+            }
+            zc base = zmul(Xb, r1);
+            if (k2 > 0.0) {
+                base.x += k2 * r2.x;
+                base.y += k2 * r2.y;
+            }
+            cur[0] = origin ? sd[3 + n] : base;
+        }
+        if (own) {
+            zc* dst = r0 + Q.r0_off + (long long)local * NHL;
+            const int tmax = L - u - v;
+#pragma unroll
+            for (int t = 0; t <= L; ++t)
+                if (t <= tmax) dst[plan::lexoff(L, t, u) + v] = cur[t];
+        }
+    }
+}
+
+// R0 entries of degree <= L sorted by degree: t | u << 6 | v << 12 | lexidx << 18.
+template <int L>
+struct RList {
+    int e[plan::nh(L)];
+};
+template <int L>
+constexpr RList<L> make_rlist() {
+    RList<L> r{};
+    int k = 0;
+    for (int d = 0; d <= L; ++d)
+        for (int t = d; t >= 0; --t)
+            for (int u = d - t; u >= 0; --u) {
+                const int v = d - t - u;
+                r.e[k++] = t | (u << 6) | (v << 12) | (plan::lexidx(L, t, u, v) << 18);
+            }
+    return r;
+}
+template <int L>
+__constant__ RList<L> c_rlist = make_rlist<L>();
+
+// Tables with more than 64 columns (L >= 10): one wavefront per table runs
+// the levels in LDS, each lane taking entries of the degree-sorted list.
 // One R0 table: the lane's seed value (lane < L + 4: X then s_0..s_L) ->
 // dst (nh(L) entries). pk holds the lane's entries e = 64 i + lane of the
 // degree-sorted list (the same at every level: level n uses its first
@@ -265,7 +348,7 @@ __device__ inline void r0_table(const zc seed, zc* __restrict__ dst, zc* const (
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restrict__ pq,
+__global__ __launch_bounds__(256) void r0_lds_kernel(const FastPairQuartet* __restrict__ pq,
                                                  const zc* __restrict__ seeds, zc* __restrict__ r0) {
     constexpr int NHL = plan::nh(L);
     constexpr int NIT = (NHL + 63) / 64;
@@ -648,7 +731,10 @@ hipError_t launch_produce(const FastLaunchArgs& a, hipStream_t s) {
                                                                                      a.kappa, a.etab);
         pair_kernel<LC, LD, C::NCHUNK, 0, true><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, 1, a.shells,
                                                                                            a.exps, a.kappa, a.etab);
-        r0_kernel<C::L><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, a.seeds, a.r0);
+        if constexpr ((C::L + 1) * (C::L + 2) / 2 <= 64)
+            r0_kernel<C::L><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, a.seeds, a.r0);
+        else
+            r0_lds_kernel<C::L><<<dim3((unsigned)a.npq), dim3(256), 0, s>>>(a.pq, a.seeds, a.r0);
     }
     return hipGetLastError();
 }
