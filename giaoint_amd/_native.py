@@ -242,8 +242,8 @@ class Context:
 
     def set_mfma(self, mode: str):
         """Classes on the FP64 matrix cores under the batched variant: "off",
-        "default" (classes without a class-specialized VALU kernel) or "all"
-        (every class with an MFMA instance)."""
+        "default" (bra and ket pair momenta >= 7, where they are measured
+        faster) or "all" (every class with an MFMA instance)."""
         self.check(self.lib.giaohip_set_option(self.handle, 7, self.MFMA_MODES[mode]))
 
     PIPELINES = {"overlap": 0, "serial": 1}

@@ -99,6 +99,25 @@ def test_chunk_sizes_bitwise():
     assert np.array_equal(got[16], got[2048]) and np.array_equal(got[64], got[2048])
 
 
+def test_pipeline_forms_bitwise():
+    """Producers beside the previous chunk's contraction or serially on the
+    context stream: the same kernels on the same data, bitwise equal, over
+    several chunks of (dd|dd) and a sample of the mixed classes."""
+    ctx = _native.Context(0)
+    ctx.set_chunk_mib(16)
+    for batch in (workloads.dddd_batch(600), workloads.mixed_batch(300)):
+        db = _device(ctx, batch)
+        f = _native.field6(batch.B, batch.origin)
+        got = {}
+        for mode in ("overlap", "serial"):
+            ctx.set_pipeline(mode)
+            got[mode] = db.eri_batch(batch.quartets, f)
+            print(f"{batch.description}: {mode}, {ctx.stats()['contract_launches']} contraction launches", flush=True)
+        assert np.array_equal(got["overlap"], got["serial"])
+    with pytest.raises(ValueError):
+        ctx.check(ctx.lib.giaohip_set_option(ctx.handle, 8, 7))
+
+
 def test_fast_waves_rejects_out_of_range():
     ctx = _native.Context(0)
     for bad in (1, 3, 9):
