@@ -281,9 +281,12 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
     """The high-angular-momentum precision workloads: (gg|gg) and (ff|dd),
     contraction degree 2, exponents 10^U(log10 0.05, log10 5e3), each with
     its E.R.E contraction on FP64 VALU (the generic kernel for (gg|gg), the
-    class-specialized (dd|ff) kernel for (ff|dd)) and on the FP64 matrix
-    cores (v_mfma_f64_16x16x4f64, dense complex Hermite products). A warmup
-    call, then one timed call per variant, output resident on the device.
+    class-specialized (dd|ff) kernel for (ff|dd)), on the FP64 matrix
+    cores (v_mfma_f64_16x16x4f64, dense complex Hermite products) and in
+    split FP32 on the FP32 matrix cores (hi/lo products on
+    v_mfma_f32_16x16x4f32, accumulated in FP32 per step of four and
+    compensated into FP64). A warmup call, then one timed call per variant,
+    output resident on the device.
     Errors: max and RMS of |variant - FP64 VALU| over the first
     `probe_values` integrals of the batch (the VALU path is held to the C
     oracle at 1e-11 by the GPU tests)."""
@@ -307,7 +310,7 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
         rec = {"workload": batch.description, "quartets": n, "variants": {}}
         ref = None
         try:
-            for variant, mode in (("fp64_valu", "off"), ("fp64_mfma", "all")):
+            for variant, mode in (("fp64_valu", "off"), ("fp64_mfma", "all"), ("split_fp32_mfma", "split32")):
                 ctx.set_mfma(mode)
                 db.eri_batch(batch.quartets[: min(n, 256)], f, device_out=dev)
                 ctx.synchronize()
@@ -323,6 +326,8 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
                     err = np.abs(vals - ref)
                     v["max_abs_err_vs_fp64_valu"] = float(err.max())
                     v["rms_err_vs_fp64_valu"] = float(np.sqrt(np.mean(err**2)))
+                    v["max_rel_err_vs_fp64_valu"] = float((err / np.maximum(np.abs(ref), 1e-300)).max())
+                    v["within_1e-10"] = bool(err.max() <= 1e-10)
                     v["max_abs_value"] = float(np.abs(ref).max())
                     v["error_sample"] = f"first {probe} integrals ({probe * n // max(total, 1)} quartets)"
                 rec["variants"][variant] = v
@@ -330,12 +335,12 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
         finally:
             ctx.set_mfma("default")
             ctx.free(dev)
-        best = max(rec["variants"].items(), key=lambda kv: kv[1]["quartets_per_s"])
+        # the headline rate of the leg is the fastest variant within the 1e-10 bar
+        ok = {k: v for k, v in rec["variants"].items() if v.get("within_1e-10", True)}
+        best = max(ok.items(), key=lambda kv: kv[1]["quartets_per_s"])
         rec["quartets_per_s"] = best[1]["quartets_per_s"]
         rec["fastest"] = best[0]
         out[name] = rec
-    out["split_fp32"] = ("not built: FP32 accumulation rounds each partial sum at 2^-24 relative, far above "
-                         "the 1e-10 bar unless compensated in FP64 (which moves the work back to FP64 units)")
     return out
 
 

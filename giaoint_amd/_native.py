@@ -238,12 +238,14 @@ class Context:
         (0 = library default, else 4..8); it sets the bra-pair tile per pass."""
         self.check(self.lib.giaohip_set_option(self.handle, 6, int(waves)))
 
-    MFMA_MODES = {"off": 0, "default": 1, "all": 2}
+    MFMA_MODES = {"off": 0, "default": 1, "all": 2, "split32": 3}
 
     def set_mfma(self, mode: str):
         """Classes on the FP64 matrix cores under the batched variant: "off",
         "default" (bra and ket pair momenta >= 7, where they are measured
-        faster) or "all" (every class with an MFMA instance)."""
+        faster), "all" (every class with an MFMA instance) or "split32" (as
+        "all", with the (gg|gg)- and (ff|dd)-type pair momenta (8, 8) and
+        (6, 4) in split FP32 -- a precision study, not within the 1e-11 bar)."""
         self.check(self.lib.giaohip_set_option(self.handle, 7, self.MFMA_MODES[mode]))
 
     PIPELINES = {"overlap": 0, "serial": 1}

@@ -35,6 +35,7 @@ SOURCES = [
     "gh_mfma_a.hip",
     "gh_mfma_b.hip",
     "gh_mfma_c.hip",
+    "gh_mfma_s.hip",
 ]
 
 HIPCC_FLAGS = [

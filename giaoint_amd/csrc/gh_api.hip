@@ -502,15 +502,20 @@ int plain_plan(giaohip_ctx* ctx, const giaohip_basis* b, const Request& r, gh::C
 // are used: below that the measured FP64 VALU kernels are faster (the dense
 // products carry the structural zeros of the Hermite tables).
 constexpr int kMfmaDefaultMinL = 7;
+// Instance keys of the split-FP32 form carry this flag.
+constexpr int kMfmaSplitKey = 256;
 
 int mfma_orient(const giaohip_basis* b, Request& r, int mode) {
     const DevShell *S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
     const int lab = S[0]->l + S[1]->l, lcd = S[2]->l + S[3]->l;
     if (mode == GIAOHIP_MFMA_OFF) return -1;
     if (mode == GIAOHIP_MFMA_DEFAULT && std::min(lab, lcd) < kMfmaDefaultMinL) return -1;
-    const bool keep = lab >= lcd && gh::mfma_instance(lab, lcd);
-    const bool turn = r.cd1 < 0 && lcd >= lab && gh::mfma_instance(lcd, lab);
+    const bool split = mode == GIAOHIP_MFMA_SPLIT32 &&
+                       (gh::mfma_split_instance(lab, lcd) || (r.cd1 < 0 && gh::mfma_split_instance(lcd, lab)));
+    const bool keep = lab >= lcd && (split ? gh::mfma_split_instance(lab, lcd) : gh::mfma_instance(lab, lcd));
+    const bool turn = r.cd1 < 0 && lcd >= lab && (split ? gh::mfma_split_instance(lcd, lab) : gh::mfma_instance(lcd, lab));
     if (!keep && !turn) return -1;
+    const int flag = split ? kMfmaSplitKey : 0;
     bool do_turn = turn && !keep;
     if (keep && turn) {
         // equal momenta: the ket with fewer components and primitive pairs costs less
@@ -530,9 +535,9 @@ int mfma_orient(const giaohip_basis* b, Request& r, int mode) {
             t.stride[k] = r.stride[map[k]];
         }
         r = t;
-        return lcd * 16 + lab;
+        return (lcd * 16 + lab) | flag;
     }
-    return lab * 16 + lcd;
+    return (lab * 16 + lcd) | flag;
 }
 
 // Fixed device memory of the plain path: slices of the library's static
@@ -674,7 +679,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const int cd0 = r.cd1 < 0 ? 0 : r.cd0, cd1 = r.cd1 < 0 ? ncd_all : r.cd1;
             const int NHab = gh::nherm(S[0]->l + S[1]->l);
             if (!mfk.empty() && mfk[i] >= 0) {
-                const int key = mfk[i], tile_w = gh::mfma_cd_tile(key >> 4);
+                const int key = mfk[i], tile_w = gh::mfma_cd_tile((key >> 4) & 15);
                 const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
                 int lo[4], hi[4];
                 for (int k = 0; k < 4; ++k) {
@@ -831,9 +836,10 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             queued[side] += L.cost;
             hipStream_t ls = side ? ctx->aux : s;
             if (L.mf >= 0) {
-                HIP_TRY(gh::launch_mfma(L.mf >> 4, L.mf & 15, d_items + L.first, (int)L.count,
-                                        b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
-                                        b->d_norms.as<double>(), d_kappa, d_seeds, d_out, ls),
+                auto launch = (L.mf & kMfmaSplitKey) ? gh::launch_mfma_split : gh::launch_mfma;
+                HIP_TRY(launch((L.mf >> 4) & 15, L.mf & 15, d_items + L.first, (int)L.count,
+                               b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                               b->d_norms.as<double>(), d_kappa, d_seeds, d_out, ls),
                         "MFMA transform kernel launch");
                 ctx->stats.launches += 1;
                 continue;
@@ -1186,7 +1192,7 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
     parallel_for(n, [&](size_t lo, size_t hi) {
         for (size_t i = lo; i < hi; ++i) {
             is_fast[i] = batched && reqs[i].cd1 < 0 && orient(b, reqs[i], fast_all[i]);
-            if (is_fast[i] && ctx->mfma_mode == GIAOHIP_MFMA_ALL) {
+            if (is_fast[i] && ctx->mfma_mode >= GIAOHIP_MFMA_ALL) {
                 Request r = reqs[i];
                 is_fast[i] = mfma_orient(b, r, ctx->mfma_mode) < 0;
             }
@@ -1436,7 +1442,7 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
         ctx->fast_waves_cap = value;
         return GIAOHIP_OK;
     }
-    if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_ALL) {
+    if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_SPLIT32) {
         ctx->mfma_mode = value;
         return GIAOHIP_OK;
     }

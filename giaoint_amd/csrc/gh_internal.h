@@ -150,6 +150,12 @@ int mfma_cd_tile(int lab);
 hipError_t launch_mfma(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                        const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                        hipStream_t s);
+// The split-FP32 form of the same contraction (FP32 hi/lo products on
+// v_mfma_f32_16x16x4f32, compensated into FP64), instances (8, 8) and (6, 4).
+bool mfma_split_instance(int lab, int lcd);
+hipError_t launch_mfma_split(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells,
+                             const double* exps, const double* coefs, const double* norms, const double* kappa,
+                             const zc* seeds, zc* out, hipStream_t s);
 // p[0..n) = 0 by a kernel on stream s.
 // ERI centre derivatives (gh_deriv.hip): one derivative quartet's block of
 // 12 x size values formed from the shifted-momentum integrals in `t`.

@@ -17,6 +17,19 @@
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[l/16 + 4r][l%16] in
 // register r.
 //
+// Split-FP32 form (SPLIT = true, the precision study of the high angular
+// momentum classes): every operand is split into FP32 hi + lo parts
+// (hi = fl32(x), lo = fl32(x - hi)), the products hi*hi, hi*lo and lo*hi of a
+// GEMM step (K = 4) are accumulated from zero in FP32 on
+// v_mfma_f32_16x16x4f32 (hi*hi and the cross terms in separate
+// accumulators), and each step's sums are added to FP64 accumulators
+// (compensated into FP64). FP32's range is kept by powers of two: the R0
+// seeds of a primitive quartet are scaled by 2^-e (e the largest exponent of
+// s_0..s_L) and W of a bra pair by 2^-e' (e' the exponent of max |W|), undone
+// exactly when a step's sums enter the FP64 accumulators; so are the bra and
+// ket factors (contraction coefficients times normalizations) of a pair. Output lane
+// layout of the FP32 instruction: lane l holds D[4(l/16) + r][l%16].
+//
 // One workgroup of 8 wavefronts takes one work item: a shell quartet, one
 // combination of contraction rows and a range of at most 32 ket components
 // (two 16-wide n-tiles; 16 components, one n-tile, for bra momentum 7, 8).
@@ -33,6 +46,7 @@ namespace gh {
 namespace mf {
 
 typedef double mf4 __attribute__((ext_vector_type(4)));
+typedef float mf4f __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 8;
 constexpr int kNT = 64 * kWaves;
@@ -85,7 +99,8 @@ struct Shape {
     // ket Hermite indices as (t * (L + 1) + u) | v << 16; the bra Hermite
     // indices as t | u << 8 | v << 16; per ket column its three 1-D table rows
     static constexpr int I_OFFL = 0, I_HCD = (L + 1) * (L + 1), I_HAB = I_HCD + JROWS,
-                         I_COL = I_HAB + 4 * KS2 + 4, I_TUV = I_COL + 3 * CDT, I_END = I_TUV + NHCD;
+                         I_COL = I_HAB + 4 * KS2 + 4, I_TUV = I_COL + 3 * CDT, I_EXP = I_TUV + NHCD,
+                         I_END = I_EXP + kWaves + 1;  // split form: per-wave W exponents, the seed exponent
     static constexpr int LDS_BYTES = Z_END * 16 + I_END * 4;
 };
 
@@ -154,7 +169,52 @@ __device__ inline void hermite_row(int i, int j, double PA, double PB, double p,
     for (int t = 0; t <= D; ++t) row[t] = t <= i + j ? tmp[t] : zc{0.0, 0.0};
 }
 
-template <int LAB, int LCD>
+// Row of the accumulator register r of lane group lk (lane / 16): the FP64
+// instruction holds D[lk + 4r][.], the FP32 one D[4 lk + r][.].
+template <bool SPLIT>
+__device__ constexpr int drow(int lk, int r) {
+    return SPLIT ? 4 * lk + r : lk + 4 * r;
+}
+
+__device__ inline float hi32(double x) { return (float)x; }
+__device__ inline float lo32(double x, float h) { return (float)(x - (double)h); }
+
+// One GEMM step (K = 4) of a complex product in split FP32, added to the FP64
+// accumulators times 2^scale_exp (ldexp: exact, and safe where the power of
+// two alone would overflow or underflow).
+__device__ inline void split_step(mf4& re, mf4& im, const zc a, const zc b, const int scale_exp) {
+    const float arh = hi32(a.x), arl = lo32(a.x, arh), aih = hi32(a.y), ail = lo32(a.y, aih);
+    const float brh = hi32(b.x), brl = lo32(b.x, brh), bih = hi32(b.y), bil = lo32(b.y, bih);
+    const mf4f z = {0.0f, 0.0f, 0.0f, 0.0f};
+    mf4f rh = __builtin_amdgcn_mfma_f32_16x16x4f32(arh, brh, z, 0, 0, 0);
+    mf4f ih = __builtin_amdgcn_mfma_f32_16x16x4f32(arh, bih, z, 0, 0, 0);
+    mf4f rx = __builtin_amdgcn_mfma_f32_16x16x4f32(arh, brl, z, 0, 0, 0);
+    mf4f ix = __builtin_amdgcn_mfma_f32_16x16x4f32(arh, bil, z, 0, 0, 0);
+    rh = __builtin_amdgcn_mfma_f32_16x16x4f32(-aih, bih, rh, 0, 0, 0);
+    ih = __builtin_amdgcn_mfma_f32_16x16x4f32(aih, brh, ih, 0, 0, 0);
+    rx = __builtin_amdgcn_mfma_f32_16x16x4f32(arl, brh, rx, 0, 0, 0);
+    ix = __builtin_amdgcn_mfma_f32_16x16x4f32(arl, bih, ix, 0, 0, 0);
+    rx = __builtin_amdgcn_mfma_f32_16x16x4f32(-aih, bil, rx, 0, 0, 0);
+    ix = __builtin_amdgcn_mfma_f32_16x16x4f32(aih, brl, ix, 0, 0, 0);
+    rx = __builtin_amdgcn_mfma_f32_16x16x4f32(-ail, bih, rx, 0, 0, 0);
+    ix = __builtin_amdgcn_mfma_f32_16x16x4f32(ail, brh, ix, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        re[r] += ldexp((double)rh[r] + (double)rx[r], scale_exp);
+        im[r] += ldexp((double)ih[r] + (double)ix[r], scale_exp);
+    }
+}
+
+// Exponent of the largest magnitude over a wavefront (lanes pass m >= 0),
+// or INT_MIN / 2 when every lane passes zero.
+__device__ inline int wave_max_exponent(double m) {
+    int e = m > 0.0 ? ilogb(m) : -(1 << 30);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) e = max(e, __shfl_xor(e, o));
+    return e;
+}
+
+template <int LAB, int LCD, bool SPLIT = false>
 __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __restrict__ items,
                                                              const DevShell* __restrict__ shells,
                                                              const double* __restrict__ exps,
@@ -179,6 +239,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
     int* hab = is + S::I_HAB;
     int* colrow = is + S::I_COL;
     int* tuvcd = is + S::I_TUV;  // ket Hermite indices as t | u << 8 | v << 16
+    int* wexp = is + S::I_EXP;   // split form: W exponent per wavefront, then the seed exponent
 
     const WorkItem it = items[blockIdx.x];
     const int tid = threadIdx.x;
@@ -316,7 +377,15 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             hermite_row<LAB>(i, j, Pc - Ac[axis], Pc - Bc[axis], p, qv,
                              e1b + (axis * S::RB + i * rb_i + j) * (LAB + 1));
         }
-        // the lane's bra factors (coefficients x normalizations) for this pair
+        // the lane's bra factors (coefficients x normalizations) for this pair;
+        // the split form scales them by 2^-e_ab, e_ab the exponent of the
+        // pair's factor for its first components (the same in every lane)
+        int e_ab = 0;
+        if constexpr (SPLIT) {
+            const double f0 = fabs(coefs[A.coef_off + it.x[0] * A.nprim + pa] * norms[A.norm_off + pa] *
+                                   coefs[B.coef_off + it.x[1] * B.nprim + pb] * norms[B.norm_off + pb]);
+            e_ab = f0 > 0.0 ? ilogb(f0) : 0;
+        }
         double fab[S::TW2];
 #pragma unroll
         for (int i = 0; i < S::TW2; ++i) {
@@ -325,6 +394,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                 const int ca = g2_ab[i] / ncb, cb = g2_ab[i] % ncb;
                 fab[i] = coefs[A.coef_off + it.x[0] * A.nprim + pa] * norms[A.norm_off + ca * A.nprim + pa] *
                          coefs[B.coef_off + it.x[1] * B.nprim + pb] * norms[B.norm_off + cb * B.nprim + pb];
+                if constexpr (SPLIT) fab[i] = ldexp(fab[i], -e_ab);
             }
         }
 
@@ -352,8 +422,24 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             const double ec = exps[C.prim_off + pc], ed = exps[D.prim_off + pd];
             const double q = ec + ed;
             __syncthreads();  // the previous ket pair's tables, R0 and products are no longer read
-            if (tid < L + 4) sseed[tid] = seed_next;
-            if (tid < S::CDT) fcol[tid] = f_next;
+            if constexpr (SPLIT) {
+                // seeds s_n scaled by 2^-e, e their largest exponent, and the ket
+                // column factors by 2^-e_c, e_c theirs (wavefront 0 holds them all)
+                if (wave == 0) {
+                    const bool sn = tid >= 3 && tid < L + 4;
+                    int e = wave_max_exponent(sn ? fmax(fabs(seed_next.x), fabs(seed_next.y)) : 0.0);
+                    if (e < -(1 << 29)) e = 0;
+                    int e_c = wave_max_exponent(fabs(f_next));
+                    if (e_c < -(1 << 29)) e_c = 0;
+                    if (tid < L + 4)
+                        sseed[tid] = sn ? zc{ldexp(seed_next.x, -e), ldexp(seed_next.y, -e)} : seed_next;
+                    if (tid < S::CDT) fcol[tid] = ldexp(f_next, -e_c);
+                    if (tid == 0) wexp[kWaves] = e + e_c;
+                }
+            } else {
+                if (tid < L + 4) sseed[tid] = seed_next;
+                if (tid < S::CDT) fcol[tid] = f_next;
+            }
             for (int e = tid; e < 3 * (lc + 1) * (ld + 1); e += kNT) {
                 const int axis = e / ((lc + 1) * (ld + 1));
                 const int i = (e / (ld + 1)) % (lc + 1), j = e % (ld + 1);
@@ -411,6 +497,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             }
             __syncthreads();
             const zc* R0 = r0buf[0];
+            const int sc1 = SPLIT ? wexp[kWaves] : 0;  // exponent that undoes the seed and ket scaling
             // GEMM1, one step of operands loaded ahead; the two products of
             // every tile that share no accumulator are issued back to back
             if (pcd + 1 < Kcd) {
@@ -437,18 +524,24 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                 zc an[S::TW1];
 #pragma unroll
                 for (int i = 0; i < S::TW1; ++i) an[i] = gather(i, hjn);
+                if constexpr (SPLIT) {
 #pragma unroll
-                for (int i = 0; i < S::TW1; ++i) {
-                    if (g1_on[i]) {
-                        wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.x, wre[i], 0, 0, 0);
-                        wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.y, wim[i], 0, 0, 0);
+                    for (int i = 0; i < S::TW1; ++i)
+                        if (g1_on[i]) split_step(wre[i], wim[i], a[i], b, sc1);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < S::TW1; ++i) {
+                        if (g1_on[i]) {
+                            wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.x, wre[i], 0, 0, 0);
+                            wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.y, wim[i], 0, 0, 0);
+                        }
                     }
-                }
 #pragma unroll
-                for (int i = 0; i < S::TW1; ++i) {
-                    if (g1_on[i]) {
-                        wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i].y, b.y, wre[i], 0, 0, 0);
-                        wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b.x, wim[i], 0, 0, 0);
+                    for (int i = 0; i < S::TW1; ++i) {
+                        if (g1_on[i]) {
+                            wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i].y, b.y, wre[i], 0, 0, 0);
+                            wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b.x, wim[i], 0, 0, 0);
+                        }
                     }
                 }
                 b = bn;
@@ -457,16 +550,36 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             }
         }
         __syncthreads();  // every wavefront finished GEMM1 reading R0
-        // W to LDS: tile i holds rows mt*16 + lk + 4r, column nt1*16 + li
+        // W to LDS: tile i holds rows mt*16 + drow(lk, r), column nt1*16 + li
 #pragma unroll
         for (int i = 0; i < S::TW1; ++i) {
             if (g1_on[i]) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    Wl[(g1_mt[i] * 16 + lk + 4 * r) * S::CDT + cdl] = zc{wre[i][r], wim[i][r]};
+                    Wl[(g1_mt[i] * 16 + drow<SPLIT>(lk, r)) * S::CDT + cdl] = zc{wre[i][r], wim[i][r]};
             }
         }
+        if constexpr (SPLIT) {
+            double m = 0.0;
+#pragma unroll
+            for (int i = 0; i < S::TW1; ++i)
+                if (g1_on[i])
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) m = fmax(m, fmax(fabs(wre[i][r]), fabs(wim[i][r])));
+            const int e = wave_max_exponent(m);
+            if (lane == 0) wexp[wave] = e;
+        }
         __syncthreads();
+        // split form: W enters the FP32 products scaled by 2^-e', e' the exponent of max |W|
+        int w_in = 0, w_out = 0;
+        if constexpr (SPLIT) {
+            int e = wexp[0];
+#pragma unroll
+            for (int w8 = 1; w8 < kWaves; ++w8) e = max(e, wexp[w8]);
+            if (e < -(1 << 29)) e = 0;
+            w_in = -e;
+            w_out = e + e_ab;  // undoes the W and bra factor scaling
+        }
         // GEMM2, one step of operands loaded ahead (W has a zero step past the end)
         // rows k >= NHAB of W are zero, invalid bra components have fab = 0
         auto bra_operand = [&](int k, int i) -> zc {
@@ -476,28 +589,36 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             return zscale(fab[i], e);
         };
         zc w = Wl[lk * S::CDT + cdl];
+        if constexpr (SPLIT) w = zc{ldexp(w.x, w_in), ldexp(w.y, w_in)};
         zc a2[S::TW2];
 #pragma unroll
         for (int i = 0; i < S::TW2; ++i) a2[i] = bra_operand(lk, i);
 #pragma unroll S::UNROLL
         for (int s = 0; s < S::KS2; ++s) {
             const int kn = 4 * (s + 1) + lk;
-            const zc wn = Wl[kn * S::CDT + cdl];
+            zc wn = Wl[kn * S::CDT + cdl];
+            if constexpr (SPLIT) wn = zc{ldexp(wn.x, w_in), ldexp(wn.y, w_in)};
             zc an[S::TW2];
 #pragma unroll
             for (int i = 0; i < S::TW2; ++i) an[i] = bra_operand(kn, i);
+            if constexpr (SPLIT) {
 #pragma unroll
-            for (int i = 0; i < S::TW2; ++i) {
-                if (g2_on[i]) {
-                    vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.x, vre[i], 0, 0, 0);
-                    vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.y, vim[i], 0, 0, 0);
+                for (int i = 0; i < S::TW2; ++i)
+                    if (g2_on[i]) split_step(vre[i], vim[i], a2[i], w, w_out);
+            } else {
+#pragma unroll
+                for (int i = 0; i < S::TW2; ++i) {
+                    if (g2_on[i]) {
+                        vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.x, vre[i], 0, 0, 0);
+                        vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.y, vim[i], 0, 0, 0);
+                    }
                 }
-            }
 #pragma unroll
-            for (int i = 0; i < S::TW2; ++i) {
-                if (g2_on[i]) {
-                    vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a2[i].y, w.y, vre[i], 0, 0, 0);
-                    vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].y, w.x, vim[i], 0, 0, 0);
+                for (int i = 0; i < S::TW2; ++i) {
+                    if (g2_on[i]) {
+                        vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a2[i].y, w.y, vre[i], 0, 0, 0);
+                        vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].y, w.x, vim[i], 0, 0, 0);
+                    }
                 }
             }
             w = wn;
@@ -505,7 +626,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             for (int i = 0; i < S::TW2; ++i) a2[i] = an[i];
         }
     }
-    // write: tile i holds bra components mt*16 + lk + 4r, ket component cdl
+    // write: tile i holds bra components mt*16 + drow(lk, r), ket component cdl
     if (cd_on) {
         const long long ic = (long long)it.x[2] * ncart(lc) + cc, id = (long long)it.x[3] * ncd_d + dc;
         const long long base = it.out_base + ic * it.stride[2] + id * it.stride[3];
@@ -514,7 +635,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             if (g2_on[i]) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int ab = g2_mt[i] * 16 + lk + 4 * r;
+                    const int ab = g2_mt[i] * 16 + drow<SPLIT>(lk, r);
                     if (ab < nab) {
                         const long long ia = (long long)it.x[0] * ncart(la) + ab / ncb,
                                         ib = (long long)it.x[1] * ncb + ab % ncb;
@@ -534,7 +655,7 @@ namespace mf {
 
 // Launch of one (LAB, LCD) instance over `n` items (dynamic LDS of
 // Shape::LDS_BYTES; above 64 KB the kernel's limit is raised once).
-template <int LAB, int LCD>
+template <int LAB, int LCD, bool SPLIT = false>
 hipError_t launch_instance(const WorkItem* items, int n, const DevShell* shells, const double* exps,
                            const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                            hipStream_t s) {
@@ -544,14 +665,14 @@ hipError_t launch_instance(const WorkItem* items, int n, const DevShell* shells,
     if constexpr (S::LDS_BYTES > 64 * 1024) {
         static bool raised = false;
         if (!raised) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_transform_kernel<LAB, LCD>),
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_transform_kernel<LAB, LCD, SPLIT>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, S::LDS_BYTES);
             if (e != hipSuccess) return e;
             raised = true;
         }
     }
-    mfma_transform_kernel<LAB, LCD><<<dim3((unsigned)n), dim3(kNT), S::LDS_BYTES, s>>>(items, shells, exps, coefs,
-                                                                                     norms, kappa, seeds, out);
+    mfma_transform_kernel<LAB, LCD, SPLIT><<<dim3((unsigned)n), dim3(kNT), S::LDS_BYTES, s>>>(
+        items, shells, exps, coefs, norms, kappa, seeds, out);
     return hipGetLastError();
 }
 

@@ -104,7 +104,11 @@ enum {
 enum {
     GIAOHIP_MFMA_OFF = 0,     /* FP64 VALU kernels only */
     GIAOHIP_MFMA_DEFAULT = 1, /* matrix cores where they are measured faster: bra and ket momenta >= 7 */
-    GIAOHIP_MFMA_ALL = 2      /* matrix cores for every class with an instance (precision studies) */
+    GIAOHIP_MFMA_ALL = 2,     /* matrix cores for every class with an instance (precision studies) */
+    GIAOHIP_MFMA_SPLIT32 = 3  /* as ALL, but (gg|gg) and (ff|dd)-type classes ((LAB, LCD) = (8, 8), (6, 4))
+                                 in split FP32: hi/lo FP32 products on v_mfma_f32_16x16x4f32 accumulated in
+                                 FP32 per step of 4 and compensated into FP64. A precision study: its errors
+                                 (about 1e-7 relative) are far above the 1e-11 bar of the FP64 path. */
 };
 enum {
     GIAOHIP_QUEUE_NONE = 0,    /* one workgroup per item, one launch per LDS-size bucket */

@@ -45,3 +45,38 @@ def test_mfma_class_against_oracle(gpu, port, cls):
             assert err <= ATOL
     finally:
         gpu.set_mfma("default")
+
+
+# The split-FP32 form (hi/lo FP32 products on v_mfma_f32_16x16x4f32, FP32
+# sums per step of four compensated into FP64): instances (8, 8) and (6, 4),
+# the latter also turned. It is a precision study, held to its own bound
+# relative to the block's largest integral, not to the FP64 bar.
+SPLIT_CLASSES = [(4, 4, 4, 4), (3, 3, 2, 2), (2, 2, 3, 3)]
+SPLIT_REL = 1e-5
+
+
+@pytest.mark.parametrize("cls", SPLIT_CLASSES, ids=lambda c: "".join(map(str, c)))
+def test_split_fp32_class_against_oracle(gpu, port, cls):
+    rng = np.random.default_rng(sum((i + 1) * l for i, l in enumerate(cls)) + 11)
+    shells = random_shells(rng, list(cls), nprim=(1, 2), ncontr=(1, 2), box=0.6, log_exp=(-1.0, 0.3))
+    b = make_basis(shells, port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    q = [(0, 1, 2, 3)]
+    try:
+        for B in ((0.1, -0.3, 0.25), (0.0, 0.0, 0.0)):
+            f = _native.field6(B, (0.2, 0.1, -0.1))
+            gpu.set_mfma("split32")
+            got = db.eri_batch(q, f)
+            gpu.set_mfma("all")
+            fp64 = db.eri_batch(q, f)
+            want, _ = port.eri_batch(b, q, port.field6(B, (0.2, 0.1, -0.1)), 16)
+            err = float(np.max(np.abs(got - want)))
+            scale = float(np.abs(want).max())
+            print(f"{cls} B={B}: max |split - oracle| = {err:.2e} (FP64 MFMA {np.max(np.abs(fp64 - want)):.2e}), "
+                  f"max |value| {scale:.2e}")
+            assert scale > 1e-3
+            assert np.max(np.abs(fp64 - want)) <= ATOL
+            assert err <= SPLIT_REL * scale
+            assert not np.array_equal(got, fp64)  # the split form really ran
+    finally:
+        gpu.set_mfma("default")
