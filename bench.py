@@ -219,15 +219,17 @@ def cpu_baselines(mixed_batch, target_s: float) -> dict:
     return out
 
 
-def mixed_leg(n: int, tune: bool = True, tune_n: int = 200_000):
+def mixed_leg(n: int, tune: bool = True, tune_n: int = 500_000):
     """Mixed classes l <= 3 (weights (1 + sum l)^-2), contraction degree U{1..8}:
-    the 2M-quartet workload. With `tune`, the launch form of the generic
-    kernel is autotuned on its first tune_n quartets (per-bucket grids,
-    per-bucket work queues, one persistent launch); then one warmup call and
-    one timed call of the batched variant (class-specialized kernels where
-    they exist) and of the plain variant (the generic kernel for every
-    class), inputs and output resident on the device. Returns the leg's
-    record and the batch."""
+    the 2M-quartet workload. With `tune`, two kernel choices are autotuned on
+    the first tune_n quartets: the largest total angular momentum whose
+    generic-transform items run on one-wavefront workgroups, and the smallest
+    whose classes run on the FP64 matrix cores; then one warmup call and one
+    timed call of the batched variant (class-specialized kernels where they
+    exist, matrix cores from the tuned momentum, the generic kernel for the
+    rest) and of the plain variant (the generic kernel for every class),
+    inputs and output resident on the device. Returns the leg's record and
+    the batch."""
     from giaoint_amd import _native, autotune, workloads
 
     t0 = time.perf_counter()
@@ -243,7 +245,7 @@ def mixed_leg(n: int, tune: bool = True, tune_n: int = 200_000):
     tuned = None
     if tune:
         m = min(n, tune_n)
-        tuned = autotune.tune_eri(db, batch.quartets[:m], f, knobs=("eri_queue",), reps=1,
+        tuned = autotune.tune_eri(db, batch.quartets[:m], f, knobs=("wave_max_l", "mfma_min_l"), reps=1,
                                   workload=f"mixed l<=3 degree 1..8, first {m} of {n} quartets")
         log(f"mixed: autotuned {tuned.values} " +
             " ".join(f"{t.value}={t.seconds:.3f}s" for t in tuned.trials))

@@ -242,11 +242,21 @@ class Context:
 
     def set_mfma(self, mode: str):
         """Classes on the FP64 matrix cores under the batched variant: "off",
-        "default" (bra and ket pair momenta >= 7, where they are measured
-        faster), "all" (every class with an MFMA instance) or "split32" (as
+        "default" (classes from the total momentum set by set_mfma_min_l,
+        where they are measured faster), "all" (every class with an MFMA instance) or "split32" (as
         "all", with the (gg|gg)- and (ff|dd)-type pair momenta (8, 8) and
         (6, 4) in split FP32 -- a precision study, not within the 1e-11 bar)."""
         self.check(self.lib.giaohip_set_option(self.handle, 7, self.MFMA_MODES[mode]))
+
+    def set_wave_max_l(self, value: int):
+        """Largest la + lb + lc + ld whose generic-transform items run on
+        one-wavefront workgroups (-1: none). Bitwise identical results."""
+        self.check(self.lib.giaohip_set_option(self.handle, 9, int(value)))
+
+    def set_mfma_min_l(self, value: int):
+        """Smallest la + lb + lc + ld whose classes (without a class-specialized
+        kernel) use the FP64 matrix cores in the "default" MFMA mode."""
+        self.check(self.lib.giaohip_set_option(self.handle, 10, int(value)))
 
     PIPELINES = {"overlap": 0, "serial": 1}
 

@@ -17,11 +17,19 @@ device in use:
   trades pipeline fill and drain against launch count).
 * ``cholesky_block`` -- candidates per pivot block of the streamed Cholesky
   (projection tile width against passes over the factors).
+* ``wave_max_l`` -- largest total angular momentum whose generic-transform
+  items run on one-wavefront workgroups (four times the items in flight per
+  CU, against fewer threads per item).
+* ``mfma_min_l`` -- smallest total angular momentum whose classes (without a
+  class-specialized kernel) run their contraction on the FP64 matrix cores
+  (dense products) instead of the generic FP64 VALU kernel.
 
 Every candidate is checked before it can win: ERI candidates must reproduce
 the default configuration's integrals on a sample of the workload (bitwise
-for the launch forms and the chunk size, which do not change any sum; within ``tolerance`` for
-the workgroup size, which changes the order of the bra-pair sums), and
+for the launch forms, the chunk size and the one-wavefront workgroups, which
+do not change any sum; within ``tolerance`` for the class-specialized
+workgroup size, which changes the order of the bra-pair sums, and for the
+matrix-core threshold, which changes the kernel), and
 Cholesky candidates must give the same pivots and factors bitwise (the block
 size is designed not to change them). Choices can be kept in a JSON cache
 keyed by device and workload (``save`` / ``load``).
@@ -43,8 +51,12 @@ FAST_WAVES = (0, 4, 5, 6, 7)
 ERI_QUEUE = ("none", "buckets", "single")
 CHUNK_MIB = (512, 1024, 2048, 4096)
 CHOLESKY_BLOCK = (8, 16, 24, 32)
-DEFAULTS = {"fast_waves": 0, "eri_queue": "none", "chunk_mib": 2048, "cholesky_block": 32}
-EXACT = ("eri_queue", "chunk_mib")  # knobs that do not change any sum: candidates must match bitwise
+WAVE_MAX_L = (-1, 3, 5, 7)
+MFMA_MIN_L = (7, 9, 11, 14)
+DEFAULTS = {"fast_waves": 0, "eri_queue": "none", "chunk_mib": 2048, "cholesky_block": 32, "wave_max_l": 5,
+            "mfma_min_l": 9}
+# knobs that do not change any sum: candidates must match bitwise
+EXACT = ("eri_queue", "chunk_mib", "wave_max_l")
 
 
 @dataclass
@@ -76,6 +88,10 @@ def apply(ctx: "_native.Context", values: dict) -> None:
         ctx.set_chunk_mib(int(values["chunk_mib"]))
     if "cholesky_block" in values:
         ctx.set_cholesky_block(int(values["cholesky_block"]))
+    if "wave_max_l" in values:
+        ctx.set_wave_max_l(int(values["wave_max_l"]))
+    if "mfma_min_l" in values:
+        ctx.set_mfma_min_l(int(values["mfma_min_l"]))
 
 
 def _best_of(fn, reps: int) -> float:
@@ -106,7 +122,8 @@ def tune_eri(basis: "_native.DeviceBasis", quartets, field6, knobs=("fast_waves"
     out = ctx.alloc(max(total, 1) * 16)
     try:
         for knob in knobs:
-            space = {"fast_waves": FAST_WAVES, "eri_queue": ERI_QUEUE, "chunk_mib": CHUNK_MIB}[knob]
+            space = {"fast_waves": FAST_WAVES, "eri_queue": ERI_QUEUE, "chunk_mib": CHUNK_MIB,
+                     "wave_max_l": WAVE_MAX_L, "mfma_min_l": MFMA_MIN_L}[knob]
             exact = knob in EXACT
             # an exact knob is compared with the integrals of the choices made so far
             base = basis.eri_batch(sample, field6) if exact else reference

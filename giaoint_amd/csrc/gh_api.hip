@@ -222,7 +222,18 @@ struct giaohip_ctx {
     int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
     int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
     int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
+    int wave_max_l = 5;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
+    int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
+    // pipelined chunks of the plain path (run_plain): per slot the seeds,
+    // descriptors and work items of one chunk in flight, their page-locked
+    // host staging, and events; the
+    // seed kernels run on plain_stream (default priority)
+    hipStream_t plain_stream = nullptr;
+    DevBuffer pl_seeds[2], pl_sq[2], pl_items[2];
+    PinnedBuffer pl_hsq[2], pl_hitems[2];
+    hipEvent_t pl_ev[2][4]{};  // seeds start, seeds end, transforms done on the context and on the aux stream
+    hipEvent_t pl_init = nullptr;
     void* noalloc = nullptr;                 // device address of `scratch` on this context's device
     rocblas_handle blas = nullptr;           // library GEMMs (exchange contraction), created on first use
 };
@@ -473,15 +484,29 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 // Ket-component tile, LDS bytes and number of work items of one request on
 // the plain transform kernel.
+// Work items of a request on the generic transform: the class, the ket
+// components per item (tile), LDS bytes, item count and workgroup size nt.
+// Classes with la + lb + lc + ld <= ctx->wave_max_l run on one wavefront
+// (nt = 64) when a tile of at least one component fits its registers and the
+// whole ket range takes no more passes than on four wavefronts.
 int plain_plan(giaohip_ctx* ctx, const giaohip_basis* b, const Request& r, gh::ClassArgs& ca, int& lds,
-               long long& nitems) {
-    const int maxw = gh::transform_max_w(), maxo = gh::transform_max_o();
+               long long& nitems, int& nt) {
     const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
     const int ncd_all = S[2]->ncart * S[3]->ncart;
     const int cd0 = r.cd1 < 0 ? 0 : r.cd0, cd1 = r.cd1 < 0 ? ncd_all : r.cd1;
     const int nab = S[0]->ncart * S[1]->ncart;
     const int NHab = gh::nherm(S[0]->l + S[1]->l);
-    int tile = std::min({cd1 - cd0, maxw / NHab, maxo / nab});
+    const int L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
+    auto tile_for = [&](int threads) {
+        return std::min({cd1 - cd0, gh::transform_max_w(threads) / NHab, gh::transform_max_o(threads) / nab});
+    };
+    nt = 256;
+    // (the single persistent launch form keeps every item on four wavefronts)
+    if (L <= ctx->wave_max_l && ctx->eri_queue != GIAOHIP_QUEUE_SINGLE) {
+        const int t64 = tile_for(64), t256 = tile_for(256);
+        if (t64 >= 1 && t256 >= 1 && (cd1 - cd0 + t64 - 1) / t64 == (cd1 - cd0 + t256 - 1) / t256) nt = 64;
+    }
+    int tile = tile_for(nt);
     if (tile < 1) return set_error(ctx, GIAOHIP_UNSUPPORTED, "ERI class too large for the GPU transform kernel");
     ca = gh::ClassArgs{{S[0]->l, S[1]->l, S[2]->l, S[3]->l}, tile};
     while (tile > 1 && gh::transform_lds_bytes(ca) > 160 * 1024) ca.tile = --tile;
@@ -498,18 +523,17 @@ int plain_plan(giaohip_ctx* ctx, const giaohip_basis* b, const Request& r, gh::C
 // image when that image's bra side is the larger one (the kernels are
 // instantiated for lab >= lcd): returns lab * 16 + lcd of the instance, or -1.
 // A request restricted to part of its ket components is not turned.
-// Under GIAOHIP_MFMA_DEFAULT only instances with both momenta >= kMfmaDefaultMinL
-// are used: below that the measured FP64 VALU kernels are faster (the dense
-// products carry the structural zeros of the Hermite tables).
-constexpr int kMfmaDefaultMinL = 7;
+// Under GIAOHIP_MFMA_DEFAULT only classes of total momentum >= min_l are used:
+// below that the FP64 VALU kernels are faster (the dense products carry the
+// structural zeros of the Hermite tables).
 // Instance keys of the split-FP32 form carry this flag.
 constexpr int kMfmaSplitKey = 256;
 
-int mfma_orient(const giaohip_basis* b, Request& r, int mode) {
+int mfma_orient(const giaohip_basis* b, Request& r, int mode, int min_l) {
     const DevShell *S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
     const int lab = S[0]->l + S[1]->l, lcd = S[2]->l + S[3]->l;
     if (mode == GIAOHIP_MFMA_OFF) return -1;
-    if (mode == GIAOHIP_MFMA_DEFAULT && std::min(lab, lcd) < kMfmaDefaultMinL) return -1;
+    if (mode == GIAOHIP_MFMA_DEFAULT && lab + lcd < min_l) return -1;
     const bool split = mode == GIAOHIP_MFMA_SPLIT32 &&
                        (gh::mfma_split_instance(lab, lcd) || (r.cd1 < 0 && gh::mfma_split_instance(lcd, lab)));
     const bool keep = lab >= lcd && (split ? gh::mfma_split_instance(lab, lcd) : gh::mfma_instance(lab, lcd));
@@ -562,7 +586,11 @@ size_t seed_scratch_zc(long long pq_total) {
 
 // Runs requests with the plain per-class transform kernel. With `fixed`, all
 // device memory comes from its slices (no allocation); chunks are then also
-// bounded by the slices' capacities.
+// bounded by the slices' capacities, and run one after the other. Otherwise
+// chunks are pipelined over two slots: while the transforms of chunk c run
+// on the context and aux streams, the host plans chunk c + 1 and its seed
+// kernel runs on the plain-path stream; a slot is reused once the chunk two
+// back has finished.
 int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs_in,
               zc* d_out, const FixedPlainMem* fixed = nullptr, bool mfma = false) {
     if (reqs_in.empty()) return GIAOHIP_OK;
@@ -573,7 +601,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         oriented = reqs_in;
         mfk.assign(oriented.size(), -1);
         parallel_for(oriented.size(), [&](size_t lo, size_t hi) {
-            for (size_t i = lo; i < hi; ++i) mfk[i] = mfma_orient(b, oriented[i], ctx->mfma_mode);
+            for (size_t i = lo; i < hi; ++i) mfk[i] = mfma_orient(b, oriented[i], ctx->mfma_mode, ctx->mfma_min_l);
         });
     }
     const std::vector<Request>& reqs = mfk.empty() ? reqs_in : oriented;
@@ -598,8 +626,30 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
     }
     HIP_TRY(hipMemcpyAsync(d_kappa, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
             "hipMemcpyAsync");
+    const bool piped = !fixed;
+    hipStream_t ps = piped ? ctx->plain_stream : s;  // the seed kernels' stream
+    if (piped) {
+        // the producer stream sees the zeroed error record and the wave vectors
+        HIP_TRY(hipEventRecord(ctx->pl_init, s), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(ps, ctx->pl_init, 0), "hipStreamWaitEvent");
+    }
+    // chunk timing of a finished slot into the statistics
+    auto slot_stats = [&](int k) -> hipError_t {
+        hipError_t e = hipEventSynchronize(ctx->pl_ev[k][2]);
+        if (e == hipSuccess) e = hipEventSynchronize(ctx->pl_ev[k][3]);
+        if (e != hipSuccess) return e;
+        const float t = std::max(elapsed(ctx->pl_ev[k][1], ctx->pl_ev[k][2]), elapsed(ctx->pl_ev[k][1], ctx->pl_ev[k][3]));
+        ctx->stats.seed_ms += elapsed(ctx->pl_ev[k][0], ctx->pl_ev[k][1]);
+        ctx->stats.transform_ms += t;
+        ctx->stats.contract_ms += t;
+        return hipSuccess;
+    };
+    // a chunk's seeds stay within four chunk budgets (per slot when pipelined:
+    // the longest items' tails are paid once per chunk, so chunks are large)
+    const size_t seed_cap = 4 * ctx->seed_budget_bytes;
 
     size_t begin = 0;
+    int chunk = 0;
     while (begin < reqs.size()) {
         // Chunk so that the seed buffer stays within budget.
         size_t end = begin;
@@ -614,13 +664,13 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const int L = A.l + B.l + C.l + D.l;
             const long long kab = (long long)A.nprim * B.nprim, kcd = (long long)C.nprim * D.nprim;
             const size_t need = (size_t)(kab * kcd) * (size_t)(L + 4);
-            // seeds are this path's only large buffer: four chunk budgets
-            if (end > begin && (seed_zc + need) * sizeof(zc) > 4 * ctx->seed_budget_bytes) break;
+            // seeds are this path's only large buffer
+            if (end > begin && (seed_zc + need) * sizeof(zc) > seed_cap) break;
             if (fixed) {
                 gh::ClassArgs ca;
-                int lds;
+                int lds, nt;
                 long long nit;
-                const int st = plain_plan(ctx, b, r, ca, lds, nit);
+                const int st = plain_plan(ctx, b, r, ca, lds, nit, nt);
                 if (st) return st;
                 const bool fits = sq.size() + 1 <= fixed->sq_cap && (size_t)(item_total + nit) <= fixed->items_cap &&
                                   seed_zc + need + seed_scratch_zc(pq_total + kab * kcd) <= fixed->seeds_cap;
@@ -646,20 +696,6 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             ++end;
         }
         const size_t scr_zc = seed_scratch_zc(pq_total);
-        zc* d_seeds;
-        SeedQuartet* d_sq;
-        if (fixed) {
-            d_seeds = fixed->seeds;
-            d_sq = fixed->sq;
-        } else {
-            HIP_TRY(ctx->seeds.ensure((seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
-            HIP_TRY(ctx->squartets.ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
-            d_seeds = ctx->seeds.as<zc>();
-            d_sq = ctx->squartets.as<SeedQuartet>();
-        }
-        int* d_scratch = reinterpret_cast<int*>(d_seeds + seed_zc);
-        HIP_TRY(hipMemcpyAsync(d_sq, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
-                "hipMemcpyAsync");
 
         // Work items of every class, one per pass of `tile` ket components,
         // each carrying its class; launched in a few LDS-size buckets, the
@@ -669,6 +705,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             WorkItem w;
             int lds;
             double cost;
+            int nt = 256;  // workgroup size of the generic transform
         };
         std::vector<Planned> planned;
         std::map<int, std::vector<Planned>> mf_planned;  // FP64-MFMA items per instance
@@ -711,9 +748,9 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                 continue;
             }
             gh::ClassArgs ca;
-            int lds;
+            int lds, nt;
             long long nit;
-            const int st = plain_plan(ctx, b, r, ca, lds, nit);
+            const int st = plain_plan(ctx, b, r, ca, lds, nit, nt);
             if (st) return st;
             const int L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
             const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
@@ -742,6 +779,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                                 for (int k = 0; k < 4; ++k) w.stride[k] = r.stride[k];
                                 w.cls = cls;
                                 pl.lds = lds;
+                                pl.nt = nt;
                                 pl.cost = kq * (gh::nherm(L) + 0.5 * NHab * gh::nherm(S[2]->l + S[3]->l) *
                                                                    (w.cd1 - w.cd0));
                                 planned.push_back(pl);
@@ -754,25 +792,28 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             int lds;
             size_t first, count;
             double cost;
-            int mf = -1;  // FP64-MFMA instance key, or -1 for the generic kernel
+            int mf = -1;   // FP64-MFMA instance key, or -1 for the generic kernel
+            int nt = 256;  // workgroup size of the generic kernel
         };
         std::vector<MixedLaunch> launches;
-        for (size_t bk = 0; bk < std::size(kBuckets); ++bk) {
-            const int lo_b = bk ? kBuckets[bk - 1] : 0, hi_b = kBuckets[bk];
-            std::vector<const Planned*> sel;
-            int lds_max = 0;
-            for (const Planned& pl : planned)
-                if (pl.lds > lo_b && pl.lds <= hi_b) {
-                    sel.push_back(&pl);
-                    lds_max = std::max(lds_max, pl.lds);
-                }
-            if (sel.empty()) continue;
-            std::stable_sort(sel.begin(), sel.end(), [](const Planned* x, const Planned* y) { return x->cost > y->cost; });
-            double cost = 0.0;
-            for (const Planned* pl : sel) cost += pl->cost;
-            launches.push_back({lds_max, all.size(), sel.size(), cost});
-            for (const Planned* pl : sel) all.push_back(pl->w);
-        }
+        for (const int nt : {64, 256})
+            for (size_t bk = 0; bk < std::size(kBuckets); ++bk) {
+                const int lo_b = bk ? kBuckets[bk - 1] : 0, hi_b = kBuckets[bk];
+                std::vector<const Planned*> sel;
+                int lds_max = 0;
+                for (const Planned& pl : planned)
+                    if (pl.nt == nt && pl.lds > lo_b && pl.lds <= hi_b) {
+                        sel.push_back(&pl);
+                        lds_max = std::max(lds_max, pl.lds);
+                    }
+                if (sel.empty()) continue;
+                std::stable_sort(sel.begin(), sel.end(),
+                                 [](const Planned* x, const Planned* y) { return x->cost > y->cost; });
+                double cost = 0.0;
+                for (const Planned* pl : sel) cost += pl->cost;
+                launches.push_back({lds_max, all.size(), sel.size(), cost, -1, nt});
+                for (const Planned* pl : sel) all.push_back(pl->w);
+            }
         if (ctx->eri_queue == GIAOHIP_QUEUE_SINGLE && launches.size() > 1) {
             // one persistent launch over every item, longest first, with the
             // largest LDS footprint of any of them
@@ -806,24 +847,51 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                 d_queue = ctx->queue.as<int>();
             }
         }
+        const int k = chunk & 1;  // slot of a pipelined chunk
+        zc* d_seeds;
+        SeedQuartet* d_sq;
         WorkItem* d_items;
+        hipEvent_t ev_s0 = ctx->ev[0], ev_s1 = ctx->ev[1];
         if (fixed) {
+            d_seeds = fixed->seeds;
+            d_sq = fixed->sq;
             d_items = fixed->items;
+            HIP_TRY(hipMemcpyAsync(d_sq, sq.data(), sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, s),
+                    "hipMemcpyAsync");
+            HIP_TRY(hipMemcpyAsync(d_items, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
+                    "hipMemcpyAsync");
         } else {
-            HIP_TRY(ctx->items.ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
-            d_items = ctx->items.as<WorkItem>();
+            // the slot is free once the chunk two back has finished
+            if (chunk >= 2) HIP_TRY(slot_stats(k), "kernel execution");
+            HIP_TRY(ctx->pl_seeds[k].ensure((seed_zc + scr_zc) * sizeof(zc)), "hipMalloc(seeds)");
+            HIP_TRY(ctx->pl_sq[k].ensure(sq.size() * sizeof(SeedQuartet)), "hipMalloc");
+            HIP_TRY(ctx->pl_items[k].ensure(all.size() * sizeof(WorkItem)), "hipMalloc");
+            HIP_TRY(ctx->pl_hsq[k].ensure(sq.size() * sizeof(SeedQuartet)), "hipHostMalloc");
+            HIP_TRY(ctx->pl_hitems[k].ensure(all.size() * sizeof(WorkItem)), "hipHostMalloc");
+            d_seeds = ctx->pl_seeds[k].as<zc>();
+            d_sq = ctx->pl_sq[k].as<SeedQuartet>();
+            d_items = ctx->pl_items[k].as<WorkItem>();
+            std::memcpy(ctx->pl_hsq[k].p, sq.data(), sq.size() * sizeof(SeedQuartet));
+            std::memcpy(ctx->pl_hitems[k].p, all.data(), all.size() * sizeof(WorkItem));
+            HIP_TRY(hipMemcpyAsync(d_sq, ctx->pl_hsq[k].p, sq.size() * sizeof(SeedQuartet), hipMemcpyHostToDevice, ps),
+                    "hipMemcpyAsync");
+            HIP_TRY(hipMemcpyAsync(d_items, ctx->pl_hitems[k].p, all.size() * sizeof(WorkItem),
+                                   hipMemcpyHostToDevice, ps),
+                    "hipMemcpyAsync");
+            ev_s0 = ctx->pl_ev[k][0];
+            ev_s1 = ctx->pl_ev[k][1];
         }
-        HIP_TRY(hipMemcpyAsync(d_items, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice, s),
-                "hipMemcpyAsync");
+        int* d_scratch = reinterpret_cast<int*>(d_seeds + seed_zc);
 
-        HIP_TRY(hipEventRecord(ctx->ev[0], s), "hipEventRecord");
+        HIP_TRY(hipEventRecord(ev_s0, ps), "hipEventRecord");
         HIP_TRY(gh::launch_seed(nmax_bucket(maxL), d_sq, (int)sq.size(), pq_total, b->d_shells.as<DevShell>(),
-                                b->d_exps.as<double>(), d_kappa, d_seeds, d_err, d_scratch, s),
+                                b->d_exps.as<double>(), d_kappa, d_seeds, d_err, d_scratch, ps),
                 "seed kernel launch");
-        HIP_TRY(hipEventRecord(ctx->ev[1], s), "hipEventRecord");
+        HIP_TRY(hipEventRecord(ev_s1, ps), "hipEventRecord");
         // class launches alternate between the two streams (disjoint outputs),
         // so that a launch with few workgroups does not leave the GPU idle
-        HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev[1], 0), "hipStreamWaitEvent");
+        if (piped) HIP_TRY(hipStreamWaitEvent(s, ev_s1, 0), "hipStreamWaitEvent");
+        HIP_TRY(hipStreamWaitEvent(ctx->aux, ev_s1, 0), "hipStreamWaitEvent");
         // greedy: the next launch goes to the stream with less work queued
         double queued[2] = {0.0, 0.0};
         std::vector<size_t> order(launches.size());
@@ -844,27 +912,44 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                 ctx->stats.launches += 1;
                 continue;
             }
-            HIP_TRY(gh::launch_transform_mixed(L.lds, d_items + L.first, (int)L.count, b->d_shells.as<DevShell>(),
+            HIP_TRY(gh::launch_transform_mixed(L.nt, L.lds, d_items + L.first, (int)L.count, b->d_shells.as<DevShell>(),
                                                b->d_exps.as<double>(), b->d_coefs.as<double>(),
                                                b->d_norms.as<double>(), d_kappa, d_seeds, d_out,
                                                d_queue ? d_queue + side : nullptr, ls),
                     "transform kernel launch");
             ctx->stats.launches += 1;
         }
-        HIP_TRY(hipEventRecord(ctx->ev[4], ctx->aux), "hipEventRecord");
-        HIP_TRY(hipStreamWaitEvent(s, ctx->ev[4], 0), "hipStreamWaitEvent");
-        HIP_TRY(hipEventRecord(ctx->ev[2], s), "hipEventRecord");
-        HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
-        ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
-        ctx->stats.transform_ms += elapsed(ctx->ev[1], ctx->ev[2]);
-        ctx->stats.contract_ms += elapsed(ctx->ev[1], ctx->ev[2]);
         ctx->stats.contract_launches += launches.size();
         ctx->stats.launches += 1;
         ctx->stats.primitive_quartets += (uint64_t)pq_total;
+        if (piped) {
+            // no join: each stream runs on into the next chunk; the slot is
+            // free once both have passed this chunk
+            HIP_TRY(hipEventRecord(ctx->pl_ev[k][2], s), "hipEventRecord");
+            HIP_TRY(hipEventRecord(ctx->pl_ev[k][3], ctx->aux), "hipEventRecord");
+        } else {
+            HIP_TRY(hipEventRecord(ctx->ev[4], ctx->aux), "hipEventRecord");
+            HIP_TRY(hipStreamWaitEvent(s, ctx->ev[4], 0), "hipStreamWaitEvent");
+            HIP_TRY(hipEventRecord(ctx->ev[2], s), "hipEventRecord");
+            HIP_TRY(hipEventSynchronize(ctx->ev[2]), "kernel execution");
+            ctx->stats.seed_ms += elapsed(ctx->ev[0], ctx->ev[1]);
+            ctx->stats.transform_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+            ctx->stats.contract_ms += elapsed(ctx->ev[1], ctx->ev[2]);
+            DevError de{};
+            HIP_TRY(hipMemcpy(&de, d_err, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
+            if (de.code != 0) return boys_error(ctx, de);
+        }
+        begin = end;
+        ++chunk;
+    }
+    if (piped) {
+        for (int c = std::max(0, chunk - 2); c < chunk; ++c) HIP_TRY(slot_stats(c & 1), "kernel execution");
+        // later work on the context stream follows the aux stream's transforms
+        HIP_TRY(hipEventRecord(ctx->ev[4], ctx->aux), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(s, ctx->ev[4], 0), "hipStreamWaitEvent");
         DevError de{};
         HIP_TRY(hipMemcpy(&de, d_err, sizeof de, hipMemcpyDeviceToHost), "hipMemcpy");
         if (de.code != 0) return boys_error(ctx, de);
-        begin = end;
     }
     return GIAOHIP_OK;
 }
@@ -1194,7 +1279,7 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
             is_fast[i] = batched && reqs[i].cd1 < 0 && orient(b, reqs[i], fast_all[i]);
             if (is_fast[i] && ctx->mfma_mode >= GIAOHIP_MFMA_ALL) {
                 Request r = reqs[i];
-                is_fast[i] = mfma_orient(b, r, ctx->mfma_mode) < 0;
+                is_fast[i] = mfma_orient(b, r, ctx->mfma_mode, ctx->mfma_min_l) < 0;
             }
         }
     });
@@ -1356,7 +1441,10 @@ int giaohip_create(int device, giaohip_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->plain_stream, hipStreamNonBlocking);
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->pl_ev[i / 4][i % 4]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->pl_init, hipEventDisableTiming);
     for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ctx->copy_ev[i], hipEventDisableTiming);
     if (e == hipSuccess) e = gh::upload_constants();
     if (e == hipSuccess) e = noalloc_scratch_init(device, &ctx->noalloc);
@@ -1375,6 +1463,14 @@ int giaohip_destroy(giaohip_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    if (ctx->plain_stream) (void)hipStreamSynchronize(ctx->plain_stream);
+    for (int k = 0; k < 2; ++k) {
+        for (DevBuffer* b : {&ctx->pl_seeds[k], &ctx->pl_sq[k], &ctx->pl_items[k]}) b->release();
+        for (PinnedBuffer* b : {&ctx->pl_hsq[k], &ctx->pl_hitems[k]}) b->release();
+        for (auto& ev : ctx->pl_ev[k])
+            if (ev) (void)hipEventDestroy(ev);
+    }
+    if (ctx->pl_init) (void)hipEventDestroy(ctx->pl_init);
     ctx->staging2.release();
     for (PinnedBuffer& b : ctx->bounce) b.release();
     for (auto& ev : ctx->copy_ev)
@@ -1394,6 +1490,7 @@ int giaohip_destroy(giaohip_ctx* ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+    if (ctx->plain_stream) (void)hipStreamDestroy(ctx->plain_stream);
     delete ctx;
     return GIAOHIP_OK;
 }
@@ -1444,6 +1541,14 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_SPLIT32) {
         ctx->mfma_mode = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_MFMA_MIN_L && value >= 0 && value <= 33) {
+        ctx->mfma_min_l = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_WAVE_MAX_L && value >= -1 && value <= gh::kMaxTotalL) {
+        ctx->wave_max_l = value;
         return GIAOHIP_OK;
     }
     if (option == GIAOHIP_OPTION_PIPELINE && (value == GIAOHIP_PIPELINE_OVERLAP || value == GIAOHIP_PIPELINE_SERIAL)) {

@@ -19,7 +19,8 @@
 //                     (overlap.cpp:60-91), Boys argument and scaling choice
 //                     (eri.cpp:162-210), complex Boys F_0..F_L, and the R0
 //                     seeds s_n = pref * common * (-2 rho)^n F_n plus X.
-//   transform_kernel: one 256-thread workgroup per work item (shell quartet x
+//   transform_kernel: one 256-thread (or, for small items, 64-thread)
+//                     workgroup per work item (shell quartet x
 //                     contraction rows x ket-component range). Per primitive
 //                     quartet the R0 table is built level by level in LDS,
 //                     the ket transform W[k][cd] += sum_j Et_cd R0[k+j] is
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(64) void boys_probe_kernel(const zc* __restrict__ z
 // Transform kernel
 // ---------------------------------------------------------------------------
 constexpr int kNT = 256;
+constexpr int kWaveNT = 64;  // the one-wavefront form
 constexpr int kMaxW = 8;  // W entries per thread
 constexpr int kMaxO = 8;  // output entries per thread
 
@@ -373,8 +375,13 @@ __device__ __host__ inline int lex_off(int D, int t, int u) {
     return nherm(D) - nherm(D - t) + u * (D - t + 1) - u * (u - 1) / 2;
 }
 
-// One work item of the generic transform. Every thread of the 256-thread
+// One work item of the generic transform. Every thread of the NT-thread
 // workgroup calls it with the same item; smem_raw is the dynamic LDS region.
+// NT = 256 (four wavefronts) or 64 (one wavefront: items whose table entries
+// and transform rows fit one wavefront's registers; a CU then holds four
+// times as many items, which hides the latency of the per-primitive-quartet
+// steps, and the workgroup barriers are a single wavefront's).
+template <int NT>
 __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkItem it,
                                                const DevShell* __restrict__ shells, const double* __restrict__ exps,
                                                const double* __restrict__ coefs, const double* __restrict__ norms,
@@ -400,11 +407,11 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
     const int tid = threadIdx.x;
 
     // Index tables (parallel fills).
-    for (int e = tid; e < (L + 1) * (L + 1); e += kNT) {
+    for (int e = tid; e < (L + 1) * (L + 1); e += NT) {
         const int t = e / (L + 1), u = e % (L + 1);
         offL[e] = t + u <= L ? lex_off(L, t, u) : -1;
     }
-    for (int e = tid; e < (Lab + 1) * (Lab + 1); e += kNT) {
+    for (int e = tid; e < (Lab + 1) * (Lab + 1); e += NT) {
         const int t = e / (Lab + 1), u = e % (Lab + 1);
         if (t + u <= Lab) {
             const int o = lex_off(Lab, t, u);
@@ -415,7 +422,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
         }
     }
     // R0 entries sorted by degree: degree d, then t descending, then u descending.
-    for (int e = tid; e < nherm(L); e += kNT) {
+    for (int e = tid; e < nherm(L); e += NT) {
         int d = 0;
         while (nherm(d) <= e) ++d;
         int r = e - (d > 0 ? nherm(d - 1) : 0);
@@ -447,13 +454,13 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
     const int ebra_i = (lb + 1) * (Lab + 1), ebra_a = (la + 1) * ebra_i;
     const int eket_i = (ld + 1) * (Lcd + 1), eket_a = (lc + 1) * eket_i;
 
-    // The R0 recurrence step of entries tid and tid + kNT (fixed for the
+    // The R0 recurrence step of entries tid and tid + NT (fixed for the
     // item): rs_a = lex | src1 << 16, rs_b = src2 | axis << 16 | (coef) << 18,
     // axis 3 marking the seed entry.
     int rs_a[2], rs_b[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        const int e = tid + r * kNT;
+        const int e = tid + r * NT;
         rs_a[r] = 0;
         rs_b[r] = 3 << 16;
         if (e < nherm(L)) {
@@ -478,6 +485,14 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
         }
     }
 
+    // Seeds of the next primitive quartet (lanes below L + 4), loaded one
+    // primitive quartet ahead so that their latency overlaps the current one;
+    // the order is the loop order below (bra pair, ket pair), restarting for
+    // each pass over the ket components.
+    const long long npq = (long long)Kab * Kcd;
+    const zc* seed_lane = seeds + it.seed_off + (tid < stride_pq ? tid : 0);
+    zc seed_next = tid < stride_pq ? seed_lane[0] : zc{0.0, 0.0};
+
     for (int t0 = 0; t0 < nsel; t0 += tile) {
         const int tl = min(tile, nsel - t0);
         zc acc[kMaxO];
@@ -489,7 +504,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
             const double ea = exps[A.prim_off + pa], eb = exps[B.prim_off + pb];
             const double p = ea + eb;
             // bra 1-D complex Hermite tables
-            for (int e = tid; e < 3 * (la + 1) * (lb + 1); e += kNT) {
+            for (int e = tid; e < 3 * (la + 1) * (lb + 1); e += NT) {
                 const int axis = e / ((la + 1) * (lb + 1));
                 const int i = (e / (lb + 1)) % (la + 1), j = e % (lb + 1);
                 const double Pc = (ea * Ac[axis] + eb * Bc[axis]) / p;
@@ -506,9 +521,13 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                 const double ec = exps[C.prim_off + pc], ed = exps[D.prim_off + pd];
                 const double q = ec + ed;
                 __syncthreads();  // previous primitive quartet finished with LDS
-                const zc* sp = seeds + it.seed_off + (long long)(pab * Kcd + pcd) * stride_pq;
-                if (tid < stride_pq) sseed[tid] = sp[tid];
-                for (int e = tid; e < 3 * (lc + 1) * (ld + 1); e += kNT) {
+                if (tid < stride_pq) {
+                    sseed[tid] = seed_next;
+                    long long nx = (long long)pab * Kcd + pcd + 1;
+                    if (nx == npq) nx = 0;
+                    seed_next = seed_lane[nx * stride_pq];
+                }
+                for (int e = tid; e < 3 * (lc + 1) * (ld + 1); e += NT) {
                     const int axis = e / ((lc + 1) * (ld + 1));
                     const int i = (e / (ld + 1)) % (lc + 1), j = e % (ld + 1);
                     const double Qc = (ec * Cc[axis] + ed * Dc[axis]) / q;
@@ -529,7 +548,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                     const int cnt = nherm(L - n);
 #pragma unroll
                     for (int r = 0; r < 2; ++r) {
-                        if (tid + r * kNT < cnt) {
+                        if (tid + r * NT < cnt) {
                             const int lex = rs_a[r] & 0xffff, s1 = rs_a[r] >> 16;
                             const int s2 = rs_b[r] & 0xffff, ax = (rs_b[r] >> 16) & 3, cf = rs_b[r] >> 18;
                             zc val;
@@ -546,7 +565,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                             nxt[lex] = val;
                         }
                     }
-                    for (int e = tid + 2 * kNT; e < cnt; e += kNT) {
+                    for (int e = tid + 2 * NT; e < cnt; e += NT) {
                         const int pk = listL[e];
                         const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63, lex = pk >> 18;
                         zc val;
@@ -587,7 +606,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                 const int tid_k = opaque_tid(tid);
 #pragma unroll
                 for (int r = 0; r < kMaxW; ++r) {
-                    const int e = tid_k + r * kNT;
+                    const int e = tid_k + r * NT;
                     if (e < nW) {
                         const int cdl = e / NHab, k = e % NHab;
                         const int pk = tuvAB[k];
@@ -628,7 +647,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                 const int nW = NHab * tl;
 #pragma unroll
                 for (int r = 0; r < kMaxW; ++r) {
-                    const int e = tid + r * kNT;
+                    const int e = tid + r * NT;
                     if (e < nW) {
                         const int cdl = e / NHab, k = e % NHab;
                         work[k * tl + cdl] = w[r];
@@ -641,7 +660,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
             const int tid_b = opaque_tid(tid);
 #pragma unroll
             for (int r = 0; r < kMaxO; ++r) {
-                const int o = tid_b + r * kNT;
+                const int o = tid_b + r * NT;
                 if (o < nO) {
                     const int abcomp = o / tl, cdl = o % tl;
                     const int cac = abcomp / ncb, cbc = abcomp % ncb;
@@ -677,7 +696,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
         const int nO = nab * tl;
 #pragma unroll
         for (int r = 0; r < kMaxO; ++r) {
-            const int o = tid + r * kNT;
+            const int o = tid + r * NT;
             if (o < nO) {
                 const int abcomp = o / tl, cdl = o % tl;
                 const int cac = abcomp / ncb, cbc = abcomp % ncb;
@@ -693,12 +712,13 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
 }
 
 // One workgroup per work item.
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void transform_kernel(
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void transform_kernel(
     ClassArgs ca, const WorkItem* __restrict__ items, const DevShell* __restrict__ shells,
     const double* __restrict__ exps, const double* __restrict__ coefs, const double* __restrict__ norms,
     const double* __restrict__ kappa, const zc* __restrict__ seeds, zc* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    transform_item(ca, items[blockIdx.x], shells, exps, coefs, norms, kappa, seeds, out, smem_raw);
+    transform_item<NT>(ca, items[blockIdx.x], shells, exps, coefs, norms, kappa, seeds, out, smem_raw);
 }
 
 // Persistent form: the grid's workgroups take items from a device work queue
@@ -717,7 +737,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(4))) void t
         __syncthreads();
         const int i = next;
         if (i >= nitems) break;
-        transform_item(ca, items[i], shells, exps, coefs, norms, kappa, seeds, out, smem_raw);
+        transform_item<kNT>(ca, items[i], shells, exps, coefs, norms, kappa, seeds, out, smem_raw);
         __syncthreads();  // LDS and `next` are free for the next item
     }
 }
@@ -817,8 +837,8 @@ int transform_lds_bytes(const ClassArgs& ca) {
     return bytes;
 }
 
-int transform_max_w() { return kMaxW * kNT; }
-int transform_max_o() { return kMaxO * kNT; }
+int transform_max_w(int nt) { return kMaxW * nt; }
+int transform_max_o(int nt) { return kMaxO * nt; }
 
 hipError_t upload_constants() {
     // Gauss-Legendre nodes by Newton iteration on P_n, the same procedure as
@@ -945,32 +965,46 @@ hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nite
     if (bytes > 64 * 1024) {
         static bool raised = false;
         if (!raised) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel),
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel<kNT>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
             if (e != hipSuccess) return e;
             raised = true;
         }
     }
-    transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), bytes, s>>>(ca, items, shells, exps, coefs, norms, kappa,
-                                                                      seeds, out);
+    transform_kernel<kNT><<<dim3((unsigned)nitems), dim3(kNT), bytes, s>>>(ca, items, shells, exps, coefs, norms,
+                                                                           kappa, seeds, out);
     return hipGetLastError();
 }
 
-hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
+hipError_t launch_transform_mixed(int nt, int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
                                   const double* exps, const double* coefs, const double* norms, const double* kappa,
                                   const zc* seeds, zc* out, int* queue, hipStream_t s) {
     if (nitems == 0) return hipSuccess;
-    if (lds_bytes > 160 * 1024) return hipErrorInvalidValue;
+    if (lds_bytes > 160 * 1024 || (nt != kNT && nt != kWaveNT)) return hipErrorInvalidValue;
+    const ClassArgs none{{0, 0, 0, 0}, 1};  // every item carries its class
+    if (nt == kWaveNT) {
+        if (lds_bytes > 64 * 1024) {
+            static bool raised_w = false;
+            if (!raised_w) {
+                hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel<kWaveNT>),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                if (e != hipSuccess) return e;
+                raised_w = true;
+            }
+        }
+        transform_kernel<kWaveNT><<<dim3((unsigned)nitems), dim3(kWaveNT), lds_bytes, s>>>(
+            none, items, shells, exps, coefs, norms, kappa, seeds, out);
+        return hipGetLastError();
+    }
     if (lds_bytes > 64 * 1024) {
         static bool raised = false;
         if (!raised) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel),
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&transform_kernel<kNT>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
             if (e != hipSuccess) return e;
             raised = true;
         }
     }
-    const ClassArgs none{{0, 0, 0, 0}, 1};  // every item carries its class
     // the queue form also holds one int of static LDS
     if (queue && lds_bytes + 256 <= 160 * 1024) {
         hipError_t e = hipSuccess;
@@ -996,8 +1030,8 @@ hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nite
                                                                          coefs, norms, kappa, seeds, out);
         return hipGetLastError();
     }
-    transform_kernel<<<dim3((unsigned)nitems), dim3(kNT), lds_bytes, s>>>(none, items, shells, exps, coefs, norms,
-                                                                          kappa, seeds, out);
+    transform_kernel<kNT><<<dim3((unsigned)nitems), dim3(kNT), lds_bytes, s>>>(none, items, shells, exps, coefs,
+                                                                               norms, kappa, seeds, out);
     return hipGetLastError();
 }
 

@@ -134,8 +134,9 @@ int fast_waves(int nhab, int kab, int cap = 0);
 
 hipError_t upload_constants();
 int transform_lds_bytes(const ClassArgs& ca);
-int transform_max_w();
-int transform_max_o();
+// Transform entries per workgroup (W entries, output entries) for nt threads.
+int transform_max_w(int nt);
+int transform_max_o(int nt);
 
 // Seeds of `total` primitive quartets. `scratch` holds seed_scratch_ints(total)
 // ints (quadrature work lists); it is reused by the next launch in stream order.
@@ -182,11 +183,12 @@ hipError_t launch_transform(const ClassArgs& ca, const WorkItem* items, int nite
                             const double* exps, const double* coefs, const double* norms, const double* kappa,
                             const zc* seeds, zc* out, hipStream_t s);
 // Transform of items of several classes in one launch: every item carries
-// its class and tile in WorkItem::cls; lds_bytes covers the largest. With
+// its class and tile in WorkItem::cls; lds_bytes covers the largest; nt is the
+// workgroup size the items were planned for (256, or 64: one wavefront). With
 // `queue` (one device int, used only by launches on stream s) the launch is
 // persistent: as many workgroups as fit on the device take items from a
 // work queue in list order; without it, one workgroup per item.
-hipError_t launch_transform_mixed(int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
+hipError_t launch_transform_mixed(int nt, int lds_bytes, const WorkItem* items, int nitems, const DevShell* shells,
                                   const double* exps, const double* coefs, const double* norms, const double* kappa,
                                   const zc* seeds, zc* out, int* queue, hipStream_t s);
 // Cholesky column ERIs for one (la, lb) group of bra shell pairs and the

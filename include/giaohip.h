@@ -56,7 +56,9 @@ enum {
     GIAOHIP_OPTION_ERI_VARIANT = 0,
     /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables),
        in MiB, 16..65536; default 2048. Two chunks are in flight at once (one
-       being produced, one transformed), so the path holds twice this much. */
+       being produced, one transformed), so the path holds twice this much.
+       Chunks of the generic transform hold up to four times this much of
+       seeds, also two at once (eight times in all). */
     GIAOHIP_OPTION_CHUNK_MIB = 1,
     /* Candidates per pivot block of giaohip_cholesky_run, 1..32; default 32.
        A block projects its candidates (the largest residuals) onto the
@@ -95,7 +97,17 @@ enum {
     /* Scheduling of the batched path's producer kernels (seeds, pair
        tables, R0 tables), one of GIAOHIP_PIPELINE_*; default
        GIAOHIP_PIPELINE_OVERLAP. Results are bitwise identical for both. */
-    GIAOHIP_OPTION_PIPELINE = 8
+    GIAOHIP_OPTION_PIPELINE = 8,
+    /* Largest total angular momentum la + lb + lc + ld of the classes whose
+       generic-transform items run on one-wavefront workgroups (when their
+       transform rows fit one wavefront); -1 puts every item on four-wavefront
+       workgroups. Default 5. Results are bitwise identical for every value. */
+    GIAOHIP_OPTION_WAVE_MAX_L = 9,
+    /* Under GIAOHIP_MFMA_DEFAULT: smallest total angular momentum
+       la + lb + lc + ld of the classes (without a class-specialized kernel,
+       both pair momenta >= 2) whose contraction runs on the FP64 matrix
+       cores; 0..33, default 9. */
+    GIAOHIP_OPTION_MFMA_MIN_L = 10
 };
 enum {
     GIAOHIP_PIPELINE_OVERLAP = 0, /* on a second stream, beside the previous chunk's contraction */
@@ -103,7 +115,8 @@ enum {
 };
 enum {
     GIAOHIP_MFMA_OFF = 0,     /* FP64 VALU kernels only */
-    GIAOHIP_MFMA_DEFAULT = 1, /* matrix cores where they are measured faster: bra and ket momenta >= 7 */
+    GIAOHIP_MFMA_DEFAULT = 1, /* matrix cores where they are measured faster: total momentum >=
+                                 GIAOHIP_OPTION_MFMA_MIN_L */
     GIAOHIP_MFMA_ALL = 2,     /* matrix cores for every class with an instance (precision studies) */
     GIAOHIP_MFMA_SPLIT32 = 3  /* as ALL, but (gg|gg) and (ff|dd)-type classes ((LAB, LCD) = (8, 8), (6, 4))
                                  in split FP32: hi/lo FP32 products on v_mfma_f32_16x16x4f32 accumulated in

@@ -69,6 +69,29 @@ def test_queue_forms_bitwise(port, variant):
     assert err.max() <= ATOL
 
 
+def test_wave_workgroups_bitwise(port):
+    """One-wavefront workgroups for the generic transform's small items give
+    the integrals of four-wavefront workgroups bitwise (the per-output sums
+    are the same; only the ket components per item and the occupancy
+    change)."""
+    ctx = _native.Context(0)
+    batch = workloads.mixed_batch(3000)
+    db = _device(ctx, batch)
+    f = _native.field6(batch.B, batch.origin)
+    ctx.set_variant("plain")
+    got = {}
+    for lmax in (-1, 7, 12, 3):
+        print(f"one-wavefront items up to L = {lmax}, 3000 mixed quartets", flush=True)
+        ctx.set_wave_max_l(lmax)
+        got[lmax] = db.eri_batch(batch.quartets, f)
+    for lmax in (7, 12, 3):
+        assert np.array_equal(got[lmax], got[-1]), lmax
+    rows = np.arange(0, 3000, 15)
+    err = np.abs(_rows_of(db, batch, got[12], rows) - _oracle_rows(port, batch, rows))
+    print(f"one-wavefront items vs oracle on {len(rows)} quartets: max |err| {err.max():.2e}")
+    assert err.max() <= ATOL
+
+
 def test_fast_waves_tiles(port):
     ctx = _native.Context(0)
     batch = workloads.dddd_batch(96)
