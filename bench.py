@@ -432,10 +432,12 @@ def projection_roofline(nbytes: float, flops: float, block_ms: float, unblocked_
             "frac": t_hbm / sec, **common}
 
 
-def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0):
+def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0, graphs_rank: int = 1500):
     """Streamed pivoted Cholesky of the 30-atom 4s3p2d1f set (1050 AOs), tau = 1e-8.
 
-    Runs the device driver in steps of 1000 pivots (a progress line after
+    First the column launches as hipGraphs against direct launches on the
+    first graphs_rank pivots (wall time of each, identical pivots required);
+    then the device driver in steps of 1000 pivots (a progress line after
     each) until convergence, or until rank_cap pivots when rank_cap > 0.
     """
     from giaoint_amd import _native, workloads
@@ -455,6 +457,26 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0):
                                        workload=f"30-atom 4s3p2d1f Cholesky, first {tune_rank} pivots")
         log(f"cholesky: autotuned {tuned.values} " +
             " ".join(f"B={t.value}:{t.seconds:.2f}s" for t in tuned.trials))
+    graphs_cmp = None
+    if graphs_rank > 0:
+        graphs_cmp = {"pivots": graphs_rank}
+        piv = {}
+        for on in (False, True, False, True):
+            ctx.set_cholesky_graphs(on)
+            t = time.perf_counter()
+            pre = _native.cholesky_run(db, field, 1e-8, 0, graphs_rank)
+            dt = time.perf_counter() - t
+            key = "graphs_s" if on else "direct_s"
+            graphs_cmp[key] = min(dt, graphs_cmp.get(key, dt))
+            if on:
+                graphs_cmp["graph_use"] = pre.graphs
+            piv[on] = pre.pivots()
+            pre.close()
+        graphs_cmp["pivots_identical"] = bool(np.array_equal(piv[True], piv[False]))
+        graphs_cmp["note"] = "best of two runs each, from scratch to the pivot count"
+        ctx.set_cholesky_graphs(True)
+        log(f"cholesky: first {graphs_rank} pivots, direct {graphs_cmp['direct_s']:.2f} s, "
+            f"graphs {graphs_cmp['graphs_s']:.2f} s")
     step = 1000
     t0 = time.perf_counter()
     first = step if rank_cap <= 0 else min(step, rank_cap)
@@ -503,6 +525,7 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0):
             "note": "K = sum_P L_P D L_P^H: two rocBLAS zgemm of order n per factor (8 FLOP per complex MAC), "
                     "density and K over PCIe",
         },
+        "column_graphs": {**ch.graphs, "prefix_comparison": graphs_cmp},
         "pivot_blocks": blk["blocks"],
         "pivots_per_block": m / max(blk["blocks"], 1),
         "candidates_projected_per_block": blk["projected"] / max(blk["blocks"], 1),

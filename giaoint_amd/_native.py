@@ -104,6 +104,7 @@ _SIGNATURES = {
     "giaohip_cholesky_vectors": ([_vp, _vp, C.c_uint64, C.c_uint64, _dp], C.c_int),
     "giaohip_cholesky_resume": ([_vp, _vp, C.c_uint64], C.c_int),
     "giaohip_cholesky_blocks": ([_vp, _u64p, _u64p, _dp, _u64p, _u64p], C.c_int),
+    "giaohip_cholesky_graphs": ([_vp, _u64p, _u64p, _u64p], C.c_int),
     "giaohip_cholesky_reconstruct": ([_vp, _vp, C.POINTER(C.c_int64), _dp], C.c_int),
     "giaohip_cholesky_coulomb": ([_vp, _vp, _dp, _dp], C.c_int),
     "giaohip_cholesky_exchange": ([_vp, _vp, _dp, _dp], C.c_int),
@@ -252,6 +253,10 @@ class Context:
         """Largest la + lb + lc + ld whose generic-transform items run on
         one-wavefront workgroups (-1: none). Bitwise identical results."""
         self.check(self.lib.giaohip_set_option(self.handle, 9, int(value)))
+
+    def set_cholesky_graphs(self, on: bool):
+        """Cholesky column launches as hipGraphs (default) or launched directly."""
+        self.check(self.lib.giaohip_set_option(self.handle, 11, 1 if on else 0))
 
     def set_mfma_min_l(self, value: int):
         """Smallest la + lb + lc + ld whose classes (without a class-specialized
@@ -505,6 +510,9 @@ class DeviceCholesky:
                                                             C.byref(proj), C.byref(crows)))
         self.blocks = {"blocks": int(blocks.value), "factor_rows_read": int(rows.value), "block_ms": bms.value,
                        "projected": int(proj.value), "candidate_rows": int(crows.value)}
+        gl, gi, gu = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.lib.giaohip_cholesky_graphs(self.handle, C.byref(gl), C.byref(gi), C.byref(gu)))
+        self.graphs = {"launches": int(gl.value), "instantiated": int(gi.value), "updated": int(gu.value)}
 
     def remeasure(self) -> float:
         """Final residual max(D - sum |L|^2), all diagonal blocks measured again (cholesky.py:283-288)."""

@@ -224,6 +224,7 @@ struct giaohip_ctx {
     int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
     int wave_max_l = 5;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
     int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
+    int chol_graphs = 1;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     // pipelined chunks of the plain path (run_plain): per slot the seeds,
     // descriptors and work items of one chunk in flight, their page-locked
@@ -296,7 +297,21 @@ struct giaohip_cholesky {
     hipEvent_t ev[10]{};
     hipStream_t s3 = nullptr;  // second stream of the column launches
     void* host = nullptr;  // page-locked copy of the state and the PSD flag
+    // A pivot's column launches as hipGraphs, one executable graph per launch
+    // topology (ket momenta, whether the seeds are computed, launch stream,
+    // kernel and stream of each bra group): the first pivot of a topology
+    // launches directly, later ones capture their launches and update the
+    // graph.
+    struct ColumnGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t uses = 0;
+    };
+    std::map<std::vector<int>, ColumnGraph> graphs;
+    uint64_t graph_launches = 0, graph_instantiations = 0, graph_updates = 0;
     void release() {
+        for (auto& [key, g] : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
         for (zc* c : chunks) (void)hipFree(c);
         chunks.clear();
         for (DevBuffer* x : {&d_units, &d_sq, &d_wi, &d_res, &d_col, &d_coef, &d_pmax, &d_pidx, &d_state, &d_flags,
@@ -1541,6 +1556,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_SPLIT32) {
         ctx->mfma_mode = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_CHOL_GRAPHS && (value == 0 || value == 1)) {
+        ctx->chol_graphs = value;
         return GIAOHIP_OK;
     }
     if (option == GIAOHIP_OPTION_MFMA_MIN_L && value >= 0 && value <= 33) {
@@ -2912,9 +2931,6 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         // stream, beside the block's candidate selection and projection.
         hipStream_t sc = new_block ? ctx->aux : s;
         CHOL_STEP_TRY(hipEventRecord(c->ev[0], sc), "hipEventRecord");
-        CHOL_STEP_TRY(gh::launch_chol_expand(c->d_units.as<gh::CholUnit>(), c->nunits, ket, c->d_sq.as<SeedQuartet>(),
-                                             c->d_wi.as<WorkItem>(), sc),
-                      "expand launch");
         const int maxL = c->max_lab + ket.lcd;
         // the seeds of this ket shell pair, computed now unless still buffered
         const long long skey = (long long)sb * (long long)b->shells.size() + sa;
@@ -2930,49 +2946,101 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
         }
         c->seed_used[sslot] = c->rank + 1;
         zc* seeds = c->d_seeds.as<zc>() + (size_t)sslot * c->seed_slot_zc;
-        if (seeds_ready) {
-            c->seed_hits += 1;
-        } else {
-            c->seed_misses += 1;
-            CHOL_STEP_TRY(gh::launch_seed(nmax_bucket(maxL), c->d_sq.as<SeedQuartet>(), c->nunits, c->p1 * ket.kcd,
-                                          b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
-                                          seeds, c->d_err.as<DevError>(), c->d_scratch, sc),
-                          "seed launch");
-        }
+        if (seeds_ready) c->seed_hits += 1;
+        else c->seed_misses += 1;
         // the (la, lb) groups of bra pairs write disjoint rows of the column:
-        // the larger ones first, spread over two streams
-        CHOL_STEP_TRY(hipEventRecord(c->ev[8], sc), "hipEventRecord");
-        CHOL_STEP_TRY(hipStreamWaitEvent(c->s3, c->ev[8], 0), "hipStreamWaitEvent");
+        // the larger ones first, spread over two streams; the launch topology
+        // (the graph key) is the ket momenta, whether the seeds are computed,
+        // and per group its kernel and stream
         std::vector<std::pair<double, size_t>> gorder;
         for (size_t gi = 0; gi < c->groups.size(); ++gi) {
             const auto& g = c->groups[gi];
             gorder.push_back({(double)g[3] * gh::nherm(g[0] + g[1] + C.l + D.l), gi});
         }
         std::sort(gorder.begin(), gorder.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-        double gq[2] = {0.0, 0.0};
-        for (const auto& [gcost, gi] : gorder) {
-            const auto& g = c->groups[gi];
-            const int side = gq[1] < gq[0] ? 1 : 0;
-            gq[side] += gcost;
-            hipStream_t gs = side ? c->s3 : sc;
-            gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
-            const bool column_kernel = g[0] + g[1] + C.l + D.l <= gh::kColumnMaxL &&
-                                       gh::chol_column_lds_bytes(g[0], g[1], C.l, D.l, ket.comp) <= 64 * 1024;
-            if (column_kernel)
-                CHOL_STEP_TRY(gh::launch_chol_column(ca, ket.comp, c->d_wi.as<WorkItem>() + g[2], g[3],
-                                                     b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
-                                                     b->d_coefs.as<double>(), b->d_norms.as<double>(),
-                                                     c->d_kappa.as<double>(), seeds, c->d_col.as<zc>(), gs),
-                              "column launch");
-            else
-                CHOL_STEP_TRY(gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
-                                                   b->d_exps.as<double>(), b->d_coefs.as<double>(),
-                                                   b->d_norms.as<double>(), c->d_kappa.as<double>(), seeds,
-                                                   c->d_col.as<zc>(), gs),
-                              "transform launch");
+        // (an executable graph is only ever launched on one stream: updated in
+        // place and launched on another stream than its previous launch, its
+        // kernels were measured to run with stale arguments on this runtime)
+        std::vector<int> topo = {C.l, D.l, seeds_ready ? 1 : 0, c->nunits > 0 ? 1 : 0, new_block ? 1 : 0};
+        std::vector<int> gside(gorder.size()), gcol(gorder.size());
+        {
+            double gq[2] = {0.0, 0.0};
+            for (size_t o = 0; o < gorder.size(); ++o) {
+                const auto& g = c->groups[gorder[o].second];
+                gside[o] = gq[1] < gq[0] ? 1 : 0;
+                gq[gside[o]] += gorder[o].first;
+                gcol[o] = g[0] + g[1] + C.l + D.l <= gh::kColumnMaxL &&
+                          gh::chol_column_lds_bytes(g[0], g[1], C.l, D.l, ket.comp) <= 64 * 1024;
+                topo.push_back((int)gorder[o].second * 4 + gside[o] * 2 + gcol[o]);
+            }
         }
-        CHOL_STEP_TRY(hipEventRecord(c->ev[9], c->s3), "hipEventRecord");
-        CHOL_STEP_TRY(hipStreamWaitEvent(sc, c->ev[9], 0), "hipStreamWaitEvent");
+        auto issue_column = [&]() -> hipError_t {
+            hipError_t e = gh::launch_chol_expand(c->d_units.as<gh::CholUnit>(), c->nunits, ket,
+                                                  c->d_sq.as<SeedQuartet>(), c->d_wi.as<WorkItem>(), sc);
+            if (e == hipSuccess && !seeds_ready)
+                e = gh::launch_seed(nmax_bucket(maxL), c->d_sq.as<SeedQuartet>(), c->nunits, c->p1 * ket.kcd,
+                                    b->d_shells.as<DevShell>(), b->d_exps.as<double>(), c->d_kappa.as<double>(),
+                                    seeds, c->d_err.as<DevError>(), c->d_scratch, sc);
+            if (e == hipSuccess) e = hipEventRecord(c->ev[8], sc);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->s3, c->ev[8], 0);
+            for (size_t o = 0; o < gorder.size() && e == hipSuccess; ++o) {
+                const auto& g = c->groups[gorder[o].second];
+                hipStream_t gs = gside[o] ? c->s3 : sc;
+                gh::ClassArgs ca{{g[0], g[1], C.l, D.l}, 1};
+                if (gcol[o])
+                    e = gh::launch_chol_column(ca, ket.comp, c->d_wi.as<WorkItem>() + g[2], g[3],
+                                               b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
+                                               b->d_coefs.as<double>(), b->d_norms.as<double>(),
+                                               c->d_kappa.as<double>(), seeds, c->d_col.as<zc>(), gs);
+                else
+                    e = gh::launch_transform(ca, c->d_wi.as<WorkItem>() + g[2], g[3], b->d_shells.as<DevShell>(),
+                                             b->d_exps.as<double>(), b->d_coefs.as<double>(), b->d_norms.as<double>(),
+                                             c->d_kappa.as<double>(), seeds, c->d_col.as<zc>(), gs);
+            }
+            if (e == hipSuccess) e = hipEventRecord(c->ev[9], c->s3);
+            if (e == hipSuccess) e = hipStreamWaitEvent(sc, c->ev[9], 0);
+            return e;
+        };
+        giaohip_cholesky::ColumnGraph* cg = ctx->chol_graphs ? &c->graphs[topo] : nullptr;
+        if (!cg || cg->uses++ == 0) {
+            // direct launches (also the first pivot of a topology, so that
+            // kernel attributes are set outside any capture)
+            CHOL_STEP_TRY(issue_column(), "column launches");
+        } else {
+            // capture this pivot's launches, update the topology's graph with
+            // them (or instantiate it), launch it
+            CHOL_STEP_TRY(hipStreamBeginCapture(sc, hipStreamCaptureModeThreadLocal), "graph capture");
+            const hipError_t e1 = issue_column();
+            hipGraph_t graph = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(sc, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                CHOL_STEP_TRY(e1 != hipSuccess ? e1 : e2, "graph capture");
+            }
+            if (cg->exec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+                if (hipGraphExecUpdate(cg->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+                    c->graph_updates += 1;
+                } else {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(cg->exec);
+                    cg->exec = nullptr;
+                }
+            }
+            if (!cg->exec) {
+                const hipError_t e3 = hipGraphInstantiate(&cg->exec, graph, nullptr, nullptr, 0);
+                if (e3 != hipSuccess) {
+                    cg->exec = nullptr;
+                    (void)hipGraphDestroy(graph);
+                    CHOL_STEP_TRY(e3, "graph instantiate");
+                }
+                c->graph_instantiations += 1;
+            }
+            (void)hipGraphDestroy(graph);
+            CHOL_STEP_TRY(hipGraphLaunch(cg->exec, sc), "graph launch");
+            c->graph_launches += 1;
+        }
         CHOL_STEP_TRY(hipEventRecord(c->ev[1], sc), "hipEventRecord");
         if (new_block) {
             // New block: the largest residuals as candidates. Candidates still
@@ -3344,6 +3412,15 @@ int giaohip_cholesky_info(const giaohip_cholesky* c, uint64_t* rank, double* max
     if (column_ms) *column_ms = c->column_ms;
     if (update_ms) *update_ms = c->update_ms;
     if (diagonal_ms) *diagonal_ms = c->diagonal_ms;
+    return GIAOHIP_OK;
+}
+
+int giaohip_cholesky_graphs(const giaohip_cholesky* c, uint64_t* launches, uint64_t* instantiations,
+                            uint64_t* updates) {
+    if (!c) return GIAOHIP_INVALID_ARGUMENT;
+    if (launches) *launches = c->graph_launches;
+    if (instantiations) *instantiations = c->graph_instantiations;
+    if (updates) *updates = c->graph_updates;
     return GIAOHIP_OK;
 }
 

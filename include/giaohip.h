@@ -107,7 +107,13 @@ enum {
        la + lb + lc + ld of the classes (without a class-specialized kernel,
        both pair momenta >= 2) whose contraction runs on the FP64 matrix
        cores; 0..33, default 9. */
-    GIAOHIP_OPTION_MFMA_MIN_L = 10
+    GIAOHIP_OPTION_MFMA_MIN_L = 10,
+    /* Cholesky column launches of a pivot (expansion, Boys seeds unless
+       buffered, the column kernels of every bra group over two streams) as
+       hipGraphs: 1 (default) or 0 (direct launches). One executable graph
+       per launch topology, updated from each pivot's captured launches.
+       Pivots and factors are bitwise identical either way. */
+    GIAOHIP_OPTION_CHOL_GRAPHS = 11
 };
 enum {
     GIAOHIP_PIPELINE_OVERLAP = 0, /* on a second stream, beside the previous chunk's contraction */
@@ -369,6 +375,14 @@ int giaohip_cholesky_info(const giaohip_cholesky* chol, uint64_t* rank, double* 
  * No counterpart in the reference (its projection is one GEMV per pivot). */
 int giaohip_cholesky_blocks(const giaohip_cholesky* chol, uint64_t* blocks, uint64_t* factor_rows_read,
                             double* block_ms, uint64_t* projected, uint64_t* candidate_rows);
+/* hipGraph use of a run's column launches (GIAOHIP_OPTION_CHOL_GRAPHS): graph
+ * launches, graphs instantiated, in-place updates from a pivot's captured
+ * launches. Any output pointer may be NULL. Replaces the per-pivot native
+ * column batch calls of the reference driver
+ * (python/giao_integrals/cholesky.py:249-254 calling
+ * src/cholesky.cpp:203-282), which launch no graphs. */
+int giaohip_cholesky_graphs(const giaohip_cholesky* chol, uint64_t* launches, uint64_t* instantiations,
+                            uint64_t* updates);
 /* Reconstructed ERIs from the device factors, (mu nu|lam sig) ~
  * sum_P L_P[mu,nu] conj(L_P[sig,lam]), on the AO ranges mu in
  * [ranges[0], ranges[1]), nu in [ranges[2], ranges[3]), lam in [ranges[4],

@@ -124,6 +124,41 @@ def test_device_cholesky_block_sizes_bitwise(gpu, port):
             gpu.set_cholesky_block(bad)
 
 
+@pytest.mark.parametrize("natoms,zero_field,stop", [(3, False, 0), (3, True, 0), (10, False, 900)])
+def test_device_cholesky_graphs_bitwise(gpu, port, natoms, zero_field, stop):
+    """The column launches of each pivot as hipGraphs (one executable graph
+    per launch topology, updated from every pivot's captured launches and
+    replayed between the pivot's own direct launches) give the pivots and
+    factors of direct launches bitwise, in a field and at zero field (real
+    columns)."""
+    sh, B, O = workloads.cholesky_atoms(natoms=natoms)
+    if zero_field:
+        B = (0.0, 0.0, 0.0)
+    b = make_basis(sh.shell_list(), port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    f = _native.field6(B, O)
+    runs = {}
+    try:
+        for on in (False, True):
+            print(f"{natoms} atoms, B={B}, graphs {on}, stop at {stop}", flush=True)
+            gpu.set_cholesky_graphs(on)
+            runs[on] = _native.cholesky_run(db, f, 1e-8, stop_after=stop)
+    finally:
+        gpu.set_cholesky_graphs(True)
+    direct, graphs = runs[False], runs[True]
+    print(f"rank {graphs.rank}: graph use {graphs.graphs}")
+    assert direct.graphs["launches"] == 0
+    g = graphs.graphs
+    # every pivot after the first of its topology replays a graph
+    assert g["launches"] > graphs.rank // 2
+    assert g["instantiated"] + g["updated"] == g["launches"]
+    assert g["instantiated"] < g["launches"]
+    assert graphs.rank == direct.rank and graphs.converged == direct.converged
+    np.testing.assert_array_equal(graphs.pivots(), direct.pivots())
+    np.testing.assert_array_equal(graphs.vectors(), direct.vectors())
+    assert graphs.max_residual == direct.max_residual
+
+
 def test_device_cholesky_blocks_across_factor_chunks(gpu, port):
     """Ten atoms (350 AOs): the factors span two device chunks of 1095 rows,
     so projections cross a chunk boundary. Blocked (16) and one-at-a-time
