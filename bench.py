@@ -279,7 +279,7 @@ def mixed_leg(n: int, tune: bool = True, tune_n: int = 500_000):
     }, batch
 
 
-def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
+def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000, tune: bool = True):
     """The high-angular-momentum precision workloads: (gg|gg) and (ff|dd),
     contraction degree 2, exponents 10^U(log10 0.05, log10 5e3), each with
     its E.R.E contraction on FP64 VALU (the generic kernel for (gg|gg), the
@@ -296,6 +296,7 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
 
     g, fd = workloads.precision_batches(n_gggg=n_gggg, n_ffdd=n_ffdd)
     out = {}
+    mfma_var = 0  # compile-time variant of the matrix-core kernels, tuned on the first (gg|gg) quartets
     for name, batch in (("gggg", g), ("ffdd", fd)):
         n = len(batch.quartets)
         if n == 0:
@@ -311,6 +312,17 @@ def high_l_leg(n_gggg: int, n_ffdd: int, probe_values: int = 4_000_000):
         probe = min(total, probe_values)
         rec = {"workload": batch.description, "quartets": n, "variants": {}}
         ref = None
+        if tune and name == "gggg":
+            from giaoint_amd import autotune
+
+            ctx.set_mfma("all")
+            m = min(n, 4000)
+            choice = autotune.tune_eri(db, batch.quartets[:m], f, knobs=("mfma_variant",), reps=2, check_quartets=16,
+                                       workload=f"(gg|gg) K=2, first {m} quartets, FP64 matrix cores")
+            mfma_var = int(choice.values["mfma_variant"])
+            rec["autotune"] = choice.as_dict()
+            log(f"gggg: MFMA variant {mfma_var} " + " ".join(f"{t.value}={t.seconds:.3f}s" for t in choice.trials))
+        ctx.set_mfma_variant(mfma_var)
         try:
             for variant, mode in (("fp64_valu", "off"), ("fp64_mfma", "all"), ("split_fp32_mfma", "split32")):
                 ctx.set_mfma(mode)
@@ -716,7 +728,7 @@ def main():
     if args.mixed > 0:
         result["mixed_class"], mixed = mixed_leg(args.mixed, tune=not args.no_autotune)
     if args.high_l_gggg > 0 or args.high_l_ffdd > 0:
-        result["high_l"] = high_l_leg(args.high_l_gggg, args.high_l_ffdd)
+        result["high_l"] = high_l_leg(args.high_l_gggg, args.high_l_ffdd, tune=not args.no_autotune)
     if not args.no_derivatives:
         result["eri_derivatives"] = derivative_leg(args.cpu_seconds, not args.no_cpu and not args.no_cpu_legs)
     if args.cholesky_rank >= 0:

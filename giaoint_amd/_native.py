@@ -254,6 +254,11 @@ class Context:
         one-wavefront workgroups (-1: none). Bitwise identical results."""
         self.check(self.lib.giaohip_set_option(self.handle, 9, int(value)))
 
+    def set_mfma_variant(self, variant: int):
+        """Compile-time variant (0..3: GEMM unroll, LDS row padding) of the
+        matrix-core contraction kernels; bitwise identical results."""
+        self.check(self.lib.giaohip_set_option(self.handle, 12, int(variant)))
+
     def set_cholesky_graphs(self, on: bool):
         """Cholesky column launches as hipGraphs (default) or launched directly."""
         self.check(self.lib.giaohip_set_option(self.handle, 11, 1 if on else 0))

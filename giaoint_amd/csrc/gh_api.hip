@@ -225,6 +225,7 @@ struct giaohip_ctx {
     int wave_max_l = 5;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
     int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
     int chol_graphs = 1;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
+    int mfma_variant = 0;                    // compile-time variant of the matrix-core kernels (GIAOHIP_OPTION_MFMA_VARIANT)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     // pipelined chunks of the plain path (run_plain): per slot the seeds,
     // descriptors and work items of one chunk in flight, their page-locked
@@ -919,11 +920,18 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             queued[side] += L.cost;
             hipStream_t ls = side ? ctx->aux : s;
             if (L.mf >= 0) {
-                auto launch = (L.mf & kMfmaSplitKey) ? gh::launch_mfma_split : gh::launch_mfma;
-                HIP_TRY(launch((L.mf >> 4) & 15, L.mf & 15, d_items + L.first, (int)L.count,
-                               b->d_shells.as<DevShell>(), b->d_exps.as<double>(), b->d_coefs.as<double>(),
-                               b->d_norms.as<double>(), d_kappa, d_seeds, d_out, ls),
-                        "MFMA transform kernel launch");
+                const int lab = (L.mf >> 4) & 15, lcd = L.mf & 15;
+                if (L.mf & kMfmaSplitKey)
+                    HIP_TRY(gh::launch_mfma_split(lab, lcd, d_items + L.first, (int)L.count, b->d_shells.as<DevShell>(),
+                                                  b->d_exps.as<double>(), b->d_coefs.as<double>(),
+                                                  b->d_norms.as<double>(), d_kappa, d_seeds, d_out, ls),
+                            "MFMA transform kernel launch");
+                else
+                    HIP_TRY(gh::launch_mfma(lab, lcd, ctx->mfma_variant, d_items + L.first, (int)L.count,
+                                            b->d_shells.as<DevShell>(), b->d_exps.as<double>(),
+                                            b->d_coefs.as<double>(), b->d_norms.as<double>(), d_kappa, d_seeds,
+                                            d_out, ls),
+                            "MFMA transform kernel launch");
                 ctx->stats.launches += 1;
                 continue;
             }
@@ -1556,6 +1564,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_SPLIT32) {
         ctx->mfma_mode = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_MFMA_VARIANT && value >= 0 && value < gh::kMfmaVariantCount) {
+        ctx->mfma_variant = value;
         return GIAOHIP_OK;
     }
     if (option == GIAOHIP_OPTION_CHOL_GRAPHS && (value == 0 || value == 1)) {

@@ -145,10 +145,13 @@ size_t seed_scratch_ints(long long total);
 // E.R.E contraction on the FP64 matrix cores (gh_mfma.h) for bra / ket
 // pair momenta (lab, lcd), lab >= lcd: whether an instance exists, the ket
 // components one work item covers (items carry cd1 - cd0 <= that), and the
-// launch (seeds as for launch_transform, one 512-thread workgroup per item).
+// launch (seeds as for launch_transform, one 512-thread workgroup per item)
+// in compile-time variant var (GEMM unroll, LDS row padding; 0..3, the same
+// integrals for every variant).
+constexpr int kMfmaVariantCount = 4;
 bool mfma_instance(int lab, int lcd);
 int mfma_cd_tile(int lab);
-hipError_t launch_mfma(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+hipError_t launch_mfma(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                        const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                        hipStream_t s);
 // The split-FP32 form of the same contraction (FP32 hi/lo products on

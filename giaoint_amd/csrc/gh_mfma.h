@@ -73,7 +73,13 @@ constexpr int max_pair_rows(int LP) {
     return m;
 }
 
-template <int LAB, int LCD>
+// Compile-time variants of an instance (VAR bits), chosen at run time by the
+// autotuner: bit 0 flips the GEMM step unroll (1 <-> 2), bit 1 pads the row
+// stride of the W and Ec tiles in LDS by one complex entry (16 bytes), which
+// moves consecutive rows to other banks. Neither changes any sum.
+static_assert(kMfmaVariantCount == 4, "variants 0..3");
+
+template <int LAB, int LCD, int VAR = 0>
 struct Shape {
     static constexpr int L = LAB + LCD;
     static constexpr int NHAB = plan::nh(LAB), NHCD = plan::nh(LCD), NHL = plan::nh(L);
@@ -88,12 +94,14 @@ struct Shape {
     static constexpr int E1B = 3 * RB * (LAB + 1), E1K = 3 * RK * (LCD + 1);
     static constexpr int WROWS = 16 * MT1, JROWS = 4 * (KS1 + 1);  // one zero step past the end (prefetch)
     static constexpr int RPT = (NHL + kNT - 1) / kNT;             // R0 entries per thread
-    static constexpr int UNROLL = TW1 * TW2 > 2 ? 1 : 2;          // GEMM step unroll (register budget)
+    static constexpr int UNROLL0 = TW1 * TW2 > 2 ? 1 : 2;         // GEMM step unroll (register budget)
+    static constexpr int UNROLL = (VAR & 1) ? 3 - UNROLL0 : UNROLL0;
+    static constexpr int CDS = CDT + ((VAR & 2) ? 1 : 0);          // row stride of W and Ec in LDS
     // LDS, complex entries: two R0 level buffers, the bra and ket 1-D
     // tables, W, the dense ket products Ec[j][cd]; then ints: lexicographic
     // offsets, the bra Hermite decode
     static constexpr int Z_R0 = 0, Z_E1B = 2 * (NHL + 1), Z_E1K = Z_E1B + E1B, Z_W = Z_E1K + E1K,
-                         Z_EC = Z_W + (WROWS + 4) * CDT, Z_FCOL = Z_EC + JROWS * CDT, Z_SEED = Z_FCOL + CDT,
+                         Z_EC = Z_W + (WROWS + 4) * CDS, Z_FCOL = Z_EC + JROWS * CDS, Z_SEED = Z_FCOL + CDT,
                          Z_END = Z_SEED + L + 4;
     // ints: lexicographic offsets (t * (L + 1) + u -> lex of (t, u, 0)); the
     // ket Hermite indices as (t * (L + 1) + u) | v << 16; the bra Hermite
@@ -214,7 +222,7 @@ __device__ inline int wave_max_exponent(double m) {
     return e;
 }
 
-template <int LAB, int LCD, bool SPLIT = false>
+template <int LAB, int LCD, bool SPLIT = false, int VAR = 0>
 __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __restrict__ items,
                                                              const DevShell* __restrict__ shells,
                                                              const double* __restrict__ exps,
@@ -222,7 +230,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                                                              const double* __restrict__ norms,
                                                              const double* __restrict__ kappa,
                                                              const zc* __restrict__ seeds, zc* __restrict__ out) {
-    using S = Shape<LAB, LCD>;
+    using S = Shape<LAB, LCD, VAR>;
     constexpr int L = S::L;
     extern __shared__ __align__(16) unsigned char smem[];
     zc* zs = reinterpret_cast<zc*>(smem);
@@ -493,7 +501,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                     const double f = ((tj + uj + vj) & 1) ? -fcol[col] : fcol[col];
                     v = zscale(f, zmul(zmul(ex, ey), ez));
                 }
-                Ec[e] = v;
+                Ec[j * S::CDS + col] = v;
             }
             __syncthreads();
             const zc* R0 = r0buf[0];
@@ -509,7 +517,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                 const int idx = offL[(g1_k[i] & 0xffff) + (hj & 0xffff)] + (g1_k[i] >> 16) + (hj >> 16);
                 return R0[g1_row[i] ? idx : S::NHL];
             };
-            zc b = Ec[lk * S::CDT + cdl];
+            zc b = Ec[lk * S::CDS + cdl];
             zc a[S::TW1];
             {
                 const int hj = hcd[lk];
@@ -519,7 +527,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
 #pragma unroll S::UNROLL
             for (int s = 0; s < S::KS1; ++s) {
                 const int jn = 4 * (s + 1) + lk;  // step s + 1 (a zero row past the end)
-                const zc bn = Ec[jn * S::CDT + cdl];
+                const zc bn = Ec[jn * S::CDS + cdl];
                 const int hjn = hcd[jn];
                 zc an[S::TW1];
 #pragma unroll
@@ -556,7 +564,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
             if (g1_on[i]) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    Wl[(g1_mt[i] * 16 + drow<SPLIT>(lk, r)) * S::CDT + cdl] = zc{wre[i][r], wim[i][r]};
+                    Wl[(g1_mt[i] * 16 + drow<SPLIT>(lk, r)) * S::CDS + cdl] = zc{wre[i][r], wim[i][r]};
             }
         }
         if constexpr (SPLIT) {
@@ -588,7 +596,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                               e1b[g2_bz[i] + (hk >> 16)]);
             return zscale(fab[i], e);
         };
-        zc w = Wl[lk * S::CDT + cdl];
+        zc w = Wl[lk * S::CDS + cdl];
         if constexpr (SPLIT) w = zc{ldexp(w.x, w_in), ldexp(w.y, w_in)};
         zc a2[S::TW2];
 #pragma unroll
@@ -596,7 +604,7 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
 #pragma unroll S::UNROLL
         for (int s = 0; s < S::KS2; ++s) {
             const int kn = 4 * (s + 1) + lk;
-            zc wn = Wl[kn * S::CDT + cdl];
+            zc wn = Wl[kn * S::CDS + cdl];
             if constexpr (SPLIT) wn = zc{ldexp(wn.x, w_in), ldexp(wn.y, w_in)};
             zc an[S::TW2];
 #pragma unroll
@@ -655,25 +663,43 @@ namespace mf {
 
 // Launch of one (LAB, LCD) instance over `n` items (dynamic LDS of
 // Shape::LDS_BYTES; above 64 KB the kernel's limit is raised once).
-template <int LAB, int LCD, bool SPLIT = false>
-hipError_t launch_instance(const WorkItem* items, int n, const DevShell* shells, const double* exps,
-                           const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
-                           hipStream_t s) {
-    using S = Shape<LAB, LCD>;
+template <int LAB, int LCD, bool SPLIT = false, int VAR = 0>
+hipError_t launch_one(const WorkItem* items, int n, const DevShell* shells, const double* exps,
+                      const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
+                      hipStream_t s) {
+    using S = Shape<LAB, LCD, VAR>;
     static_assert(S::LDS_BYTES <= 160 * 1024, "MFMA transform exceeds the LDS");
     if (n == 0) return hipSuccess;
     if constexpr (S::LDS_BYTES > 64 * 1024) {
         static bool raised = false;
         if (!raised) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_transform_kernel<LAB, LCD, SPLIT>),
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_transform_kernel<LAB, LCD, SPLIT, VAR>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, S::LDS_BYTES);
             if (e != hipSuccess) return e;
             raised = true;
         }
     }
-    mfma_transform_kernel<LAB, LCD, SPLIT><<<dim3((unsigned)n), dim3(kNT), S::LDS_BYTES, s>>>(
+    mfma_transform_kernel<LAB, LCD, SPLIT, VAR><<<dim3((unsigned)n), dim3(kNT), S::LDS_BYTES, s>>>(
         items, shells, exps, coefs, norms, kappa, seeds, out);
     return hipGetLastError();
+}
+
+// Launch of one (LAB, LCD) instance in compile-time variant `var`
+// (0..kMfmaVariants-1; the split form has variant 0 only).
+template <int LAB, int LCD, bool SPLIT = false>
+hipError_t launch_instance(int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+                           const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
+                           hipStream_t s) {
+    if constexpr (SPLIT) {
+        return launch_one<LAB, LCD, true, 0>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+    } else {
+        switch (var) {
+            case 1: return launch_one<LAB, LCD, false, 1>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+            case 2: return launch_one<LAB, LCD, false, 2>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+            case 3: return launch_one<LAB, LCD, false, 3>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+            default: return launch_one<LAB, LCD, false, 0>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+        }
+    }
 }
 
 }  // namespace mf

@@ -4,18 +4,18 @@
 
 namespace gh {
 
-hipError_t launch_mfma_b(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+hipError_t launch_mfma_b(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                          const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                          hipStream_t s);
-hipError_t launch_mfma_c(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+hipError_t launch_mfma_c(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                          const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                          hipStream_t s);
 
-hipError_t launch_mfma_a(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+hipError_t launch_mfma_a(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                          const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                          hipStream_t s) {
 #define GH_MF(A, B) \
-    if (lab == A && lcd == B) return mf::launch_instance<A, B>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+    if (lab == A && lcd == B) return mf::launch_instance<A, B>(var, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
     GH_MF(2, 2)
     GH_MF(3, 2)
     GH_MF(3, 3)
@@ -37,13 +37,13 @@ bool mfma_instance(int lab, int lcd) {
 
 int mfma_cd_tile(int lab) { return 16 * mf::n_tiles(lab); }
 
-hipError_t launch_mfma(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+hipError_t launch_mfma(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                        const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                        hipStream_t s) {
     if (!mfma_instance(lab, lcd)) return hipErrorInvalidValue;
-    if (lab <= 4) return launch_mfma_a(lab, lcd, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
-    if (lab <= 6) return launch_mfma_b(lab, lcd, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
-    return launch_mfma_c(lab, lcd, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+    if (lab <= 4) return launch_mfma_a(lab, lcd, var, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+    if (lab <= 6) return launch_mfma_b(lab, lcd, var, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+    return launch_mfma_c(lab, lcd, var, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
 }
 
 }  // namespace gh

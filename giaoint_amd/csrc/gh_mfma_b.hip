@@ -3,11 +3,11 @@
 
 namespace gh {
 
-hipError_t launch_mfma_b(int lab, int lcd, const WorkItem* items, int n, const DevShell* shells, const double* exps,
+hipError_t launch_mfma_b(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                          const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                          hipStream_t s) {
 #define GH_MF(A, B) \
-    if (lab == A && lcd == B) return mf::launch_instance<A, B>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+    if (lab == A && lcd == B) return mf::launch_instance<A, B>(var, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
     GH_MF(5, 2)
     GH_MF(5, 3)
     GH_MF(5, 4)

@@ -11,9 +11,9 @@ hipError_t launch_mfma_split(int lab, int lcd, const WorkItem* items, int n, con
                              const double* exps, const double* coefs, const double* norms, const double* kappa,
                              const zc* seeds, zc* out, hipStream_t s) {
     if (lab == 8 && lcd == 8)
-        return mf::launch_instance<8, 8, true>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+        return mf::launch_instance<8, 8, true>(0, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
     if (lab == 6 && lcd == 4)
-        return mf::launch_instance<6, 4, true>(items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
+        return mf::launch_instance<6, 4, true>(0, items, n, shells, exps, coefs, norms, kappa, seeds, out, s);
     return hipErrorInvalidValue;
 }
 

@@ -113,7 +113,12 @@ enum {
        hipGraphs: 1 (default) or 0 (direct launches). One executable graph
        per launch topology, updated from each pivot's captured launches.
        Pivots and factors are bitwise identical either way. */
-    GIAOHIP_OPTION_CHOL_GRAPHS = 11
+    GIAOHIP_OPTION_CHOL_GRAPHS = 11,
+    /* Compile-time variant of the FP64 matrix-core contraction kernels,
+       0..3 (default 0): bit 0 flips the GEMM step unroll, bit 1 pads the
+       LDS row stride of its W and ket-product tiles by 16 bytes. Results
+       are bitwise identical for every variant (the autotuner picks one). */
+    GIAOHIP_OPTION_MFMA_VARIANT = 12
 };
 enum {
     GIAOHIP_PIPELINE_OVERLAP = 0, /* on a second stream, beside the previous chunk's contraction */

@@ -80,3 +80,33 @@ def test_split_fp32_class_against_oracle(gpu, port, cls):
             assert not np.array_equal(got, fp64)  # the split form really ran
     finally:
         gpu.set_mfma("default")
+
+
+@pytest.mark.parametrize("cls", [(4, 4, 4, 4), (3, 3, 2, 2), (2, 1, 2, 1)], ids=lambda c: "".join(map(str, c)))
+def test_mfma_compile_time_variants_bitwise(gpu, port, cls):
+    """The four compile-time variants of the matrix-core kernels (GEMM step
+    unroll, padded LDS rows) compute the same sums: bitwise identical
+    integrals, and the oracle's at 1e-11."""
+    rng = np.random.default_rng(sum((i + 1) * l for i, l in enumerate(cls)) + 23)
+    shells = random_shells(rng, list(cls), nprim=(1, 2), ncontr=(1, 2), box=0.6, log_exp=(-1.0, 0.3))
+    b = make_basis(shells, port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    q = [(0, 1, 2, 3), (2, 3, 0, 1)]
+    f = _native.field6((0.1, -0.3, 0.25), (0.2, 0.1, -0.1))
+    want, _ = port.eri_batch(b, q, port.field6((0.1, -0.3, 0.25), (0.2, 0.1, -0.1)), 16)
+    gpu.set_mfma("all")
+    try:
+        got = {}
+        for var in range(4):
+            gpu.set_mfma_variant(var)
+            got[var] = db.eri_batch(q, f)
+        for var in range(1, 4):
+            assert np.array_equal(got[var], got[0]), var
+        err = float(np.max(np.abs(got[0] - want)))
+        print(f"{cls}: four variants bitwise equal, max |MFMA - oracle| = {err:.2e}")
+        assert err <= ATOL
+        with pytest.raises(ValueError):
+            gpu.set_mfma_variant(4)
+    finally:
+        gpu.set_mfma_variant(0)
+        gpu.set_mfma("default")
