@@ -471,24 +471,20 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0, graphs_rank:
             " ".join(f"B={t.value}:{t.seconds:.2f}s" for t in tuned.trials))
     graphs_cmp = None
     if graphs_rank > 0:
-        graphs_cmp = {"pivots": graphs_rank}
+        # the first graphs_rank pivots with the column launches as hipGraphs and direct
+        graphs_cmp = {"prefix_pivots": graphs_rank}
         piv = {}
-        for on in (False, True, False, True):
+        for on in (False, True):
             ctx.set_cholesky_graphs(on)
             t = time.perf_counter()
             pre = _native.cholesky_run(db, field, 1e-8, 0, graphs_rank)
-            dt = time.perf_counter() - t
-            key = "graphs_s" if on else "direct_s"
-            graphs_cmp[key] = min(dt, graphs_cmp.get(key, dt))
-            if on:
-                graphs_cmp["graph_use"] = pre.graphs
+            graphs_cmp["prefix_graphs_s" if on else "prefix_direct_s"] = time.perf_counter() - t
             piv[on] = pre.pivots()
             pre.close()
-        graphs_cmp["pivots_identical"] = bool(np.array_equal(piv[True], piv[False]))
-        graphs_cmp["note"] = "best of two runs each, from scratch to the pivot count"
-        ctx.set_cholesky_graphs(True)
-        log(f"cholesky: first {graphs_rank} pivots, direct {graphs_cmp['direct_s']:.2f} s, "
-            f"graphs {graphs_cmp['graphs_s']:.2f} s")
+        graphs_cmp["prefix_pivots_identical"] = bool(np.array_equal(piv[True], piv[False]))
+        ctx.set_cholesky_graphs(False)
+        log(f"cholesky: first {graphs_rank} pivots, direct {graphs_cmp['prefix_direct_s']:.2f} s, "
+            f"graphs {graphs_cmp['prefix_graphs_s']:.2f} s")
     step = 1000
     t0 = time.perf_counter()
     first = step if rank_cap <= 0 else min(step, rank_cap)
@@ -500,6 +496,7 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0, graphs_rank:
     n = db.nao
     m = ch.rank
     t = ch.timings_ms
+    t_col = t["column"]
     # a device-resident consumer of the factors: J[mu,nu] = sum (mu nu|lam sig) D[lam,sig]
     rng = np.random.default_rng(0)
     dens = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
@@ -537,12 +534,25 @@ def cholesky_leg(rank_cap: int, block: int = 0, tune_rank: int = 0, graphs_rank:
             "note": "K = sum_P L_P D L_P^H: two rocBLAS zgemm of order n per factor (8 FLOP per complex MAC), "
                     "density and K over PCIe",
         },
-        "column_graphs": {**ch.graphs, "prefix_comparison": graphs_cmp},
+        "column_graphs": graphs_cmp,
         "pivot_blocks": blk["blocks"],
         "pivots_per_block": m / max(blk["blocks"], 1),
         "candidates_projected_per_block": blk["projected"] / max(blk["blocks"], 1),
         "projection": projection_roofline(proj_bytes, proj_flops, blk["block_ms"], unblocked_bytes),
     }
+    if graphs_cmp is not None and rank_cap <= 0:
+        # the whole run once more with the column launches as hipGraphs
+        ctx.set_cholesky_graphs(True)
+        t = time.perf_counter()
+        chg = _native.cholesky_run(db, field, 1e-8)
+        graphs_cmp["full_graphs_s"] = time.perf_counter() - t
+        graphs_cmp["full_direct_s"] = wall
+        graphs_cmp["full_pivots_identical"] = bool(np.array_equal(chg.pivots(), ch.pivots()))
+        graphs_cmp["full_graph_use"] = chg.graphs
+        graphs_cmp["full_column_ms"] = {"graphs": chg.timings_ms["column"], "direct": t_col}
+        chg.close()
+        ctx.set_cholesky_graphs(False)
+        log(f"cholesky: whole run with graphs {graphs_cmp['full_graphs_s']:.2f} s (direct {wall:.2f} s)")
     ch.close()
     return result
 

@@ -259,9 +259,10 @@ class Context:
         matrix-core contraction kernels; bitwise identical results."""
         self.check(self.lib.giaohip_set_option(self.handle, 12, int(variant)))
 
-    def set_cholesky_graphs(self, on: bool):
-        """Cholesky column launches as hipGraphs (default) or launched directly."""
-        self.check(self.lib.giaohip_set_option(self.handle, 11, 1 if on else 0))
+    def set_cholesky_graphs(self, on):
+        """Cholesky column launches as hipGraphs: True (every pivot), "inside"
+        (pivots inside a block only) or False (launched directly)."""
+        self.check(self.lib.giaohip_set_option(self.handle, 11, 2 if on == "inside" else (1 if on else 0)))
 
     def set_mfma_min_l(self, value: int):
         """Smallest la + lb + lc + ld whose classes (without a class-specialized

@@ -224,7 +224,7 @@ struct giaohip_ctx {
     int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
     int wave_max_l = 5;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
     int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
-    int chol_graphs = 1;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
+    int chol_graphs = 0;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
     int mfma_variant = 0;                    // compile-time variant of the matrix-core kernels (GIAOHIP_OPTION_MFMA_VARIANT)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     // pipelined chunks of the plain path (run_plain): per slot the seeds,
@@ -1570,7 +1570,7 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
         ctx->mfma_variant = value;
         return GIAOHIP_OK;
     }
-    if (option == GIAOHIP_OPTION_CHOL_GRAPHS && (value == 0 || value == 1)) {
+    if (option == GIAOHIP_OPTION_CHOL_GRAPHS && value >= 0 && value <= 2) {
         ctx->chol_graphs = value;
         return GIAOHIP_OK;
     }
@@ -3013,7 +3013,10 @@ int chol_steps(giaohip_ctx* ctx, giaohip_cholesky* c, uint64_t target) {
             if (e == hipSuccess) e = hipStreamWaitEvent(sc, c->ev[9], 0);
             return e;
         };
-        giaohip_cholesky::ColumnGraph* cg = ctx->chol_graphs ? &c->graphs[topo] : nullptr;
+        // graphs for every pivot (1) or only inside a block (2: a block's
+        // first column runs beside the candidate projection)
+        const bool use_graph = ctx->chol_graphs == 1 || (ctx->chol_graphs == 2 && !new_block);
+        giaohip_cholesky::ColumnGraph* cg = use_graph ? &c->graphs[topo] : nullptr;
         if (!cg || cg->uses++ == 0) {
             // direct launches (also the first pivot of a topology, so that
             // kernel attributes are set outside any capture)

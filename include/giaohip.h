@@ -110,9 +110,11 @@ enum {
     GIAOHIP_OPTION_MFMA_MIN_L = 10,
     /* Cholesky column launches of a pivot (expansion, Boys seeds unless
        buffered, the column kernels of every bra group over two streams) as
-       hipGraphs: 1 (default) or 0 (direct launches). One executable graph
-       per launch topology, updated from each pivot's captured launches.
-       Pivots and factors are bitwise identical either way. */
+       hipGraphs: 1 (every pivot), 2 (pivots inside a candidate block only)
+       or 0 (default: direct launches, measured faster over a whole run).
+       One executable graph per launch topology, updated from each pivot's
+       captured launches. Pivots and factors are bitwise identical for every
+       value. */
     GIAOHIP_OPTION_CHOL_GRAPHS = 11,
     /* Compile-time variant of the FP64 matrix-core contraction kernels,
        0..3 (default 0): bit 0 flips the GEMM step unroll, bit 1 pads the

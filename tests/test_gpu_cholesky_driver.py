@@ -139,24 +139,26 @@ def test_device_cholesky_graphs_bitwise(gpu, port, natoms, zero_field, stop):
     f = _native.field6(B, O)
     runs = {}
     try:
-        for on in (False, True):
+        for on in (False, True, "inside"):
             print(f"{natoms} atoms, B={B}, graphs {on}, stop at {stop}", flush=True)
             gpu.set_cholesky_graphs(on)
             runs[on] = _native.cholesky_run(db, f, 1e-8, stop_after=stop)
     finally:
-        gpu.set_cholesky_graphs(True)
-    direct, graphs = runs[False], runs[True]
-    print(f"rank {graphs.rank}: graph use {graphs.graphs}")
+        gpu.set_cholesky_graphs(False)
+    direct = runs[False]
     assert direct.graphs["launches"] == 0
-    g = graphs.graphs
-    # every pivot after the first of its topology replays a graph
-    assert g["launches"] > graphs.rank // 2
-    assert g["instantiated"] + g["updated"] == g["launches"]
-    assert g["instantiated"] < g["launches"]
-    assert graphs.rank == direct.rank and graphs.converged == direct.converged
-    np.testing.assert_array_equal(graphs.pivots(), direct.pivots())
-    np.testing.assert_array_equal(graphs.vectors(), direct.vectors())
-    assert graphs.max_residual == direct.max_residual
+    for mode in (True, "inside"):
+        graphs = runs[mode]
+        print(f"graphs {mode}: rank {graphs.rank}, graph use {graphs.graphs}")
+        g = graphs.graphs
+        # every pivot after the first of its topology replays a graph
+        assert g["launches"] > graphs.rank // 2
+        assert g["instantiated"] + g["updated"] == g["launches"]
+        assert g["instantiated"] < g["launches"]
+        assert graphs.rank == direct.rank and graphs.converged == direct.converged
+        np.testing.assert_array_equal(graphs.pivots(), direct.pivots())
+        np.testing.assert_array_equal(graphs.vectors(), direct.vectors())
+        assert graphs.max_residual == direct.max_residual
 
 
 def test_device_cholesky_blocks_across_factor_chunks(gpu, port):
