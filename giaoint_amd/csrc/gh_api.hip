@@ -1073,7 +1073,9 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
             n.seed = (size_t)(n.kab * n.kcd) * (size_t)(n.L + 4);
             n.r0 = (size_t)(n.kab * n.kcd) * (size_t)f.info.nhl;
             n.tab = (size_t)(n.kab * f.info.bra_ne + n.kcd * f.info.ket_ne);
-            n.scr = ((size_t)gh::kBoysBins * (size_t)(n.kab * n.kcd) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
+            // quadrature lists (index and quartet per entry) and the block map of the seed kernel
+            const size_t units = (size_t)(n.kab * n.kcd);
+            n.scr = ((2 * (size_t)gh::kBoysBins * units + units / 128 + 2) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
             n.items = 1;
             for (int k = 0; k < 4; ++k)
                 if (f.r->xfix[f.perm[k]] < 0) n.items *= (size_t)S[k]->ncontr;

@@ -112,7 +112,10 @@ __device__ inline void record_error(DevError* err, int code, long long item, zc 
 // not depend on the order of the lists.
 //
 // Scratch layout (ints): [0, kBoysBins) bin counts, [15] the work counter,
-// then bin b at 16 + b * total.
+// then bin b's primitive-quartet indices at 16 + b * total, the quartet index
+// of each entry at 16 + (kBoysBins + b) * total, and the block map (the
+// quartet holding the first primitive quartet of each 128-lane block of
+// phase A, then the last quartet) at 16 + 2 * kBoysBins * total.
 // ---------------------------------------------------------------------------
 __device__ inline int boys_bin(zc z) {
     const double d = fmax(fabs(z.x), fabs(z.y));
@@ -228,17 +231,40 @@ __device__ inline int find_quartet(const SeedQuartet* __restrict__ quartets, int
     return lo;
 }
 
+// Quartet index of the primitive quartet gid among quartets [lo, hi].
+__device__ inline int find_quartet_in(const SeedQuartet* __restrict__ quartets, int lo, int hi, long long gid) {
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (quartets[mid].pq_begin <= gid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+constexpr int kSeedBlock = 128;  // lanes per workgroup of phase A
+
+// Block map of phase A: the quartet of each block's first primitive quartet,
+// and (entry nblocks) the last quartet, so that a lane searches only the few
+// quartets its block spans instead of the whole list.
+__global__ void seed_blockmap_kernel(const SeedQuartet* __restrict__ quartets, int nq, long long total, int nblocks,
+                                     int* __restrict__ blkq) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblocks) blkq[b] = find_quartet(quartets, nq, (long long)b * kSeedBlock);
+    else if (b == nblocks) blkq[b] = nq - 1;
+}
+
 template <int NMAX>
-__global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict__ quartets, int nq,
-                                                   long long total, const DevShell* __restrict__ shells,
-                                                   const double* __restrict__ exps,
-                                                   const double* __restrict__ kappa, zc* __restrict__ seeds,
-                                                   DevError* err, const double* __restrict__ radial,
-                                                   int* __restrict__ scratch) {
+__global__ __launch_bounds__(kSeedBlock) void seed_kernel(const SeedQuartet* __restrict__ quartets, int nq,
+                                                          long long total, const DevShell* __restrict__ shells,
+                                                          const double* __restrict__ exps,
+                                                          const double* __restrict__ kappa, zc* __restrict__ seeds,
+                                                          DevError* err, const double* __restrict__ radial,
+                                                          int* __restrict__ scratch) {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int bin = -1;
+    const int* blkq = scratch + 16 + 2 * (long long)kBoysBins * total;
+    int bin = -1, qi = 0;
     if (gid < total) {
-        const int qi = find_quartet(quartets, nq, gid);
+        qi = find_quartet_in(quartets, blkq[blockIdx.x], blkq[blockIdx.x + 1], gid);
         const SeedQuartet Q = quartets[qi];
         seed_unit<NMAX, false>(Q, qi, gid - Q.pq_begin, shells, exps, kappa, seeds, err, radial, bin);
     }
@@ -253,7 +279,11 @@ __global__ __launch_bounds__(128) void seed_kernel(const SeedQuartet* __restrict
         int base = 0;
         if (lane == leader) base = atomicAdd(scratch + bb, __popcll(mask));
         base = __shfl(base, leader);
-        if (bin == bb) scratch[16 + (long long)bb * total + base + __popcll(mask & below)] = (int)gid;
+        if (bin == bb) {
+            const long long at = base + __popcll(mask & below);
+            scratch[16 + (long long)bb * total + at] = (int)gid;
+            scratch[16 + (long long)(kBoysBins + bb) * total + at] = qi;
+        }
     }
 }
 
@@ -289,7 +319,7 @@ __global__ __launch_bounds__(64) void seed_quad_kernel(const SeedQuartet* __rest
         }
         if (bin >= 0) {
             const long long gid = scratch[16 + (long long)bin * total + pos];
-            const int qi = find_quartet(quartets, nq, gid);
+            const int qi = scratch[16 + (long long)(kBoysBins + bin) * total + pos];
             const SeedQuartet Q = quartets[qi];
             int unused;
             seed_unit<NMAX, true>(Q, qi, gid - Q.pq_begin, shells, exps, kappa, seeds, err, radial, unused);
@@ -927,7 +957,10 @@ hipError_t upload_constants() {
     return hipSuccess;
 }
 
-size_t seed_scratch_ints(long long total) { return 16 + (size_t)kBoysBins * (size_t)total; }
+size_t seed_scratch_ints(long long total) {
+    const size_t nblocks = (size_t)((total + kSeedBlock - 1) / kSeedBlock);
+    return 16 + 2 * (size_t)kBoysBins * (size_t)total + nblocks + 1;
+}
 
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
                        const double* exps, const double* kappa, zc* seeds, DevError* err, int* scratch,
@@ -937,8 +970,11 @@ hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long 
     // sequence holds no memset node when a caller captures it)
     hipError_t e = launch_zero_ints(scratch, 16, s);
     if (e != hipSuccess) return e;
-    const dim3 block(128);
-    const dim3 grid((unsigned)((total + 127) / 128));
+    const dim3 block(kSeedBlock);
+    const int nblocks = (int)((total + kSeedBlock - 1) / kSeedBlock);
+    const dim3 grid((unsigned)nblocks);
+    seed_blockmap_kernel<<<dim3((unsigned)((nblocks + 1 + 255) / 256)), dim3(256), 0, s>>>(
+        q, nq, total, nblocks, scratch + 16 + 2 * (long long)kBoysBins * total);
     const long long qwaves = (total + 63) / 64;
     const dim3 qgrid((unsigned)(qwaves < 4096 ? qwaves : 4096));
 #define GH_SEED(N)                                                                                              \
