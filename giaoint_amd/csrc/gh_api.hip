@@ -1053,7 +1053,7 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
     // the producers run on the second stream (overlapping the previous
     // chunk's contraction) or, serially, on the context stream
     hipStream_t sc = ctx->stream, sp = ctx->pipeline == GIAOHIP_PIPELINE_SERIAL ? ctx->stream : ctx->aux;
-    constexpr size_t kScratchHead = 4;  // zc holding the 16 counter ints of the quadrature lists
+    constexpr size_t kScratchHead = (gh::kSeedScratchHead * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);  // counters
     struct Need {
         size_t seed, r0, tab, scr;
         long long kab, kcd;
@@ -1075,7 +1075,7 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
             n.tab = (size_t)(n.kab * f.info.bra_ne + n.kcd * f.info.ket_ne);
             // quadrature lists (index and quartet per entry) and the block map of the seed kernel
             const size_t units = (size_t)(n.kab * n.kcd);
-            n.scr = ((2 * (size_t)gh::kBoysBins * units + units / 128 + 2) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
+            n.scr = ((2 * (size_t)gh::kBoysBins * units + units / 256 + 2) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
             n.items = 1;
             for (int k = 0; k < 4; ++k)
                 if (f.r->xfix[f.perm[k]] < 0) n.items *= (size_t)S[k]->ncontr;

@@ -103,19 +103,24 @@ __device__ inline void record_error(DevError* err, int code, long long item, zc 
 // Seed kernels
 //
 // Phase A (seed_kernel): one lane per primitive quartet. Arguments in the
-// power-series and asymptotic regions (and domain errors) are finished here;
-// arguments that need the adaptive quadrature are appended to one of
-// kBoysBins lists by their starting segment count (wave-aggregated atomics).
+// asymptotic region (and domain errors) are finished here; arguments that
+// need the adaptive quadrature are appended to one of 11 lists by their
+// starting segment count, power-series arguments to a twelfth
+// (workgroup-aggregated atomics).
 // Phase B (seed_quad_kernel): persistent wavefronts take 64 list entries at
-// a time, most expensive bin first, so a wavefront's lanes run quadratures of
+// a time, the series bin first, then the quadrature bins most expensive
+// first, so a wavefront's lanes run arguments of
 // similar length. Each argument is evaluated on its own, so the results do
 // not depend on the order of the lists.
 //
-// Scratch layout (ints): [0, kBoysBins) bin counts, [15] the work counter,
-// then bin b's primitive-quartet indices at 16 + b * total, the quartet index
-// of each entry at 16 + (kBoysBins + b) * total, and the block map (the
-// quartet holding the first primitive quartet of each 128-lane block of
-// phase A, then the last quartet) at 16 + 2 * kBoysBins * total.
+// Scratch layout (ints): bin b's count at b * kSeedCounterStride and the
+// work counter at kBoysBins * kSeedCounterStride (each on a cache line of its
+// own: every workgroup of phase A adds to them), then from kSeedScratchHead
+// bin b's primitive-quartet indices at kSeedScratchHead + b * total, the
+// quartet index of each entry at kSeedScratchHead + (kBoysBins + b) * total,
+// and the block map (the quartet holding the first primitive quartet of each
+// workgroup of phase A, then the last quartet) at
+// kSeedScratchHead + 2 * kBoysBins * total.
 // ---------------------------------------------------------------------------
 __device__ inline int boys_bin(zc z) {
     const double d = fmax(fabs(z.x), fabs(z.y));
@@ -126,17 +131,19 @@ __device__ inline int boys_bin(zc z) {
     if (s <= 14) return 9;
     return 10;
 }
-static_assert(kBoysBins == 11, "boys_bin covers 11 bins");
+// The power-series arguments (|z| <= 0.75) also go to phase B, in a bin of
+// their own: their per-order series cost each wavefront holding one of them
+// far more than the asymptotic form costs the others.
+constexpr int kSeriesBin = 11;
+static_assert(kBoysBins == 12, "boys_bin covers 11 quadrature bins and the series bin");
 
-// Series / asymptotic / domain error, as boys_eval decides them.
+// Asymptotic form and domain error, as boys_eval decides them; false when the
+// argument needs the power series or the quadrature (phase B).
 template <int NMAX>
 __device__ inline bool boys_direct(zc z, int nmax, bool scaled, zc (&F)[NMAX + 1], int& st) {
     const double mag = zabs(z);
     double err = 0.0;
-    if (mag <= 0.75) {
-        st = boys_series<NMAX>(z, nmax, scaled, F, err);
-        return true;
-    }
+    if (mag <= 0.75) return false;
     const double asym = 2.0 * (double)nmax + 60.0;
     if (z.x >= asym && fabs(z.y) <= 0.5 * z.x) {
         st = boys_asymptotic<NMAX>(z, nmax, scaled, F, err);
@@ -180,7 +187,7 @@ __device__ inline bool seed_unit(const SeedQuartet& Q, int qidx, long long local
         st = boys_eval<NMAX>(T, L, scaled, F, region, berr, segs);
     } else {
         if (!boys_direct<NMAX>(T, L, scaled, F, st)) {
-            bin = boys_bin(T);
+            bin = zabs(T) <= 0.75 ? kSeriesBin : boys_bin(T);
             return false;
         }
     }
@@ -241,7 +248,9 @@ __device__ inline int find_quartet_in(const SeedQuartet* __restrict__ quartets, 
     return lo;
 }
 
-constexpr int kSeedBlock = 128;  // lanes per workgroup of phase A
+constexpr int kSeedBlock = 256;  // lanes per workgroup of phase A
+constexpr int kSeedCounterStride = 32;
+static_assert((kBoysBins + 1) * kSeedCounterStride <= kSeedScratchHead, "seed counters fit the scratch head");
 
 // Block map of phase A: the quartet of each block's first primitive quartet,
 // and (entry nblocks) the last quartet, so that a lane searches only the few
@@ -260,30 +269,41 @@ __global__ __launch_bounds__(kSeedBlock) void seed_kernel(const SeedQuartet* __r
                                                           const double* __restrict__ kappa, zc* __restrict__ seeds,
                                                           DevError* err, const double* __restrict__ radial,
                                                           int* __restrict__ scratch) {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int* blkq = scratch + 16 + 2 * (long long)kBoysBins * total;
+    __shared__ int s_cnt[kBoysBins];   // the workgroup's count per bin
+    __shared__ int s_base[kBoysBins];  // the workgroup's first position per bin
+    const int tid = threadIdx.x;
+    if (tid < kBoysBins) s_cnt[tid] = 0;
+    __syncthreads();
+    const long long gid = (long long)blockIdx.x * blockDim.x + tid;
+    const int* blkq = scratch + kSeedScratchHead + 2 * (long long)kBoysBins * total;
     int bin = -1, qi = 0;
     if (gid < total) {
         qi = find_quartet_in(quartets, blkq[blockIdx.x], blkq[blockIdx.x + 1], gid);
         const SeedQuartet Q = quartets[qi];
         seed_unit<NMAX, false>(Q, qi, gid - Q.pq_begin, shells, exps, kappa, seeds, err, radial, bin);
     }
-    // append the deferred arguments, one atomic per (wavefront, bin)
-    const int lane = threadIdx.x % 64;
+    // append the deferred arguments: positions within the workgroup from LDS
+    // atomics (one per wavefront and bin), then one global atomic per bin
+    const int lane = tid % 64;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int my_off = 0;
 #pragma unroll
     for (int bb = 0; bb < kBoysBins; ++bb) {
         const unsigned long long mask = __ballot(bin == bb);
         if (mask == 0) continue;
         const int leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(scratch + bb, __popcll(mask));
-        base = __shfl(base, leader);
-        if (bin == bb) {
-            const long long at = base + __popcll(mask & below);
-            scratch[16 + (long long)bb * total + at] = (int)gid;
-            scratch[16 + (long long)(kBoysBins + bb) * total + at] = qi;
-        }
+        int off = 0;
+        if (lane == leader) off = atomicAdd(&s_cnt[bb], __popcll(mask));
+        off = __shfl(off, leader);
+        if (bin == bb) my_off = off + __popcll(mask & below);
+    }
+    __syncthreads();
+    if (tid < kBoysBins && s_cnt[tid] > 0) s_base[tid] = atomicAdd(scratch + tid * kSeedCounterStride, s_cnt[tid]);
+    __syncthreads();
+    if (bin >= 0) {
+        const long long at = s_base[bin] + my_off;
+        scratch[kSeedScratchHead + (long long)bin * total + at] = (int)gid;
+        scratch[kSeedScratchHead + (long long)(kBoysBins + bin) * total + at] = qi;
     }
 }
 
@@ -298,19 +318,19 @@ __global__ __launch_bounds__(64) void seed_quad_kernel(const SeedQuartet* __rest
     int all = 0;
 #pragma unroll
     for (int bb = 0; bb < kBoysBins; ++bb) {
-        counts[bb] = scratch[bb];
+        counts[bb] = scratch[bb * kSeedCounterStride];
         all += counts[bb];
     }
     const int lane = threadIdx.x;
     for (;;) {
         int start = 0;
-        if (lane == 0) start = atomicAdd(scratch + 15, 64);
+        if (lane == 0) start = atomicAdd(scratch + kBoysBins * kSeedCounterStride, 64);
         start = __shfl(start, 0);
         if (start >= all) break;  // every wavefront reaches this once the lists are taken
         const int idx = start + lane;
         int off = 0, bin = -1, pos = 0;
 #pragma unroll
-        for (int bb = kBoysBins - 1; bb >= 0; --bb) {  // most expensive bin first
+        for (int bb = kBoysBins - 1; bb >= 0; --bb) {  // series, then most expensive bin first
             if (bin < 0 && idx < off + counts[bb]) {
                 bin = bb;
                 pos = idx - off;
@@ -318,8 +338,8 @@ __global__ __launch_bounds__(64) void seed_quad_kernel(const SeedQuartet* __rest
             off += counts[bb];
         }
         if (bin >= 0) {
-            const long long gid = scratch[16 + (long long)bin * total + pos];
-            const int qi = scratch[16 + (long long)(kBoysBins + bin) * total + pos];
+            const long long gid = scratch[kSeedScratchHead + (long long)bin * total + pos];
+            const int qi = scratch[kSeedScratchHead + (long long)(kBoysBins + bin) * total + pos];
             const SeedQuartet Q = quartets[qi];
             int unused;
             seed_unit<NMAX, true>(Q, qi, gid - Q.pq_begin, shells, exps, kappa, seeds, err, radial, unused);
@@ -959,7 +979,7 @@ hipError_t upload_constants() {
 
 size_t seed_scratch_ints(long long total) {
     const size_t nblocks = (size_t)((total + kSeedBlock - 1) / kSeedBlock);
-    return 16 + 2 * (size_t)kBoysBins * (size_t)total + nblocks + 1;
+    return kSeedScratchHead + 2 * (size_t)kBoysBins * (size_t)total + nblocks + 1;
 }
 
 hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long total, const DevShell* shells,
@@ -968,13 +988,13 @@ hipError_t launch_seed(int nmax_bucket, const SeedQuartet* q, int nq, long long 
     if (total == 0) return hipSuccess;
     // the quadrature list counters start at zero (a fill kernel, so the
     // sequence holds no memset node when a caller captures it)
-    hipError_t e = launch_zero_ints(scratch, 16, s);
+    hipError_t e = launch_zero_ints(scratch, kSeedScratchHead, s);
     if (e != hipSuccess) return e;
     const dim3 block(kSeedBlock);
     const int nblocks = (int)((total + kSeedBlock - 1) / kSeedBlock);
     const dim3 grid((unsigned)nblocks);
     seed_blockmap_kernel<<<dim3((unsigned)((nblocks + 1 + 255) / 256)), dim3(256), 0, s>>>(
-        q, nq, total, nblocks, scratch + 16 + 2 * (long long)kBoysBins * total);
+        q, nq, total, nblocks, scratch + kSeedScratchHead + 2 * (long long)kBoysBins * total);
     const long long qwaves = (total + 63) / 64;
     const dim3 qgrid((unsigned)(qwaves < 4096 ? qwaves : 4096));
 #define GH_SEED(N)                                                                                              \

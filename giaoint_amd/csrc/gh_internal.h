@@ -140,7 +140,8 @@ int transform_max_o(int nt);
 
 // Seeds of `total` primitive quartets. `scratch` holds seed_scratch_ints(total)
 // ints (quadrature work lists); it is reused by the next launch in stream order.
-constexpr int kBoysBins = 11;
+constexpr int kBoysBins = 12;          // Boys phase-B lists: 11 quadrature bins, the power series
+constexpr int kSeedScratchHead = 448;  // ints of counters before the phase-B lists
 size_t seed_scratch_ints(long long total);
 // E.R.E contraction on the FP64 matrix cores (gh_mfma.h) for bra / ket
 // pair momenta (lab, lcd), lab >= lcd: whether an instance exists, the ket
