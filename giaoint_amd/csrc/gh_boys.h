@@ -24,37 +24,48 @@ extern __constant__ double c_gl32w[32];
 constexpr double kBoysAtol = 2.0e-15;
 constexpr double kBoysRtol = 7.5e-14;
 
+// Compensated power series of every order n <= nmax. Each order sums
+// (-z)^k / k! / (2n + 2k + 1); the powers (-z)^k / k! are formed once for all
+// orders by the recurrence each order would use on its own, so every order's
+// terms, sums and stopping index are those of its own series.
 template <int NMAX>
 __device__ inline int boys_series(zc z, int nmax, bool scaled, zc (&F)[NMAX + 1], double& worst) {
     worst = 0.0;
     const zc mz = zneg(z);
-    bool ok = true;
+    zc sum[NMAX + 1], comp[NMAX + 1];
+    double last[NMAX + 1];
+    unsigned long long open = 0;  // bit n: order n still summing
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+        sum[n] = comp[n] = zc{0.0, 0.0};
+        last[n] = 0.0;
+        if (n <= nmax) open |= 1ull << n;
+    }
+    zc pof{1.0, 0.0};
+    for (int k = 0; k < 256 && open; ++k) {
+#pragma unroll
+        for (int n = 0; n <= NMAX; ++n) {
+            if (!((open >> n) & 1ull)) continue;
+            const double den = (double)(2 * n + 2 * k + 1);
+            const zc term{pof.x / den, pof.y / den};
+            const zc corrected = zsub(term, comp[n]);
+            const zc next = zadd(sum[n], corrected);
+            comp[n] = zsub(zsub(next, sum[n]), corrected);
+            sum[n] = next;
+            last[n] = zabs(term);
+            if (k > 2 && last[n] <= kBoysAtol + kBoysRtol * zabs(sum[n])) open &= ~(1ull << n);
+        }
+        const double kk = (double)(k + 1);
+        pof = zmul(pof, zc{mz.x / kk, mz.y / kk});
+    }
+    const zc ez = scaled ? zexp(z) : zc{1.0, 0.0};
 #pragma unroll
     for (int n = 0; n <= NMAX; ++n) {
         if (n > nmax) continue;
-        zc pof{1.0, 0.0}, sum{0.0, 0.0}, comp{0.0, 0.0};
-        double last = 0.0;
-        bool converged = false;
-        for (int k = 0; k < 256; ++k) {
-            const double den = (double)(2 * n + 2 * k + 1);
-            const zc term{pof.x / den, pof.y / den};
-            const zc corrected = zsub(term, comp);
-            const zc next = zadd(sum, corrected);
-            comp = zsub(zsub(next, sum), corrected);
-            sum = next;
-            last = zabs(term);
-            if (k > 2 && last <= kBoysAtol + kBoysRtol * zabs(sum)) {
-                converged = true;
-                break;
-            }
-            const double kk = (double)(k + 1);
-            pof = zmul(pof, zc{mz.x / kk, mz.y / kk});
-        }
-        ok = ok && converged;
-        F[n] = scaled ? zmul(zexp(z), sum) : sum;
-        worst = fmax(worst, last);
+        F[n] = scaled ? zmul(ez, sum[n]) : sum[n];
+        worst = fmax(worst, last[n]);
     }
-    return ok ? kErrNone : kErrBoysSeries;
+    return open == 0 ? kErrNone : kErrBoysSeries;
 }
 
 // One pass of composite 16- and 32-point Gauss-Legendre over `segments`.
