@@ -259,6 +259,11 @@ class Context:
         matrix-core contraction kernels; bitwise identical results."""
         self.check(self.lib.giaohip_set_option(self.handle, 12, int(variant)))
 
+    def set_small_max_l(self, value: int):
+        """Largest la + lb + lc + ld (-1..2) evaluated by the
+        lane-per-primitive-quartet kernel."""
+        self.check(self.lib.giaohip_set_option(self.handle, 13, int(value)))
+
     def set_cholesky_graphs(self, on):
         """Cholesky column launches as hipGraphs: True (every pivot), "inside"
         (pivots inside a block only) or False (launched directly)."""

@@ -23,6 +23,8 @@ device in use:
 * ``mfma_min_l`` -- smallest total angular momentum whose classes (without a
   class-specialized kernel) run their contraction on the FP64 matrix cores
   (dense products) instead of the generic FP64 VALU kernel.
+* ``small_max_l`` -- largest total angular momentum of the classes on the
+  lane-per-primitive-quartet kernel (-1 turns it off).
 * ``mfma_variant`` -- a compile-time choice of the matrix-core kernels: every
   (LAB, LCD) instance is built in four variants (GEMM step unroll 1 or 2,
   LDS row stride of its W and ket-product tiles padded by 16 bytes or not),
@@ -58,8 +60,9 @@ CHOLESKY_BLOCK = (8, 16, 24, 32)
 WAVE_MAX_L = (-1, 3, 5, 7)
 MFMA_MIN_L = (7, 9, 11, 14)
 MFMA_VARIANT = (0, 1, 2, 3)
+SMALL_MAX_L = (-1, 2)
 DEFAULTS = {"fast_waves": 0, "eri_queue": "none", "chunk_mib": 2048, "cholesky_block": 32, "wave_max_l": 5,
-            "mfma_min_l": 9, "mfma_variant": 0}
+            "mfma_min_l": 9, "mfma_variant": 0, "small_max_l": 2}
 # knobs that do not change any sum: candidates must match bitwise
 EXACT = ("eri_queue", "chunk_mib", "wave_max_l", "mfma_variant")
 
@@ -99,6 +102,8 @@ def apply(ctx: "_native.Context", values: dict) -> None:
         ctx.set_mfma_min_l(int(values["mfma_min_l"]))
     if "mfma_variant" in values:
         ctx.set_mfma_variant(int(values["mfma_variant"]))
+    if "small_max_l" in values:
+        ctx.set_small_max_l(int(values["small_max_l"]))
 
 
 def _best_of(fn, reps: int) -> float:
@@ -130,7 +135,8 @@ def tune_eri(basis: "_native.DeviceBasis", quartets, field6, knobs=("fast_waves"
     try:
         for knob in knobs:
             space = {"fast_waves": FAST_WAVES, "eri_queue": ERI_QUEUE, "chunk_mib": CHUNK_MIB,
-                     "wave_max_l": WAVE_MAX_L, "mfma_min_l": MFMA_MIN_L, "mfma_variant": MFMA_VARIANT}[knob]
+                     "wave_max_l": WAVE_MAX_L, "mfma_min_l": MFMA_MIN_L, "mfma_variant": MFMA_VARIANT,
+                     "small_max_l": SMALL_MAX_L}[knob]
             exact = knob in EXACT
             # an exact knob is compared with the integrals of the choices made so far
             base = basis.eri_batch(sample, field6) if exact else reference

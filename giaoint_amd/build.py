@@ -36,6 +36,7 @@ SOURCES = [
     "gh_mfma_b.hip",
     "gh_mfma_c.hip",
     "gh_mfma_s.hip",
+    "gh_small.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -226,6 +226,7 @@ struct giaohip_ctx {
     int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
     int chol_graphs = 0;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
     int mfma_variant = 0;                    // compile-time variant of the matrix-core kernels (GIAOHIP_OPTION_MFMA_VARIANT)
+    int small_max_l = gh::kSmallMaxL;        // lane-per-primitive-quartet classes up to this L (GIAOHIP_OPTION_SMALL_MAX_L)
     DevBuffer queue;                         // work-queue counters: [0] context stream, [1] aux stream
     // pipelined chunks of the plain path (run_plain): per slot the seeds,
     // descriptors and work items of one chunk in flight, their page-locked
@@ -600,7 +601,9 @@ size_t seed_scratch_zc(long long pq_total) {
     return (gh::seed_scratch_ints(pq_total) * sizeof(int) + sizeof(zc) - 1) / sizeof(zc);
 }
 
-// Runs requests with the plain per-class transform kernel. With `fixed`, all
+// Runs requests with the plain per-class transform kernel (with `mfma`, the
+// classes the context's MFMA mode selects on the matrix cores; with `small`,
+// the smallest classes on the lane-per-primitive-quartet kernel). With `fixed`, all
 // device memory comes from its slices (no allocation); chunks are then also
 // bounded by the slices' capacities, and run one after the other. Otherwise
 // chunks are pipelined over two slots: while the transforms of chunk c run
@@ -608,7 +611,7 @@ size_t seed_scratch_zc(long long pq_total) {
 // kernel runs on the plain-path stream; a slot is reused once the chunk two
 // back has finished.
 int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs_in,
-              zc* d_out, const FixedPlainMem* fixed = nullptr, bool mfma = false) {
+              zc* d_out, const FixedPlainMem* fixed = nullptr, bool mfma = false, bool small = false) {
     if (reqs_in.empty()) return GIAOHIP_OK;
     // requests with an FP64-MFMA instance, oriented for it (mfk: instance key or -1)
     std::vector<Request> oriented;
@@ -725,6 +728,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         };
         std::vector<Planned> planned;
         std::map<int, std::vector<Planned>> mf_planned;  // FP64-MFMA items per instance
+        std::map<int, std::vector<Planned>> sm_planned;  // lane-per-primitive-quartet items per class
         for (size_t i = begin; i < end; ++i) {
             const Request& r = reqs[i];
             const DevShell* S[4] = {&b->shells[r.sh[0]], &b->shells[r.sh[1]], &b->shells[r.sh[2]], &b->shells[r.sh[3]]};
@@ -763,12 +767,43 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
                                 }
                 continue;
             }
+            const int L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
+            if (small && !fixed && L <= ctx->small_max_l) {
+                // one lane-per-primitive-quartet item per combination of contraction rows
+                const int key = S[0]->l | (S[1]->l << 4) | (S[2]->l << 8) | (S[3]->l << 12);
+                const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
+                int lo[4], hi[4];
+                for (int k = 0; k < 4; ++k) {
+                    lo[k] = r.xfix[k] < 0 ? 0 : r.xfix[k];
+                    hi[k] = r.xfix[k] < 0 ? S[k]->ncontr : r.xfix[k] + 1;
+                }
+                for (int xa = lo[0]; xa < hi[0]; ++xa)
+                    for (int xb = lo[1]; xb < hi[1]; ++xb)
+                        for (int xc = lo[2]; xc < hi[2]; ++xc)
+                            for (int xd = lo[3]; xd < hi[3]; ++xd) {
+                                Planned pl{};
+                                WorkItem& w = pl.w;
+                                for (int k = 0; k < 4; ++k) w.sh[k] = r.sh[k];
+                                w.x[0] = xa;
+                                w.x[1] = xb;
+                                w.x[2] = xc;
+                                w.x[3] = xd;
+                                w.cd0 = cd0;
+                                w.cd1 = cd1;
+                                w.seed_off = sq[i - begin].seed_off;
+                                w.out_base = r.out_base;
+                                for (int k = 0; k < 4; ++k) w.stride[k] = r.stride[k];
+                                w.cls = 0;
+                                pl.cost = 0.05 * kq;
+                                sm_planned[key].push_back(pl);
+                            }
+                continue;
+            }
             gh::ClassArgs ca;
             int lds, nt;
             long long nit;
             const int st = plain_plan(ctx, b, r, ca, lds, nit, nt);
             if (st) return st;
-            const int L = S[0]->l + S[1]->l + S[2]->l + S[3]->l;
             const double kq = (double)S[0]->nprim * S[1]->nprim * S[2]->nprim * S[3]->nprim;
             const int cls = S[0]->l | (S[1]->l << 4) | (S[2]->l << 8) | (S[3]->l << 12) | (ca.tile << 16);
             int lo[4], hi[4];
@@ -810,6 +845,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             double cost;
             int mf = -1;   // FP64-MFMA instance key, or -1 for the generic kernel
             int nt = 256;  // workgroup size of the generic kernel
+            int sm = -1;   // class of the lane-per-primitive-quartet kernel, or -1
         };
         std::vector<MixedLaunch> launches;
         for (const int nt : {64, 256})
@@ -852,6 +888,14 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             double cost = 0.0;
             for (const Planned& pl : list) cost += pl.cost;
             launches.push_back({0, all.size(), list.size(), cost, key});
+            for (const Planned& pl : list) all.push_back(pl.w);
+        }
+        for (auto& [key, list] : sm_planned) {
+            std::stable_sort(list.begin(), list.end(),
+                             [](const Planned& x, const Planned& y) { return x.cost > y.cost; });
+            double cost = 0.0;
+            for (const Planned& pl : list) cost += pl.cost;
+            launches.push_back({0, all.size(), list.size(), cost, -1, 64, key});
             for (const Planned& pl : list) all.push_back(pl.w);
         }
         int* d_queue = nullptr;
@@ -919,6 +963,15 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
             const int side = queued[1] < queued[0] ? 1 : 0;
             queued[side] += L.cost;
             hipStream_t ls = side ? ctx->aux : s;
+            if (L.sm >= 0) {
+                HIP_TRY(gh::launch_small(L.sm & 15, (L.sm >> 4) & 15, (L.sm >> 8) & 15, (L.sm >> 12) & 15,
+                                         d_items + L.first, (int)L.count, b->d_shells.as<DevShell>(),
+                                         b->d_exps.as<double>(), b->d_coefs.as<double>(), b->d_norms.as<double>(),
+                                         d_kappa, d_seeds, d_out, ls),
+                        "small-class kernel launch");
+                ctx->stats.launches += 1;
+                continue;
+            }
             if (L.mf >= 0) {
                 const int lab = (L.mf >> 4) & 15, lcd = L.mf & 15;
                 if (L.mf & kMfmaSplitKey)
@@ -1318,7 +1371,7 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
         if (is_fast[i]) fast.push_back(fast_all[i]);
         else plain.push_back(reqs[i]);
     }
-    int st = run_plain(ctx, b, field, plain, d_out, nullptr, mfma);
+    int st = run_plain(ctx, b, field, plain, d_out, nullptr, mfma, batched);
     if (st || fast.empty()) return st;
     // kappa and the error record were prepared by run_plain only if it ran
     const std::vector<double> kap = wave_vectors(b, field);
@@ -1566,6 +1619,10 @@ int giaohip_set_option(giaohip_ctx* ctx, int option, int value) {
     }
     if (option == GIAOHIP_OPTION_MFMA && value >= GIAOHIP_MFMA_OFF && value <= GIAOHIP_MFMA_SPLIT32) {
         ctx->mfma_mode = value;
+        return GIAOHIP_OK;
+    }
+    if (option == GIAOHIP_OPTION_SMALL_MAX_L && value >= -1 && value <= gh::kSmallMaxL) {
+        ctx->small_max_l = value;
         return GIAOHIP_OK;
     }
     if (option == GIAOHIP_OPTION_MFMA_VARIANT && value >= 0 && value < gh::kMfmaVariantCount) {

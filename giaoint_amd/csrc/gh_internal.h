@@ -155,6 +155,14 @@ int mfma_cd_tile(int lab);
 hipError_t launch_mfma(int lab, int lcd, int var, const WorkItem* items, int n, const DevShell* shells, const double* exps,
                        const double* coefs, const double* norms, const double* kappa, const zc* seeds, zc* out,
                        hipStream_t s);
+// Lane-per-primitive-quartet kernel of the smallest classes (gh_small.hip):
+// la + lb + lc + ld <= kSmallMaxL, one 64-thread workgroup per item (a shell
+// quartet x contraction rows, any ket range), seeds as for launch_transform.
+constexpr int kSmallMaxL = 2;
+bool small_class(int la, int lb, int lc, int ld);
+hipError_t launch_small(int la, int lb, int lc, int ld, const WorkItem* items, int n, const DevShell* shells,
+                        const double* exps, const double* coefs, const double* norms, const double* kappa,
+                        const zc* seeds, zc* out, hipStream_t s);
 // The split-FP32 form of the same contraction (FP32 hi/lo products on
 // v_mfma_f32_16x16x4f32, compensated into FP64), instances (8, 8) and (6, 4).
 bool mfma_split_instance(int lab, int lcd);

@@ -120,7 +120,12 @@ enum {
        0..3 (default 0): bit 0 flips the GEMM step unroll, bit 1 pads the
        LDS row stride of its W and ket-product tiles by 16 bytes. Results
        are bitwise identical for every variant (the autotuner picks one). */
-    GIAOHIP_OPTION_MFMA_VARIANT = 12
+    GIAOHIP_OPTION_MFMA_VARIANT = 12,
+    /* Largest total angular momentum la + lb + lc + ld (-1..2, default 2)
+       of the classes evaluated by the lane-per-primitive-quartet kernel (each
+       lane one primitive quartet, the block in registers, one wavefront
+       reduction) instead of the generic transform. */
+    GIAOHIP_OPTION_SMALL_MAX_L = 13
 };
 enum {
     GIAOHIP_PIPELINE_OVERLAP = 0, /* on a second stream, beside the previous chunk's contraction */
