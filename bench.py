@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--fast-waves", type=int, default=-1,
                     help="workgroup waves of the class-specialized kernel for the headline batch "
                          "(0 = library default, 4..8; default: autotuned)")
+    ap.add_argument("--pipeline", choices=("overlap", "serial"), default="overlap",
+                    help="producer kernels of the batched path beside the contraction (default) or serially")
     ap.add_argument("--no-host-output", action="store_true",
                     help="skip the PCIe-inclusive run into host memory (profiles of the timed steps only)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -587,6 +589,7 @@ def main():
         return ctx.stats()
 
     tuned = None
+    ctx.set_pipeline(args.pipeline)
     if args.fast_waves >= 0:
         ctx.set_fast_waves(args.fast_waves)
     elif not args.no_autotune:
