@@ -55,9 +55,9 @@ from . import _native
 
 FAST_WAVES = (0, 4, 5, 6, 7)
 ERI_QUEUE = ("none", "buckets", "single")
-CHUNK_MIB = (512, 1024, 2048, 4096)
+CHUNK_MIB = (1024, 2048, 4096, 8192, 16384)
 CHOLESKY_BLOCK = (8, 16, 24, 32)
-WAVE_MAX_L = (-1, 3, 5, 7)
+WAVE_MAX_L = (-1, 5, 6, 7, 8)
 MFMA_MIN_L = (7, 9, 11, 14)
 MFMA_VARIANT = (0, 1, 2, 3)
 SMALL_MAX_L = (-1, 2)

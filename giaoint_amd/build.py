@@ -32,6 +32,16 @@ SOURCES = [
     "gh_fast_pppp.hip",
     "gh_fast_psps.hip",
     "gh_fast_ssss.hip",
+    "gh_fast_g0.hip",
+    "gh_fast_g1.hip",
+    "gh_fast_g2.hip",
+    "gh_fast_g3.hip",
+    "gh_fast_g4.hip",
+    "gh_fast_g5.hip",
+    "gh_fast_g6.hip",
+    "gh_fast_g7.hip",
+    "gh_fast_g8.hip",
+    "gh_fast_g9.hip",
     "gh_mfma_a.hip",
     "gh_mfma_b.hip",
     "gh_mfma_c.hip",

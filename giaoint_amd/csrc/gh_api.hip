@@ -214,7 +214,7 @@ struct giaohip_ctx {
     hipEvent_t copy_ev[2]{};
     DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
     std::set<void*> user_allocs;
-    size_t seed_budget_bytes = size_t(2048) << 20;
+    size_t seed_budget_bytes = size_t(8192) << 20;
     int chol_block = 32;  // candidates per Cholesky pivot block
     int chol_pivot_batch = 32;  // cholesky_eri's pivot_batch_size
     size_t staging_budget_bytes = size_t(1) << 30;
@@ -222,7 +222,7 @@ struct giaohip_ctx {
     int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
     int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
     int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
-    int wave_max_l = 5;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
+    int wave_max_l = 7;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
     int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
     int chol_graphs = 0;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
     int mfma_variant = 0;                    // compile-time variant of the matrix-core kernels (GIAOHIP_OPTION_MFMA_VARIANT)
@@ -1365,12 +1365,35 @@ int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6]
     std::vector<FastReq> fast;
     size_t nfast = 0;
     for (size_t i = 0; i < n; ++i) nfast += is_fast[i] ? 1 : 0;
-    fast.reserve(nfast);
     plain.reserve(n - nfast);
+    // The batched requests grouped by class kernel (input order kept within
+    // a class): a chunk then holds few classes and each launch many items.
+    std::vector<gh::FastLaunch> kinds;
+    std::vector<int> kind(n, -1);
+    std::vector<size_t> fill;
     for (size_t i = 0; i < n; ++i) {
-        if (is_fast[i]) fast.push_back(fast_all[i]);
-        else plain.push_back(reqs[i]);
+        if (!is_fast[i]) {
+            plain.push_back(reqs[i]);
+            continue;
+        }
+        const gh::FastLaunch t = fast_all[i].info.transform;
+        int k = 0;
+        while (k < (int)kinds.size() && kinds[k] != t) ++k;
+        if (k == (int)kinds.size()) {
+            kinds.push_back(t);
+            fill.push_back(0);
+        }
+        kind[i] = k;
+        ++fill[k];
     }
+    for (size_t k = 0, at = 0; k < fill.size(); ++k) {
+        const size_t c = fill[k];
+        fill[k] = at;
+        at += c;
+    }
+    fast.resize(nfast);
+    for (size_t i = 0; i < n; ++i)
+        if (kind[i] >= 0) fast[fill[kind[i]]++] = fast_all[i];
     int st = run_plain(ctx, b, field, plain, d_out, nullptr, mfma, batched);
     if (st || fast.empty()) return st;
     // kappa and the error record were prepared by run_plain only if it ran

@@ -384,7 +384,9 @@ __device__ inline int opaque_tid(int v) {
 
 struct LdsLayout {
     int offL, listL, offAB, tuvAB;  // int offsets (in ints)
+    int skey, sdec;                 // ket slots of a pass: sort keys, decode (int2)
     int seeds, eket, ebra, work;    // zc offsets (in zc) from zc base
+    int kfac, bfac;                 // contraction x normalization factors (doubles)
     int zc_base;                    // byte offset of the zc region
 };
 
@@ -402,6 +404,11 @@ __device__ __host__ inline LdsLayout lds_layout(int la, int lb, int lc, int ld, 
     ints += (Lab + 1) * (Lab + 1);
     s.tuvAB = ints;
     ints += nherm(Lab);
+    s.skey = ints;
+    ints += tile;
+    ints = (ints + 1) & ~1;
+    s.sdec = ints;
+    ints += 2 * tile;
     s.zc_base = align16(ints * 4);
     int z = 0;
     s.seeds = z;
@@ -410,6 +417,10 @@ __device__ __host__ inline LdsLayout lds_layout(int la, int lb, int lc, int ld, 
     z += 3 * (lc + 1) * (ld + 1) * (Lcd + 1);
     s.ebra = z;
     z += 3 * (la + 1) * (lb + 1) * (Lab + 1);
+    s.kfac = z;
+    z += (tile + 1) / 2;
+    s.bfac = z;
+    z += (ncart(la) * ncart(lb) + 1) / 2;
     s.work = z;
     const int r0 = 2 * nherm(L), wz = nherm(Lab) * tile;
     z += r0 > wz ? r0 : wz;
@@ -450,10 +461,14 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
     int* listL = ibase + lay.listL;
     int* offAB = ibase + lay.offAB;
     int* tuvAB = ibase + lay.tuvAB;
+    int* skey = ibase + lay.skey;
+    int2* sdec = reinterpret_cast<int2*>(ibase + lay.sdec);
     zc* sseed = zbase + lay.seeds;
     zc* eket = zbase + lay.eket;
     zc* ebra = zbase + lay.ebra;
     zc* work = zbase + lay.work;
+    double* kfac = reinterpret_cast<double*>(zbase + lay.kfac);
+    double* bfac = reinterpret_cast<double*>(zbase + lay.bfac);
     const int tid = threadIdx.x;
 
     // Index tables (parallel fills).
@@ -503,6 +518,8 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
     const int stride_pq = L + 4;
     const int ebra_i = (lb + 1) * (Lab + 1), ebra_a = (la + 1) * ebra_i;
     const int eket_i = (ld + 1) * (Lcd + 1), eket_a = (lc + 1) * eket_i;
+    // e / NHab as a multiply-high: exact for every e < 2^32 / NHab
+    const unsigned nh_inv = NHab > 1 ? 0xffffffffu / (unsigned)NHab + 1u : 0u;
 
     // The R0 recurrence step of entries tid and tid + NT (fixed for the
     // item): rs_a = lex | src1 << 16, rs_b = src2 | axis << 16 | (coef) << 18,
@@ -548,6 +565,31 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
         zc acc[kMaxO];
 #pragma unroll
         for (int r = 0; r < kMaxO; ++r) acc[r] = zc{0.0, 0.0};
+        // The pass's ket components in slot order: sorted by their loop
+        // bounds (cx + dx, cy + dy, cz + dz), so that the lanes of a wavefront
+        // mostly run the same trip counts. sdec[slot] = (the six Cartesian
+        // exponents, 4 bits each; cc | dc << 8). (The previous pass's reads of
+        // sdec all precede the barrier below.)
+        for (int sl = tid; sl < tl; sl += NT) {
+            const int cdcomp = it.cd0 + t0 + sl;
+            const unsigned pcp = cart_of(lc, cdcomp / ncd), pdp = cart_of(ld, cdcomp % ncd);
+            const int a = (pcp & 255) + (pdp & 255), b = ((pcp >> 8) & 255) + ((pdp >> 8) & 255);
+            const int c = (pcp >> 16) + (pdp >> 16);
+            skey[sl] = (((a << 10) | (b << 5) | c) << 12) | sl;
+        }
+        __syncthreads();
+        for (int sl = tid; sl < tl; sl += NT) {
+            const int key = skey[sl];
+            int rank = 0;
+            for (int s2 = 0; s2 < tl; ++s2) rank += skey[s2] < key ? 1 : 0;
+            const int cdcomp = it.cd0 + t0 + sl;
+            const int cc = cdcomp / ncd, dc = cdcomp % ncd;
+            const unsigned pcp = cart_of(lc, cc), pdp = cart_of(ld, dc);
+            sdec[rank] = make_int2((int)((pcp & 15) | ((pdp & 15) << 4) | (((pcp >> 8) & 15) << 8) |
+                                         (((pdp >> 8) & 15) << 12) | ((pcp >> 16) << 16) | ((pdp >> 16) << 20)),
+                                   cc | (dc << 8));
+        }
+        // (the first ket step's barrier orders these writes before their reads)
 
         for (int pab = 0; pab < Kab; ++pab) {
             const int pa = pab / Kb, pb = pab % Kb;
@@ -561,6 +603,13 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                 const double qv = kappa[3 * it.sh[1] + axis] - kappa[3 * it.sh[0] + axis];
                 build_phase_hermite(i, j, Pc - Ac[axis], Pc - Bc[axis], p, qv,
                                     ebra + axis * ebra_a + i * ebra_i + j * (Lab + 1));
+            }
+            // contraction x normalization of every bra component pair, the
+            // product formed in the order ((cA nA) cB) nB
+            for (int ab = tid; ab < nab; ab += NT) {
+                const int cac = ab / ncb, cbc = ab % ncb;
+                bfac[ab] = coefs[A.coef_off + it.x[0] * A.nprim + pa] * norms[A.norm_off + cac * A.nprim + pa] *
+                           coefs[B.coef_off + it.x[1] * B.nprim + pb] * norms[B.norm_off + cbc * B.nprim + pb];
             }
             zc w[kMaxW];
 #pragma unroll
@@ -582,8 +631,15 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                     const int i = (e / (ld + 1)) % (lc + 1), j = e % (ld + 1);
                     const double Qc = (ec * Cc[axis] + ed * Dc[axis]) / q;
                     const double qv = kappa[3 * it.sh[3] + axis] - kappa[3 * it.sh[2] + axis];
-                    build_phase_hermite(i, j, Qc - Cc[axis], Qc - Dc[axis], q, qv,
-                                        eket + axis * eket_a + i * eket_i + j * (Lcd + 1));
+                    zc* dst = eket + axis * eket_a + i * eket_i + j * (Lcd + 1);
+                    build_phase_hermite(i, j, Qc - Cc[axis], Qc - Dc[axis], q, qv, dst);
+                    // the ket side's (-1)^(tau + phi + chi), one axis at a time
+                    for (int h = 1; h <= i + j; h += 2) dst[h] = zneg(dst[h]);
+                }
+                for (int sl = tid; sl < tl; sl += NT) {
+                    const int cc = sdec[sl].y & 255, dc = sdec[sl].y >> 8;
+                    kfac[sl] = coefs[C.coef_off + it.x[2] * C.nprim + pc] * norms[C.norm_off + cc * C.nprim + pc] *
+                               coefs[D.coef_off + it.x[3] * D.nprim + pd] * norms[D.norm_off + dc * D.nprim + pd];
                 }
                 __syncthreads();
                 // R0 table: level L holds s_L; level n from level n+1.
@@ -658,14 +714,13 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                 for (int r = 0; r < kMaxW; ++r) {
                     const int e = tid_k + r * NT;
                     if (e < nW) {
-                        const int cdl = e / NHab, k = e % NHab;
+                        const int cdl = NHab > 1 ? (int)__umulhi((unsigned)e, nh_inv) : e;
+                        const int k = e - cdl * NHab;
                         const int pk = tuvAB[k];
                         const int t = pk & 63, u = (pk >> 6) & 63, v = (pk >> 12) & 63;
-                        const int cdcomp = it.cd0 + t0 + cdl;
-                        const int cc = cdcomp / ncd, dc = cdcomp % ncd;
-                        const unsigned pcp = cart_of(lc, cc), pdp = cart_of(ld, dc);
-                        const int cx = pcp & 255, cy = (pcp >> 8) & 255, cz = pcp >> 16;
-                        const int dx = pdp & 255, dy = (pdp >> 8) & 255, dz = pdp >> 16;
+                        const int dx6 = sdec[cdl].x;
+                        const int cx = dx6 & 15, dx = (dx6 >> 4) & 15, cy = (dx6 >> 8) & 15;
+                        const int dy = (dx6 >> 12) & 15, cz = (dx6 >> 16) & 15, dz = dx6 >> 20;
                         const zc* Ex = eket + 0 * eket_a + cx * eket_i + dx * (Lcd + 1);
                         const zc* Ey = eket + 1 * eket_a + cy * eket_i + dy * (Lcd + 1);
                         const zc* Ez = eket + 2 * eket_a + cz * eket_i + dz * (Lcd + 1);
@@ -676,16 +731,12 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                             for (int phi = 0; phi <= cy + dy; ++phi) {
                                 const zc* rr = R0 + offL[(t + tau) * (L + 1) + (u + phi)] + v;
                                 zc stp{0.0, 0.0};
-                                for (int chi = 0; chi <= cz + dz; ++chi)
-                                    zmac(stp, (chi & 1) ? zneg(Ez[chi]) : Ez[chi], rr[chi]);
-                                zmac(st, (phi & 1) ? zneg(Ey[phi]) : Ey[phi], stp);
+                                for (int chi = 0; chi <= cz + dz; ++chi) zmac(stp, Ez[chi], rr[chi]);
+                                zmac(st, Ey[phi], stp);
                             }
-                            zmac(s, (tau & 1) ? zneg(Ex[tau]) : Ex[tau], st);
+                            zmac(s, Ex[tau], st);
                         }
-                        const double sc = coefs[C.coef_off + it.x[2] * C.nprim + pc] *
-                                          norms[C.norm_off + cc * C.nprim + pc] *
-                                          coefs[D.coef_off + it.x[3] * D.nprim + pd] *
-                                          norms[D.norm_off + dc * D.nprim + pd];
+                        const double sc = kfac[cdl];
                         w[r].x = fma(sc, s.x, w[r].x);
                         w[r].y = fma(sc, s.y, w[r].y);
                     }
@@ -699,8 +750,8 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                 for (int r = 0; r < kMaxW; ++r) {
                     const int e = tid + r * NT;
                     if (e < nW) {
-                        const int cdl = e / NHab, k = e % NHab;
-                        work[k * tl + cdl] = w[r];
+                        const int cdl = NHab > 1 ? (int)__umulhi((unsigned)e, nh_inv) : e;
+                        work[(e - cdl * NHab) * tl + cdl] = w[r];
                     }
                 }
             }
@@ -732,10 +783,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
                         }
                         zmac(s, Ex[t], st);
                     }
-                    const double sc = coefs[A.coef_off + it.x[0] * A.nprim + pa] *
-                                      norms[A.norm_off + cac * A.nprim + pa] *
-                                      coefs[B.coef_off + it.x[1] * B.nprim + pb] *
-                                      norms[B.norm_off + cbc * B.nprim + pb];
+                    const double sc = bfac[abcomp];
                     acc[r].x = fma(sc, s.x, acc[r].x);
                     acc[r].y = fma(sc, s.y, acc[r].y);
                 }
@@ -750,8 +798,7 @@ __device__ __forceinline__ void transform_item(const ClassArgs ca, const WorkIte
             if (o < nO) {
                 const int abcomp = o / tl, cdl = o % tl;
                 const int cac = abcomp / ncb, cbc = abcomp % ncb;
-                const int cdcomp = it.cd0 + t0 + cdl;
-                const int cc = cdcomp / ncd, dc = cdcomp % ncd;
+                const int cc = sdec[cdl].y & 255, dc = sdec[cdl].y >> 8;
                 const long long ia = (long long)it.x[0] * nca + cac, ib = (long long)it.x[1] * ncb + cbc;
                 const long long ic = (long long)it.x[2] * ncc + cc, id = (long long)it.x[3] * ncd + dc;
                 out[it.out_base + ia * it.stride[0] + ib * it.stride[1] + ic * it.stride[2] + id * it.stride[3]] =

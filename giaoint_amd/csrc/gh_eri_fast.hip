@@ -9,6 +9,17 @@ FastClassInfo fast_info_2233();
 FastClassInfo fast_info_1111();
 FastClassInfo fast_info_1010();
 FastClassInfo fast_info_0000();
+// groups of classes with la + lb + lc + ld = 3..8 (gh_fast_g*.hip)
+FastClassInfo fast_info_g0(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g1(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g2(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g3(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g4(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g5(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g6(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g7(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g8(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g9(int la, int lb, int lc, int ld);
 
 int fast_waves(int nhab, int kab, int cap) {
     const int maxw = cap >= kFastBraGroups && cap <= kFastMaxWaves ? cap : kFastMaxWaves;
@@ -29,6 +40,12 @@ FastClassInfo fast_class_info(int la, int lb, int lc, int ld) {
     GH_FAST(1, 0, 1, 0)
     GH_FAST(0, 0, 0, 0)
 #undef GH_FAST
+    for (FastClassInfo (*group)(int, int, int, int) :
+         {fast_info_g0, fast_info_g1, fast_info_g2, fast_info_g3, fast_info_g4, fast_info_g5, fast_info_g6,
+          fast_info_g7, fast_info_g8, fast_info_g9}) {
+        const FastClassInfo f = group(la, lb, lc, ld);
+        if (f.available) return f;
+    }
     return FastClassInfo{false, 0, 0, 0, 0, nullptr, nullptr};
 }
 

@@ -1,0 +1,17 @@
+// Batched transform kernels of the classes (11|10), (20|11), (22|20), (30|30), (32|10), (31|21).
+#include "gh_fast.h"
+
+namespace gh {
+FastClassInfo fast_info_g7(int la, int lb, int lc, int ld) {
+#define GH_FAST(a, b, c, d) \
+    if (la == a && lb == b && lc == c && ld == d) return fast::info<a, b, c, d>();
+    GH_FAST(1, 1, 1, 0)
+    GH_FAST(2, 0, 1, 1)
+    GH_FAST(2, 2, 2, 0)
+    GH_FAST(3, 0, 3, 0)
+    GH_FAST(3, 2, 1, 0)
+    GH_FAST(3, 1, 2, 1)
+#undef GH_FAST
+    return FastClassInfo{false, 0, 0, 0, 0, nullptr, nullptr};
+}
+}  // namespace gh

@@ -55,7 +55,7 @@ enum {
 enum {
     GIAOHIP_OPTION_ERI_VARIANT = 0,
     /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables),
-       in MiB, 16..65536; default 2048. Two chunks are in flight at once (one
+       in MiB, 16..65536; default 8192. Two chunks are in flight at once (one
        being produced, one transformed), so the path holds twice this much.
        Chunks of the generic transform hold up to four times this much of
        seeds, also two at once (eight times in all). */
@@ -101,7 +101,7 @@ enum {
     /* Largest total angular momentum la + lb + lc + ld of the classes whose
        generic-transform items run on one-wavefront workgroups (when their
        transform rows fit one wavefront); -1 puts every item on four-wavefront
-       workgroups. Default 5. Results are bitwise identical for every value. */
+       workgroups. Default 7. Results are bitwise identical for every value. */
     GIAOHIP_OPTION_WAVE_MAX_L = 9,
     /* Under GIAOHIP_MFMA_DEFAULT: smallest total angular momentum
        la + lb + lc + ld of the classes (without a class-specialized kernel,
