@@ -42,6 +42,9 @@ SOURCES = [
     "gh_fast_g7.hip",
     "gh_fast_g8.hip",
     "gh_fast_g9.hip",
+    "gh_fast_g10.hip",
+    "gh_fast_g11.hip",
+    "gh_fast_g12.hip",
     "gh_mfma_a.hip",
     "gh_mfma_b.hip",
     "gh_mfma_c.hip",

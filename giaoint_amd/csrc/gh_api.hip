@@ -214,7 +214,7 @@ struct giaohip_ctx {
     hipEvent_t copy_ev[2]{};
     DevBuffer work[2];                 // seeds, R0 tables and pair tables of two chunks in flight
     std::set<void*> user_allocs;
-    size_t seed_budget_bytes = size_t(8192) << 20;
+    size_t seed_budget_bytes = size_t(16384) << 20;
     int chol_block = 32;  // candidates per Cholesky pivot block
     int chol_pivot_batch = 32;  // cholesky_eri's pivot_batch_size
     size_t staging_budget_bytes = size_t(1) << 30;
@@ -663,9 +663,9 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         ctx->stats.contract_ms += t;
         return hipSuccess;
     };
-    // a chunk's seeds stay within four chunk budgets (per slot when pipelined:
+    // a chunk's seeds stay within two chunk budgets (per slot when pipelined:
     // the longest items' tails are paid once per chunk, so chunks are large)
-    const size_t seed_cap = 4 * ctx->seed_budget_bytes;
+    const size_t seed_cap = 2 * ctx->seed_budget_bytes;
 
     size_t begin = 0;
     int chunk = 0;

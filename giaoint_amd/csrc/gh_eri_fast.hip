@@ -20,6 +20,10 @@ FastClassInfo fast_info_g6(int la, int lb, int lc, int ld);
 FastClassInfo fast_info_g7(int la, int lb, int lc, int ld);
 FastClassInfo fast_info_g8(int la, int lb, int lc, int ld);
 FastClassInfo fast_info_g9(int la, int lb, int lc, int ld);
+// and the (fd) classes with la + lb + lc + ld = 9, 10
+FastClassInfo fast_info_g10(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g11(int la, int lb, int lc, int ld);
+FastClassInfo fast_info_g12(int la, int lb, int lc, int ld);
 
 int fast_waves(int nhab, int kab, int cap) {
     const int maxw = cap >= kFastBraGroups && cap <= kFastMaxWaves ? cap : kFastMaxWaves;
@@ -42,7 +46,7 @@ FastClassInfo fast_class_info(int la, int lb, int lc, int ld) {
 #undef GH_FAST
     for (FastClassInfo (*group)(int, int, int, int) :
          {fast_info_g0, fast_info_g1, fast_info_g2, fast_info_g3, fast_info_g4, fast_info_g5, fast_info_g6,
-          fast_info_g7, fast_info_g8, fast_info_g9}) {
+          fast_info_g7, fast_info_g8, fast_info_g9, fast_info_g10, fast_info_g11, fast_info_g12}) {
         const FastClassInfo f = group(la, lb, lc, ld);
         if (f.available) return f;
     }

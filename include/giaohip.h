@@ -55,10 +55,10 @@ enum {
 enum {
     GIAOHIP_OPTION_ERI_VARIANT = 0,
     /* Work memory of one batched-path chunk (seeds, R0 tables, pair tables),
-       in MiB, 16..65536; default 8192. Two chunks are in flight at once (one
+       in MiB, 16..65536; default 16384. Two chunks are in flight at once (one
        being produced, one transformed), so the path holds twice this much.
-       Chunks of the generic transform hold up to four times this much of
-       seeds, also two at once (eight times in all). */
+       Chunks of the generic transform hold up to twice this much of seeds,
+       also two at once (four times in all). */
     GIAOHIP_OPTION_CHUNK_MIB = 1,
     /* Candidates per pivot block of giaohip_cholesky_run, 1..32; default 32.
        A block projects its candidates (the largest residuals) onto the
