@@ -45,6 +45,10 @@ SOURCES = [
     "gh_fast_g10.hip",
     "gh_fast_g11.hip",
     "gh_fast_g12.hip",
+    "gh_fast_g13.hip",
+    "gh_fast_g14.hip",
+    "gh_fast_g15.hip",
+    "gh_fast_g16.hip",
     "gh_mfma_a.hip",
     "gh_mfma_b.hip",
     "gh_mfma_c.hip",

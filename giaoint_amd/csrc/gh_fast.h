@@ -404,24 +404,43 @@ template <class P, int G, int D, class Slot>
 struct Pipe {
     static constexpr int S = P::T.start[G][0], END = P::T.start[G][P::NH];
     static constexpr int NB = (END - S + 15) / 16;
-    static constexpr int first_batch(int h) {
+    // The Hermite indices whose first entry lies in batch b, ascending:
+    // h[start[b] .. start[b + 1]) (tabulated once per group).
+    struct FirstUse {
+        int start[NB + 1];
+        int h[P::NH];
+    };
+    static constexpr FirstUse make_first_use() {
+        FirstUse f{};
+        int fb[P::NH];
+        for (int h = 0; h < P::NH; ++h) fb[h] = -1;
         for (int e = S; e < END; ++e)
-            if (P::T.hidx[e] == h) return (e - S) / 16;
-        return -1;
+            if (fb[P::T.hidx[e]] < 0) fb[P::T.hidx[e]] = (e - S) / 16;
+        int n = 0;
+        for (int b = 0; b < NB; ++b) {
+            f.start[b] = n;
+            for (int h = 0; h < P::NH; ++h)
+                if (fb[h] == b) f.h[n++] = h;
+        }
+        f.start[NB] = n;
+        return f;
     }
+    static constexpr FirstUse FU = make_first_use();
 
-    template <int B, int H, class Src, int NV>
+    template <int B, int I, class Src, int NV>
     __device__ static inline void load_vals(zc (&val)[NV], const Src& src) {
-        if constexpr (H < NV) {
-            if constexpr (first_batch(H) == B) val[H] = src.template load<H>();
-            load_vals<B, H + 1>(val, src);
+        if constexpr (FU.start[B] + I < FU.start[B + 1]) {
+            constexpr int H = FU.h[FU.start[B] + I];
+            val[H] = src.template load<H>();
+            load_vals<B, I + 1>(val, src);
         }
     }
-    template <int B, int H, class Src, int NV>
+    template <int B, int I, class Src, int NV>
     __device__ static inline void finish_vals(zc (&val)[NV], const Src& src) {
-        if constexpr (H < NV) {
-            if constexpr (first_batch(H) == B) val[H] = src.finish(val[H]);
-            finish_vals<B, H + 1>(val, src);
+        if constexpr (FU.start[B] + I < FU.start[B + 1]) {
+            constexpr int H = FU.h[FU.start[B] + I];
+            val[H] = src.finish(val[H]);
+            finish_vals<B, I + 1>(val, src);
         }
     }
     template <int B, class Src, int NV>
@@ -544,14 +563,16 @@ constexpr Norms<C> make_norms() {
     return n;
 }
 template <class C>
-__constant__ Norms<C> c_norms = make_norms<C>();
+inline constexpr Norms<C> kNorms = make_norms<C>();
+template <class C>
+__constant__ Norms<C> c_norms = kNorms<C>;
 
 // Stores (or, for later passes over the bra pairs, adds) the group's outputs.
 template <class C, int G, int AB>
 __device__ inline void write_ab(const zc (&OUT)[C::NOUT], double gcd, long long base, const FastItem& it,
                                 bool accumulate, zc* __restrict__ out) {
     if constexpr (AB < C::NAB) {
-        constexpr double g = make_norms<C>().ab[AB];
+        constexpr double g = kNorms<C>.ab[AB];
         constexpr int ca = AB / C::NCB, cb = AB % C::NCB;
         zc v = zscale(g * gcd, OUT[AB / kG]);
         if (it.conj) v.y = -v.y;

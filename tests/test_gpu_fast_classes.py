@@ -1,9 +1,7 @@
-"""The batched variant on every class with l <= 3 and la + lb + lc + ld =
-3..10 against the oracle: class-specialized kernels (each ordered class
-evaluated through the symmetry image that has one; all but the classes with
-an (ff) pair below L = 9, and the (fd) classes at L = 9, 10) and the
-matrix-core kernels for the rest, in a field and at zero field, with
-general contractions; and the mixed-class batch on the batched and the
+"""The class-specialized kernels of every class with l <= 3 and
+la + lb + lc + ld = 3..12 against the oracle (each ordered class evaluated
+through the symmetry image that has a kernel), in a field and at zero field,
+with general contractions; and the mixed-class batch on the batched and the
 plain variant. Reference contraction: src/eri.cpp:341-368."""
 
 import itertools
@@ -20,7 +18,7 @@ pytestmark = pytest.mark.gpu
 ATOL = 1e-11
 
 
-@pytest.mark.parametrize("L", range(3, 11))
+@pytest.mark.parametrize("L", range(3, 13))
 def test_fast_classes_against_oracle(gpu, port, L):
     classes = [c for c in itertools.product(range(4), repeat=4) if sum(c) == L]
     worst = 0.0
