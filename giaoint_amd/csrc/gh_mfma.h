@@ -42,6 +42,8 @@
 #include "gh_internal.h"
 #include "gh_plan.h"
 
+#include <type_traits>
+
 namespace gh {
 namespace mf {
 
@@ -50,6 +52,19 @@ typedef float mf4f __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 8;
 constexpr int kNT = 64 * kWaves;
+
+// f(std::integral_constant<int, n>) for a wave-uniform n in 0..N: the GEMM
+// step loops run over a compile-time number of owned tiles, so no branch
+// around a matrix instruction sits inside them.
+template <int N, class F>
+__device__ inline void with_tile_count(int n, F&& f) {
+    if constexpr (N == 0) {
+        f(std::integral_constant<int, 0>{});
+    } else {
+        if (n == N) f(std::integral_constant<int, N>{});
+        else with_tile_count<N - 1>(n, f);
+    }
+}
 
 // Ket n-tiles (16 components each) of one work item: two, or one where the
 // bra side is so large that the accumulators of two would not fit in the
@@ -335,6 +350,9 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
         const int h = k < S::NHAB ? c_htab<LAB>.e[k] : 0;
         g1_k[i] = ((h & 255) * (L + 1) + ((h >> 8) & 255)) | ((h >> 16) << 16);
     }
+    int n1 = 0;  // owned GEMM1 tiles: a prefix of the TW1 slots (t grows with i)
+#pragma unroll
+    for (int i = 0; i < S::TW1; ++i) n1 += g1_on[i] ? 1 : 0;
     const int nt1 = ntv == 2 ? (wave & 1) : 0;  // the wavefront's n-tile (fixed: kWaves is even)
     const int cdl = nt1 * 16 + li;              // the lane's ket column (GEMM B operands, output)
     const bool cd_on = cdl < ncomp;
@@ -366,6 +384,10 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
         g2_by[i] = (S::RB + ((apk >> 8) & 255) * rb_i + ((bpk >> 8) & 255)) * (LAB + 1);
         g2_bz[i] = (2 * S::RB + (apk >> 16) * rb_i + (bpk >> 16)) * (LAB + 1);
     }
+
+    int n2 = 0;  // owned GEMM2 tiles, likewise a prefix
+#pragma unroll
+    for (int i = 0; i < S::TW2; ++i) n2 += g2_on[i] ? 1 : 0;
 
     mf4 vre[S::TW2], vim[S::TW2];
 #pragma unroll
@@ -517,45 +539,45 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                 const int idx = offL[(g1_k[i] & 0xffff) + (hj & 0xffff)] + (g1_k[i] >> 16) + (hj >> 16);
                 return R0[g1_row[i] ? idx : S::NHL];
             };
-            zc b = Ec[lk * S::CDS + cdl];
-            zc a[S::TW1];
-            {
-                const int hj = hcd[lk];
+            with_tile_count<S::TW1>(n1, [&](auto na_c) {
+                constexpr int NA = decltype(na_c)::value;
+                if constexpr (NA > 0) {
+                    zc b = Ec[lk * S::CDS + cdl];
+                    zc a[NA];
+                    {
+                        const int hj = hcd[lk];
 #pragma unroll
-                for (int i = 0; i < S::TW1; ++i) a[i] = gather(i, hj);
-            }
-#pragma unroll S::UNROLL
-            for (int s = 0; s < S::KS1; ++s) {
-                const int jn = 4 * (s + 1) + lk;  // step s + 1 (a zero row past the end)
-                const zc bn = Ec[jn * S::CDS + cdl];
-                const int hjn = hcd[jn];
-                zc an[S::TW1];
-#pragma unroll
-                for (int i = 0; i < S::TW1; ++i) an[i] = gather(i, hjn);
-                if constexpr (SPLIT) {
-#pragma unroll
-                    for (int i = 0; i < S::TW1; ++i)
-                        if (g1_on[i]) split_step(wre[i], wim[i], a[i], b, sc1);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < S::TW1; ++i) {
-                        if (g1_on[i]) {
-                            wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.x, wre[i], 0, 0, 0);
-                            wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.y, wim[i], 0, 0, 0);
-                        }
+                        for (int i = 0; i < NA; ++i) a[i] = gather(i, hj);
                     }
+#pragma unroll S::UNROLL
+                    for (int s = 0; s < S::KS1; ++s) {
+                        const int jn = 4 * (s + 1) + lk;  // step s + 1 (a zero row past the end)
+                        const zc bn = Ec[jn * S::CDS + cdl];
+                        const int hjn = hcd[jn];
+                        zc an[NA];
 #pragma unroll
-                    for (int i = 0; i < S::TW1; ++i) {
-                        if (g1_on[i]) {
-                            wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i].y, b.y, wre[i], 0, 0, 0);
-                            wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b.x, wim[i], 0, 0, 0);
+                        for (int i = 0; i < NA; ++i) an[i] = gather(i, hjn);
+                        if constexpr (SPLIT) {
+#pragma unroll
+                            for (int i = 0; i < NA; ++i) split_step(wre[i], wim[i], a[i], b, sc1);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < NA; ++i) {
+                                wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.x, wre[i], 0, 0, 0);
+                                wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b.y, wim[i], 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int i = 0; i < NA; ++i) {
+                                wre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i].y, b.y, wre[i], 0, 0, 0);
+                                wim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b.x, wim[i], 0, 0, 0);
+                            }
                         }
+                        b = bn;
+#pragma unroll
+                        for (int i = 0; i < NA; ++i) a[i] = an[i];
                     }
                 }
-                b = bn;
-#pragma unroll
-                for (int i = 0; i < S::TW1; ++i) a[i] = an[i];
-            }
+            });
         }
         __syncthreads();  // every wavefront finished GEMM1 reading R0
         // W to LDS: tile i holds rows mt*16 + drow(lk, r), column nt1*16 + li
@@ -596,43 +618,43 @@ __global__ __launch_bounds__(kNT) void mfma_transform_kernel(const WorkItem* __r
                               e1b[g2_bz[i] + (hk >> 16)]);
             return zscale(fab[i], e);
         };
-        zc w = Wl[lk * S::CDS + cdl];
-        if constexpr (SPLIT) w = zc{ldexp(w.x, w_in), ldexp(w.y, w_in)};
-        zc a2[S::TW2];
+        with_tile_count<S::TW2>(n2, [&](auto na_c) {
+            constexpr int NA = decltype(na_c)::value;
+            if constexpr (NA > 0) {
+                zc w = Wl[lk * S::CDS + cdl];
+                if constexpr (SPLIT) w = zc{ldexp(w.x, w_in), ldexp(w.y, w_in)};
+                zc a2[NA];
 #pragma unroll
-        for (int i = 0; i < S::TW2; ++i) a2[i] = bra_operand(lk, i);
+                for (int i = 0; i < NA; ++i) a2[i] = bra_operand(lk, i);
 #pragma unroll S::UNROLL
-        for (int s = 0; s < S::KS2; ++s) {
-            const int kn = 4 * (s + 1) + lk;
-            zc wn = Wl[kn * S::CDS + cdl];
-            if constexpr (SPLIT) wn = zc{ldexp(wn.x, w_in), ldexp(wn.y, w_in)};
-            zc an[S::TW2];
+                for (int s = 0; s < S::KS2; ++s) {
+                    const int kn = 4 * (s + 1) + lk;
+                    zc wn = Wl[kn * S::CDS + cdl];
+                    if constexpr (SPLIT) wn = zc{ldexp(wn.x, w_in), ldexp(wn.y, w_in)};
+                    zc an[NA];
 #pragma unroll
-            for (int i = 0; i < S::TW2; ++i) an[i] = bra_operand(kn, i);
-            if constexpr (SPLIT) {
+                    for (int i = 0; i < NA; ++i) an[i] = bra_operand(kn, i);
+                    if constexpr (SPLIT) {
 #pragma unroll
-                for (int i = 0; i < S::TW2; ++i)
-                    if (g2_on[i]) split_step(vre[i], vim[i], a2[i], w, w_out);
-            } else {
+                        for (int i = 0; i < NA; ++i) split_step(vre[i], vim[i], a2[i], w, w_out);
+                    } else {
 #pragma unroll
-                for (int i = 0; i < S::TW2; ++i) {
-                    if (g2_on[i]) {
-                        vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.x, vre[i], 0, 0, 0);
-                        vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.y, vim[i], 0, 0, 0);
+                        for (int i = 0; i < NA; ++i) {
+                            vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.x, vre[i], 0, 0, 0);
+                            vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].x, w.y, vim[i], 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int i = 0; i < NA; ++i) {
+                            vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a2[i].y, w.y, vre[i], 0, 0, 0);
+                            vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].y, w.x, vim[i], 0, 0, 0);
+                        }
                     }
-                }
+                    w = wn;
 #pragma unroll
-                for (int i = 0; i < S::TW2; ++i) {
-                    if (g2_on[i]) {
-                        vre[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a2[i].y, w.y, vre[i], 0, 0, 0);
-                        vim[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i].y, w.x, vim[i], 0, 0, 0);
-                    }
+                    for (int i = 0; i < NA; ++i) a2[i] = an[i];
                 }
             }
-            w = wn;
-#pragma unroll
-            for (int i = 0; i < S::TW2; ++i) a2[i] = an[i];
-        }
+        });
     }
     // write: tile i holds bra components mt*16 + drow(lk, r), ket component cdl
     if (cd_on) {
