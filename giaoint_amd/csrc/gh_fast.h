@@ -182,13 +182,14 @@ __global__ __launch_bounds__(256) void pair_kernel(const FastPairQuartet* __rest
 }
 
 // ---------------------------------------------------------------------------
-// R0 kernel: one wavefront per primitive quartet. Seeds X (3) and
+// R0 kernel: a wavefront builds the tables of G = 64 / ((L+1)(L+2)/2)
+// primitive quartets at once, one per group of lanes. Seeds X (3) and
 // s_0..s_L come from the seed kernel; levels n = L-1 .. 0 of
 //   R^n_{t,u,v} = X0 R^{n+1}_{t-1,u,v} + (t-1) R^{n+1}_{t-2,u,v}    (t > 0)
 //   R^n_{0,u,v} = X1 R^{n+1}_{0,u-1,v} + (u-1) R^{n+1}_{0,u-2,v}    (u > 0)
 //   R^n_{0,0,v} = X2 R^{n+1}_{0,0,v-1} + (v-1) R^{n+1}_{0,0,v-2}    (v > 0)
 //   R^n_{0,0,0} = s_n
-// Lane c < (L+1)(L+2)/2 owns the column (u, v) of the table and keeps its
+// Lane c < (L+1)(L+2)/2 of a group owns the column (u, v) of the table and keeps its
 // entries t = 0 .. L-n-u-v of the current and the previous level in
 // registers: the t recurrence stays in the lane, and only the t = 0 entries
 // of the two neighbouring columns cross lanes (shuffles), twice per level.
@@ -200,14 +201,17 @@ __global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restri
     constexpr int NHL = plan::nh(L);
     constexpr int NCOL = (L + 1) * (L + 2) / 2;
     static_assert(NCOL <= 64, "one column per lane (r0_lds_kernel above that)");
+    constexpr int G = 64 / NCOL;  // tables per wavefront, one per group of NCOL lanes
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
     const FastPairQuartet& Q = pq[blockIdx.x];
     const int units = Q.kab * Q.kcd;
+    const int grp = lane / NCOL, col = lane - grp * NCOL;
+    const bool own = grp < G;
     // the lane's column (u, v) and the lanes holding the columns its t = 0
-    // entry is built from: (u-1, v), (u-2, v) or (0, v-1), (0, v-2)
+    // entry is built from: (u-1, v), (u-2, v) or (0, v-1), (0, v-2), in its group
     int u = 0, v = 0;
     {
-        int c = lane < NCOL ? lane : 0, uu = 0;
+        int c = own ? col : 0, uu = 0;
         while (c > L - uu) {
             c -= L - uu + 1;
             ++uu;
@@ -215,28 +219,31 @@ __global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restri
         u = uu;
         v = c;
     }
-    const bool own = lane < NCOL;
     auto column = [](int cu, int cv) { return cu * (L + 1) - cu * (cu - 1) / 2 + cv; };
-    const int src1 = u > 0 ? column(u - 1, v) : (v > 0 ? column(0, v - 1) : 0);
-    const int src2 = u > 1 ? column(u - 2, v) : (u == 0 && v > 1 ? column(0, v - 2) : 0);
+    const int g0 = own ? grp * NCOL : 0;
+    const int src1 = g0 + (u > 0 ? column(u - 1, v) : (v > 0 ? column(0, v - 1) : 0));
+    const int src2 = g0 + (u > 1 ? column(u - 2, v) : (u == 0 && v > 1 ? column(0, v - 2) : 0));
     const double k2 = u > 0 ? (double)(u - 1) : (double)(v - 1);
     const bool origin = u == 0 && v == 0;
-    // the four wavefronts of the workgroup take the quartet's primitive quartets in turn
-    for (int local = wave; local < units; local += 4) {
-        // the table's seeds: a wave-uniform address, read into scalar registers
-        const zc* sp = seeds + Q.seed_off + (long long)local * (L + 4);
-        zc sd[L + 4];
+    // the wavefronts of the workgroup take the quartet's primitive quartets in
+    // turn, G at a time (the loop is wave-uniform: every lane joins the shuffles)
+    for (int base0 = wave * G; base0 < units; base0 += 4 * G) {
+        const int local = base0 + (own ? grp : 0);
+        const bool active = own && local < units;
+        // the seeds this lane uses: X0, X1 or X2, and s_0..s_L in the origin lane
+        const zc* sp = seeds + Q.seed_off + (long long)(active ? local : base0) * (L + 4);
+        const zc X0 = sp[0];
+        const zc Xb = u > 0 ? sp[1] : sp[2];
+        zc sn[L + 1];
 #pragma unroll
-        for (int i = 0; i < L + 4; ++i) sd[i] = sp[i];
-        const zc X0 = sd[0];
-        const zc Xb = u > 0 ? sd[1] : sd[2];
+        for (int i = 0; i <= L; ++i) sn[i] = origin ? sp[3 + i] : zc{0.0, 0.0};
         // the column at the current level, updated in place from high t down:
         // entries above the column's last (L - n - u - v) are never read by
         // entries that exist
         zc cur[L + 1];
 #pragma unroll
         for (int t = 0; t <= L; ++t) cur[t] = zc{0.0, 0.0};
-        if (origin) cur[0] = sd[3 + L];
+        if (origin) cur[0] = sn[L];
 #pragma unroll
         for (int n = L - 1; n >= 0; --n) {
             const zc r1 = zc{__shfl(cur[0].x, src1), __shfl(cur[0].y, src1)};
@@ -255,9 +262,9 @@ __global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restri
                 base.x += k2 * r2.x;
                 base.y += k2 * r2.y;
             }
-            cur[0] = origin ? sd[3 + n] : base;
+            cur[0] = origin ? sn[n] : base;
         }
-        if (own) {
+        if (active) {
             zc* dst = r0 + Q.r0_off + (long long)local * NHL;
             const int tmax = L - u - v;
 #pragma unroll
