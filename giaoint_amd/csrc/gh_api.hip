@@ -29,6 +29,7 @@
 #include <thread>
 #include <tuple>
 #include <unordered_map>
+#include <span>
 #include <vector>
 
 using gh::DerivItem;
@@ -223,6 +224,7 @@ struct giaohip_ctx {
     int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
     int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
     int wave_max_l = 7;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)
+    size_t fast_pending = 0;                 // batched-path chunks queued, statistics not yet collected
     int mfma_min_l = 9;                      // default-mode matrix-core classes from this L (GIAOHIP_OPTION_MFMA_MIN_L)
     int chol_graphs = 0;                     // Cholesky column launches as hipGraphs (GIAOHIP_OPTION_CHOL_GRAPHS)
     int mfma_variant = 0;                    // compile-time variant of the matrix-core kernels (GIAOHIP_OPTION_MFMA_VARIANT)
@@ -473,6 +475,30 @@ void parallel_for(size_t n, F&& f) {
     for (auto& th : threads) th.join();
 }
 
+// Allocator whose value-less construction leaves trivial elements
+// uninitialized: large request and plan arrays are filled by parallel loops
+// right after allocation, so zeroing them first (single-threaded) is waste.
+template <class T>
+struct NoInit : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <class U>
+    NoInit(const NoInit<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+template <class T>
+using BigVec = std::vector<T, NoInit<T>>;
+
 int boys_error(giaohip_ctx* ctx, const DevError& e) {
     switch (e.code) {
         case gh::kErrBoysDomain: {
@@ -610,20 +636,51 @@ size_t seed_scratch_zc(long long pq_total) {
 // on the context and aux streams, the host plans chunk c + 1 and its seed
 // kernel runs on the plain-path stream; a slot is reused once the chunk two
 // back has finished.
-int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs_in,
-              zc* d_out, const FixedPlainMem* fixed = nullptr, bool mfma = false, bool small = false) {
+// Chunk timing of a finished slot of the pipelined generic path into the statistics.
+hipError_t plain_slot_stats(giaohip_ctx* ctx, int k) {
+    hipError_t e = hipEventSynchronize(ctx->pl_ev[k][2]);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->pl_ev[k][3]);
+    if (e != hipSuccess) return e;
+    const float t = std::max(elapsed(ctx->pl_ev[k][1], ctx->pl_ev[k][2]), elapsed(ctx->pl_ev[k][1], ctx->pl_ev[k][3]));
+    ctx->stats.seed_ms += elapsed(ctx->pl_ev[k][0], ctx->pl_ev[k][1]);
+    ctx->stats.transform_ms += t;
+    ctx->stats.contract_ms += t;
+    return hipSuccess;
+}
+
+// With `prepared`, the caller has already queued the wave vectors and the
+// zeroed error record on the context stream (work queued before this call
+// may have recorded an error there, which this call's check then reports).
+int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], std::span<const Request> reqs_in,
+              zc* d_out, const FixedPlainMem* fixed = nullptr, bool mfma = false, bool small = false,
+              bool prepared = false) {
     if (reqs_in.empty()) return GIAOHIP_OK;
     // requests with an FP64-MFMA instance, oriented for it (mfk: instance key or -1)
-    std::vector<Request> oriented;
+    BigVec<Request> oriented;
     std::vector<int> mfk;
     if (mfma && !fixed) {
-        oriented = reqs_in;
-        mfk.assign(oriented.size(), -1);
-        parallel_for(oriented.size(), [&](size_t lo, size_t hi) {
-            for (size_t i = lo; i < hi; ++i) mfk[i] = mfma_orient(b, oriented[i], ctx->mfma_mode, ctx->mfma_min_l);
+        mfk.assign(reqs_in.size(), -1);
+        std::atomic<bool> any{false};
+        parallel_for(reqs_in.size(), [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) {
+                Request r = reqs_in[i];
+                mfk[i] = mfma_orient(b, r, ctx->mfma_mode, ctx->mfma_min_l);
+                if (mfk[i] >= 0) any.store(true, std::memory_order_relaxed);
+            }
         });
+        if (any) {
+            oriented.resize(reqs_in.size());
+            parallel_for(reqs_in.size(), [&](size_t lo, size_t hi) {
+                for (size_t i = lo; i < hi; ++i) {
+                    oriented[i] = reqs_in[i];
+                    if (mfk[i] >= 0) (void)mfma_orient(b, oriented[i], ctx->mfma_mode, ctx->mfma_min_l);
+                }
+            });
+        } else {
+            mfk.clear();
+        }
     }
-    const std::vector<Request>& reqs = mfk.empty() ? reqs_in : oriented;
+    const std::span<const Request> reqs = mfk.empty() ? reqs_in : std::span<const Request>(oriented);
     hipStream_t s = ctx->stream;
     const std::vector<double> kap = wave_vectors(b, field);
     double* d_kappa;
@@ -641,10 +698,11 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
         d_kappa = ctx->kappa.as<double>();
         d_err = ctx->errrec.as<DevError>();
-        HIP_TRY(hipMemsetAsync(d_err, 0, sizeof(DevError), s), "hipMemsetAsync");
+        if (!prepared) HIP_TRY(hipMemsetAsync(d_err, 0, sizeof(DevError), s), "hipMemsetAsync");
     }
-    HIP_TRY(hipMemcpyAsync(d_kappa, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
-            "hipMemcpyAsync");
+    if (!prepared || fixed)
+        HIP_TRY(hipMemcpyAsync(d_kappa, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
     const bool piped = !fixed;
     hipStream_t ps = piped ? ctx->plain_stream : s;  // the seed kernels' stream
     if (piped) {
@@ -652,17 +710,7 @@ int run_plain(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], c
         HIP_TRY(hipEventRecord(ctx->pl_init, s), "hipEventRecord");
         HIP_TRY(hipStreamWaitEvent(ps, ctx->pl_init, 0), "hipStreamWaitEvent");
     }
-    // chunk timing of a finished slot into the statistics
-    auto slot_stats = [&](int k) -> hipError_t {
-        hipError_t e = hipEventSynchronize(ctx->pl_ev[k][2]);
-        if (e == hipSuccess) e = hipEventSynchronize(ctx->pl_ev[k][3]);
-        if (e != hipSuccess) return e;
-        const float t = std::max(elapsed(ctx->pl_ev[k][1], ctx->pl_ev[k][2]), elapsed(ctx->pl_ev[k][1], ctx->pl_ev[k][3]));
-        ctx->stats.seed_ms += elapsed(ctx->pl_ev[k][0], ctx->pl_ev[k][1]);
-        ctx->stats.transform_ms += t;
-        ctx->stats.contract_ms += t;
-        return hipSuccess;
-    };
+    auto slot_stats = [&](int k) -> hipError_t { return plain_slot_stats(ctx, k); };
     // a chunk's seeds stay within two chunk budgets (per slot when pipelined:
     // the longest items' tails are paid once per chunk, so chunks are large)
     const size_t seed_cap = 2 * ctx->seed_budget_bytes;
@@ -1101,7 +1149,10 @@ bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
 // a chunk's transform after its producer and a buffer's reuse after the
 // transform that read it. Descriptors are planned on the host into
 // page-locked staging while the device works, and the host waits once.
-int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq>& reqs, zc* d_out) {
+int fast_finish(giaohip_ctx* ctx);
+
+int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, std::span<const FastReq> reqs, zc* d_out,
+             bool finish = true) {
     if (reqs.empty()) return GIAOHIP_OK;
     // the producers run on the second stream (overlapping the previous
     // chunk's contraction) or, serially, on the context stream
@@ -1114,7 +1165,7 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
         size_t items;
     };
     // pass 1: sizes
-    std::vector<Need> need(reqs.size());
+    BigVec<Need> need(reqs.size());
     parallel_for(reqs.size(), [&](size_t lo, size_t hi) {
         for (size_t i = lo; i < hi; ++i) {
             const FastReq& f = reqs[i];
@@ -1326,6 +1377,18 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
         ++nchunks;
         begin = end;
     }
+    ctx->fast_pending = nchunks;
+    return finish ? fast_finish(ctx) : GIAOHIP_OK;
+}
+
+// Waits for the queued batched-path chunks, collects their timings and checks
+// the error record.
+int fast_finish(giaohip_ctx* ctx) {
+    const size_t nchunks = ctx->fast_pending;
+    ctx->fast_pending = 0;
+    if (nchunks == 0) return GIAOHIP_OK;
+    constexpr int kEv = 5;
+    auto event = [&](size_t c, int k) -> hipEvent_t& { return ctx->chunk_ev[kEv * c + k]; };
     HIP_TRY(hipEventSynchronize(event(nchunks - 1, 4)), "kernel execution");
     for (size_t c = 0; c < nchunks; ++c) {
         ctx->stats.seed_ms += elapsed(event(c, 0), event(c, 1));
@@ -1343,68 +1406,76 @@ int run_fast(giaohip_ctx* ctx, const giaohip_basis* b, const std::vector<FastReq
 // Runs a list of requests; outputs go to d_out (device). Full-block requests
 // of classes with a specialized kernel take the batched path (unless the
 // plain variant is selected); the rest the plain per-class kernel.
-int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], const std::vector<Request>& reqs,
+int run_requests(giaohip_ctx* ctx, const giaohip_basis* b, const double field[6], std::span<const Request> reqs,
                  zc* d_out) {
     if (reqs.empty()) return GIAOHIP_OK;
-    // split into batched and plain requests, in parallel and in input order
+    // Split into plain and batched requests, the batched ones grouped by class
+    // kernel (input order kept within a class, classes in class-table order):
+    // a chunk then holds few classes and each launch many items. Two parallel
+    // passes over contiguous thread ranges: the class key of every request and
+    // per-range counts, then a stable scatter.
     const size_t n = reqs.size();
     const bool batched = ctx->eri_variant == GIAOHIP_ERI_BATCHED;
-    std::vector<FastReq> fast_all(n);
-    std::vector<char> is_fast(n);
     const bool mfma = batched && ctx->mfma_mode != GIAOHIP_MFMA_OFF;
-    parallel_for(n, [&](size_t lo, size_t hi) {
-        for (size_t i = lo; i < hi; ++i) {
-            is_fast[i] = batched && reqs[i].cd1 < 0 && orient(b, reqs[i], fast_all[i]);
-            if (is_fast[i] && ctx->mfma_mode >= GIAOHIP_MFMA_ALL) {
-                Request r = reqs[i];
-                is_fast[i] = mfma_orient(b, r, ctx->mfma_mode, ctx->mfma_min_l) < 0;
-            }
+    const size_t nkey = b->cls_orient.size() + 1;  // key 0: plain, 1 + class index: batched
+    auto key_of = [&](const Request& r, FastReq& f) -> size_t {
+        if (!batched || r.cd1 >= 0 || !orient(b, r, f)) return 0;
+        if (ctx->mfma_mode >= GIAOHIP_MFMA_ALL) {
+            Request m = r;
+            if (mfma_orient(b, m, ctx->mfma_mode, ctx->mfma_min_l) >= 0) return 0;
+        }
+        return 1 + class_index(b, f.sh);
+    };
+    const size_t T = std::max<size_t>(1, std::min(host_threads(), (n + 8191) / 8192));
+    const size_t per = (n + T - 1) / T;
+    std::vector<size_t> count(T * nkey, 0);
+    auto in_ranges = [&](auto&& body) {
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < T; ++t) th.emplace_back([&, t] { body(t); });
+        body(0);
+        for (auto& x : th) x.join();
+    };
+    in_ranges([&](size_t t) {
+        size_t* c = count.data() + t * nkey;
+        FastReq f;
+        for (size_t i = t * per, e = std::min(n, (t + 1) * per); i < e; ++i) ++c[key_of(reqs[i], f)];
+    });
+    // exclusive offsets: plain list for key 0, batched list in key order
+    size_t nplain = 0, nfast = 0;
+    for (size_t k = 0; k < nkey; ++k)
+        for (size_t t = 0; t < T; ++t) {
+            size_t& c = count[t * nkey + k];
+            const size_t v = c;
+            c = k == 0 ? nplain : nfast;
+            (k == 0 ? nplain : nfast) += v;
+        }
+    BigVec<Request> plain(nplain);
+    BigVec<FastReq> fast(nfast);
+    in_ranges([&](size_t t) {
+        size_t* c = count.data() + t * nkey;
+        FastReq f;
+        for (size_t i = t * per, e = std::min(n, (t + 1) * per); i < e; ++i) {
+            const size_t k = key_of(reqs[i], f);
+            if (k == 0) plain[c[0]++] = reqs[i];
+            else fast[c[k]++] = f;
         }
     });
-    std::vector<Request> plain;
-    std::vector<FastReq> fast;
-    size_t nfast = 0;
-    for (size_t i = 0; i < n; ++i) nfast += is_fast[i] ? 1 : 0;
-    plain.reserve(n - nfast);
-    // The batched requests grouped by class kernel (input order kept within
-    // a class): a chunk then holds few classes and each launch many items.
-    std::vector<gh::FastLaunch> kinds;
-    std::vector<int> kind(n, -1);
-    std::vector<size_t> fill;
-    for (size_t i = 0; i < n; ++i) {
-        if (!is_fast[i]) {
-            plain.push_back(reqs[i]);
-            continue;
-        }
-        const gh::FastLaunch t = fast_all[i].info.transform;
-        int k = 0;
-        while (k < (int)kinds.size() && kinds[k] != t) ++k;
-        if (k == (int)kinds.size()) {
-            kinds.push_back(t);
-            fill.push_back(0);
-        }
-        kind[i] = k;
-        ++fill[k];
-    }
-    for (size_t k = 0, at = 0; k < fill.size(); ++k) {
-        const size_t c = fill[k];
-        fill[k] = at;
-        at += c;
-    }
-    fast.resize(nfast);
-    for (size_t i = 0; i < n; ++i)
-        if (kind[i] >= 0) fast[fill[kind[i]]++] = fast_all[i];
-    int st = run_plain(ctx, b, field, plain, d_out, nullptr, mfma, batched);
-    if (st || fast.empty()) return st;
-    // kappa and the error record were prepared by run_plain only if it ran
-    const std::vector<double> kap = wave_vectors(b, field);
+    if (fast.empty()) return run_plain(ctx, b, field, plain, d_out, nullptr, mfma, batched);
+    // The batched part first, queued without waiting: the plain part's host
+    // planning then overlaps its execution, and the plain part's final wait
+    // and error check cover both (shared context stream and error record).
     hipStream_t s = ctx->stream;
+    const std::vector<double> kap = wave_vectors(b, field);
     HIP_TRY(ctx->kappa.ensure(kap.size() * sizeof(double)), "hipMalloc");
     HIP_TRY(hipMemcpyAsync(ctx->kappa.p, kap.data(), kap.size() * sizeof(double), hipMemcpyHostToDevice, s),
             "hipMemcpyAsync");
     HIP_TRY(ctx->errrec.ensure(sizeof(DevError)), "hipMalloc");
     HIP_TRY(hipMemsetAsync(ctx->errrec.p, 0, sizeof(DevError), s), "hipMemsetAsync");
-    return run_fast(ctx, b, fast, d_out);
+    int st = run_fast(ctx, b, fast, d_out, plain.empty());
+    if (st || plain.empty()) return st;
+    st = run_plain(ctx, b, field, plain, d_out, nullptr, mfma, batched, true);
+    const int fst = fast_finish(ctx);
+    return st ? st : fst;
 }
 
 void reset_stats(giaohip_ctx* ctx) { ctx->stats = giaohip_stats{}; }
@@ -2070,7 +2141,7 @@ int giaohip_eri_batch(giaohip_ctx* ctx, const giaohip_basis* b, const uint32_t* 
     auto make_req = [&](size_t i, long long base) { return quartet_request(b, quartets, i, base); };
     int st = GIAOHIP_OK;
     if (out_on_device) {
-        std::vector<Request> reqs(n);
+        BigVec<Request> reqs(n);
         parallel_for(n, [&](size_t lo, size_t hi) {
             for (size_t i = lo; i < hi; ++i) reqs[i] = make_req(i, (long long)offs[i]);
         });
