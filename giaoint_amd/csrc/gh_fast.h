@@ -230,13 +230,25 @@ __global__ __launch_bounds__(256) void r0_kernel(const FastPairQuartet* __restri
     for (int base0 = wave * G; base0 < units; base0 += 4 * G) {
         const int local = base0 + (own ? grp : 0);
         const bool active = own && local < units;
-        // the seeds this lane uses: X0, X1 or X2, and s_0..s_L in the origin lane
-        const zc* sp = seeds + Q.seed_off + (long long)(active ? local : base0) * (L + 4);
-        const zc X0 = sp[0];
-        const zc Xb = u > 0 ? sp[1] : sp[2];
-        zc sn[L + 1];
+        // the seeds this lane uses: X0, X1 or X2, and s_0..s_L in the origin
+        // lane; with one table per wavefront the address is wave-uniform and
+        // every seed is read into scalar registers
+        const zc* sp = seeds + Q.seed_off + (long long)(G == 1 || !active ? base0 : local) * (L + 4);
+        zc X0, Xb, sn[L + 1];
+        if constexpr (G == 1) {
+            zc sd[L + 4];
 #pragma unroll
-        for (int i = 0; i <= L; ++i) sn[i] = origin ? sp[3 + i] : zc{0.0, 0.0};
+            for (int i = 0; i < L + 4; ++i) sd[i] = sp[i];
+            X0 = sd[0];
+            Xb = u > 0 ? sd[1] : sd[2];
+#pragma unroll
+            for (int i = 0; i <= L; ++i) sn[i] = sd[3 + i];
+        } else {
+            X0 = sp[0];
+            Xb = u > 0 ? sp[1] : sp[2];
+#pragma unroll
+            for (int i = 0; i <= L; ++i) sn[i] = origin ? sp[3 + i] : zc{0.0, 0.0};
+        }
         // the column at the current level, updated in place from high t down:
         // entries above the column's last (L - n - u - v) are never read by
         // entries that exist
