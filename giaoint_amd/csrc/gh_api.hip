@@ -1128,8 +1128,22 @@ void build_class_tables(giaohip_basis* b) {
 
 bool orient(const giaohip_basis* b, const Request& r, FastReq& f) {
     const size_t ci = class_index(b, r.sh);
-    const int o = b->cls_orient[ci];
+    int o = b->cls_orient[ci];
     if (o < 0) return false;
+    // Where the image with the pairs exchanged has the same class (bra and
+    // ket pairs of equal momenta), it uses the same kernel: the pair with
+    // more primitive pairs goes to the bra side, whose primitive pairs are
+    // the kernel's rows.
+    {
+        const int o2 = o ^ 1;
+        bool same = true;
+        for (int k = 0; k < 4; ++k)
+            same = same && b->shells[r.sh[kPerms[o2][k]]].l == b->shells[r.sh[kPerms[o][k]]].l;
+        auto kbra = [&](int img) {
+            return b->shells[r.sh[kPerms[img][0]]].nprim * b->shells[r.sh[kPerms[img][1]]].nprim;
+        };
+        if (same && kbra(o2) > kbra(o)) o = o2;
+    }
     for (int k = 0; k < 4; ++k) {
         f.sh[k] = r.sh[kPerms[o][k]];
         f.perm[k] = kPerms[o][k];
