@@ -52,7 +52,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
-    ap.add_argument("--chunk-mib", type=int, default=0, help="batched-path chunk size (default: library's)")
+    ap.add_argument("--chunk-mib", type=int, default=8192,
+                    help="batched-path chunk size of the headline leg in MiB (0: autotuned with the workgroup size); "
+                         "fixed by default so that the contraction launches of every run have one size")
     ap.add_argument("--cholesky-rank", type=int, default=0,
                     help="Cholesky leg: stop after this many pivots (0: run to convergence, -1: skip the leg)")
     ap.add_argument("--cholesky-block", type=int, default=0,
