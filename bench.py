@@ -228,12 +228,12 @@ def mixed_leg(n: int, tune: bool = True, tune_n: int = 500_000):
     the 2M-quartet workload. With `tune`, two kernel choices are autotuned on
     the first tune_n quartets: the largest total angular momentum whose
     generic-transform items run on one-wavefront workgroups, and the smallest
-    whose classes run on the FP64 matrix cores; then one warmup call and one
-    timed call of the batched variant (class-specialized kernels where they
-    exist, matrix cores from the tuned momentum, the generic kernel for the
-    rest) and of the plain variant (the generic kernel for every class),
-    inputs and output resident on the device. Returns the leg's record and
-    the batch."""
+    whose classes run on the FP64 matrix cores; then one untimed and one
+    timed call over the whole batch with the batched variant (class-specialized
+    kernels where they exist, matrix cores from the tuned momentum, the
+    generic kernel for the rest) and with the plain variant (the generic
+    kernel for every class), inputs and output resident on the device.
+    Returns the leg's record and the batch."""
     from giaoint_amd import _native, autotune, workloads
 
     t0 = time.perf_counter()
@@ -258,7 +258,9 @@ def mixed_leg(n: int, tune: bool = True, tune_n: int = 500_000):
     try:
         for variant in ("batched", "plain"):
             ctx.set_variant(variant)
-            db.eri_batch(batch.quartets[: min(n, 20_000)], f, device_out=out)
+            # one untimed call over the whole batch: the first call at a new
+            # size grows the context's device and page-locked buffers
+            db.eri_batch(batch.quartets, f, device_out=out)
             ctx.synchronize()
             t = time.perf_counter()
             db.eri_batch(batch.quartets, f, device_out=out)
