@@ -60,3 +60,35 @@ def test_fast_classes_on_mixed_batch(gpu, port):
     diff = float(np.max(np.abs(fast - plain)))
     print(f"max |batched - plain| over 4000 mixed quartets = {diff:.2e}")
     assert diff <= ATOL
+
+
+def test_batch_error_from_either_part(gpu, port):
+    """A Boys domain error in a batch that has both batched (class-specialized)
+    and plain (lane-per-primitive-quartet) requests is reported whichever part
+    it comes from, and the context computes the next batch correctly. The
+    failing quartet's Boys argument is -225 (|z| > 160, the reference's
+    verified direct domain), as the oracle reports for it too."""
+    from oracle.pyoracle import CheckerError
+
+    far = [(np.array([3.0, 0.0, 0.0]), 0, np.array([0.01]), np.array([[1.0]])),
+           (np.array([-3.0, 0.0, 0.0]), 0, np.array([0.01]), np.array([[1.0]])),
+           (np.array([3.0, 0.0, 0.0]), 1, np.array([0.01]), np.array([[1.0]]))]
+    rng = np.random.default_rng(17)
+    near = random_shells(rng, [2, 2, 1, 0], nprim=(1, 3), ncontr=(1, 2), box=1.0, log_exp=(-0.3, 1.0))
+    b = make_basis(far + near, port)
+    db = _native.DeviceBasis(gpu, b.centers, b.L, b.nprim, b.ncontr, b.exps, b.coefs)
+    B, O = (0.0, 0.0, 1.0), (0.0, 0.0, 0.0)
+    f = _native.field6(B, O)
+    good = [(3, 4, 3, 4), (3, 4, 4, 3), (5, 6, 5, 6), (3, 3, 5, 6), (6, 6, 6, 6), (5, 6, 6, 6)]
+    with pytest.raises(CheckerError, match="outside the verified"):
+        port.eri_batch(b, [(0, 1, 1, 0)], port.field6(B, O), 1)
+    gpu.set_variant("batched")
+    # the failing quartet in the batched part ((ss|ss) has a class kernel)
+    with pytest.raises(_native.BoysNumericalError, match="outside the verified"):
+        db.eri_batch(good + [(0, 1, 1, 0)], f)
+    # in the plain part ((ss|sp) runs on the lane-per-primitive-quartet kernel)
+    with pytest.raises(_native.BoysNumericalError, match="outside the verified"):
+        db.eri_batch(good + [(0, 1, 1, 2)], f)
+    got = db.eri_batch(good, f)
+    want, _ = port.eri_batch(b, good, port.field6(B, O), 16)
+    assert float(np.max(np.abs(got - want))) <= ATOL
