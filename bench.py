@@ -249,7 +249,7 @@ def mixed_leg(n: int, tune: bool = True, tune_n: int = 500_000):
     tuned = None
     if tune:
         m = min(n, tune_n)
-        tuned = autotune.tune_eri(db, batch.quartets[:m], f, knobs=("wave_max_l", "mfma_min_l"), reps=1,
+        tuned = autotune.tune_eri(db, batch.quartets[:m], f, knobs=("fast_waves", "wave_max_l", "mfma_min_l"), reps=1,
                                   workload=f"mixed l<=3 degree 1..8, first {m} of {n} quartets")
         log(f"mixed: autotuned {tuned.values} " +
             " ".join(f"{t.value}={t.seconds:.3f}s" for t in tuned.trials))

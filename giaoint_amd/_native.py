@@ -236,7 +236,7 @@ class Context:
 
     def set_fast_waves(self, waves: int):
         """Largest workgroup of the class-specialized transform in waves
-        (0 = library default, else 4..8); it sets the bra-pair tile per pass."""
+        (4..8, or 0 for 8; default 4); it sets the bra-pair tile per pass."""
         self.check(self.lib.giaohip_set_option(self.handle, 6, int(waves)))
 
     MFMA_MODES = {"off": 0, "default": 1, "all": 2, "split32": 3}

@@ -53,7 +53,7 @@ import numpy as np
 
 from . import _native
 
-FAST_WAVES = (0, 4, 5, 6, 7)
+FAST_WAVES = (4, 5, 6, 7, 0)
 ERI_QUEUE = ("none", "buckets", "single")
 CHUNK_MIB = (1024, 2048, 4096, 8192, 16384)
 CHOLESKY_BLOCK = (8, 16, 24, 32)
@@ -61,7 +61,7 @@ WAVE_MAX_L = (-1, 5, 6, 7, 8)
 MFMA_MIN_L = (7, 9, 11, 14)
 MFMA_VARIANT = (0, 1, 2, 3)
 SMALL_MAX_L = (-1, 2)
-DEFAULTS = {"fast_waves": 0, "eri_queue": "none", "chunk_mib": 2048, "cholesky_block": 32, "wave_max_l": 5,
+DEFAULTS = {"fast_waves": 4, "eri_queue": "none", "chunk_mib": 16384, "cholesky_block": 32, "wave_max_l": 7,
             "mfma_min_l": 9, "mfma_variant": 0, "small_max_l": 2}
 # knobs that do not change any sum: candidates must match bitwise
 EXACT = ("eri_queue", "chunk_mib", "wave_max_l", "mfma_variant")

@@ -220,7 +220,7 @@ struct giaohip_ctx {
     int chol_pivot_batch = 32;  // cholesky_eri's pivot_batch_size
     size_t staging_budget_bytes = size_t(1) << 30;
     int eri_queue = GIAOHIP_QUEUE_NONE;  // generic-kernel launch form (GIAOHIP_OPTION_ERI_QUEUE)
-    int fast_waves_cap = 0;                  // class-specialized workgroup cap (0: kFastMaxWaves)
+    int fast_waves_cap = 4;                  // class-specialized workgroup cap (0: kFastMaxWaves)
     int mfma_mode = GIAOHIP_MFMA_DEFAULT;    // FP64-MFMA contraction use (GIAOHIP_OPTION_MFMA)
     int pipeline = GIAOHIP_PIPELINE_OVERLAP; // producer stream of the batched path (GIAOHIP_OPTION_PIPELINE)
     int wave_max_l = 7;                      // one-wavefront generic items up to this L (GIAOHIP_OPTION_WAVE_MAX_L)

@@ -84,8 +84,9 @@ enum {
        identical for every form. */
     GIAOHIP_OPTION_ERI_QUEUE = 5,
     /* Largest workgroup of the class-specialized transform, in waves:
-       0 (default: 8) or 4..8. A workgroup takes as many bra primitive pairs
-       per pass as its rows hold, so the value sets the tile of bra pairs. */
+       0 (meaning 8) or 4..8; default 4. A workgroup takes as many bra
+       primitive pairs per pass as its rows hold, so the value sets the tile
+       of bra pairs. */
     GIAOHIP_OPTION_FAST_WAVES = 6,
     /* Classes whose E.R.E contraction runs on the FP64 matrix cores
        (v_mfma_f64_16x16x4f64, dense complex Hermite products) under the
