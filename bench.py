@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # scaled by a copy-kernel calibration, and FP64 operations issued from the
 # SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64 counters (64 lanes, FMA = 2)
 # (profiles/fast_transform_kernel_dddd_200k_pmc.md, profiles/fp64_mfma_lds_counters.md).
-TRANSFORM_HBM_BYTES_PER_QUARTET = {4: 474_894, 5: 370_995}
+TRANSFORM_HBM_BYTES_PER_QUARTET = {4: 477_171, 5: 370_995}
 TRANSFORM_FP64_ISSUED_PER_QUARTET = {4: 1.1159e7}
 TRANSFORM_TRAFFIC_NOTE = ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), "
                           "factors from a 256 MiB copy; {b:,} B per (dd|dd) quartet at {w} waves x quartets per launch")
