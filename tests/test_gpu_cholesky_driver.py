@@ -5,7 +5,10 @@ python/giao_integrals/cholesky.py:60-297) runs on the C oracle's diagonal
 and columns. Pivots must agree except where the two candidates' residuals
 tie to 1e-12 relative (a rounding-level tie may reorder them); factors must
 agree to 1e-10 up to that point; the final residual must be below the
-stopping tolerance.
+stopping tolerance. Exact ties are common: the ordered pairs (mu nu) and
+(nu mu) have equal diagonals, so the pivot order within such a pair is a
+rounding decision; the tests bound the number of tie decisions taken the
+other way (at most half the rank) and print it.
 """
 
 import numpy as np
@@ -40,6 +43,7 @@ def test_device_cholesky_matches_checker(gpu, port, natoms):
         real=False, single_center=False, threshold=1e-8, follow=gp, stats=stats,
     )
     print(f"rank {res.rank}; pivots decided by rounding-level ties: {stats.get('tie_flips', 0)}")
+    assert stats.get("tie_flips", 0) <= res.rank // 2
     assert abs(res.tolerance - tol) <= 1e-12 * tol
     assert len(piv) == res.rank
     worst = 0.0
@@ -204,6 +208,7 @@ def test_device_cholesky_real_arithmetic(gpu, port, case):
         real=True, single_center=(case == "one-centre"), threshold=1e-8, follow=gp, stats=stats,
     )
     print(f"{case}: rank {res.rank}; rounding-level tie decisions {stats.get('tie_flips', 0)}")
+    assert stats.get("tie_flips", 0) <= res.rank // 2
     assert len(piv) == res.rank
     got = res.vectors()
     assert np.all(got.imag == 0.0)
@@ -281,6 +286,7 @@ def test_device_cholesky_general_contractions(gpu, port, case):
     )
     print(f"{case}: rank {res.rank}; {stats.get('overrides', 0)} pivots from the block instead of the global "
           f"maximum; pivots decided by rounding-level ties {stats.get('tie_flips', 0)}")
+    assert stats.get("tie_flips", 0) <= res.rank // 2
     assert stats.get("overrides", 0) > 0  # the policy's local override is exercised
     assert len(piv) == res.rank
     assert abs(res.tolerance - tol) <= 1e-12 * tol
